@@ -1,0 +1,293 @@
+// Device-side data layout and per-pair maths of the CryoSPH step on gfx950.
+//
+// Layout in HBM (structure of arrays, capacity-sized, one allocation each):
+//   persistent (permuted by the cell sort, double-buffered for the gather):
+//     x, v, vt, nrm : float4 (xyz, w = 0)      m, m0, rho, T : float
+//     phase, flags  : uint8                     id            : uint32
+//   scratch (indexed like the reference: never permuted, rewritten before use):
+//     a, ab, ns, vw, fw, mw : float4            p, drho, dTdt, sigma, kappa, nu_t,
+//     alpha, wall_w, heat_w, out_m, out_h, w_norm, p_wv, rmin : float
+// float4 keeps every vector load/store a single 16-byte access.
+//
+// Arithmetic: FP32 in the same operation order as the reference per-pair
+// bodies (SPH_kernels.cu, navier_stokes.cu, surface_tension.cu,
+// phase_change.cu); everything is compiled with -ffp-contract=off so that no
+// multiply-add is fused, and FP32 division / sqrt are IEEE correctly rounded.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace cryo {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr uint8_t kLiquid = 0, kVapor = 1, kWall = 2, kDead = 3;
+constexpr uint8_t kHeated = 1, kSource = 2, kTargetVapor = 4, kDying = 8, kScratchMask = 2 | 4 | 8;
+
+// ------------------------------------------------------------------ vectors
+struct V3 {
+  float x, y, z;
+};
+__host__ __device__ inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__host__ __device__ inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__host__ __device__ inline V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+__host__ __device__ inline V3 operator*(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__host__ __device__ inline V3 operator/(V3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__host__ __device__ inline V3& operator+=(V3& a, V3 b) {
+  a.x += b.x;
+  a.y += b.y;
+  a.z += b.z;
+  return a;
+}
+__host__ __device__ inline V3& operator-=(V3& a, V3 b) {
+  a.x -= b.x;
+  a.y -= b.y;
+  a.z -= b.z;
+  return a;
+}
+__host__ __device__ inline V3& operator*=(V3& a, float s) {
+  a.x *= s;
+  a.y *= s;
+  a.z *= s;
+  return a;
+}
+__host__ __device__ inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__host__ __device__ inline V3 cross(V3 a, V3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__host__ __device__ inline float norm2(V3 a) { return dot(a, a); }
+__host__ __device__ inline float norm(V3 a) { return sqrtf(dot(a, a)); }
+__host__ __device__ inline V3 normalized(V3 a, float eps = 1e-20f) {
+  const float n = norm(a);
+  return n > eps ? a / n : V3{0, 0, 0};
+}
+__host__ __device__ inline V3 ld3(const float4& q) { return V3{q.x, q.y, q.z}; }
+__host__ __device__ inline float4 st3(V3 a) { return make_float4(a.x, a.y, a.z, 0.0f); }
+template <class T>
+__host__ __device__ inline T clampv(T v, T lo, T hi) {
+  return v < lo ? lo : (v > hi ? hi : v);
+}
+__host__ __device__ inline bool is_fluid(uint8_t ph) { return ph <= kVapor; }
+__host__ __device__ inline int pidx(uint8_t ph) { return ph == kVapor ? 1 : 0; }
+
+// ------------------------------------------------------------ smoothing kernel
+struct KernelP {
+  int type;  // 0 cubic spline, 1 Wendland C2
+  float h, inv_h, support, norm;
+};
+__host__ __device__ inline float kW(const KernelP& k, float r) {
+  const float q = r * k.inv_h;
+  if (q >= 2.0f) return 0.0f;
+  if (k.type == 0) {
+    if (q < 1.0f) return k.norm * (1.0f - 1.5f * q * q + 0.75f * q * q * q);
+    const float t = 2.0f - q;
+    return k.norm * 0.25f * t * t * t;
+  }
+  const float t = 1.0f - 0.5f * q;
+  const float t2 = t * t;
+  return k.norm * t2 * t2 * (2.0f * q + 1.0f);
+}
+__host__ __device__ inline float kdW(const KernelP& k, float r) {
+  const float q = r * k.inv_h;
+  if (q >= 2.0f) return 0.0f;
+  if (k.type == 0) {
+    if (q < 1.0f) return k.norm * k.inv_h * (-3.0f * q + 2.25f * q * q);
+    const float t = 2.0f - q;
+    return k.norm * k.inv_h * -0.75f * t * t;
+  }
+  const float t = 1.0f - 0.5f * q;
+  return k.norm * k.inv_h * -5.0f * q * t * t * t;
+}
+__host__ __device__ inline V3 kgradW(const KernelP& k, V3 rij, float r) {
+  if (r <= 1e-12f * k.h) return V3{0, 0, 0};
+  return rij * (kdW(k, r) / r);
+}
+__host__ __device__ inline float akinci_coh(float r, float c) {
+  if (r >= c || r <= 0.0f) return 0.0f;
+  const float c3 = c * c * c;
+  const float c9 = c3 * c3 * c3;
+  const float coeff = 32.0f / (kPi * c9);
+  const float t = (c - r);
+  const float base = t * t * t * r * r * r;
+  if (2.0f * r > c) return coeff * base;
+  return coeff * (2.0f * base - c3 * c3 / 64.0f);
+}
+
+// ---------------------------------------------------------- static params
+// Everything init_params derives once (sph_solver.cu:292-405).  Passed by
+// value to every kernel; values that change during a run live in DevState.
+struct SimParams {
+  KernelP kernel;
+  float dx, vol0, support2;
+  float rho0[2], mu[2], k[2], cp[2], mref[2];
+  float c0, tait_B, tait_gamma, p_bg, gas_R, vapor_K;
+  int clamp_neg;
+  float alpha_av, alpha_av_vap, delta_sph;
+  int density_mode, no_slip, sps, tvf, tvf_stress;
+  float tvf_factor, tvf_sigma_min, shift_strength;
+  int tvf_interface_only;
+  float smag_cs;
+  int st_model;
+  float sigma, cos_theta, sin_theta, normal_eps, wall_adh_w;
+  int wetting, wall_colour, csf_smooth, csf_balanced;
+  float ak_gamma, ak_beta, ak_adh_scale;  // ak_adh_scale = 0.007 / c^3.25 (host)
+  int thermal, phase_change, wall_bc;
+  float T_wall, T_ref, h_fg, lee_evap, lee_cond, max_transfer, vapor_rebalance, pc_density_limit, merge_ratio,
+      split_ratio, heat_power;
+  int has_vapor;
+};
+
+// ------------------------------------------------------- device state block
+// Device-resident per-step state: the host never needs it to launch a step.
+struct DevState {
+  double time, last_dt;
+  long long step;
+  double heat_input, evaporated, condensed, vented, vented_h, p_ullage, fluid_volume0;
+  float dt;             // P.dt of the reference (time step of the current step)
+  float T_sat;          // current saturation temperature
+  float heat_inv_norm;  // 1 / sum heat_w
+  float v_max, a_max, Tv_max, nu_t_max, spacing_ratio;  // evaluate_rates maxima
+  // frame (sph_solver.cu:660-667)
+  V3 cfg_gravity, frame_acc, gravity, omega, omega_dot, origin;
+  int frame_rot;
+  int n;        // live particles
+  int n_alive;  // after the last cell sort
+  int splits, merges;
+  unsigned next_id;
+  unsigned split_requested;
+  unsigned max_bits[5];  // atomicMax scratch for the maxima (float bits, all >= 0)
+  int pad;
+};
+
+// --------------------------------------------------------------- equations
+__host__ __device__ inline float eos_pressure(const SimParams& P, uint8_t ph, float rho, float T) {
+  float p;
+  if (ph == kVapor) {
+    p = rho * P.gas_R * T + (P.p_bg - P.vapor_K);
+  } else {
+    const float ratio = rho / P.rho0[0];
+    float rg;
+    if (P.tait_gamma == 7.0f) {
+      const float r2 = ratio * ratio;
+      rg = r2 * r2 * r2 * ratio;
+    } else {
+      rg = (float)pow((double)ratio, (double)P.tait_gamma);
+    }
+    p = P.tait_B * (rg - 1.0f) + P.p_bg;
+  }
+  return (P.clamp_neg && p < 0.0f) ? 0.0f : p;
+}
+__host__ __device__ inline float sound_speed(const SimParams& P, uint8_t ph, float T) {
+  if (ph == kVapor) return sqrtf(P.gas_R * (T > 0.0f ? T : 0.0f));
+  return P.c0;
+}
+__host__ __device__ inline float enthalpy(const SimParams& P, uint8_t ph, float T) {
+  return ph == kVapor ? P.h_fg + P.cp[1] * (T - P.T_ref) : P.cp[0] * (T - P.T_ref);
+}
+__host__ __device__ inline float temp_from_h(const SimParams& P, uint8_t ph, float h) {
+  return ph == kVapor ? P.T_ref + (h - P.h_fg) / P.cp[1] : P.T_ref + h / P.cp[0];
+}
+
+// Time-step limits (sph_solver.cu:482-504); cfg numbers passed in DtLimits.
+struct DtLimits {
+  float cfl, viscous_number, force_number, capillary_number, thermal_number, dt_max, dt_min, fixed_dt;
+};
+__host__ __device__ inline float compute_dt(const SimParams& P, const DtLimits& L, float v_max, float a_max,
+                                            float Tv_max, float nu_t_max, float spacing_ratio) {
+  if (L.fixed_dt > 0.0f) return L.fixed_dt;
+  const float h = P.kernel.h;
+  float c_max = P.c0;
+  if (P.has_vapor) {
+    const float tv = Tv_max > P.T_ref ? Tv_max : P.T_ref;
+    const float cv = sqrtf(P.gas_R * tv);
+    c_max = c_max < cv ? cv : c_max;
+  }
+  const float sr = 1.0f < spacing_ratio ? spacing_ratio : 1.0f;
+  float dt = L.cfl * h / ((c_max + v_max) * sr);
+  const float n0 = P.mu[0] / P.rho0[0], n1 = P.mu[1] / P.rho0[1];
+  const float nu = (n0 < n1 ? n1 : n0) + nu_t_max;
+  if (nu > 0.0f) {
+    const float d = L.viscous_number * h * h / nu;
+    dt = d < dt ? d : dt;
+  }
+  if (a_max > 0.0f) {
+    const float d = L.force_number * sqrtf(h / a_max);
+    dt = d < dt ? d : dt;
+  }
+  if (P.sigma > 0.0f) {
+    const float rho_avg = 0.5f * (P.rho0[0] + (P.has_vapor ? P.rho0[1] : P.rho0[0]));
+    const float d = L.capillary_number * sqrtf(rho_avg * h * h * h / (2.0f * kPi * P.sigma));
+    dt = d < dt ? d : dt;
+  }
+  if (P.thermal) {
+    float alpha = P.k[0] / (P.rho0[0] * P.cp[0]);
+    if (P.has_vapor) {
+      const float a1 = P.k[1] / (P.rho0[1] * P.cp[1]);
+      alpha = alpha < a1 ? a1 : alpha;
+    }
+    if (alpha > 0.0f) {
+      const float d = L.thermal_number * h * h / alpha;
+      dt = d < dt ? d : dt;
+    }
+  }
+  return dt < L.dt_min ? L.dt_min : (L.dt_max < dt ? L.dt_max : dt);
+}
+
+// --------------------------------------------------------------------- grid
+// Uniform grid, cell = 2h (spatial_hash.hpp:14-56).  `off` is the exclusive
+// prefix of particles per cell (num_cells + 1 entries): the particles of cell c
+// are [off[c], off[c+1]).  Cells x0..x1 of one (y, z) row are consecutive keys,
+// so a row is one contiguous index range, and the 9 rows in z, y order visit
+// neighbours in the reference's order (27 cells z->y->x, index ascending).
+struct GridDev {
+  float ox, oy, oz, inv_cell;
+  int nx, ny, nz, num_cells;
+  const int* off;
+};
+__host__ __device__ inline void cell_coords(const GridDev& g, V3 p, int& cx, int& cy, int& cz) {
+  cx = clampv(static_cast<int>(floorf((p.x - g.ox) * g.inv_cell)), 0, g.nx - 1);
+  cy = clampv(static_cast<int>(floorf((p.y - g.oy) * g.inv_cell)), 0, g.ny - 1);
+  cz = clampv(static_cast<int>(floorf((p.z - g.oz) * g.inv_cell)), 0, g.nz - 1);
+}
+__host__ __device__ inline int cell_index(const GridDev& g, V3 p) {
+  int cx, cy, cz;
+  cell_coords(g, p, cx, cy, cz);
+  return (cz * g.ny + cy) * g.nx + cx;
+}
+
+// Visits every particle of the 27 cells around p, in the reference order.
+template <class F>
+__device__ inline void for_each_nbr(const GridDev& g, V3 p, F&& f) {
+  int cx, cy, cz;
+  cell_coords(g, p, cx, cy, cz);
+  const int z0 = cz > 0 ? cz - 1 : 0, z1 = cz < g.nz - 1 ? cz + 1 : cz;
+  const int y0 = cy > 0 ? cy - 1 : 0, y1 = cy < g.ny - 1 ? cy + 1 : cy;
+  const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx < g.nx - 1 ? cx + 1 : cx;
+  for (int z = z0; z <= z1; ++z) {
+    for (int y = y0; y <= y1; ++y) {
+      const int row = (z * g.ny + y) * g.nx;
+      const int s = g.off[row + x0], e = g.off[row + x1 + 1];
+      for (int j = s; j < e; ++j) f(j);
+    }
+  }
+}
+
+// ------------------------------------------------------------- particle SoA
+struct Persist {
+  float4 *x, *v, *vt, *nrm;
+  float *m, *m0, *rho, *T;
+  uint8_t *phase, *flags;
+  uint32_t* id;
+};
+struct Scratch {
+  float4 *a, *ab, *ns, *vw, *fw, *mw;
+  float *p, *drho, *dTdt, *sigma, *kappa, *nu_t, *alpha, *wall_w, *heat_w, *out_m, *out_h, *w_norm, *p_wv, *rmin;
+};
+struct PV {  // kernel view
+  Persist s;
+  Scratch t;
+  int capacity;
+};
+
+}  // namespace cryo

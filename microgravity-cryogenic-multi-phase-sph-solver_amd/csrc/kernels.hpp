@@ -1,0 +1,81 @@
+// Host-callable launchers of the gfx950 kernels (all asynchronous on `s`).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "cryo_device.hpp"
+
+namespace cryo {
+
+// Prescribed tank motion table on the device (ManeuverProfile, sph_solver.hpp:96-105).
+struct Maneuver {
+  int n;               // 0 = no manoeuvre
+  const double* time;  // n
+  const V3* acc;       // n
+  const V3* omega;     // n or nullptr
+  V3 origin;
+};
+
+// Fluid constants used by the ullage update (FluidProperties, fluid_properties.hpp:11-38).
+struct FluidConst {
+  float rho_liquid, rho_vapor, T_sat, p_sat, R_gas, h_fg;
+};
+
+// grid
+void launch_kick_drift_keys(const PV& pv, const GridDev& g, DevState* st, int n, bool do_kick, bool continuity,
+                            bool tvf, uint32_t* keys, uint32_t* vals, hipStream_t s);
+int radix_tiles(int n);
+void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, int n,
+                       int key_bits, uint32_t* tile_hist, hipStream_t s, bool* result_in_alt);
+void launch_cell_offsets(const uint32_t* keys, int n, int num_cells, int* off, DevState* st, hipStream_t s);
+void launch_gather(const Persist& dst, const Persist& src, const uint32_t* perm, const DevState* st, int n_ub,
+                   hipStream_t s);
+void launch_neighbor_stats(const float4* x, const uint32_t* id, const GridDev& g, int n, float r2max, int* counts,
+                           unsigned long long* sums, hipStream_t s);
+void launch_neighbor_fill(const float4* x, const GridDev& g, int n, float r2max, const long long* offsets,
+                          int* indices, hipStream_t s);
+void launch_scan_one_block(uint32_t* data, int total, uint32_t* total_out, hipStream_t s);
+
+// physics
+void launch_density(const PV& pv, const GridDev& g, const SimParams& P, int n, hipStream_t s);
+void launch_normalize_masses(const PV& pv, const GridDev& g, const SimParams& P, int n, hipStream_t s);
+void launch_wall_bc(const PV& pv, const GridDev& g, const SimParams& P, const DevState* st, int n, hipStream_t s);
+int launch_interface(const PV& pv, const GridDev& g, const SimParams& P, DevState* st, int n, bool csf_mode,
+                     double* heat_partials, hipStream_t s);
+void launch_smooth_normals(const PV& pv, const GridDev& g, const SimParams& P, int n, hipStream_t s);
+void launch_heat_norm(DevState* st, const double* partials, int nb, bool active, hipStream_t s);
+void launch_forces(const PV& pv, const GridDev& g, const SimParams& P, DevState* st, int n, bool csf_mode,
+                   bool akinci, bool continuity, bool thermal, hipStream_t s);
+void launch_probe(const PV& pv, const GridDev& g, const SimParams& P, const float* pts, float* out, int np,
+                  hipStream_t s);
+void launch_shifting(const PV& pv, const GridDev& g, const SimParams& P, const DevState* st, int n, hipStream_t s);
+
+// step
+int diag_quantities();
+void launch_begin_step(DevState* st, const SimParams& P, const DtLimits& L, const Maneuver& M, hipStream_t s);
+void launch_kick_sources(const PV& pv, const SimParams& P, DevState* st, int n, bool do_kick, bool thermal,
+                         bool sources, hipStream_t s);
+int launch_pc_normalize(const PV& pv, const GridDev& g, const SimParams& P, int n, double* partials, int stride,
+                        hipStream_t s);
+void launch_pc_apply(const PV& pv, const GridDev& g, const SimParams& P, int n, hipStream_t s);
+void launch_rebalance(const PV& pv, const GridDev& g, const SimParams& P, int n, hipStream_t s);
+void launch_rebalance_apply_mark(const PV& pv, const SimParams& P, int n, bool rebalance, uint32_t* flag,
+                                 hipStream_t s);
+void launch_exclusive_scan(const uint32_t* in, uint32_t* out, int n, uint32_t* block_sums, uint32_t* total,
+                           hipStream_t s);
+void launch_split(const PV& pv, const SimParams& P, const DevState* st, int n, const uint32_t* flag,
+                  const uint32_t* offset, hipStream_t s);
+void launch_split_commit(DevState* st, int n, int capacity, const uint32_t* total, hipStream_t s);
+int launch_venting(const PV& pv, const SimParams& P, const DevState* st, int n_ub, float relax, double* partials,
+                   int stride, hipStream_t s);
+int launch_inventory(const PV& pv, const DevState* st, int n_ub, double* partials, int stride, hipStream_t s);
+void launch_end_step(DevState* st, const double* partials, int stride, int nb_pc, int nb_vent, int nb_inv,
+                     const FluidConst& fl, bool closed, bool advance_clock, hipStream_t s);
+void launch_scale_velocities(const PV& pv, int n, float f, hipStream_t s);
+void launch_diagnostics(const PV& pv, const SimParams& P, int n, double* partials, int stride, unsigned* maxbits,
+                        double* out, hipStream_t s);
+void launch_telemetry_pack(const PV& pv, int n, float* rec, hipStream_t s);
+
+}  // namespace cryo

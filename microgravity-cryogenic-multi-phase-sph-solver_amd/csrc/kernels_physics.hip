@@ -1,0 +1,688 @@
+// Neighbour-loop physics on gfx950 (one thread per target particle).
+//
+// The reference runs one kernel per operator (evaluate_rates,
+// sph_solver.cu:449-480).  Here operators whose inputs are ready at the same
+// point share one neighbour sweep:
+//   k_density    density summation + EOS          (SPH_kernels.cu:6-34, navier_stokes.cu:5-14)
+//   k_wall_bc    Adami wall extrapolation          (navier_stokes.cu:16-69)
+//   k_interface  colour gradient, alpha, wall/heat weights, contact angle, SPS,
+//                pointwise CSF normals             (surface_tension.cu:5-118, navier_stokes.cu:71-119)
+//   k_forces     transport velocity, momentum with wall loads, CSF or Akinci,
+//                continuity, conduction, dt maxima (SPH_kernels.cu:56-104,162-203,
+//                navier_stokes.cu:121-245, surface_tension.cu:120-213,
+//                phase_change.cu:31-75, sph_solver.cu:62-99)
+// Every accumulator adds its neighbours in the reference order (ascending
+// sorted index), with the reference's per-pair expressions, so each field is
+// the same float the reference computes (given equal inputs).
+#include "kernels.hpp"
+
+namespace cryo {
+
+namespace {
+
+constexpr int kBlock = 256;
+__device__ inline int gtid() { return static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); }
+inline int blocks_for(long long n) { return static_cast<int>((n + kBlock - 1) / kBlock); }
+
+// Deterministic FP64 block sum: fixed shuffle tree per wave, waves in order.
+__device__ inline void block_sum_store(double v, double* out) {
+  __shared__ double wsum[kBlock / 64];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) wsum[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0;
+    for (int w = 0; w < kBlock / 64; ++w) s += wsum[w];
+    *out = s;
+  }
+}
+
+// Max of non-negative floats (NaN ignored, as `v > acc ? v : acc` does).
+__device__ inline float wave_max(float v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const float o = __shfl_down(v, off, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+__device__ inline void atomic_max_nonneg(unsigned* addr, float v) {
+  if (v > 0.0f) atomicMax(addr, __float_as_uint(v));
+}
+
+// pow for positive float arguments: evaluated in double and rounded once.
+__device__ inline float powf_cr(float x, float y) { return static_cast<float>(pow(static_cast<double>(x), static_cast<double>(y))); }
+
+// ------------------------------------------------------- density + EOS
+__global__ __launch_bounds__(kBlock) void k_density(PV pv, GridDev g, SimParams P, int n) {
+  const int i = gtid();
+  if (i >= n) return;
+  const uint8_t ph = pv.s.phase[i];
+  if (!is_fluid(ph)) {
+    pv.t.sigma[i] = 1.0f / P.vol0;
+    pv.t.rmin[i] = P.kernel.support;
+    return;
+  }
+  const V3 xi = ld3(pv.s.x[i]);
+  float sum = 0.0f, r2min = P.support2;
+  for_each_nbr(g, xi, [&](int j) {
+    const float r2 = norm2(xi - ld3(pv.s.x[j]));
+    if (r2 >= P.support2) return;
+    sum += kW(P.kernel, sqrtf(r2));
+    if (j != i && r2 < r2min) r2min = r2;
+  });
+  pv.t.sigma[i] = sum;
+  pv.t.rmin[i] = sqrtf(r2min);
+  float rho = pv.s.rho[i];
+  if (P.density_mode == 0) {
+    rho = (ph == kLiquid ? pv.s.m0[i] : pv.s.m[i]) * sum;
+    pv.s.rho[i] = rho;
+  }
+  pv.t.p[i] = eos_pressure(P, ph, rho, pv.s.T[i]);
+}
+
+// SPH_kernels.cu:36-54 (start-up only).
+__global__ __launch_bounds__(kBlock) void k_normalize_masses(PV pv, GridDev g, SimParams P, int n) {
+  const int i = gtid();
+  if (i >= n) return;
+  if (!is_fluid(pv.s.phase[i])) return;
+  const V3 xi = ld3(pv.s.x[i]);
+  float sum = 0.0f;
+  for_each_nbr(g, xi, [&](int j) {
+    const float r2 = norm2(xi - ld3(pv.s.x[j]));
+    if (r2 < P.support2) sum += kW(P.kernel, sqrtf(r2));
+  });
+  if (sum > 0.0f) {
+    const float m = pv.s.rho[i] / sum;
+    pv.s.m[i] = m;
+    pv.s.m0[i] = m;
+  }
+}
+
+// --------------------------------------------------------------- wall BC
+__global__ __launch_bounds__(kBlock) void k_wall_bc(PV pv, GridDev g, SimParams P, const DevState* st, int n) {
+  const int i = gtid();
+  if (i >= n) return;
+  if (pv.s.phase[i] != kWall) return;
+  const V3 grav = st->gravity;
+  const V3 xi = ld3(pv.s.x[i]);
+  float sw[2] = {0.0f, 0.0f}, sp[2] = {0.0f, 0.0f}, sgr[2] = {0.0f, 0.0f};
+  float srho = 0.0f;
+  V3 sv{0, 0, 0};
+  for_each_nbr(g, xi, [&](int j) {
+    const uint8_t phj = pv.s.phase[j];
+    if (!is_fluid(phj)) return;
+    const V3 rwf = xi - ld3(pv.s.x[j]);
+    const float r2 = norm2(rwf);
+    if (r2 >= P.support2) return;
+    const float w = kW(P.kernel, sqrtf(r2));
+    const float rhoj = pv.s.rho[j];
+    const int k = pidx(phj);
+    sw[k] += w;
+    sp[k] += pv.t.p[j] * w;
+    sgr[k] += dot(grav, rwf) * rhoj * w;
+    srho += rhoj * w;
+    sv += ld3(pv.s.v[j]) * w;
+  });
+  float pw[2];
+  for (int k = 0; k < 2; ++k) {
+    pw[k] = sw[k] > 0.0f ? (sp[k] + sgr[k]) / sw[k] : P.p_bg;
+    if (P.clamp_neg && pw[k] < 0.0f) pw[k] = 0.0f;
+  }
+  if (!(sw[0] > 0.0f) && sw[1] > 0.0f) pw[0] = pw[1];
+  if (!(sw[1] > 0.0f) && sw[0] > 0.0f) pw[1] = pw[0];
+  pv.t.p[i] = pw[0];
+  pv.t.p_wv[i] = pw[1];
+  pv.t.alpha[i] = (sw[0] + sw[1]) > 0.0f ? sw[0] / (sw[0] + sw[1]) : 0.0f;
+  const float swt = sw[0] + sw[1];
+  if (swt > 0.0f) {
+    pv.s.rho[i] = srho / swt;
+    pv.t.vw[i] = st3(sv * (-1.0f / swt));
+  } else {
+    pv.s.rho[i] = P.rho0[0];
+    pv.t.vw[i] = st3(V3{0, 0, 0});
+  }
+  pv.t.a[i] = st3(V3{0, 0, 0});
+}
+
+// ------------------------------------------------------- interface fields
+// Also resets the dt maxima for k_forces and writes FP64 block partials of
+// sum_fluid heat_w (sph_solver.cu:101-104) when heat_partials != nullptr.
+__global__ __launch_bounds__(kBlock) void k_interface(PV pv, GridDev g, SimParams P, DevState* st, int n,
+                                                     int csf_mode, double* heat_partials) {
+  const int i = gtid();
+  if (i == 0) {
+    for (int k = 0; k < 5; ++k) st->max_bits[k] = 0u;
+  }
+  double heat_contrib = 0.0;
+  if (i < n) {
+    const uint8_t ph = pv.s.phase[i];
+    if (!is_fluid(ph)) {
+      if (P.sps) pv.t.nu_t[i] = 0.0f;
+    } else {
+      const V3 xi = ld3(pv.s.x[i]);
+      const float ci = ph == kLiquid ? 1.0f : 0.0f;
+      V3 gradc{0, 0, 0}, wall_n{0, 0, 0}, akn{0, 0, 0}, gradphi{0, 0, 0};
+      float s_all = 0.0f, s_other = 0.0f, ww = 0.0f, hw = 0.0f;
+      // SPS velocity gradient (navier_stokes.cu:71-119)
+      const V3 vi = ld3(pv.s.v[i]);
+      float Lm[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      for_each_nbr(g, xi, [&](int j) {
+        const V3 rij = xi - ld3(pv.s.x[j]);
+        const float r2 = norm2(rij);
+        if (r2 >= P.support2) return;
+        const float r = sqrtf(r2);
+        const float w = kW(P.kernel, r);
+        const uint8_t phj = pv.s.phase[j];
+        const V3 gw = kgradW(P.kernel, rij, r);
+        if (P.sps && j != i && !(phj == kWall && !P.no_slip)) {
+          V3 vj;
+          float Vj;
+          if (phj == kWall) {
+            vj = ld3(pv.t.vw[j]);
+            Vj = P.vol0;
+          } else {
+            vj = ld3(pv.s.v[j]);
+            Vj = pv.s.m[j] / pv.s.rho[j];
+          }
+          const V3 dv = (vj - vi) * Vj;
+          const float dvc[3] = {dv.x, dv.y, dv.z};
+          const float gwc[3] = {gw.x, gw.y, gw.z};
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) Lm[a][b] += dvc[a] * gwc[b];
+        }
+        if (phj == kWall) {
+          const float vw = P.vol0 * w;
+          ww += vw;
+          if (pv.s.flags[j] & kHeated) hw += vw;
+          wall_n += ld3(pv.s.nrm[j]) * vw;
+          gradphi += gw * P.vol0;
+          if (P.wall_colour) gradc += gw * (P.vol0 * (pv.t.alpha[j] - ci));
+          return;
+        }
+        const float Vj = pv.s.m[j] / pv.s.rho[j];
+        s_all += Vj * w;
+        if (phj != ph) s_other += Vj * w;
+        if (j == i) return;
+        const float cj = phj == kLiquid ? 1.0f : 0.0f;
+        gradc += gw * (Vj * (cj - ci));
+        if (phj == kLiquid) akn += gw * Vj;
+      });
+      if (P.sps) {
+        float ss = 0.0f;
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) {
+            const float sab = 0.5f * (Lm[a][b] + Lm[b][a]);
+            ss += sab * sab;
+          }
+        const float smag = sqrtf(2.0f * ss);
+        const float l = P.smag_cs * P.dx;
+        pv.t.nu_t[i] = l * l * smag;
+      }
+      pv.t.alpha[i] = s_all > 0.0f ? s_other / s_all : 0.0f;
+      pv.t.wall_w[i] = ww;
+      pv.t.heat_w[i] = hw;
+      heat_contrib = static_cast<double>(hw);
+      if (P.st_model == 2) {
+        pv.s.nrm[i] = st3(ph == kLiquid ? akn * (-P.kernel.h) : V3{0, 0, 0});
+      } else {
+        const float gmag = norm(gradc);
+        V3 vwi{0, 0, 0};
+        if (P.wetting != 0 && ww > 0.0f && gmag > P.normal_eps && norm2(wall_n) > 0.0f) {
+          const V3 nw = normalized(wall_n);
+          const V3 nh = gradc / gmag;
+          const V3 tan_in = nh - nw * dot(nh, nw);
+          const float tm = norm(tan_in);
+          if (tm > 1e-6f) {
+            const float cos_d = -dot(nh, nw);
+            const float delta_w = 2.0f * norm(gradphi);
+            const float coeff = 2.0f * P.sigma / (P.rho0[0] + P.rho0[1]);
+            vwi = tan_in * (-(P.cos_theta - cos_d) * coeff * gmag * delta_w / tm);
+          }
+        }
+        pv.t.vw[i] = st3(vwi);
+        if (P.wetting != 1 && ww > 0.0f && gmag > P.normal_eps) {
+          const V3 nw = normalized(wall_n);
+          const V3 nh = gradc / gmag;
+          V3 t = nh - nw * dot(nh, nw);
+          const float tmag = norm(t);
+          if (norm2(nw) > 0.0f && tmag > 1e-6f) {
+            t = t / tmag;
+            const V3 target = nw * (-P.cos_theta) + t * P.sin_theta;
+            const float beta = clampv(ww / P.wall_adh_w, 0.0f, 1.0f);
+            const V3 blended = normalized(nh * (1.0f - beta) + target * beta);
+            gradc = blended * gmag;
+          }
+        }
+        pv.s.nrm[i] = st3(gradc);
+        // smooth_interface_normals without smoothing (surface_tension.cu:87-118)
+        if (csf_mode && !P.csf_smooth) {
+          const float gm = norm(gradc);
+          pv.t.ns[i] = st3(!(gm > P.normal_eps) ? V3{0, 0, 0} : gradc / gm);
+        }
+      }
+    }
+  }
+  if (heat_partials) block_sum_store(heat_contrib, heat_partials + blockIdx.x);
+}
+
+// surface_tension.cu:87-118 with neighbour smoothing on.
+__global__ __launch_bounds__(kBlock) void k_smooth_normals(PV pv, GridDev g, SimParams P, int n) {
+  const int i = gtid();
+  if (i >= n) return;
+  if (!is_fluid(pv.s.phase[i])) return;
+  const V3 gi = ld3(pv.s.nrm[i]);
+  const float gmag = norm(gi);
+  if (!(gmag > P.normal_eps)) {
+    pv.t.ns[i] = st3(V3{0, 0, 0});
+    return;
+  }
+  const V3 xi = ld3(pv.s.x[i]);
+  V3 acc{0, 0, 0};
+  for_each_nbr(g, xi, [&](int j) {
+    if (!is_fluid(pv.s.phase[j])) return;
+    const V3 gj = ld3(pv.s.nrm[j]);
+    const float gjm = norm(gj);
+    if (!(gjm > P.normal_eps)) return;
+    const float r2 = norm2(xi - ld3(pv.s.x[j]));
+    if (r2 >= P.support2) return;
+    acc += gj * ((pv.s.m[j] / pv.s.rho[j]) * kW(P.kernel, sqrtf(r2)) / gjm);
+  });
+  pv.t.ns[i] = st3(normalized(acc));
+}
+
+// ------------------------------------------------------------------ forces
+struct ForceFlags {
+  int csf_mode;     // CSF with vapour present
+  int akinci;       // Akinci model
+  int continuity;   // continuity density rate
+  int thermal;      // conduction + heater
+};
+
+__global__ __launch_bounds__(kBlock) void k_forces(PV pv, GridDev g, SimParams P, DevState* st, int n,
+                                                  ForceFlags F) {
+  const int i = gtid();
+  float m_v = 0.0f, m_a = 0.0f, m_tv = 0.0f, m_nu = 0.0f, m_sp = 0.0f;
+  if (i < n) {
+    const uint8_t ph = pv.s.phase[i];
+    if (!is_fluid(ph)) {
+      pv.t.ab[i] = st3(V3{0, 0, 0});
+      pv.t.a[i] = st3(V3{0, 0, 0});
+      pv.t.fw[i] = st3(V3{0, 0, 0});
+      pv.t.mw[i] = st3(V3{0, 0, 0});
+      if (F.continuity) pv.t.drho[i] = 0.0f;
+      if (F.thermal) pv.t.dTdt[i] = 0.0f;
+    } else {
+      const V3 grav = st->gravity;
+      const float dt_now = st->dt;
+      const int pi_idx = pidx(ph);
+      const V3 xi = ld3(pv.s.x[i]);
+      const V3 vi = ld3(pv.s.v[i]);
+      const float mi = pv.s.m[i];
+      const float rhoi = pv.s.rho[i];
+      const float Ti = pv.s.T[i];
+      const float sigmai = pv.t.sigma[i];
+      const float Vi = mi / rhoi;
+      const float Vi2 = Vi * Vi;
+      const float pi = pv.t.p[i];
+      const float ci = sound_speed(P, ph, Ti);
+      const float h = P.kernel.h;
+      const float eta2 = 0.01f * h * h;
+      // transport velocity (SPH_kernels.cu:56-104)
+      const bool tvf_on = P.tvf && !(sigmai < P.tvf_sigma_min) &&
+                          !(P.tvf_interface_only && !(pv.t.alpha[i] > 0.0f));
+      const float Vt = 1.0f / sigmai;
+      const float Vt2 = Vt * Vt;
+      const float mt = rhoi * Vt;
+      const float pb = P.tvf_factor * P.rho0[pi_idx] * ci * ci;
+      const float inv_gamma = 1.0f / P.tait_gamma;
+      V3 sum_t{0, 0, 0};
+      // momentum (navier_stokes.cu:121-245)
+      const float etai = P.mu[pi_idx] + (P.sps ? rhoi * pv.t.nu_t[i] : 0.0f);
+      const float alpha_i = ph == kVapor ? P.alpha_av_vap : P.alpha_av;
+      V3 force{0, 0, 0}, acc_av{0, 0, 0};
+      const float r_guard = 0.6f * P.dx;
+      const bool tvf_stress = P.tvf && P.tvf_stress;
+      const V3 dvi = ld3(pv.s.vt[i]) - vi;
+      V3 fwall{0, 0, 0}, mwall{0, 0, 0};
+      const V3 origin = st->origin;
+      // CSF (surface_tension.cu:120-177)
+      const V3 ni = F.csf_mode ? ld3(pv.t.ns[i]) : V3{0, 0, 0};
+      const bool csf_on = F.csf_mode && norm2(ni) > 0.25f;
+      const float cci = ph == kLiquid ? 1.0f : 0.0f;
+      float num = 0.0f, den = 0.0f;
+      V3 G{0, 0, 0};
+      // Akinci (surface_tension.cu:179-213)
+      const bool ak_on = F.akinci && ph == kLiquid;
+      const V3 nak = ak_on ? ld3(pv.s.nrm[i]) : V3{0, 0, 0};
+      const float psi = P.rho0[0] * P.vol0;
+      V3 acc_ak{0, 0, 0};
+      // continuity (SPH_kernels.cu:162-203)
+      const float4* vel = P.tvf ? pv.s.vt : pv.s.v;
+      const V3 vci = F.continuity ? ld3(vel[i]) : V3{0, 0, 0};
+      float div = 0.0f, diff = 0.0f;
+      // conduction (phase_change.cu:31-75)
+      const float ki = P.k[pi_idx];
+      const bool iso = P.wall_bc == 1;
+      float q = 0.0f;
+
+      for_each_nbr(g, xi, [&](int j) {
+        if (j == i) return;
+        const V3 xj = ld3(pv.s.x[j]);
+        const V3 rij = xi - xj;
+        const float r2 = norm2(rij);
+        if (r2 >= P.support2) return;
+        const float r = sqrtf(r2);
+        const float dW = kdW(P.kernel, r);
+        const bool separated = r > 1e-12f * h;
+        const V3 gw = separated ? rij * (dW / r) : V3{0, 0, 0};
+        const uint8_t phj = pv.s.phase[j];
+        if (phj == kWall) {
+          if (tvf_on) {
+            float rho_w;
+            if (ph == kVapor) {
+              rho_w = (pv.t.p_wv[j] - (P.p_bg - P.vapor_K)) / (P.gas_R * Ti);
+            } else {
+              const float ratio = (pv.t.p[j] - P.p_bg) / P.tait_B + 1.0f;
+              rho_w = P.rho0[0] * powf_cr(ratio > 0.5f ? ratio : 0.5f, inv_gamma);
+            }
+            const float Vj = rho_w > 0.0f ? Vt * rhoi / rho_w : Vt;
+            sum_t += gw * (Vt2 + Vj * Vj);
+          }
+          {
+            const V3 f_before = force, a_before = acc_av;
+            const float Vj = P.vol0;
+            const float vsum = Vi2 + Vj * Vj;
+            const float pw = ph == kVapor ? pv.t.p_wv[j] : pv.t.p[j];
+            force -= gw * (vsum * 0.5f * (pi + pw));
+            if (tvf_stress) force += vi * (0.5f * vsum * rhoi * dot(dvi, gw));
+            if (r < r_guard && separated) {
+              const float qg = 1.0f - r / r_guard;
+              const V3 nn = rij / r;
+              acc_av += nn * (4.0f * ci * ci * qg * qg / r_guard);
+              const float vn = dot(vi, nn);
+              if (vn < 0.0f && dt_now > 0.0f) acc_av -= nn * (qg * vn / dt_now);
+            }
+            if (P.no_slip) {
+              const V3 vij = vi - ld3(pv.t.vw[j]);
+              if (separated) force += vij * (vsum * etai * dW / r);
+              const float vr = dot(vij, rij);
+              if (alpha_i > 0.0f && vr < 0.0f) {
+                const float mu_ij = h * vr / (r2 + eta2);
+                const float Pi = -alpha_i * ci * mu_ij / rhoi;
+                acc_av -= gw * (rhoi * P.vol0 * Pi);
+              }
+            }
+            const V3 dF = (force - f_before) + (acc_av - a_before) * mi;
+            fwall += dF;
+            mwall += cross(xj - origin, dF);
+          }
+          if (ak_on && r2 > 0.0f) {
+            const float arg = -4.0f * r * r / P.kernel.support + 6.0f * r - 2.0f * P.kernel.support;
+            float adh = 0.0f;
+            if (!(r >= P.kernel.support || 2.0f * r <= P.kernel.support) && arg > 0.0f)
+              adh = P.ak_adh_scale * powf_cr(arg, 0.25f);
+            acc_ak -= (rij / r) * (P.ak_beta * psi * adh);
+          }
+          if (F.continuity) div += P.vol0 * dot(vci, gw);
+          if (F.thermal && iso) {
+            const float rgw = r * dW;
+            q += Vi * P.vol0 * (2.0f * ki) * (Ti - pv.s.T[j]) * rgw / (r2 + eta2);
+          }
+          return;
+        }
+        // fluid neighbour
+        const float mj = pv.s.m[j];
+        const float rhoj = pv.s.rho[j];
+        if (tvf_on) {
+          const float Vj = 1.0f / pv.t.sigma[j];
+          sum_t += gw * (Vt2 + Vj * Vj);
+        }
+        const float Vj = mj / rhoj;
+        const float vsum = Vi2 + Vj * Vj;
+        const float pt = (rhoj * pi + rhoi * pv.t.p[j]) / (rhoi + rhoj);
+        force -= gw * (vsum * pt);
+        const V3 vj = ld3(pv.s.v[j]);
+        if (tvf_stress) {
+          const V3 Ag = vi * (rhoi * dot(dvi, gw)) + vj * (rhoj * dot(ld3(pv.s.vt[j]) - vj, gw));
+          force += Ag * (0.5f * vsum);
+        }
+        const V3 vij = vi - vj;
+        const float etaj = P.mu[pidx(phj)] + (P.sps ? rhoj * pv.t.nu_t[j] : 0.0f);
+        const float eta_sum = etai + etaj;
+        if (eta_sum > 0.0f && separated) {
+          const float eta_ij = 2.0f * etai * etaj / eta_sum;
+          force += vij * (vsum * eta_ij * dW / r);
+        }
+        const float Tj = pv.s.T[j];
+        if (alpha_i > 0.0f && phj == ph) {
+          const float vr = dot(vij, rij);
+          if (vr < 0.0f) {
+            const float cj = sound_speed(P, phj, Tj);
+            const float mu_ij = h * vr / (r2 + eta2);
+            const float Pi = -alpha_i * 0.5f * (ci + cj) * mu_ij / (0.5f * (rhoi + rhoj));
+            acc_av -= gw * (mj * Pi);
+          }
+        }
+        if (csf_on && is_fluid(phj)) {
+          const V3 gwc = rij * (dW / r);
+          const V3 nj = ld3(pv.t.ns[j]);
+          if (norm2(nj) > 0.25f) {
+            num += Vj * dot(ni - nj, gwc);
+            den += Vj * r * (-dW);
+          }
+          if (P.csf_balanced) {
+            const float cj = phj == kLiquid ? 1.0f : 0.0f;
+            G += gwc * ((Vi * Vi + Vj * Vj) * (rhoj * cci + rhoi * cj) / (rhoi + rhoj));
+          }
+        }
+        if (ak_on && r2 > 0.0f && phj == kLiquid) {
+          const V3 rhat = rij / r;
+          const float K = 2.0f * P.rho0[0] / (rhoi + rhoj);
+          const V3 f_coh = rhat * (-P.ak_gamma * mi * mj * akinci_coh(r, P.kernel.support));
+          const V3 f_curv = (nak - ld3(pv.s.nrm[j])) * (-P.ak_gamma * mi);
+          acc_ak += (f_coh + f_curv) * (K / mi);
+        }
+        if (F.continuity) {
+          div += Vj * dot(vci - ld3(vel[j]), gw);
+          if (phj == ph) diff += Vj * 2.0f * (rhoj - rhoi) * dot(-rij, gw) / (r2 + eta2);
+        }
+        if (F.thermal) {
+          const float kj = P.k[pidx(phj)];
+          const float kij = 4.0f * ki * kj / (ki + kj);
+          const float rgw = r * dW;
+          q += Vi * Vj * kij * (Ti - Tj) * rgw / (r2 + eta2);
+        }
+      });
+
+      const V3 ab = tvf_on ? sum_t * (-pb / mt) : V3{0, 0, 0};
+      pv.t.ab[i] = st3(ab);
+      V3 acc = grav + force / mi + acc_av;
+      if (st->frame_rot) {
+        const V3 om = st->omega, omd = st->omega_dot;
+        const V3 rr = xi - origin;
+        acc -= cross(omd, rr) + cross(om, cross(om, rr)) + cross(om, vi) * 2.0f;
+      }
+      if (F.csf_mode) {
+        if (csf_on) {
+          const float kappa = den > 0.0f ? 3.0f * num / den : 0.0f;
+          pv.t.kappa[i] = kappa;
+          if (P.wetting != 0) acc += ld3(pv.t.vw[i]);
+          if (P.csf_balanced) {
+            acc += G * (P.sigma * kappa / mi);
+          } else {
+            const float coeff = 2.0f * P.sigma / (P.rho0[0] + P.rho0[1]);
+            acc += ld3(pv.s.nrm[i]) * (coeff * kappa);
+          }
+        } else {
+          pv.t.kappa[i] = 0.0f;
+        }
+      }
+      if (ak_on) acc += acc_ak;
+      pv.t.a[i] = st3(acc);
+      pv.t.fw[i] = st3(fwall);
+      pv.t.mw[i] = st3(mwall);
+      if (F.continuity) pv.t.drho[i] = rhoi * div + P.delta_sph * h * P.c0 * diff;
+      if (F.thermal) {
+        q += P.heat_power * pv.t.heat_w[i] * st->heat_inv_norm;
+        pv.t.dTdt[i] = q / (mi * P.cp[pi_idx]);
+      }
+      // evaluate_rates maxima (sph_solver.cu:62-99)
+      m_v = norm(vi);
+      const float na = norm(acc), nb = norm(ab);
+      m_a = na > nb ? na : nb;
+      if (ph == kVapor) m_tv = Ti;
+      if (P.sps) m_nu = pv.t.nu_t[i];
+      const float rm = pv.t.rmin[i] > 0.05f * P.dx ? pv.t.rmin[i] : 0.05f * P.dx;
+      m_sp = 0.5f * P.dx / rm;
+    }
+  }
+  m_v = wave_max(m_v);
+  m_a = wave_max(m_a);
+  m_tv = wave_max(m_tv);
+  m_nu = wave_max(m_nu);
+  m_sp = wave_max(m_sp);
+  if ((threadIdx.x & 63) == 0) {
+    atomic_max_nonneg(&st->max_bits[0], m_v);
+    atomic_max_nonneg(&st->max_bits[1], m_a);
+    atomic_max_nonneg(&st->max_bits[2], m_tv);
+    atomic_max_nonneg(&st->max_bits[3], m_nu);
+    atomic_max_nonneg(&st->max_bits[4], m_sp);
+  }
+}
+
+// Publishes the maxima (spacing ratio starts from 1, sph_solver.cu:62-71).
+__global__ void k_publish_maxima(DevState* st, int has_vapor, int sps) {
+  st->v_max = __uint_as_float(st->max_bits[0]);
+  st->a_max = __uint_as_float(st->max_bits[1]);
+  st->Tv_max = has_vapor ? __uint_as_float(st->max_bits[2]) : 0.0f;
+  st->nu_t_max = sps ? __uint_as_float(st->max_bits[3]) : 0.0f;
+  const float sp = __uint_as_float(st->max_bits[4]);
+  st->spacing_ratio = sp > 1.0f ? sp : 1.0f;
+}
+
+// sph_solver.cu:466-472: heat_inv_norm from the FP64 partials.
+__global__ __launch_bounds__(1024) void k_heat_norm(DevState* st, const double* partials, int nb, int active) {
+  __shared__ double s[1024];
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < nb; k += 1024) acc += partials[k];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = 512; off > 0; off >>= 1) {
+    if (static_cast<int>(threadIdx.x) < off) s[threadIdx.x] += s[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double S = s[0];
+    st->heat_inv_norm = active ? (S > 0 ? static_cast<float>(1.0 / S) : 0.0f) : 0.0f;
+  }
+}
+
+// probe_liquid_pressure (sph_solver.cu:197-216)
+__global__ __launch_bounds__(kBlock) void k_probe(PV pv, GridDev g, SimParams P, const float* pts, float* out,
+                                                 int np) {
+  const int k = gtid();
+  if (k >= np) return;
+  const V3 x{pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]};
+  float num = 0.0f, den = 0.0f;
+  for_each_nbr(g, x, [&](int j) {
+    if (pv.s.phase[j] != kLiquid) return;
+    const float r2 = norm2(x - ld3(pv.s.x[j]));
+    if (r2 >= P.support2) return;
+    const float w = (pv.s.m[j] / pv.s.rho[j]) * kW(P.kernel, sqrtf(r2));
+    num += pv.t.p[j] * w;
+    den += w;
+  });
+  out[k] = den > 0.0f ? num / den : __builtin_nanf("");
+}
+
+// SPH_kernels.cu:106-160 (delta+ shifting), pass 1 and pass 2.
+__global__ __launch_bounds__(kBlock) void k_shift_compute(PV pv, GridDev g, SimParams P, const DevState* st,
+                                                         int n) {
+  const int i = gtid();
+  if (i >= n) return;
+  pv.t.ns[i] = st3(V3{0, 0, 0});
+  const uint8_t ph = pv.s.phase[i];
+  if (!is_fluid(ph) || pv.t.sigma[i] < P.tvf_sigma_min) return;
+  const float h = P.kernel.h;
+  const float w_dx = kW(P.kernel, P.dx);
+  const float mach = st->v_max / P.c0;
+  const float coeff = P.shift_strength * mach * 4.0f * h * h;
+  const float cap = 0.1f * P.dx;
+  const V3 xi = ld3(pv.s.x[i]);
+  V3 sum{0, 0, 0};
+  for_each_nbr(g, xi, [&](int j) {
+    if (j == i) return;
+    const uint8_t phj = pv.s.phase[j];
+    if (phj == kDead) return;
+    const V3 rij = xi - ld3(pv.s.x[j]);
+    const float r2 = norm2(rij);
+    if (r2 >= P.support2) return;
+    const float r = sqrtf(r2);
+    const float Vj = phj == kWall ? 1.0f / pv.t.sigma[i] : 1.0f / pv.t.sigma[j];
+    const float qq = kW(P.kernel, r) / w_dx;
+    const float q2 = qq * qq;
+    sum += kgradW(P.kernel, rij, r) * (Vj * (1.0f + 0.2f * q2 * q2));
+  });
+  V3 dr = sum * (-coeff);
+  if (pv.t.alpha[i] > 0.0f) {
+    const V3 gn = ld3(pv.s.nrm[i]);
+    const float gm = norm(gn);
+    if (gm > P.normal_eps) {
+      const V3 nn = gn / gm;
+      dr -= nn * dot(dr, nn);
+    } else {
+      dr = V3{0, 0, 0};
+    }
+  }
+  const float len = norm(dr);
+  if (len > cap) dr *= cap / len;
+  pv.t.ns[i] = st3(dr);
+}
+
+__global__ __launch_bounds__(kBlock) void k_shift_apply(PV pv, int n) {
+  const int i = gtid();
+  if (i >= n) return;
+  if (is_fluid(pv.s.phase[i])) pv.s.x[i] = st3(ld3(pv.s.x[i]) + ld3(pv.t.ns[i]));
+}
+
+}  // namespace
+
+void launch_density(const PV& pv, const GridDev& g, const SimParams& P, int n, hipStream_t s) {
+  if (n > 0) k_density<<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, n);
+}
+void launch_normalize_masses(const PV& pv, const GridDev& g, const SimParams& P, int n, hipStream_t s) {
+  if (n > 0) k_normalize_masses<<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, n);
+}
+void launch_wall_bc(const PV& pv, const GridDev& g, const SimParams& P, const DevState* st, int n, hipStream_t s) {
+  if (n > 0) k_wall_bc<<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, st, n);
+}
+int launch_interface(const PV& pv, const GridDev& g, const SimParams& P, DevState* st, int n, bool csf_mode,
+                     double* heat_partials, hipStream_t s) {
+  const int nb = blocks_for(n > 0 ? n : 1);
+  k_interface<<<nb, kBlock, 0, s>>>(pv, g, P, st, n, csf_mode ? 1 : 0, heat_partials);
+  return nb;
+}
+void launch_smooth_normals(const PV& pv, const GridDev& g, const SimParams& P, int n, hipStream_t s) {
+  if (n > 0) k_smooth_normals<<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, n);
+}
+void launch_heat_norm(DevState* st, const double* partials, int nb, bool active, hipStream_t s) {
+  k_heat_norm<<<1, 1024, 0, s>>>(st, partials, nb, active ? 1 : 0);
+}
+void launch_forces(const PV& pv, const GridDev& g, const SimParams& P, DevState* st, int n, bool csf_mode,
+                   bool akinci, bool continuity, bool thermal, hipStream_t s) {
+  ForceFlags F{csf_mode ? 1 : 0, akinci ? 1 : 0, continuity ? 1 : 0, thermal ? 1 : 0};
+  k_forces<<<blocks_for(n > 0 ? n : 1), kBlock, 0, s>>>(pv, g, P, st, n, F);
+  k_publish_maxima<<<1, 1, 0, s>>>(st, P.has_vapor, P.sps);
+}
+void launch_probe(const PV& pv, const GridDev& g, const SimParams& P, const float* pts, float* out, int np,
+                  hipStream_t s) {
+  if (np > 0) k_probe<<<blocks_for(np), kBlock, 0, s>>>(pv, g, P, pts, out, np);
+}
+void launch_shifting(const PV& pv, const GridDev& g, const SimParams& P, const DevState* st, int n, hipStream_t s) {
+  if (n <= 0) return;
+  k_shift_compute<<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, st, n);
+  k_shift_apply<<<blocks_for(n), kBlock, 0, s>>>(pv, n);
+}
+
+}  // namespace cryo

@@ -1,0 +1,1489 @@
+// Solver orchestration on the host and the C ABI of include/cryosph.h.
+//
+// Mirrors Solver (sph_solver.hpp:134-232, sph_solver.cu:273-756) with the
+// step kept on the device: dt, the manoeuvre frame, bookkeeping and the
+// ullage state live in DevState, so a step is a fixed sequence of launches on
+// one blocking HIP stream.  The host synchronises only where it must know the
+// particle count (phase change can split or merge particles) or the clock
+// (run_until, relax), and when a caller reads results.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "host.hpp"
+
+namespace cryo {
+namespace {
+
+thread_local std::string g_err;
+
+// Host pow used where the reference evaluates std::pow in float
+// (akinci_adhesion, SPH_kernels.cuh:81-86).
+float akinci_adh_host(float r, float c) {
+  if (r >= c || 2.0f * r <= c) return 0.0f;
+  const float arg = -4.0f * r * r / c + 6.0f * r - 2.0f * c;
+  if (arg <= 0.0f) return 0.0f;
+  return 0.007f / std::pow(c, 3.25f) * std::pow(arg, 0.25f);
+}
+
+// Akinci Fowler integral (sph_solver.cu:181-193).
+template <class Fn>
+double radial_moment4(Fn f, double c) {
+  const int n = 2000;
+  const double hstep = c / n;
+  double acc = 0;
+  for (int k = 0; k <= n; ++k) {
+    const double r = k * hstep;
+    const double w = (k == 0 || k == n) ? 1.0 : (k % 2 ? 4.0 : 2.0);
+    acc += w * r * r * r * r * f(r);
+  }
+  return acc * hstep / 3.0;
+}
+
+float saturation_temperature(const cryo_fluid& f, float p) {  // fluid_properties.cpp:7-10
+  const double inv_T = 1.0 / f.T_sat - (double(f.R_gas) / f.h_fg) * std::log(double(p) / f.p_sat);
+  return static_cast<float>(1.0 / inv_T);
+}
+float saturation_pressure(const cryo_fluid& f, float T) {  // fluid_properties.cpp:12-14
+  return static_cast<float>(f.p_sat * std::exp(double(f.h_fg) / f.R_gas * (1.0 / f.T_sat - 1.0 / double(T))));
+}
+
+bool preset(const std::string& n, cryo_fluid* o) {  // fluid_properties.cpp:16-79
+  std::memset(o, 0, sizeof(*o));
+  auto set = [&](const char* name, float rl, float rv, float ml, float mv, float kl, float kv, float cl, float cvv,
+                 float sg, float hfg, float ts, float ps, float rg) {
+    std::strncpy(o->name, name, sizeof(o->name) - 1);
+    o->rho_liquid = rl;
+    o->rho_vapor = rv;
+    o->mu_liquid = ml;
+    o->mu_vapor = mv;
+    o->k_liquid = kl;
+    o->k_vapor = kv;
+    o->cp_liquid = cl;
+    o->cp_vapor = cvv;
+    o->sigma = sg;
+    o->h_fg = hfg;
+    o->T_sat = ts;
+    o->p_sat = ps;
+    o->R_gas = rg;
+  };
+  if (n == "custom") set("custom", 1000.f, 1.f, 1e-3f, 1e-5f, 0.6f, 0.025f, 4180.f, 2000.f, 0.072f, 2.26e6f, 373.15f, 101325.f, 461.5f);
+  else if (n == "LH2") set("LH2", 70.85f, 1.338f, 1.33e-5f, 1.11e-6f, 0.1034f, 0.0166f, 9770.f, 12200.f, 1.93e-3f, 445.6e3f, 20.37f, 101325.f, 4124.2f);
+  else if (n == "LN2") set("LN2", 806.1f, 4.61f, 1.61e-4f, 5.4e-6f, 0.1446f, 0.0074f, 2042.f, 1123.f, 8.85e-3f, 199.2e3f, 77.36f, 101325.f, 296.8f);
+  else if (n == "LOX") set("LOX", 1141.f, 4.47f, 1.96e-4f, 6.9e-6f, 0.152f, 0.0084f, 1699.f, 972.f, 13.2e-3f, 213.1e3f, 90.19f, 101325.f, 259.8f);
+  else if (n == "LCH4") set("LCH4", 422.4f, 1.816f, 1.17e-4f, 4.4e-6f, 0.184f, 0.0119f, 3481.f, 2218.f, 13.4e-3f, 510.8e3f, 111.67f, 101325.f, 518.3f);
+  else if (n == "H2O") set("H2O", 998.2f, 0.598f, 1.0e-3f, 1.23e-5f, 0.598f, 0.0248f, 4182.f, 2080.f, 0.0728f, 2.257e6f, 373.15f, 101325.f, 461.5f);
+  else return false;
+  return true;
+}
+
+void config_default(cryo_config* c) {  // sph_solver.hpp:20-91
+  std::memset(c, 0, sizeof(*c));
+  c->dx = 0.005f;
+  c->h_factor = 1.3f;
+  c->kernel = CRYO_KERNEL_WENDLAND_C2;
+  c->density_mode = CRYO_DENSITY_SUMMATION;
+  c->normalize_initial_masses = 1;
+  c->gravity = cryo_vec3{0.0f, 0.0f, -9.81f};
+  c->velocity_scale = 0.0f;
+  c->sound_speed = 0.0f;
+  c->tait_gamma = 7.0f;
+  c->background_pressure = -1.0f;
+  c->vapor_stiffness = -1.0f;
+  c->clamp_negative_pressure = 0;
+  c->transport_velocity = 1;
+  c->transport_background_factor = 2.0f;
+  c->transport_stress = 0;
+  c->transport_interface_only = 1;
+  c->particle_shifting = 0;
+  c->shifting_strength = 0.25f;
+  c->artificial_viscosity = 0.02f;
+  c->artificial_viscosity_vapor = 0.2f;
+  c->delta_sph = 0.1f;
+  c->no_slip = 1;
+  c->sps_turbulence = 0;
+  c->smagorinsky_cs = 0.12f;
+  c->surface_tension = CRYO_ST_CSF;
+  c->contact_angle_deg = 90.0f;
+  c->contact_angle_wall_weight = 0.01f;
+  c->wall_colour = 1;
+  c->wetting_model = CRYO_WET_NORMAL_ROTATION;
+  c->csf_smooth_normals = 0;
+  c->csf_balanced = 0;
+  c->akinci_gamma = -1.0f;
+  c->akinci_beta = -1.0f;
+  c->thermal = 1;
+  c->phase_change = 1;
+  c->wall_thermal_bc = CRYO_WALL_ADIABATIC;
+  c->heater_power = 0.0f;
+  c->conductivity_scale = 1.0f;
+  c->lee_evaporation = 10.0f;
+  c->lee_condensation = 10.0f;
+  c->thermo_mode = CRYO_VENTED;
+  c->vent_relaxation = 0.02f;
+  c->merge_ratio = 0.3f;
+  c->split_ratio = 2.0f;
+  c->max_transfer_fraction = 0.25f;
+  c->phase_change_density_limit = 0.01f;
+  c->vapor_mass_rebalance = 1.0f;
+  c->cfl = 0.25f;
+  c->viscous_number = 0.125f;
+  c->force_number = 0.25f;
+  c->capillary_number = 0.25f;
+  c->thermal_number = 0.1f;
+  c->dt_max = 1e-2f;
+  c->dt_min = 1e-9f;
+  c->fixed_dt = 0.0f;
+}
+
+V3 v3(cryo_vec3 a) { return V3{a.x, a.y, a.z}; }
+V3 v3p(const float* a) { return V3{a[0], a[1], a[2]}; }
+cryo_vec3 cv(V3 a) { return cryo_vec3{a.x, a.y, a.z}; }
+
+// ManeuverProfile host copy (sph_solver.hpp:96-105, sph_solver.cu:234-268).
+struct HostManeuver {
+  std::vector<double> time;
+  std::vector<V3> acc, omega;
+  V3 origin{0, 0, 0};
+  void validate() const {
+    if (time.empty()) throw std::invalid_argument("ManeuverProfile: empty time table");
+    if (acc.size() != time.size())
+      throw std::invalid_argument("ManeuverProfile: acceleration must have one entry per time");
+    if (!omega.empty() && omega.size() != time.size())
+      throw std::invalid_argument("ManeuverProfile: angular_velocity must be empty or have one entry per time");
+    for (std::size_t k = 1; k < time.size(); ++k)
+      if (!(time[k] > time[k - 1])) throw std::invalid_argument("ManeuverProfile: time must be strictly increasing");
+  }
+  void evaluate(double t, V3& a, V3& w, V3& wd) const {
+    const std::size_t n = time.size();
+    const bool rot = !omega.empty();
+    wd = V3{0, 0, 0};
+    if (t <= time.front() || n == 1) {
+      a = acc.front();
+      w = rot ? omega.front() : V3{0, 0, 0};
+      return;
+    }
+    if (t >= time.back()) {
+      a = acc.back();
+      w = rot ? omega.back() : V3{0, 0, 0};
+      return;
+    }
+    const std::size_t k = static_cast<std::size_t>(std::upper_bound(time.begin(), time.end(), t) - time.begin()) - 1;
+    const double span = time[k + 1] - time[k];
+    const float f = static_cast<float>((t - time[k]) / span);
+    a = acc[k] * (1.0f - f) + acc[k + 1] * f;
+    if (rot) {
+      w = omega[k] * (1.0f - f) + omega[k + 1] * f;
+      wd = (omega[k + 1] - omega[k]) / static_cast<float>(span);
+    } else {
+      w = V3{0, 0, 0};
+    }
+  }
+};
+
+void check_device() {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+    throw std::runtime_error("CryoSPH: no HIP device visible (this library runs on AMD Instinct MI355X, gfx950)");
+  int dev = 0;
+  CRYO_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop{};
+  CRYO_HIP(hipGetDeviceProperties(&prop, dev));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    throw std::runtime_error(std::string("CryoSPH: device arch '") + prop.gcnArchName +
+                             "' is not gfx950; this build only carries gfx950 code");
+}
+
+}  // namespace
+}  // namespace cryo
+
+using namespace cryo;
+
+// ------------------------------------------------------------------ solver
+struct cryo_solver {
+  cryo_config cfg{};
+  cryo_fluid fluid{};
+  SimParams P{};
+  DtLimits L{};
+  hipStream_t stream = nullptr;
+
+  // particle arrays: two persistent sets (gather ping-pong) + scratch
+  DevBuf<float4> x_[2], v_[2], vt_[2], nrm_[2];
+  DevBuf<float> m_[2], m0_[2], rho_[2], T_[2];
+  DevBuf<uint8_t> phase_[2], flags_[2];
+  DevBuf<uint32_t> id_[2];
+  int set = 0;
+  DevBuf<float4> a, ab, ns, vw, fw, mw;
+  DevBuf<float> p, drho, dTdt, sigma, kappa, nu_t, alpha, wall_w, heat_w, out_m, out_h, w_norm, p_wv, rmin;
+  // grid and scans
+  DevBuf<uint32_t> keys_[2], vals_[2], tile_hist, split_flag, split_off, scan_sums, scan_total;
+  DevBuf<int> cell_off;
+  const uint32_t* last_perm = nullptr;
+  int n_sorted = 0;
+  // reductions
+  DevBuf<double> partials, diag_out;
+  DevBuf<unsigned> diag_max;
+  int stride = 0;
+  DevBuf<DevState> state;
+  DevState* hs = nullptr;  // pinned host mirror
+  // manoeuvre table on the device
+  DevBuf<double> man_time;
+  DevBuf<V3> man_acc, man_omega;
+  Maneuver man{};
+  HostManeuver hman;
+  bool has_maneuver = false;
+  V3 frame_acc{0, 0, 0};
+  V3 frame_omega{0, 0, 0}, frame_omega_dot{0, 0, 0}, frame_origin{0, 0, 0};
+  int frame_rot = 0;
+  // grid configuration
+  GridDev g{};
+  float cell = 1.0f;
+  int key_bits = 1;
+  // host knowledge
+  int n = 0, capacity = 0, pending_dead = 0, merges_seen = 0;
+  uint32_t next_id = 0;
+  bool initialized = false, has_vapor = false;
+  float char_length = 0.0f;
+  // telemetry (telemetry.hpp:33-70)
+  int tele_mode = CRYO_TELEMETRY_ASYNC_COPY;
+  int tele_cap = 0;
+  float* tele_host = nullptr;
+  float* tele_host_dev = nullptr;
+  DevBuf<float> tele_staging;
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t tele_computed = nullptr, tele_landed = nullptr;
+  bool tele_pending = false;
+  double tele_time = 0;
+  long long tele_step = 0;
+  int tele_n = 0;
+
+  cryo_solver(const cryo_config& c, const cryo_fluid& f) : cfg(c), fluid(f) {
+    if (!(cfg.dx > 0)) throw std::invalid_argument("SolverConfig.dx must be > 0");
+    if (cfg.contact_angle_deg < 0 || cfg.contact_angle_deg > 180)
+      throw std::invalid_argument("contact angle must be within [0, 180] degrees");
+    check_device();
+    CRYO_HIP(hipStreamCreate(&stream));  // blocking, default priority
+    CRYO_HIP(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
+    CRYO_HIP(hipEventCreateWithFlags(&tele_computed, hipEventDisableTiming));
+    CRYO_HIP(hipEventCreateWithFlags(&tele_landed, hipEventDisableTiming));
+    CRYO_HIP(hipHostMalloc(reinterpret_cast<void**>(&hs), sizeof(DevState), hipHostMallocDefault));
+    std::memset(hs, 0, sizeof(DevState));
+    state.resize(1);
+    hs->p_ullage = fluid.p_sat;
+    hs->T_sat = fluid.T_sat;
+    hs->spacing_ratio = 1.0f;
+    hs->cfg_gravity = v3(cfg.gravity);
+    hs->gravity = v3(cfg.gravity);
+    push_state();
+  }
+
+  ~cryo_solver() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (copy_stream) (void)hipStreamSynchronize(copy_stream);
+    release_telemetry();
+    if (hs) (void)hipHostFree(hs);
+    if (tele_computed) (void)hipEventDestroy(tele_computed);
+    if (tele_landed) (void)hipEventDestroy(tele_landed);
+    if (copy_stream) (void)hipStreamDestroy(copy_stream);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  // ------------------------------------------------------- state mirror
+  void sync_state() {
+    CRYO_HIP(hipMemcpyAsync(hs, state.get(), sizeof(DevState), hipMemcpyDeviceToHost, stream));
+    CRYO_HIP(hipStreamSynchronize(stream));
+  }
+  void push_state() {
+    CRYO_HIP(hipMemcpyAsync(state.get(), hs, sizeof(DevState), hipMemcpyHostToDevice, stream));
+    CRYO_HIP(hipStreamSynchronize(stream));
+  }
+  void check_launch() { CRYO_HIP(hipGetLastError()); }
+
+  // -------------------------------------------------------------- views
+  PV view() {
+    PV pv{};
+    const int k = set;
+    pv.s = Persist{x_[k].get(), v_[k].get(), vt_[k].get(), nrm_[k].get(), m_[k].get(), m0_[k].get(),
+                   rho_[k].get(), T_[k].get(), phase_[k].get(), flags_[k].get(), id_[k].get()};
+    pv.t = Scratch{a.get(), ab.get(), ns.get(), vw.get(), fw.get(), mw.get(), p.get(), drho.get(), dTdt.get(),
+                   sigma.get(), kappa.get(), nu_t.get(), alpha.get(), wall_w.get(), heat_w.get(), out_m.get(),
+                   out_h.get(), w_norm.get(), p_wv.get(), rmin.get()};
+    pv.capacity = capacity;
+    return pv;
+  }
+  Persist persist(int k) {
+    return Persist{x_[k].get(), v_[k].get(), vt_[k].get(), nrm_[k].get(), m_[k].get(), m0_[k].get(),
+                   rho_[k].get(), T_[k].get(), phase_[k].get(), flags_[k].get(), id_[k].get()};
+  }
+  GridDev grid() {
+    GridDev gd = g;
+    gd.off = cell_off.get();
+    return gd;
+  }
+
+  template <class F>
+  void each_particle_array(F&& f) {
+    for (int k = 0; k < 2; ++k) {
+      f(x_[k], k == set);
+      f(v_[k], k == set);
+      f(vt_[k], k == set);
+      f(nrm_[k], k == set);
+      f(m_[k], k == set);
+      f(m0_[k], k == set);
+      f(rho_[k], k == set);
+      f(T_[k], k == set);
+      f(phase_[k], k == set);
+      f(flags_[k], k == set);
+      f(id_[k], k == set);
+    }
+    f(a, true); f(ab, true); f(ns, true); f(vw, true); f(fw, true); f(mw, true);
+    f(p, true); f(drho, true); f(dTdt, true); f(sigma, true); f(kappa, true); f(nu_t, true); f(alpha, true);
+    f(wall_w, true); f(heat_w, true); f(out_m, true); f(out_h, true); f(w_norm, true); f(p_wv, true); f(rmin, true);
+  }
+
+  void size_aux(int cap) {
+    for (int k = 0; k < 2; ++k) {
+      keys_[k].resize(cap);
+      vals_[k].resize(cap);
+    }
+    tile_hist.resize(static_cast<std::size_t>(256) * radix_tiles(cap));
+    split_flag.resize(cap);
+    split_off.resize(cap);
+    scan_sums.resize(static_cast<std::size_t>((cap + 1023) / 1024) + 1);
+    scan_total.resize(1);
+    stride = (cap + 255) / 256 + 1;
+    partials.resize(static_cast<std::size_t>(std::max(diag_quantities(), 8)) * stride);
+    diag_out.resize(diag_quantities());
+    diag_max.resize(2);
+  }
+
+  void allocate(int cap) {  // ParticleSystem::allocate (particle_system.cpp:50-54)
+    each_particle_array([cap](auto& b, bool) { b.resize(cap); });
+    size_aux(cap);
+    capacity = cap;
+  }
+
+  void grow(int cap) {  // ParticleSystem::grow (particle_system.cpp:62-67)
+    if (cap <= capacity) return;
+    const std::size_t keep = static_cast<std::size_t>(capacity);
+    each_particle_array([&](auto& b, bool live) {
+      if (live) b.grow(cap, keep, stream);
+      else b.resize(cap);
+    });
+    size_aux(cap);
+    capacity = cap;
+    configure_telemetry(tele_mode);
+  }
+
+  // ---------------------------------------------------------- init_params
+  void init_params() {  // sph_solver.cu:292-405
+    const cryo_config& c = cfg;
+    SimParams& Q = P;
+    Q = SimParams{};
+    const float dx = c.dx;
+    const float h = c.h_factor * dx;
+    Q.kernel.type = c.kernel;
+    Q.kernel.h = h;
+    Q.kernel.inv_h = 1.0f / h;
+    Q.kernel.support = 2.0f * h;
+    const float h3 = h * h * h;
+    Q.kernel.norm = c.kernel == CRYO_KERNEL_CUBIC ? 1.0f / (kPi * h3) : 21.0f / (16.0f * kPi * h3);
+    Q.dx = dx;
+    Q.support2 = Q.kernel.support * Q.kernel.support;
+    {
+      double sigma0 = 0;
+      const int r = static_cast<int>(std::ceil(Q.kernel.support / dx));
+      for (int k = -r; k <= r; ++k)
+        for (int j = -r; j <= r; ++j)
+          for (int i = -r; i <= r; ++i)
+            sigma0 += kW(Q.kernel, dx * std::sqrt(static_cast<float>(i * i + j * j + k * k)));
+      Q.vol0 = static_cast<float>(1.0 / sigma0);
+    }
+    Q.rho0[0] = fluid.rho_liquid;
+    Q.rho0[1] = fluid.rho_vapor;
+    Q.mu[0] = fluid.mu_liquid;
+    Q.mu[1] = fluid.mu_vapor;
+    Q.k[0] = fluid.k_liquid * c.conductivity_scale;
+    Q.k[1] = fluid.k_vapor * c.conductivity_scale;
+    Q.cp[0] = fluid.cp_liquid;
+    Q.cp[1] = fluid.cp_vapor;
+    Q.mref[0] = Q.rho0[0] * Q.vol0;
+    Q.mref[1] = Q.rho0[1] * Q.vol0;
+    const bool st_on = c.surface_tension != CRYO_ST_NONE;
+    const float gn = norm(v3(c.gravity));
+    float vref = c.velocity_scale;
+    if (!(vref > 0)) {
+      vref = std::sqrt(gn * char_length);
+      if (st_on) vref = std::max(vref, std::sqrt(fluid.sigma / (fluid.rho_liquid * h)));
+      vref = std::max(vref, 1e-3f);
+    }
+    Q.c0 = c.sound_speed > 0 ? c.sound_speed : 10.0f * vref;
+    Q.tait_gamma = c.tait_gamma;
+    Q.tait_B = Q.rho0[0] * Q.c0 * Q.c0 / Q.tait_gamma;
+    Q.p_bg = c.background_pressure >= 0 ? c.background_pressure
+                                        : (has_vapor ? 0.1f * Q.rho0[1] * Q.c0 * Q.c0 : 0.0f);
+    Q.vapor_K = c.vapor_stiffness > 0 ? c.vapor_stiffness : (has_vapor ? Q.rho0[1] * Q.c0 * Q.c0 : Q.p_bg);
+    Q.gas_R = Q.vapor_K > 0 ? Q.vapor_K / (Q.rho0[1] * fluid.T_sat) : fluid.R_gas;
+    Q.clamp_neg = c.clamp_negative_pressure ? 1 : 0;
+    Q.alpha_av = c.artificial_viscosity;
+    Q.alpha_av_vap = c.artificial_viscosity_vapor;
+    Q.delta_sph = c.delta_sph;
+    Q.density_mode = c.density_mode;
+    Q.no_slip = c.no_slip ? 1 : 0;
+    Q.sps = c.sps_turbulence ? 1 : 0;
+    Q.smag_cs = c.smagorinsky_cs;
+    Q.tvf = c.transport_velocity ? 1 : 0;
+    Q.tvf_stress = c.transport_stress ? 1 : 0;
+    Q.tvf_factor = c.transport_background_factor;
+    Q.tvf_interface_only = c.transport_interface_only ? 1 : 0;
+    Q.shift_strength = c.particle_shifting ? c.shifting_strength : 0.0f;
+    Q.tvf_sigma_min = has_vapor ? 0.0f : 0.95f / Q.vol0;
+    Q.st_model = c.surface_tension;
+    Q.sigma = st_on ? fluid.sigma : 0.0f;
+    const float theta = c.contact_angle_deg * kPi / 180.0f;
+    Q.cos_theta = std::cos(theta);
+    Q.sin_theta = std::sin(theta);
+    Q.wall_adh_w = c.contact_angle_wall_weight;
+    Q.wall_colour = c.wall_colour ? 1 : 0;
+    Q.wetting = c.wetting_model;
+    Q.csf_smooth = c.csf_smooth_normals ? 1 : 0;
+    Q.csf_balanced = c.csf_balanced ? 1 : 0;
+    Q.normal_eps = 0.1f / h;
+    {
+      const double sup = Q.kernel.support;
+      const double i_coh =
+          radial_moment4([&](double r) { return double(akinci_coh(static_cast<float>(r), static_cast<float>(sup))); }, sup);
+      const double i_adh = radial_moment4(
+          [&](double r) { return double(akinci_adh_host(static_cast<float>(r), static_cast<float>(sup))); }, sup);
+      const double rho = fluid.rho_liquid;
+      const double gamma =
+          c.akinci_gamma >= 0 ? double(c.akinci_gamma) : (i_coh > 0 ? 8.0 * fluid.sigma / (kPi * rho * rho * i_coh) : 0.0);
+      Q.ak_gamma = static_cast<float>(gamma);
+      Q.ak_beta = c.akinci_beta >= 0
+                      ? c.akinci_beta
+                      : static_cast<float>(i_adh > 0 ? gamma * (i_coh / i_adh) * 0.5 * (1.0 + Q.cos_theta) : 0.0);
+      Q.ak_adh_scale = 0.007f / std::pow(Q.kernel.support, 3.25f);
+    }
+    Q.thermal = c.thermal ? 1 : 0;
+    Q.phase_change = c.phase_change ? 1 : 0;
+    Q.wall_bc = c.wall_thermal_bc;
+    Q.T_wall = fluid.T_sat;
+    Q.T_ref = fluid.T_sat;
+    Q.h_fg = fluid.h_fg;
+    Q.lee_evap = c.lee_evaporation;
+    Q.lee_cond = c.lee_condensation;
+    Q.max_transfer = c.max_transfer_fraction;
+    Q.pc_density_limit = c.phase_change_density_limit;
+    Q.vapor_rebalance = c.vapor_mass_rebalance;
+    Q.merge_ratio = c.merge_ratio;
+    Q.split_ratio = c.split_ratio;
+    Q.heat_power = c.heater_power;
+    Q.has_vapor = has_vapor ? 1 : 0;
+    L = DtLimits{c.cfl, c.viscous_number, c.force_number, c.capillary_number, c.thermal_number,
+                 c.dt_max, c.dt_min, c.fixed_dt};
+  }
+
+  FluidConst fluid_const() const {
+    return FluidConst{fluid.rho_liquid, fluid.rho_vapor, fluid.T_sat, fluid.p_sat, fluid.R_gas, fluid.h_fg};
+  }
+
+  // ---------------------------------------------------------- particles
+  void set_particles(int n_in, const float* x, const float* v, const float* nrm, const float* m, const float* rho,
+                     const float* T, const uint8_t* phase, const uint8_t* flags, const uint32_t* id,
+                     const float* lo, const float* hi) {
+    if (n_in <= 0) throw std::invalid_argument("Solver::set_particles: no particles");
+    if (!x || !m || !rho || !T || !phase || !lo || !hi) throw std::invalid_argument("set_particles: null array");
+    bool vap = false;
+    int dead = 0;
+    for (int i = 0; i < n_in; ++i) {
+      if (phase[i] > kDead) throw std::invalid_argument("phase values must be 0 (liquid), 1 (vapour) or 2 (wall)");
+      vap = vap || phase[i] == kVapor;
+      dead += phase[i] == kDead;
+    }
+    if (vap && cfg.surface_tension == CRYO_ST_AKINCI)
+      throw std::invalid_argument(
+          "Solver: the Akinci surface-tension model is a free-surface (liquid-only) model; use CSF when vapour "
+          "particles are present (FillSpec.include_vapor = false for Akinci runs)");
+    sync_state();
+    has_vapor = vap;
+    const V3 lo3 = v3p(lo), hi3 = v3p(hi);
+    const V3 ext = hi3 - lo3;
+    char_length = std::max({ext.x, ext.y, ext.z});
+    init_params();
+    const int cap = static_cast<int>(std::max<std::size_t>(static_cast<std::size_t>(static_cast<double>(n_in) * 2.0),
+                                                           static_cast<std::size_t>(n_in) + 1024));
+    if (cap > capacity) allocate(cap);
+    // persistent arrays
+    std::vector<float4> h4(static_cast<std::size_t>(n_in));
+    auto up4 = [&](DevBuf<float4>& b, const float* src) {
+      for (int i = 0; i < n_in; ++i)
+        h4[i] = src ? make_float4(src[3 * i], src[3 * i + 1], src[3 * i + 2], 0.0f) : make_float4(0, 0, 0, 0);
+      CRYO_HIP(hipMemcpyAsync(b.get(), h4.data(), sizeof(float4) * n_in, hipMemcpyHostToDevice, stream));
+      CRYO_HIP(hipStreamSynchronize(stream));
+    };
+    auto up = [&](auto& b, const auto* src) {
+      CRYO_HIP(hipMemcpyAsync(b.get(), src, sizeof(*src) * n_in, hipMemcpyHostToDevice, stream));
+    };
+    up4(x_[set], x);
+    up4(v_[set], v);
+    up4(vt_[set], v);
+    up4(nrm_[set], nrm);
+    up(m_[set], m);
+    up(m0_[set], m);
+    up(rho_[set], rho);
+    up(T_[set], T);
+    up(phase_[set], phase);
+    std::vector<uint8_t> fl(static_cast<std::size_t>(n_in), 0);
+    if (flags) std::copy(flags, flags + n_in, fl.begin());
+    up(flags_[set], fl.data());
+    std::vector<uint32_t> ids(static_cast<std::size_t>(n_in));
+    if (id) std::copy(id, id + n_in, ids.begin());
+    else std::iota(ids.begin(), ids.end(), 0u);
+    up(id_[set], ids.data());
+    CRYO_HIP(hipStreamSynchronize(stream));
+    next_id = 0;
+    for (uint32_t k : ids) next_id = std::max(next_id, k + 1);
+    // scratch fields start from a clean state (particle_system.cpp:81-83)
+    auto zero = [&](auto& b) { CRYO_HIP(hipMemsetAsync(b.get(), 0, sizeof(*b.get()) * b.size(), stream)); };
+    zero(a); zero(ab); zero(ns); zero(vw); zero(fw); zero(mw); zero(p); zero(drho); zero(dTdt); zero(sigma);
+    zero(kappa); zero(nu_t); zero(alpha); zero(wall_w); zero(heat_w); zero(out_m); zero(out_h); zero(w_norm);
+    zero(p_wv);
+    n = n_in;
+    pending_dead = dead;
+    configure_grid(lo3, hi3, P.kernel.support);
+    configure_telemetry(CRYO_TELEMETRY_ASYNC_COPY);
+    initialized = false;
+    merges_seen = 0;
+    // clock and counters (sph_solver.cu:420-427); the frame is kept
+    DevState& s = *hs;
+    s.time = 0;
+    s.step = 0;
+    s.last_dt = 0;
+    s.heat_input = s.evaporated = s.condensed = s.vented = s.vented_h = 0;
+    s.splits = s.merges = 0;
+    s.p_ullage = fluid.p_sat;
+    s.fluid_volume0 = 0;
+    s.dt = 0;
+    s.T_sat = fluid.T_sat;
+    s.heat_inv_norm = 0;
+    s.cfg_gravity = v3(cfg.gravity);
+    s.gravity = v3(cfg.gravity) - frame_acc;
+    s.frame_acc = frame_acc;
+    s.omega = frame_omega;
+    s.omega_dot = frame_omega_dot;
+    s.origin = frame_origin;
+    s.frame_rot = frame_rot;
+    s.n = n;
+    s.n_alive = n;
+    s.next_id = next_id;
+    push_state();
+  }
+
+  void configure_grid(V3 lo, V3 hi, float cell_size) {  // spatial_hash.cu:15-30
+    if (!(cell_size > 0)) throw std::invalid_argument("SpatialHash: cell size must be positive");
+    cell = cell_size;
+    const V3 origin = lo - V3{cell, cell, cell};
+    const V3 ext = hi - lo;
+    const int nx = std::max(1, static_cast<int>(std::ceil(ext.x / cell)) + 2);
+    const int ny = std::max(1, static_cast<int>(std::ceil(ext.y / cell)) + 2);
+    const int nz = std::max(1, static_cast<int>(std::ceil(ext.z / cell)) + 2);
+    const long long cells = static_cast<long long>(nx) * ny * nz;
+    if (cells > 400000000LL) throw std::runtime_error("SpatialHash: grid too large; increase the cell size");
+    g.ox = origin.x;
+    g.oy = origin.y;
+    g.oz = origin.z;
+    g.inv_cell = 1.0f / cell;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.num_cells = static_cast<int>(cells);
+    cell_off.resize(static_cast<std::size_t>(cells) + 2);
+    key_bits = 1;
+    while ((1ll << key_bits) <= cells) ++key_bits;
+    last_perm = nullptr;
+    n_sorted = 0;
+  }
+
+  // ----------------------------------------------------------- the step
+  // SpatialHash::build (spatial_hash.cu:48-114), optionally after kick_drift.
+  void grid_build(bool do_kick) {
+    PV pv = view();
+    const GridDev gd = grid();
+    launch_kick_drift_keys(pv, gd, state.get(), n, do_kick, cfg.density_mode == CRYO_DENSITY_CONTINUITY,
+                           P.tvf != 0, keys_[0].get(), vals_[0].get(), stream);
+    bool alt = false;
+    launch_radix_sort(keys_[0].get(), vals_[0].get(), keys_[1].get(), vals_[1].get(), n, key_bits, tile_hist.get(),
+                      stream, &alt);
+    const uint32_t* keys = alt ? keys_[1].get() : keys_[0].get();
+    const uint32_t* perm = alt ? vals_[1].get() : vals_[0].get();
+    launch_cell_offsets(keys, n, g.num_cells, cell_off.get(), state.get(), stream);
+    launch_gather(persist(1 - set), persist(set), perm, state.get(), n, stream);
+    check_launch();
+    set = 1 - set;
+    last_perm = perm;
+    n_sorted = n;
+    n -= pending_dead;  // dead particles sort last and are dropped
+    pending_dead = 0;
+  }
+
+  void evaluate_rates() {  // sph_solver.cu:449-480
+    PV pv = view();
+    const GridDev gd = grid();
+    const bool csf_mode = cfg.surface_tension == CRYO_ST_CSF && has_vapor;
+    const bool heat_sum = P.thermal && P.heat_power != 0.0f;
+    launch_density(pv, gd, P, n, stream);
+    launch_wall_bc(pv, gd, P, state.get(), n, stream);
+    const int nb = launch_interface(pv, gd, P, state.get(), n, csf_mode, heat_sum ? partials.get() : nullptr, stream);
+    if (csf_mode && P.csf_smooth) launch_smooth_normals(pv, gd, P, n, stream);
+    if (P.thermal) launch_heat_norm(state.get(), partials.get(), nb, heat_sum, stream);
+    launch_forces(pv, gd, P, state.get(), n, csf_mode, cfg.surface_tension == CRYO_ST_AKINCI,
+                  cfg.density_mode == CRYO_DENSITY_CONTINUITY, P.thermal != 0, stream);
+    check_launch();
+  }
+
+  // Phase inventory + ullage update + clock (sph_solver.cu:586-607, 530-532).
+  void end_step(int nb_pc, int nb_vent, bool advance) {
+    const bool closed = cfg.thermo_mode == CRYO_CLOSED && has_vapor;
+    int nb_inv = 0;
+    if (closed) nb_inv = launch_inventory(view(), state.get(), capacity, partials.get() + 5 * stride, stride, stream);
+    launch_end_step(state.get(), partials.get(), stride, nb_pc, nb_vent, nb_inv, fluid_const(), closed, advance,
+                    stream);
+    check_launch();
+  }
+
+  void ensure_initialized() {  // sph_solver.cu:439-447
+    if (initialized) return;
+    if (n == 0) throw std::runtime_error("Solver: no particles loaded");
+    grid_build(false);
+    if (cfg.normalize_initial_masses) launch_normalize_masses(view(), grid(), P, n, stream);
+    evaluate_rates();
+    end_step(0, 0, false);
+    sync_state();
+    n = hs->n;
+    initialized = true;
+  }
+
+  void phase_change_step(int& nb_pc, int& nb_vent) {  // sph_solver.cu:545-584
+    const GridDev gd = grid();
+    PV pv = view();
+    nb_pc = launch_pc_normalize(pv, gd, P, n, partials.get(), stride, stream);
+    launch_pc_apply(pv, gd, P, n, stream);
+    const bool rebalance = P.vapor_rebalance > 0.0f && has_vapor;
+    if (rebalance) launch_rebalance(pv, gd, P, n, stream);
+    if (static_cast<std::size_t>(n) * 4 > static_cast<std::size_t>(capacity) * 3) {
+      grow(capacity * 3 / 2 + 1024);
+      pv = view();
+    }
+    launch_rebalance_apply_mark(pv, P, n, rebalance, split_flag.get(), stream);
+    launch_exclusive_scan(split_flag.get(), split_off.get(), n, scan_sums.get(), scan_total.get(), stream);
+    launch_split(pv, P, state.get(), n, split_flag.get(), split_off.get(), stream);
+    launch_split_commit(state.get(), n, capacity, scan_total.get(), stream);
+    nb_vent = 0;
+    if (cfg.thermo_mode == CRYO_VENTED && has_vapor)
+      nb_vent = launch_venting(pv, P, state.get(), capacity, cfg.vent_relaxation, partials.get() + 3 * stride, stride,
+                               stream);
+    check_launch();
+  }
+
+  void step() {  // sph_solver.cu:511-533
+    ensure_initialized();
+    launch_begin_step(state.get(), P, L, man, stream);
+    grid_build(true);
+    evaluate_rates();
+    const bool shifting = P.shift_strength > 0.0f;
+    const bool pc = cfg.phase_change != 0;
+    PV pv = view();
+    launch_kick_sources(pv, P, state.get(), n, true, P.thermal != 0, pc && !shifting, stream);
+    if (shifting) {
+      launch_shifting(pv, grid(), P, state.get(), n, stream);
+      if (pc) launch_kick_sources(pv, P, state.get(), n, false, false, true, stream);
+    }
+    int nb_pc = 0, nb_vent = 0;
+    if (pc) phase_change_step(nb_pc, nb_vent);
+    end_step(nb_pc, nb_vent, true);
+    if (pc) {
+      sync_state();
+      n = hs->n;
+      pending_dead = hs->merges - merges_seen;
+      merges_seen = hs->merges;
+    }
+  }
+
+  void scale_velocities(float f) { launch_scale_velocities(view(), n, f, stream); }
+
+  void relax(double duration, float damping) {  // sph_solver.cu:707-736
+    ensure_initialized();
+    const int thermal = P.thermal;
+    const int pc = cfg.phase_change;
+    P.thermal = 0;
+    cfg.phase_change = 0;
+    sync_state();
+    const double t_end = hs->time + duration;
+    try {
+      while (hs->time < t_end) {
+        step();
+        scale_velocities(damping);
+        sync_state();
+      }
+    } catch (...) {
+      P.thermal = thermal;
+      cfg.phase_change = pc;
+      throw;
+    }
+    P.thermal = thermal;
+    cfg.phase_change = pc;
+    scale_velocities(0.0f);
+    sync_state();
+    hs->time = 0;
+    hs->step = 0;
+    hs->last_dt = 0;
+    hs->heat_input = hs->evaporated = hs->condensed = hs->vented = hs->vented_h = 0;
+    hs->splits = hs->merges = 0;
+    merges_seen = 0;
+    push_state();
+    evaluate_rates();
+    end_step(0, 0, false);
+    sync_state();
+  }
+
+  // -------------------------------------------------------------- frame
+  void apply_frame(V3 acc, V3 w, V3 wd, V3 origin) {  // sph_solver.cu:660-667
+    sync_state();
+    frame_acc = acc;
+    frame_omega = w;
+    frame_omega_dot = wd;
+    frame_origin = origin;
+    frame_rot = (norm2(w) > 0.0f || norm2(wd) > 0.0f) ? 1 : 0;
+    hs->frame_acc = acc;
+    hs->gravity = v3(cfg.gravity) - acc;
+    hs->omega = w;
+    hs->omega_dot = wd;
+    hs->origin = origin;
+    hs->frame_rot = frame_rot;
+    push_state();
+  }
+
+  void set_maneuver(const HostManeuver& mv) {  // sph_solver.cu:674-681
+    mv.validate();
+    hman = mv;
+    const int cnt = static_cast<int>(mv.time.size());
+    man_time.resize(cnt);
+    man_acc.resize(cnt);
+    CRYO_HIP(hipMemcpyAsync(man_time.get(), mv.time.data(), sizeof(double) * cnt, hipMemcpyHostToDevice, stream));
+    CRYO_HIP(hipMemcpyAsync(man_acc.get(), mv.acc.data(), sizeof(V3) * cnt, hipMemcpyHostToDevice, stream));
+    if (!mv.omega.empty()) {
+      man_omega.resize(cnt);
+      CRYO_HIP(hipMemcpyAsync(man_omega.get(), mv.omega.data(), sizeof(V3) * cnt, hipMemcpyHostToDevice, stream));
+    }
+    man = Maneuver{cnt, man_time.get(), man_acc.get(), mv.omega.empty() ? nullptr : man_omega.get(), mv.origin};
+    has_maneuver = true;
+    sync_state();
+    V3 a3, w3, wd3;
+    hman.evaluate(hs->time, a3, w3, wd3);
+    apply_frame(a3, w3, wd3, mv.origin);
+  }
+
+  void clear_maneuver() {
+    has_maneuver = false;
+    man = Maneuver{};
+  }
+
+  // ----------------------------------------------------------- queries
+  void diagnostics(cryo_diagnostics* d) {  // sph_solver.cu:609-653
+    ensure_initialized();
+    launch_diagnostics(view(), P, n, partials.get(), stride, diag_max.get(), diag_out.get(), stream);
+    check_launch();
+    std::vector<double> q(static_cast<std::size_t>(diag_quantities()));
+    unsigned mx[2];
+    CRYO_HIP(hipMemcpyAsync(q.data(), diag_out.get(), sizeof(double) * q.size(), hipMemcpyDeviceToHost, stream));
+    CRYO_HIP(hipMemcpyAsync(mx, diag_max.get(), sizeof(mx), hipMemcpyDeviceToHost, stream));
+    sync_state();
+    float fmx[2];
+    std::memcpy(fmx, mx, sizeof(fmx));
+    std::memset(d, 0, sizeof(*d));
+    const DevState& s = *hs;
+    d->time = s.time;
+    d->step = s.step;
+    d->dt = s.last_dt;
+    d->n_total = n;
+    d->n_liquid = static_cast<int>(q[0]);
+    d->n_vapor = static_cast<int>(q[1]);
+    d->n_wall = static_cast<int>(q[2]);
+    d->mass_liquid = q[3];
+    d->mass_vapor = q[4];
+    d->mass_total = d->mass_liquid + d->mass_vapor;
+    d->kinetic_energy = q[5] + q[6];
+    d->enthalpy = q[7] + q[8];
+    d->heat_input = s.heat_input;
+    d->evaporated_mass = s.evaporated;
+    d->condensed_mass = s.condensed;
+    d->vented_mass = s.vented;
+    d->vented_enthalpy = s.vented_h;
+    d->T_sat = s.T_sat;
+    d->p_ullage = s.p_ullage;
+    d->max_speed = fmx[0];
+    if (d->mass_liquid > 0) {
+      d->T_liquid_mean = q[9] / d->mass_liquid;
+      d->T_liquid_max = fmx[1];
+      d->liquid_com = cryo_vec3{static_cast<float>(q[11] / d->mass_liquid), static_cast<float>(q[12] / d->mass_liquid),
+                                static_cast<float>(q[13] / d->mass_liquid)};
+      d->liquid_momentum = cryo_vec3{static_cast<float>(q[14]), static_cast<float>(q[15]), static_cast<float>(q[16])};
+    }
+    if (d->mass_vapor > 0) d->T_vapor_mean = q[10] / d->mass_vapor;
+    d->splits = s.splits;
+    d->merges = s.merges;
+    d->slosh_force = cryo_vec3{-static_cast<float>(q[17]), -static_cast<float>(q[18]), -static_cast<float>(q[19])};
+    d->slosh_moment = cryo_vec3{-static_cast<float>(q[20]), -static_cast<float>(q[21]), -static_cast<float>(q[22])};
+    d->frame_acceleration = cv(s.frame_acc);
+    d->frame_angular_velocity = cv(s.omega);
+  }
+
+  void params(cryo_params* o) {
+    sync_state();
+    std::memset(o, 0, sizeof(*o));
+    o->h = P.kernel.h;
+    o->inv_h = P.kernel.inv_h;
+    o->support = P.kernel.support;
+    o->kernel_norm = P.kernel.norm;
+    o->support2 = P.support2;
+    o->dx = P.dx;
+    o->vol0 = P.vol0;
+    for (int k = 0; k < 2; ++k) {
+      o->rho0[k] = P.rho0[k];
+      o->mu[k] = P.mu[k];
+      o->k[k] = P.k[k];
+      o->cp[k] = P.cp[k];
+      o->mref[k] = P.mref[k];
+    }
+    o->c0 = P.c0;
+    o->tait_B = P.tait_B;
+    o->tait_gamma = P.tait_gamma;
+    o->p_bg = P.p_bg;
+    o->gas_R = P.gas_R;
+    o->vapor_K = P.vapor_K;
+    o->gravity = cv(hs->gravity);
+    o->omega = cv(hs->omega);
+    o->omega_dot = cv(hs->omega_dot);
+    o->frame_origin = cv(hs->origin);
+    o->frame_rot = hs->frame_rot;
+    o->sigma = P.sigma;
+    o->cos_theta = P.cos_theta;
+    o->sin_theta = P.sin_theta;
+    o->normal_eps = P.normal_eps;
+    o->akinci_gamma = P.ak_gamma;
+    o->akinci_beta = P.ak_beta;
+    o->tvf_sigma_min = P.tvf_sigma_min;
+    o->shift_strength = P.shift_strength;
+    o->T_ref = P.T_ref;
+    o->T_sat = hs->T_sat;
+    o->h_fg = P.h_fg;
+    o->heat_inv_norm = hs->heat_inv_norm;
+    o->dt = hs->dt;
+  }
+
+  void download(int field, void* dst, int count) {
+    if (count < 0 || count > n) throw std::out_of_range("cryo_solver_download: count exceeds num_particles");
+    if (count == 0) return;
+    const int k = set;
+    auto d4 = [&](const DevBuf<float4>& b) {
+      std::vector<float4> h(static_cast<std::size_t>(count));
+      CRYO_HIP(hipMemcpyAsync(h.data(), b.get(), sizeof(float4) * count, hipMemcpyDeviceToHost, stream));
+      CRYO_HIP(hipStreamSynchronize(stream));
+      float* o = static_cast<float*>(dst);
+      for (int i = 0; i < count; ++i) {
+        o[3 * i] = h[i].x;
+        o[3 * i + 1] = h[i].y;
+        o[3 * i + 2] = h[i].z;
+      }
+    };
+    auto d1 = [&](const auto& b) {
+      CRYO_HIP(hipMemcpyAsync(dst, b.get(), sizeof(*b.get()) * count, hipMemcpyDeviceToHost, stream));
+      CRYO_HIP(hipStreamSynchronize(stream));
+    };
+    switch (field) {
+      case CRYO_F_X: d4(x_[k]); break;
+      case CRYO_F_V: d4(v_[k]); break;
+      case CRYO_F_VT: d4(vt_[k]); break;
+      case CRYO_F_NRM: d4(nrm_[k]); break;
+      case CRYO_F_M: d1(m_[k]); break;
+      case CRYO_F_M0: d1(m0_[k]); break;
+      case CRYO_F_RHO: d1(rho_[k]); break;
+      case CRYO_F_T: d1(T_[k]); break;
+      case CRYO_F_PHASE: d1(phase_[k]); break;
+      case CRYO_F_FLAGS: d1(flags_[k]); break;
+      case CRYO_F_ID: d1(id_[k]); break;
+      case CRYO_F_A: d4(a); break;
+      case CRYO_F_AB: d4(ab); break;
+      case CRYO_F_NS: d4(ns); break;
+      case CRYO_F_VW: d4(vw); break;
+      case CRYO_F_FW: d4(fw); break;
+      case CRYO_F_MW: d4(mw); break;
+      case CRYO_F_P: d1(p); break;
+      case CRYO_F_DRHO: d1(drho); break;
+      case CRYO_F_DTDT: d1(dTdt); break;
+      case CRYO_F_SIGMA: d1(sigma); break;
+      case CRYO_F_KAPPA: d1(kappa); break;
+      case CRYO_F_NU_T: d1(nu_t); break;
+      case CRYO_F_ALPHA: d1(alpha); break;
+      case CRYO_F_WALL_W: d1(wall_w); break;
+      case CRYO_F_HEAT_W: d1(heat_w); break;
+      case CRYO_F_OUT_M: d1(out_m); break;
+      case CRYO_F_OUT_H: d1(out_h); break;
+      case CRYO_F_W_NORM: d1(w_norm); break;
+      case CRYO_F_P_WV: d1(p_wv); break;
+      case CRYO_F_RMIN: d1(rmin); break;
+      default: throw std::invalid_argument("cryo_solver_download: unknown field");
+    }
+  }
+
+  void neighbor_counts(float radius, int32_t* counts, uint64_t* sums) {
+    DevBuf<int> dc;
+    DevBuf<unsigned long long> ds;
+    dc.resize(std::max(n, 1));
+    ds.resize(std::max(n, 1));
+    const PV pv = view();
+    launch_neighbor_stats(pv.s.x, pv.s.id, grid(), n, radius * radius, dc.get(), ds.get(), stream);
+    check_launch();
+    if (n > 0) {
+      CRYO_HIP(hipMemcpyAsync(counts, dc.get(), sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+      if (sums) CRYO_HIP(hipMemcpyAsync(sums, ds.get(), sizeof(uint64_t) * n, hipMemcpyDeviceToHost, stream));
+    }
+    CRYO_HIP(hipStreamSynchronize(stream));
+  }
+
+  void neighbor_csr(float radius, int64_t* offsets, int32_t* indices) {
+    std::vector<int32_t> c(static_cast<std::size_t>(n));
+    neighbor_counts(radius, c.data(), nullptr);
+    offsets[0] = 0;
+    for (int i = 0; i < n; ++i) offsets[i + 1] = offsets[i] + c[i];
+    if (!indices || n == 0) return;
+    DevBuf<long long> doff;
+    DevBuf<int> didx;
+    doff.resize(n + 1);
+    didx.resize(std::max<int64_t>(offsets[n], 1));
+    CRYO_HIP(hipMemcpyAsync(doff.get(), offsets, sizeof(long long) * (n + 1), hipMemcpyHostToDevice, stream));
+    launch_neighbor_fill(view().s.x, grid(), n, radius * radius, doff.get(), didx.get(), stream);
+    check_launch();
+    CRYO_HIP(hipMemcpyAsync(indices, didx.get(), sizeof(int) * offsets[n], hipMemcpyDeviceToHost, stream));
+    CRYO_HIP(hipStreamSynchronize(stream));
+  }
+
+  // ---------------------------------------------------------- telemetry
+  void release_telemetry() {
+    if (tele_pending) {
+      (void)hipEventSynchronize(tele_landed);
+      tele_pending = false;
+    }
+    if (tele_host) (void)hipHostFree(tele_host);
+    tele_host = nullptr;
+    tele_host_dev = nullptr;
+    tele_staging.release();
+    tele_cap = 0;
+  }
+  void configure_telemetry(int mode) {  // telemetry.cu:27-54
+    if (mode != CRYO_TELEMETRY_ASYNC_COPY && mode != CRYO_TELEMETRY_ZERO_COPY)
+      throw std::invalid_argument("telemetry mode must be AsyncCopy or ZeroCopy");
+    release_telemetry();
+    tele_mode = mode;
+    if (capacity <= 0) return;
+    const std::size_t bytes = sizeof(float) * 7 * static_cast<std::size_t>(capacity);
+    CRYO_HIP(hipHostMalloc(reinterpret_cast<void**>(&tele_host), bytes,
+                           mode == CRYO_TELEMETRY_ZERO_COPY ? hipHostMallocMapped : hipHostMallocDefault));
+    if (mode == CRYO_TELEMETRY_ZERO_COPY)
+      CRYO_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&tele_host_dev), tele_host, 0));
+    else
+      tele_staging.resize(7 * static_cast<std::size_t>(capacity));
+    tele_cap = capacity;
+  }
+  bool request_telemetry() {  // telemetry.cu:56-96
+    ensure_initialized();
+    if (tele_pending || n > tele_cap) return false;
+    sync_state();
+    tele_time = hs->time;
+    tele_step = hs->step;
+    tele_n = n;
+    if (tele_mode == CRYO_TELEMETRY_ZERO_COPY) {
+      launch_telemetry_pack(view(), n, tele_host_dev, stream);
+      CRYO_HIP(hipEventRecord(tele_landed, stream));
+    } else {
+      launch_telemetry_pack(view(), n, tele_staging.get(), stream);
+      CRYO_HIP(hipEventRecord(tele_computed, stream));
+      CRYO_HIP(hipStreamWaitEvent(copy_stream, tele_computed, 0));
+      CRYO_HIP(hipMemcpyAsync(tele_host, tele_staging.get(), sizeof(float) * 7 * n, hipMemcpyDeviceToHost,
+                              copy_stream));
+      CRYO_HIP(hipEventRecord(tele_landed, copy_stream));
+    }
+    check_launch();
+    tele_pending = true;
+    return true;
+  }
+  bool telemetry_ready() {
+    if (!tele_pending) return true;
+    const hipError_t e = hipEventQuery(tele_landed);
+    if (e == hipSuccess) return true;
+    if (e == hipErrorNotReady) return false;
+    CRYO_HIP(e);
+    return false;
+  }
+};
+
+// ------------------------------------------------------------ C ABI glue
+namespace {
+
+int fail(const char* what, int code) {
+  g_err = what;
+  return code;
+}
+
+#define CRYO_GUARD(...)                                                  \
+  try {                                                                  \
+    __VA_ARGS__;                                                         \
+    return CRYO_OK;                                                      \
+  } catch (const std::invalid_argument& e) {                             \
+    return fail(e.what(), CRYO_ERR_INVALID_ARGUMENT);                    \
+  } catch (const std::out_of_range& e) {                                 \
+    return fail(e.what(), CRYO_ERR_OUT_OF_RANGE);                        \
+  } catch (const std::exception& e) {                                    \
+    return fail(e.what(), CRYO_ERR_RUNTIME);                             \
+  } catch (...) {                                                        \
+    return fail("unknown error", CRYO_ERR_RUNTIME);                      \
+  }
+
+cryo_solver& S(cryo_solver* s) {
+  if (!s) throw std::invalid_argument("null solver");
+  return *s;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cryo_last_error(void) { return g_err.c_str(); }
+const char* cryo_version(void) { return "0.2.0-gfx950"; }
+
+int cryo_device_name(char* buf, size_t len) {
+  CRYO_GUARD({
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) throw std::runtime_error("no HIP device");
+    int dev = 0;
+    CRYO_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop{};
+    CRYO_HIP(hipGetDeviceProperties(&prop, dev));
+    const std::string name = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+    if (buf && len) {
+      std::strncpy(buf, name.c_str(), len - 1);
+      buf[len - 1] = 0;
+    }
+  })
+}
+int cryo_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+void cryo_config_default(cryo_config* out) {
+  if (out) config_default(out);
+}
+int cryo_fluid_preset(const char* name, cryo_fluid* out) {
+  CRYO_GUARD({
+    if (!name || !out) throw std::invalid_argument("null argument");
+    if (!preset(name, out)) throw std::invalid_argument(std::string("unknown fluid preset '") + name + "'");
+  })
+}
+float cryo_fluid_saturation_temperature(const cryo_fluid* f, float p) { return saturation_temperature(*f, p); }
+float cryo_fluid_saturation_pressure(const cryo_fluid* f, float T) { return saturation_pressure(*f, T); }
+
+int cryo_solver_create(const cryo_config* cfg, const cryo_fluid* fluid, cryo_solver** out) {
+  try {
+    if (!cfg || !fluid || !out) throw std::invalid_argument("null argument");
+    *out = new cryo_solver(*cfg, *fluid);
+    return CRYO_OK;
+  } catch (const std::invalid_argument& e) {
+    return fail(e.what(), CRYO_ERR_INVALID_ARGUMENT);
+  } catch (const std::exception& e) {
+    const std::string w = e.what();
+    return fail(e.what(), w.find("CryoSPH:") == 0 ? CRYO_ERR_NO_DEVICE : CRYO_ERR_RUNTIME);
+  }
+}
+void cryo_solver_destroy(cryo_solver* s) {
+  try {
+    delete s;
+  } catch (...) {
+  }
+}
+
+int cryo_solver_set_particles(cryo_solver* s, int32_t n, const float* x, const float* v, const float* nrm,
+                              const float* m, const float* rho, const float* T, const uint8_t* phase,
+                              const uint8_t* flags, const uint32_t* id, const float domain_lo[3],
+                              const float domain_hi[3]) {
+  CRYO_GUARD(S(s).set_particles(n, x, v, nrm, m, rho, T, phase, flags, id, domain_lo, domain_hi))
+}
+int cryo_solver_step(cryo_solver* s) { CRYO_GUARD(S(s).step()) }
+int cryo_solver_run_steps(cryo_solver* s, int64_t k) {
+  CRYO_GUARD({
+    for (int64_t q = 0; q < k; ++q) S(s).step();
+    S(s).sync_state();
+  })
+}
+int cryo_solver_run_until(cryo_solver* s, double t_end) {
+  CRYO_GUARD({
+    cryo_solver& z = S(s);
+    z.ensure_initialized();
+    z.sync_state();
+    while (z.hs->time < t_end) {
+      z.step();
+      z.sync_state();
+    }
+  })
+}
+int cryo_solver_relax(cryo_solver* s, double duration, float damping) { CRYO_GUARD(S(s).relax(duration, damping)) }
+int cryo_solver_next_dt(cryo_solver* s, double* dt) {
+  CRYO_GUARD({
+    cryo_solver& z = S(s);
+    z.ensure_initialized();
+    z.sync_state();
+    *dt = compute_dt(z.P, z.L, z.hs->v_max, z.hs->a_max, z.hs->Tv_max, z.hs->nu_t_max, z.hs->spacing_ratio);
+  })
+}
+int cryo_solver_damp_velocities(cryo_solver* s, float factor) {
+  CRYO_GUARD({
+    cryo_solver& z = S(s);
+    z.scale_velocities(factor);
+    z.sync_state();
+    z.hs->v_max *= factor;
+    z.push_state();
+  })
+}
+int cryo_solver_time(cryo_solver* s, double* t) { CRYO_GUARD(S(s).sync_state(); *t = S(s).hs->time) }
+int cryo_solver_step_count(cryo_solver* s, int64_t* k) { CRYO_GUARD(S(s).sync_state(); *k = S(s).hs->step) }
+int cryo_solver_last_dt(cryo_solver* s, double* dt) { CRYO_GUARD(S(s).sync_state(); *dt = S(s).hs->last_dt) }
+int cryo_solver_num_particles(cryo_solver* s, int32_t* n) { CRYO_GUARD(*n = S(s).n) }
+int cryo_solver_capacity(cryo_solver* s, int32_t* c) { CRYO_GUARD(*c = S(s).capacity) }
+int cryo_solver_config(cryo_solver* s, cryo_config* o) { CRYO_GUARD(*o = S(s).cfg) }
+int cryo_solver_fluid(cryo_solver* s, cryo_fluid* o) { CRYO_GUARD(*o = S(s).fluid) }
+int cryo_solver_params(cryo_solver* s, cryo_params* o) { CRYO_GUARD(S(s).params(o)) }
+int cryo_solver_diagnostics(cryo_solver* s, cryo_diagnostics* o) { CRYO_GUARD(S(s).diagnostics(o)) }
+int cryo_solver_download(cryo_solver* s, int32_t field, void* dst, int32_t count) {
+  CRYO_GUARD(S(s).download(field, dst, count))
+}
+
+int cryo_solver_set_gravity(cryo_solver* s, const float g[3]) {
+  CRYO_GUARD({
+    cryo_solver& z = S(s);
+    z.cfg.gravity = cryo_vec3{g[0], g[1], g[2]};
+    z.sync_state();
+    z.hs->cfg_gravity = v3p(g);
+    z.hs->gravity = v3p(g) - z.hs->frame_acc;
+    z.push_state();
+  })
+}
+int cryo_solver_set_frame_motion(cryo_solver* s, const float acc[3], const float omega[3], const float omega_dot[3],
+                                 const float origin[3]) {
+  CRYO_GUARD({
+    cryo_solver& z = S(s);
+    z.clear_maneuver();
+    z.apply_frame(v3p(acc), v3p(omega), v3p(omega_dot), v3p(origin));
+  })
+}
+int cryo_solver_set_maneuver(cryo_solver* s, int32_t n, const double* time, const float* acc, const float* omega,
+                             const float origin[3]) {
+  CRYO_GUARD({
+    if (n < 0 || (n > 0 && (!time || !acc))) throw std::invalid_argument("ManeuverProfile: bad table");
+    HostManeuver m;
+    m.time.assign(time, time + n);
+    for (int k = 0; k < n; ++k) m.acc.push_back(v3p(acc + 3 * k));
+    if (omega)
+      for (int k = 0; k < n; ++k) m.omega.push_back(v3p(omega + 3 * k));
+    m.origin = origin ? v3p(origin) : V3{0, 0, 0};
+    S(s).set_maneuver(m);
+  })
+}
+int cryo_solver_clear_frame_motion(cryo_solver* s) {
+  CRYO_GUARD({
+    cryo_solver& z = S(s);
+    z.clear_maneuver();
+    z.apply_frame(V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0}, V3{0, 0, 0});
+  })
+}
+int cryo_solver_set_heater_power(cryo_solver* s, float watts) {
+  CRYO_GUARD({
+    S(s).cfg.heater_power = watts;
+    S(s).P.heat_power = watts;
+  })
+}
+int cryo_solver_set_contact_angle(cryo_solver* s, float degrees) {
+  CRYO_GUARD({
+    if (degrees < 0 || degrees > 180) throw std::invalid_argument("contact angle must be within [0, 180] degrees");
+    cryo_solver& z = S(s);
+    z.cfg.contact_angle_deg = degrees;
+    const float theta = degrees * kPi / 180.0f;
+    z.P.cos_theta = std::cos(theta);
+    z.P.sin_theta = std::sin(theta);
+  })
+}
+
+int cryo_maneuver_evaluate(int32_t n, const double* time, const float* acc, const float* omega, double t,
+                           float a_out[3], float w_out[3], float wdot_out[3]) {
+  CRYO_GUARD({
+    if (n < 0 || (n > 0 && (!time || !acc))) throw std::invalid_argument("ManeuverProfile: bad table");
+    HostManeuver m;
+    m.time.assign(time, time + n);
+    for (int k = 0; k < n; ++k) m.acc.push_back(v3p(acc + 3 * k));
+    if (omega)
+      for (int k = 0; k < n; ++k) m.omega.push_back(v3p(omega + 3 * k));
+    m.validate();
+    V3 a3, w3, wd3;
+    m.evaluate(t, a3, w3, wd3);
+    a_out[0] = a3.x; a_out[1] = a3.y; a_out[2] = a3.z;
+    w_out[0] = w3.x; w_out[1] = w3.y; w_out[2] = w3.z;
+    wdot_out[0] = wd3.x; wdot_out[1] = wd3.y; wdot_out[2] = wd3.z;
+  })
+}
+
+int cryo_solver_probe_pressure(cryo_solver* s, int32_t np, const float* points, float* out) {
+  CRYO_GUARD({
+    cryo_solver& z = S(s);
+    z.ensure_initialized();
+    if (np <= 0) return CRYO_OK;
+    DevBuf<float> dp, dout;
+    dp.resize(3 * static_cast<std::size_t>(np));
+    dout.resize(np);
+    CRYO_HIP(hipMemcpyAsync(dp.get(), points, sizeof(float) * 3 * np, hipMemcpyHostToDevice, z.stream));
+    launch_probe(z.view(), z.grid(), z.P, dp.get(), dout.get(), np, z.stream);
+    z.check_launch();
+    CRYO_HIP(hipMemcpyAsync(out, dout.get(), sizeof(float) * np, hipMemcpyDeviceToHost, z.stream));
+    CRYO_HIP(hipStreamSynchronize(z.stream));
+  })
+}
+
+int cryo_solver_neighbor_stats(cryo_solver* s, float radius, int32_t* counts, uint64_t* id_sums) {
+  CRYO_GUARD(S(s).neighbor_counts(radius, counts, id_sums))
+}
+int cryo_solver_neighbor_csr(cryo_solver* s, float radius, int64_t* offsets, int32_t* indices) {
+  CRYO_GUARD(S(s).neighbor_csr(radius, offsets, indices))
+}
+int cryo_solver_grid_info(cryo_solver* s, int32_t dims[3], float origin[3], float* cell_size, int32_t* num_cells) {
+  CRYO_GUARD({
+    cryo_solver& z = S(s);
+    dims[0] = z.g.nx;
+    dims[1] = z.g.ny;
+    dims[2] = z.g.nz;
+    origin[0] = z.g.ox;
+    origin[1] = z.g.oy;
+    origin[2] = z.g.oz;
+    *cell_size = z.cell;
+    *num_cells = z.g.num_cells;
+  })
+}
+int cryo_solver_grid_tables(cryo_solver* s, int32_t* cell_start, int32_t* cell_end, uint32_t* perm,
+                            int32_t* n_sorted) {
+  CRYO_GUARD({
+    cryo_solver& z = S(s);
+    const int nc = z.g.num_cells;
+    std::vector<int> off(static_cast<std::size_t>(nc) + 1);
+    CRYO_HIP(hipMemcpyAsync(off.data(), z.cell_off.get(), sizeof(int) * (nc + 1), hipMemcpyDeviceToHost, z.stream));
+    if (perm && z.last_perm && z.n_sorted > 0)
+      CRYO_HIP(hipMemcpyAsync(perm, z.last_perm, sizeof(uint32_t) * z.n_sorted, hipMemcpyDeviceToHost, z.stream));
+    CRYO_HIP(hipStreamSynchronize(z.stream));
+    for (int c = 0; c < nc; ++c) {
+      const bool nonempty = off[c + 1] > off[c];
+      if (cell_start) cell_start[c] = nonempty ? off[c] : -1;
+      if (cell_end) cell_end[c] = nonempty ? off[c + 1] : -1;
+    }
+    if (n_sorted) *n_sorted = z.n_sorted;
+  })
+}
+
+int cryo_solver_initialize(cryo_solver* s) { CRYO_GUARD(S(s).ensure_initialized()) }
+int cryo_solver_rebuild_grid(cryo_solver* s, int32_t* removed) {
+  CRYO_GUARD({
+    cryo_solver& z = S(s);
+    const int before = z.n;
+    z.grid_build(false);
+    z.sync_state();
+    z.n = z.hs->n_alive;
+    if (removed) *removed = before - z.n;
+  })
+}
+int cryo_solver_evaluate_rates(cryo_solver* s) {
+  CRYO_GUARD({
+    cryo_solver& z = S(s);
+    z.evaluate_rates();
+    CRYO_HIP(hipStreamSynchronize(z.stream));
+  })
+}
+
+int cryo_hash_sort(int32_t n, const float* x, const uint8_t* phase, const float lo[3], const float hi[3], float cell,
+                   uint32_t* perm, int32_t* n_alive, float radius, int32_t* counts, uint64_t* id_sums) {
+  CRYO_GUARD({
+    if (n < 0) throw std::invalid_argument("cryo_hash_sort: n < 0");
+    check_device();
+    cryo_config c;
+    config_default(&c);
+    cryo_fluid f;
+    preset("custom", &f);
+    cryo_solver z(c, f);
+    std::vector<float> ones(static_cast<std::size_t>(std::max(n, 1)), 1.0f);
+    if (n == 0) {
+      *n_alive = 0;
+      return CRYO_OK;
+    }
+    z.set_particles(n, x, nullptr, nullptr, ones.data(), ones.data(), ones.data(), phase, nullptr, nullptr, lo, hi);
+    z.configure_grid(v3p(lo), v3p(hi), cell);
+    const int before = z.n;
+    z.grid_build(false);
+    z.sync_state();
+    z.n = z.hs->n_alive;
+    *n_alive = z.n;
+    (void)before;
+    if (z.n > 0)
+      CRYO_HIP(hipMemcpyAsync(perm, z.view().s.id, sizeof(uint32_t) * z.n, hipMemcpyDeviceToHost, z.stream));
+    CRYO_HIP(hipStreamSynchronize(z.stream));
+    if (counts) z.neighbor_counts(radius, counts, id_sums);
+  })
+}
+
+int cryo_solver_configure_telemetry(cryo_solver* s, int32_t mode) { CRYO_GUARD(S(s).configure_telemetry(mode)) }
+int cryo_solver_telemetry_mode(cryo_solver* s, int32_t* mode) { CRYO_GUARD(*mode = S(s).tele_mode) }
+int cryo_solver_request_telemetry(cryo_solver* s) {
+  try {
+    return S(s).request_telemetry() ? 1 : 0;
+  } catch (const std::exception& e) {
+    return fail(e.what(), CRYO_ERR_RUNTIME);
+  }
+}
+int cryo_solver_telemetry_ready(cryo_solver* s) {
+  try {
+    return S(s).telemetry_ready() ? 1 : 0;
+  } catch (const std::exception& e) {
+    return fail(e.what(), CRYO_ERR_RUNTIME);
+  }
+}
+int cryo_solver_telemetry_pending_size(cryo_solver* s, int32_t* n) {
+  CRYO_GUARD(*n = S(s).tele_pending ? S(s).tele_n : 0)
+}
+int cryo_solver_fetch_telemetry(cryo_solver* s, double* time, int64_t* step, int32_t* n, float* xyz, float* T,
+                                float* p, float* speed, uint8_t* phase) {
+  try {
+    cryo_solver& z = S(s);
+    if (!z.tele_pending) return 0;
+    CRYO_HIP(hipEventSynchronize(z.tele_landed));
+    z.tele_pending = false;
+    *time = z.tele_time;
+    *step = z.tele_step;
+    *n = z.tele_n;
+    for (int i = 0; i < z.tele_n; ++i) {
+      const float* r = z.tele_host + 7 * static_cast<std::size_t>(i);
+      if (xyz) {
+        xyz[3 * i] = r[0];
+        xyz[3 * i + 1] = r[1];
+        xyz[3 * i + 2] = r[2];
+      }
+      if (T) T[i] = r[3];
+      if (p) p[i] = r[4];
+      if (speed) speed[i] = r[5];
+      uint32_t ph;
+      std::memcpy(&ph, &r[6], sizeof(ph));
+      if (phase) phase[i] = static_cast<uint8_t>(ph);
+    }
+    return 1;
+  } catch (const std::exception& e) {
+    return fail(e.what(), CRYO_ERR_RUNTIME);
+  }
+}
+
+void cryo_fill_default(cryo_fill* f) {
+  if (!f) return;
+  std::memset(f, 0, sizeof(*f));
+  f->mode = CRYO_FILL_SETTLED;
+  f->fill_fraction = 0.5;
+  f->settle_direction = cryo_vec3{0, 0, -1};
+  f->bubble_center = cryo_vec3{0, 0, 0};
+  f->include_vapor = 1;
+  f->wall_layers = 3;
+  f->T_liquid = f->T_vapor = f->T_wall = std::numeric_limits<float>::quiet_NaN();
+  f->heater_zmin = -std::numeric_limits<double>::infinity();
+  f->heater_zmax = std::numeric_limits<double>::infinity();
+}
+
+int cryo_geo_sdf(int32_t nn, const cryo_geo_node* nodes, int32_t nm, const cryo_mesh* meshes, double x, double y,
+                 double z, double* out) {
+  CRYO_GUARD(*out = Geometry(nn, nodes, nm, meshes).sdf(x, y, z))
+}
+int cryo_geo_bounds(int32_t nn, const cryo_geo_node* nodes, int32_t nm, const cryo_mesh* meshes, float lo[3],
+                    float hi[3]) {
+  CRYO_GUARD({
+    V3 l, h;
+    Geometry(nn, nodes, nm, meshes).bounds(l, h);
+    lo[0] = l.x; lo[1] = l.y; lo[2] = l.z;
+    hi[0] = h.x; hi[1] = h.y; hi[2] = h.z;
+  })
+}
+int cryo_geo_volume(int32_t nn, const cryo_geo_node* nodes, int32_t nm, const cryo_mesh* meshes, double* out) {
+  CRYO_GUARD(*out = Geometry(nn, nodes, nm, meshes).volume())
+}
+int cryo_mesh_winding_number(const cryo_mesh* mesh, double x, double y, double z, double* out) {
+  CRYO_GUARD({
+    cryo_geo_node nd{CRYO_GEO_MESH, -1, -1, 0, {0, 0, 0}};
+    *out = Geometry(1, &nd, 1, mesh).mesh_winding(0, x, y, z);
+  })
+}
+
+int cryo_build_tank_particles(int32_t nn, const cryo_geo_node* nodes, int32_t nm, const cryo_mesh* meshes,
+                              const cryo_fluid* fluid, float dx, const cryo_fill* fill, int32_t* n, float* x,
+                              float* v, float* nrm, float* m, float* rho, float* T, uint8_t* phase, uint8_t* flags,
+                              uint32_t* id) {
+  CRYO_GUARD({
+    const HostParticles hp = build_tank_particles(Geometry(nn, nodes, nm, meshes), *fluid, dx, *fill);
+    const int cnt = static_cast<int>(hp.size());
+    if (x) {
+      if (*n < cnt) throw std::out_of_range("cryo_build_tank_particles: arrays too small");
+      for (int i = 0; i < cnt; ++i) {
+        x[3 * i] = hp.x[i].x; x[3 * i + 1] = hp.x[i].y; x[3 * i + 2] = hp.x[i].z;
+        if (v) { v[3 * i] = hp.v[i].x; v[3 * i + 1] = hp.v[i].y; v[3 * i + 2] = hp.v[i].z; }
+        if (nrm) { nrm[3 * i] = hp.nrm[i].x; nrm[3 * i + 1] = hp.nrm[i].y; nrm[3 * i + 2] = hp.nrm[i].z; }
+      }
+      if (m) std::copy(hp.m.begin(), hp.m.end(), m);
+      if (rho) std::copy(hp.rho.begin(), hp.rho.end(), rho);
+      if (T) std::copy(hp.T.begin(), hp.T.end(), T);
+      if (phase) std::copy(hp.phase.begin(), hp.phase.end(), phase);
+      if (flags) std::copy(hp.flags.begin(), hp.flags.end(), flags);
+      if (id) std::copy(hp.id.begin(), hp.id.end(), id);
+    }
+    *n = cnt;
+  })
+}
+
+int cryo_solver_build_tank(cryo_solver* s, int32_t nn, const cryo_geo_node* nodes, int32_t nm,
+                           const cryo_mesh* meshes, const cryo_fill* fill) {
+  CRYO_GUARD({
+    cryo_solver& z = S(s);
+    const Geometry geo(nn, nodes, nm, meshes);
+    const HostParticles hp = build_tank_particles(geo, z.fluid, z.cfg.dx, *fill);
+    V3 lo, hi;
+    geo.bounds(lo, hi);
+    const float pad = (fill->wall_layers + 1) * z.cfg.dx;
+    lo = lo - V3{pad, pad, pad};
+    hi = hi + V3{pad, pad, pad};
+    const int cnt = static_cast<int>(hp.size());
+    std::vector<float> xs(3 * static_cast<std::size_t>(cnt)), vs(xs.size()), ns(xs.size());
+    for (int i = 0; i < cnt; ++i) {
+      xs[3 * i] = hp.x[i].x; xs[3 * i + 1] = hp.x[i].y; xs[3 * i + 2] = hp.x[i].z;
+      vs[3 * i] = hp.v[i].x; vs[3 * i + 1] = hp.v[i].y; vs[3 * i + 2] = hp.v[i].z;
+      ns[3 * i] = hp.nrm[i].x; ns[3 * i + 1] = hp.nrm[i].y; ns[3 * i + 2] = hp.nrm[i].z;
+    }
+    const float l3[3] = {lo.x, lo.y, lo.z}, h3[3] = {hi.x, hi.y, hi.z};
+    z.set_particles(cnt, xs.data(), vs.data(), ns.data(), hp.m.data(), hp.rho.data(), hp.T.data(), hp.phase.data(),
+                    hp.flags.data(), hp.id.data(), l3, h3);
+  })
+}
+
+}  // extern "C"

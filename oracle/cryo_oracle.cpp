@@ -2119,3 +2119,14 @@ double orc_saturation_pressure(const cryo_fluid* f, double T) {
 }
 
 }  // extern "C"
+
+extern "C" {
+// Pins the OpenMP thread count: the FP64 block sums depend on it.
+void orc_set_threads(int n) { omp_set_num_threads(n); }
+// Kernel value and radial derivative (SPH_kernels.cuh:14-58) for closed-form checks.
+void orc_kernel_eval(int type, double h, double r, double* W, double* dWdr) {
+  const orc::Kern k = orc::Kern::make(type, orc::R(h));
+  *W = double(orc::kW(k, orc::R(r)));
+  *dWdr = double(orc::kdW(k, orc::R(r)));
+}
+}

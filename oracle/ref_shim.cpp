@@ -6,6 +6,8 @@
 // real=double (CRYOSPH_DOUBLE_PRECISION, cuda_utils.cuh:50-55).  Output goes
 // to oracle/_ref/ only.  Nothing in the product links or loads this; tests use
 // it to pin the oracle restatement and the HIP path against the reference.
+#include <omp.h>
+
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -533,3 +535,12 @@ int ref_build_tank_particles(int nn, const cryo_geo_node* nodes, int nm, const c
 }
 
 }  // extern "C"
+
+extern "C" {
+void ref_set_threads(int n) { omp_set_num_threads(n); }
+void ref_kernel_eval(int type, double h, double r, double* W, double* dWdr) {
+  const KernelParams k = KernelParams::make(static_cast<KernelType>(type), real(h));
+  *W = double(kernel_W(k, real(r)));
+  *dWdr = double(kernel_dWdr(k, real(r)));
+}
+}

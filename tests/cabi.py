@@ -420,3 +420,238 @@ def fluid_preset(name: str) -> Fluid:
     f = Fluid()
     _check(lib, kind + "_", getattr(lib, kind + "_fluid_preset")(name.encode(), C.byref(f)))
     return f
+
+
+class HipSolver:
+    """The product: libcryosph.so through its C ABI (include/cryosph.h)."""
+
+    _lib = None
+
+    @classmethod
+    def load(cls):
+        if cls._lib is None:
+            if not HIP_LIB.exists():
+                raise FileNotFoundError(f"{HIP_LIB} not built (python {PKG / 'build.py'})")
+            lib = C.CDLL(str(HIP_LIB))
+            lib.cryo_last_error.restype = C.c_char_p
+            lib.cryo_version.restype = C.c_char_p
+            lib.cryo_fluid_saturation_temperature.restype = C.c_float
+            lib.cryo_fluid_saturation_temperature.argtypes = [C.POINTER(Fluid), C.c_float]
+            lib.cryo_fluid_saturation_pressure.restype = C.c_float
+            lib.cryo_fluid_saturation_pressure.argtypes = [C.POINTER(Fluid), C.c_float]
+            lib.cryo_solver_relax.argtypes = [C.c_void_p, C.c_double, C.c_float]
+            lib.cryo_solver_damp_velocities.argtypes = [C.c_void_p, C.c_float]
+            lib.cryo_solver_set_heater_power.argtypes = [C.c_void_p, C.c_float]
+            lib.cryo_solver_set_contact_angle.argtypes = [C.c_void_p, C.c_float]
+            lib.cryo_solver_neighbor_stats.argtypes = [C.c_void_p, C.c_float, C.c_void_p, C.c_void_p]
+            lib.cryo_solver_neighbor_csr.argtypes = [C.c_void_p, C.c_float, C.c_void_p, C.c_void_p]
+            lib.cryo_hash_sort.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float,
+                                           C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_void_p]
+            lib.cryo_build_tank_particles.argtypes = [C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                                      C.c_float, C.c_void_p, C.c_void_p] + [C.c_void_p] * 9
+            lib.cryo_solver_build_tank.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p,
+                                                   C.c_void_p]
+            cls._lib = lib
+        return cls._lib
+
+    def __init__(self, cfg: Config, fluid: Fluid):
+        self.lib = self.load()
+        self.real = np.float32
+        h = C.c_void_p()
+        self._call("solver_create", C.byref(cfg), C.byref(fluid), C.byref(h))
+        self.h = h
+
+    def _call(self, name, *args):
+        rc = getattr(self.lib, "cryo_" + name)(*args)
+        if rc < 0:
+            msg = self.lib.cryo_last_error().decode()
+            raise {-1: ValueError, -3: IndexError}.get(rc, RuntimeError)(f"cryo_{name} rc={rc}: {msg}")
+        return rc
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.cryo_solver_destroy(self.h)
+            self.h = None
+
+    def set_particles(self, ps: ParticleSet, lo, hi):
+        lo = np.asarray(lo, np.float32)
+        hi = np.asarray(hi, np.float32)
+        self._call("solver_set_particles", self.h, C.c_int32(len(ps)), _f32p(ps.x), _f32p(ps.v), _f32p(ps.nrm),
+                   _f32p(ps.m), _f32p(ps.rho), _f32p(ps.T), _u8p(ps.phase), _u8p(ps.flags), _u32p(ps.id),
+                   _f32p(lo), _f32p(hi))
+
+    def build_tank(self, nodes, fill):
+        self._call("solver_build_tank", self.h, C.c_int32(len(nodes)), C.cast(nodes, C.c_void_p), C.c_int32(0), None,
+                   C.c_void_p(C.addressof(fill)))
+
+    def step(self):
+        self._call("solver_step", self.h)
+
+    def run_steps(self, n):
+        self._call("solver_run_steps", self.h, C.c_int64(n))
+
+    def run_until(self, t):
+        self._call("solver_run_until", self.h, C.c_double(t))
+
+    def relax(self, duration, damping=0.98):
+        self._call("solver_relax", self.h, C.c_double(duration), C.c_float(damping))
+
+    def next_dt(self):
+        d = C.c_double()
+        self._call("solver_next_dt", self.h, C.byref(d))
+        return d.value
+
+    def damp_velocities(self, f):
+        self._call("solver_damp_velocities", self.h, C.c_float(f))
+
+    def initialize(self):
+        self._call("solver_initialize", self.h)
+
+    @property
+    def time(self):
+        d = C.c_double()
+        self._call("solver_time", self.h, C.byref(d))
+        return d.value
+
+    @property
+    def step_count(self):
+        d = C.c_int64()
+        self._call("solver_step_count", self.h, C.byref(d))
+        return d.value
+
+    @property
+    def num_particles(self):
+        d = C.c_int32()
+        self._call("solver_num_particles", self.h, C.byref(d))
+        return d.value
+
+    def params(self) -> Params:
+        o = Params()
+        self._call("solver_params", self.h, C.byref(o))
+        return o
+
+    def diagnostics(self) -> Diagnostics:
+        o = Diagnostics()
+        self._call("solver_diagnostics", self.h, C.byref(o))
+        return o
+
+    def field(self, name):
+        n = self.num_particles
+        if name in ("phase", "flags"):
+            out = np.empty(n, np.uint8)
+        elif name == "id":
+            out = np.empty(n, np.uint32)
+        elif name in VEC3_FIELDS:
+            out = np.empty((n, 3), np.float32)
+        else:
+            out = np.empty(n, np.float32)
+        self._call("solver_download", self.h, C.c_int32(FIELD_ID[name]), out.ctypes.data_as(C.c_void_p), C.c_int32(n))
+        return out
+
+    def set_gravity(self, g):
+        self._call("solver_set_gravity", self.h, _f32p(np.asarray(g, np.float32)))
+
+    def set_frame_motion(self, acc=(0, 0, 0), omega=(0, 0, 0), omega_dot=(0, 0, 0), origin=(0, 0, 0)):
+        a = [np.asarray(v, np.float32) for v in (acc, omega, omega_dot, origin)]
+        self._call("solver_set_frame_motion", self.h, *[_f32p(v) for v in a])
+
+    def set_maneuver(self, time, acc, omega=None, origin=(0, 0, 0)):
+        t = np.ascontiguousarray(time, np.float64)
+        a = np.ascontiguousarray(acc, np.float32)
+        w = None if omega is None else np.ascontiguousarray(omega, np.float32)
+        o = np.asarray(origin, np.float32)
+        self._call("solver_set_maneuver", self.h, C.c_int32(len(t)), t.ctypes.data_as(C.POINTER(C.c_double)),
+                   _f32p(a), _f32p(w), _f32p(o))
+
+    def set_heater_power(self, w):
+        self._call("solver_set_heater_power", self.h, C.c_float(w))
+
+    def set_contact_angle(self, d):
+        self._call("solver_set_contact_angle", self.h, C.c_float(d))
+
+    def grid_info(self):
+        dims = (C.c_int32 * 3)()
+        origin = (C.c_float * 3)()
+        cell = C.c_float()
+        nc = C.c_int32()
+        self._call("solver_grid_info", self.h, dims, origin, C.byref(cell), C.byref(nc))
+        return tuple(dims), tuple(origin), cell.value, nc.value
+
+    def grid_tables(self):
+        _, _, _, nc = self.grid_info()
+        cs = np.empty(nc, np.int32)
+        ce = np.empty(nc, np.int32)
+        self._call("solver_grid_tables", self.h, cs.ctypes.data_as(C.c_void_p), ce.ctypes.data_as(C.c_void_p),
+                   None, None)
+        return cs, ce
+
+    def neighbor_stats(self, radius):
+        n = self.num_particles
+        c = np.empty(n, np.int32)
+        s = np.empty(n, np.uint64)
+        self._call("solver_neighbor_stats", self.h, C.c_float(radius), c.ctypes.data_as(C.c_void_p),
+                   s.ctypes.data_as(C.c_void_p))
+        return c, s
+
+    def neighbor_csr(self, radius):
+        n = self.num_particles
+        off = np.empty(n + 1, np.int64)
+        self._call("solver_neighbor_csr", self.h, C.c_float(radius), off.ctypes.data_as(C.c_void_p), None)
+        idx = np.empty(max(int(off[-1]), 1), np.int32)
+        self._call("solver_neighbor_csr", self.h, C.c_float(radius), off.ctypes.data_as(C.c_void_p),
+                   idx.ctypes.data_as(C.c_void_p))
+        return off, idx[: int(off[-1])]
+
+    def probe_pressure(self, pts):
+        p = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+        out = np.empty(len(p), np.float32)
+        self._call("solver_probe_pressure", self.h, C.c_int32(len(p)), _f32p(p), _f32p(out))
+        return out
+
+
+def hip_build_tank_particles(nodes, fluid: Fluid, dx, fill: Fill) -> ParticleSet:
+    """build_tank_particles through the product's host-side generator (no GPU needed)."""
+    lib = HipSolver.load()
+    n = C.c_int32(0)
+    rc = lib.cryo_build_tank_particles(len(nodes), C.cast(nodes, C.c_void_p), 0, None, C.byref(fluid), dx,
+                                       C.byref(fill), C.byref(n), *([None] * 9))
+    if rc < 0:
+        raise RuntimeError(lib.cryo_last_error().decode())
+    cnt = n.value
+    x = np.empty((cnt, 3), np.float32)
+    v = np.empty((cnt, 3), np.float32)
+    nrm = np.empty((cnt, 3), np.float32)
+    m = np.empty(cnt, np.float32)
+    rho = np.empty(cnt, np.float32)
+    T = np.empty(cnt, np.float32)
+    ph = np.empty(cnt, np.uint8)
+    fl = np.empty(cnt, np.uint8)
+    id_ = np.empty(cnt, np.uint32)
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    rc = lib.cryo_build_tank_particles(len(nodes), C.cast(nodes, C.c_void_p), 0, None, C.byref(fluid), dx,
+                                       C.byref(fill), C.byref(n), vp(x), vp(v), vp(nrm), vp(m), vp(rho), vp(T), vp(ph),
+                                       vp(fl), vp(id_))
+    if rc < 0:
+        raise RuntimeError(lib.cryo_last_error().decode())
+    return ParticleSet(x, v, nrm, m, rho, T, ph, fl, id_)
+
+
+def hip_fluid_preset(name: str) -> Fluid:
+    lib = HipSolver.load()
+    f = Fluid()
+    if lib.cryo_fluid_preset(name.encode(), C.byref(f)) != 0:
+        raise ValueError(lib.cryo_last_error().decode())
+    return f
+
+
+def hip_config_default() -> Config:
+    lib = HipSolver.load()
+    c = Config()
+    lib.cryo_config_default(C.byref(c))
+    return c
+
+
+def tank_domain(ps: ParticleSet, nodes_bounds_lo, nodes_bounds_hi, dx, wall_layers=3):
+    """Solver::build_tank's padded domain (sph_solver.cu:430-437), in float32."""
+    pad = np.float32((wall_layers + 1) * np.float32(dx))
+    return (np.asarray(nodes_bounds_lo, np.float32) - pad, np.asarray(nodes_bounds_hi, np.float32) + pad)
