@@ -1,0 +1,165 @@
+"""Throughput benchmark of the CryoSPH step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--dx 0.007] [--cpu-steps 2] [--no-cpu]
+
+Workload (the plain form at scale): single-phase water
+column collapse in a 3.0 x 1.0 x 0.8 m box with a 0.15 x 0.4 x 0.15 m block on
+the floor at x = 2.3 m; water column 1.0 x 1.0 x 0.5 m at x = 0; rho = 1000,
+mu = 1e-3, c0 = 10 sqrt(2 g H), continuity density with delta-SPH 0.1, alpha 0.02,
+no-slip walls, thermal / phase change / surface tension off (the reference's
+own dam-break settings, validation/spheric_test2/run.py:36-51).  dx = 0.007 m
+gives 2,233,076 particles (1,441,726 fluid + 791,350 wall); synthetic lattice
+input, no data files.
+
+One "step" is Solver::step (sph_solver.cu:511-533): kick-drift, cell sort,
+every neighbour pass, reductions and dt.  `value` counts particle-steps per
+second with walls included (as benchmarks/perf_scaling.cpp:103-104 does),
+inputs resident in HBM.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "tests"))
+
+# Algorithmic bytes per particle of the passes the dominant kernel (k_forces)
+# fuses for this workload: transport velocity (bulk early exit), momentum with
+# wall loads, continuity rate and the dt maxima.
+FORCES_BYTES = {"fluid": 21 + 89 + 33 + 44, "wall": 13 + 37 + 5 + 4}
+# Whole-step algorithmic bytes of the reference pass structure for this
+# workload (single phase, continuity, no thermal / phase change / tension).
+STEP_BYTES = {"fluid": 590, "wall": 310}
+HBM_PEAK_GBS = 8000.0
+
+
+def c2_particles(cabi, dx):
+    water = cabi.hip_fluid_preset("H2O")
+    water.rho_liquid = 1000.0
+    water.mu_liquid = 1.0e-3
+    f32 = lambda v: float(np.float32(v))  # noqa: E731  (Translated keeps float offsets, geometry.hpp:92-103)
+    nodes = cabi.geo_nodes([
+        (cabi.GEO_BOX, -1, -1, (3.0, 1.0, 0.8)),
+        (cabi.GEO_TRANSLATED, 0, -1, (f32(1.5), 0.0, f32(0.4))),
+        (cabi.GEO_BOX, -1, -1, (0.15, 0.4, 0.15)),
+        (cabi.GEO_TRANSLATED, 2, -1, (f32(2.375), 0.0, f32(0.075))),
+        (cabi.GEO_DIFFERENCE, 1, 3, ()),
+    ])
+    ps = cabi.hip_build_tank_particles(nodes, water, dx, cabi.default_fill(mode=2))
+    keep = (ps.phase == cabi.WALL) | ((ps.x[:, 0] < 1.0) & (ps.x[:, 2] < 0.5))
+    ps = cabi.ParticleSet(ps.x[keep], ps.v[keep], ps.nrm[keep], ps.m[keep], ps.rho[keep], ps.T[keep], ps.phase[keep],
+                          ps.flags[keep], None)
+    pad = np.float32(4 * dx)
+    lo = np.array([-pad, -0.5 - pad, -pad], np.float32)
+    hi = np.array([3.0 + pad, 0.5 + pad, 0.8 + pad], np.float32)
+    cfg = cabi.default_config(dx=dx, gravity=(0.0, 0.0, -9.81), surface_tension=0, thermal=0, phase_change=0,
+                              density_mode=1, velocity_scale=math.sqrt(2 * 9.81 * 0.5), artificial_viscosity=0.02,
+                              delta_sph=0.1, no_slip=1)
+    return water, cfg, ps, lo, hi
+
+
+def profile_section(lib, handle, index):
+    name = C.create_string_buffer(64)
+    ms = C.c_double()
+    cnt = C.c_int64()
+    rc = lib.cryo_solver_profile_section(handle, C.c_int32(index), name, C.c_size_t(64), C.byref(ms), C.byref(cnt))
+    if rc != 0:
+        raise RuntimeError(lib.cryo_last_error().decode())
+    return name.value.decode(), ms.value, cnt.value
+
+
+def cpu_baseline(cabi, water, cfg, ps, lo, hi, steps):
+    """Reference CPU/OpenMP build (oracle/_ref) on the host cores, a bounded sample."""
+    cores = min(16, os.cpu_count() or 1)
+    kind = "reference" if cabi.REF_LIB[32].exists() else "port"
+    pref = "ref" if kind == "reference" else "orc"
+    lib = cabi.CpuSolver.load(pref, 32)
+    getattr(lib, pref + "_set_threads")(cores)
+    s = cabi.CpuSolver(pref, cfg, water, 32)
+    s.set_particles(ps, lo, hi)
+    s.step()  # start-up evaluation + one warm-up step
+    t0 = time.perf_counter()
+    s.run_steps(steps)
+    sec = time.perf_counter() - t0
+    n = s.num_particles
+    return {"value": n * steps / sec, "unit": "particle-steps/s", "cores": cores, "kind": kind,
+            "sample": f"same workload (N={n}), {steps} timed steps after 1 warm-up step, "
+                      f"{sec:.1f} s, OMP threads={cores}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--dx", type=float, default=0.007)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported (how the step would shard "
+              "across GPUs is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import cabi
+
+    water, cfg, ps, lo, hi = c2_particles(cabi, a.dx)
+    n_fluid = int((ps.phase != cabi.WALL).sum())
+    n_wall = int((ps.phase == cabi.WALL).sum())
+    s = cabi.HipSolver(cfg, water)
+    lib = s.lib
+    s.set_particles(ps, lo, hi)
+    s.run_steps(a.warmup)  # start-up evaluation, allocations, first sorts
+    lib.cryo_solver_profile_reset(s.h)
+    lib.cryo_solver_profile_enable(s.h, C.c_uint32(1 << 6))  # forces kernel only
+    t0 = time.perf_counter()
+    s.run_steps(a.steps)  # ends with a stream synchronisation
+    sec = time.perf_counter() - t0
+    lib.cryo_solver_profile_enable(s.h, C.c_uint32(0))
+    name, ms, cnt = profile_section(lib, s.h, 6)
+    n = s.num_particles
+    ms_step = 1e3 * sec / a.steps
+    forces_ms = ms / max(cnt, 1)
+    forces_bytes = n_fluid * FORCES_BYTES["fluid"] + n_wall * FORCES_BYTES["wall"]
+    achieved = forces_bytes / (forces_ms * 1e-3) / 1e9
+    step_bytes = n_fluid * STEP_BYTES["fluid"] + n_wall * STEP_BYTES["wall"]
+    out = {
+        "metric": "particle-steps/s + % HBM roofline at 1M; ensemble speedup; adjoint/forward cost",
+        "value": n * a.steps / sec,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (lattice-generated tank, no data files)",
+        "config": {"workload": f"water column collapse 3.0x1.0x0.8 m box + 0.15x0.4x0.15 m block, column "
+                               f"1.0x1.0x0.5 m, dx={a.dx}, continuity + delta-SPH, no-slip, FP32",
+                   "n_particles": n, "n_fluid": n_fluid, "n_wall": n_wall, "dx": a.dx},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_forces (transport velocity + momentum + continuity + dt maxima)",
+                     "kernel_ms": forces_ms, "algorithmic_bytes": forces_bytes},
+        "step_roofline": {"algorithmic_bytes": step_bytes,
+                          "achieved": step_bytes / (ms_step * 1e-3) / 1e9,
+                          "frac": step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cabi, water, cfg, ps, lo, hi, a.cpu_steps)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

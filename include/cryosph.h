@@ -233,6 +233,20 @@ int cryo_hash_sort(int32_t n, const float* x, const uint8_t* phase, const float 
                    float cell, uint32_t* perm, int32_t* n_alive, float radius, int32_t* counts,
                    uint64_t* id_sums);
 
+/* ------------------------------------------------------ profiling ---- */
+/* HIP-event timing of the step's sections on the solver stream (no
+ * counterpart in the reference, which only times whole steps with
+ * steady_clock, benchmarks/perf_scaling.cpp:86-104).  mask bit k enables
+ * section k: 0 begin+kick-drift+keys, 1 cell sort, 2 gather, 3 density+EOS,
+ * 4 wall BC, 5 interface fields, 6 forces (TVF, momentum, CSF, continuity,
+ * conduction), 7 kick+sources, 8 phase change, 9 end of step. */
+int cryo_solver_profile_enable(cryo_solver* s, uint32_t mask);
+int cryo_solver_profile_reset(cryo_solver* s);
+/* Name, accumulated milliseconds and launch count of section `index`;
+ * CRYO_ERR_OUT_OF_RANGE past the last section. */
+int cryo_solver_profile_section(cryo_solver* s, int32_t index, char* name, size_t len, double* total_ms,
+                                int64_t* count);
+
 /* ---------------------------------------------------- telemetry ---- */
 /* TelemetryStream (telemetry.hpp:33-70): configure / request / ready /
  * fetch.  A frame holds n records of (x, y, z, T, p, speed) floats + phase. */

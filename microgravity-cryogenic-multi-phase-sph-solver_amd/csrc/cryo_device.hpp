@@ -273,6 +273,33 @@ __device__ inline void for_each_nbr(const GridDev& g, V3 p, F&& f) {
   }
 }
 
+// ---------------------------------------------------------- neighbour lists
+// Built once per step by the density pass from the same 27-cell sweep, in
+// the same ascending order, holding every j with r^2 < (2h)^2 (fluid targets:
+// all particles, self included; wall targets: fluid particles only).
+// Transposed layout idx[k * stride + i], so lane i of a wave reading entry k
+// touches consecutive addresses.  count[i] = -1 marks an overflow (more than
+// K neighbours): that particle's passes fall back to the cell sweep, which
+// visits the same j in the same order.
+struct NbrList {
+  int* idx;
+  int* count;
+  int stride;
+  int K;
+  int valid;  // 0: positions or cells changed since the lists were built
+};
+
+template <class F>
+__device__ inline void for_each_pair(const NbrList& L, const GridDev& g, int i, V3 xi, F&& f) {
+  const int c = L.valid ? L.count[i] : -1;
+  if (c >= 0) {
+    const int* p = L.idx + i;
+    for (int k = 0; k < c; ++k) f(p[static_cast<long long>(k) * L.stride]);
+  } else {
+    for_each_nbr(g, xi, f);
+  }
+}
+
 // ------------------------------------------------------------- particle SoA
 struct Persist {
   float4 *x, *v, *vt, *nrm;

@@ -39,28 +39,31 @@ void launch_neighbor_fill(const float4* x, const GridDev& g, int n, float r2max,
 void launch_scan_one_block(uint32_t* data, int total, uint32_t* total_out, hipStream_t s);
 
 // physics
-void launch_density(const PV& pv, const GridDev& g, const SimParams& P, int n, hipStream_t s);
+void launch_density(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, int n, hipStream_t s);
 void launch_normalize_masses(const PV& pv, const GridDev& g, const SimParams& P, int n, hipStream_t s);
-void launch_wall_bc(const PV& pv, const GridDev& g, const SimParams& P, const DevState* st, int n, hipStream_t s);
-int launch_interface(const PV& pv, const GridDev& g, const SimParams& P, DevState* st, int n, bool csf_mode,
-                     double* heat_partials, hipStream_t s);
-void launch_smooth_normals(const PV& pv, const GridDev& g, const SimParams& P, int n, hipStream_t s);
+void launch_wall_bc(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, const DevState* st, int n,
+                    hipStream_t s);
+int launch_interface(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, int n,
+                     bool csf_mode, double* heat_partials, hipStream_t s);
+void launch_smooth_normals(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, int n,
+                           hipStream_t s);
 void launch_heat_norm(DevState* st, const double* partials, int nb, bool active, hipStream_t s);
-void launch_forces(const PV& pv, const GridDev& g, const SimParams& P, DevState* st, int n, bool csf_mode,
-                   bool akinci, bool continuity, bool thermal, hipStream_t s);
+void launch_forces(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, int n,
+                   bool csf_mode, bool akinci, bool continuity, bool thermal, hipStream_t s);
 void launch_probe(const PV& pv, const GridDev& g, const SimParams& P, const float* pts, float* out, int np,
                   hipStream_t s);
-void launch_shifting(const PV& pv, const GridDev& g, const SimParams& P, const DevState* st, int n, hipStream_t s);
+void launch_shifting(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, const DevState* st, int n,
+                     hipStream_t s);
 
 // step
 int diag_quantities();
 void launch_begin_step(DevState* st, const SimParams& P, const DtLimits& L, const Maneuver& M, hipStream_t s);
 void launch_kick_sources(const PV& pv, const SimParams& P, DevState* st, int n, bool do_kick, bool thermal,
                          bool sources, hipStream_t s);
-int launch_pc_normalize(const PV& pv, const GridDev& g, const SimParams& P, int n, double* partials, int stride,
-                        hipStream_t s);
-void launch_pc_apply(const PV& pv, const GridDev& g, const SimParams& P, int n, hipStream_t s);
-void launch_rebalance(const PV& pv, const GridDev& g, const SimParams& P, int n, hipStream_t s);
+int launch_pc_normalize(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, int n,
+                        double* partials, int stride, hipStream_t s);
+void launch_pc_apply(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, int n, hipStream_t s);
+void launch_rebalance(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, int n, hipStream_t s);
 void launch_rebalance_apply_mark(const PV& pv, const SimParams& P, int n, bool rebalance, uint32_t* flag,
                                  hipStream_t s);
 void launch_exclusive_scan(const uint32_t* in, uint32_t* out, int n, uint32_t* block_sums, uint32_t* total,

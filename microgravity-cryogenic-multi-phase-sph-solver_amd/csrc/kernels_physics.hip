@@ -54,23 +54,38 @@ __device__ inline void atomic_max_nonneg(unsigned* addr, float v) {
 __device__ inline float powf_cr(float x, float y) { return static_cast<float>(pow(static_cast<double>(x), static_cast<double>(y))); }
 
 // ------------------------------------------------------- density + EOS
-__global__ __launch_bounds__(kBlock) void k_density(PV pv, GridDev g, SimParams P, int n) {
+// Also builds the step's neighbour lists from the same sweep.
+__global__ __launch_bounds__(kBlock) void k_density(PV pv, GridDev g, SimParams P, NbrList L, int n) {
   const int i = gtid();
   if (i >= n) return;
   const uint8_t ph = pv.s.phase[i];
+  const V3 xi = ld3(pv.s.x[i]);
+  int* out = L.idx + i;
+  int c = 0;
   if (!is_fluid(ph)) {
     pv.t.sigma[i] = 1.0f / P.vol0;
     pv.t.rmin[i] = P.kernel.support;
+    if (ph == kWall) {  // wall targets only ever need their fluid neighbours
+      for_each_nbr(g, xi, [&](int j) {
+        if (!is_fluid(pv.s.phase[j])) return;
+        if (norm2(xi - ld3(pv.s.x[j])) >= P.support2) return;
+        if (c < L.K) out[static_cast<long long>(c) * L.stride] = j;
+        ++c;
+      });
+    }
+    L.count[i] = c <= L.K ? c : -1;
     return;
   }
-  const V3 xi = ld3(pv.s.x[i]);
   float sum = 0.0f, r2min = P.support2;
   for_each_nbr(g, xi, [&](int j) {
     const float r2 = norm2(xi - ld3(pv.s.x[j]));
     if (r2 >= P.support2) return;
     sum += kW(P.kernel, sqrtf(r2));
     if (j != i && r2 < r2min) r2min = r2;
+    if (c < L.K) out[static_cast<long long>(c) * L.stride] = j;
+    ++c;
   });
+  L.count[i] = c <= L.K ? c : -1;
   pv.t.sigma[i] = sum;
   pv.t.rmin[i] = sqrtf(r2min);
   float rho = pv.s.rho[i];
@@ -100,7 +115,8 @@ __global__ __launch_bounds__(kBlock) void k_normalize_masses(PV pv, GridDev g, S
 }
 
 // --------------------------------------------------------------- wall BC
-__global__ __launch_bounds__(kBlock) void k_wall_bc(PV pv, GridDev g, SimParams P, const DevState* st, int n) {
+__global__ __launch_bounds__(kBlock) void k_wall_bc(PV pv, GridDev g, SimParams P, NbrList L, const DevState* st,
+                                                   int n) {
   const int i = gtid();
   if (i >= n) return;
   if (pv.s.phase[i] != kWall) return;
@@ -109,7 +125,7 @@ __global__ __launch_bounds__(kBlock) void k_wall_bc(PV pv, GridDev g, SimParams 
   float sw[2] = {0.0f, 0.0f}, sp[2] = {0.0f, 0.0f}, sgr[2] = {0.0f, 0.0f};
   float srho = 0.0f;
   V3 sv{0, 0, 0};
-  for_each_nbr(g, xi, [&](int j) {
+  for_each_pair(L, g, i, xi, [&](int j) {
     const uint8_t phj = pv.s.phase[j];
     if (!is_fluid(phj)) return;
     const V3 rwf = xi - ld3(pv.s.x[j]);
@@ -148,7 +164,8 @@ __global__ __launch_bounds__(kBlock) void k_wall_bc(PV pv, GridDev g, SimParams 
 // ------------------------------------------------------- interface fields
 // Also resets the dt maxima for k_forces and writes FP64 block partials of
 // sum_fluid heat_w (sph_solver.cu:101-104) when heat_partials != nullptr.
-__global__ __launch_bounds__(kBlock) void k_interface(PV pv, GridDev g, SimParams P, DevState* st, int n,
+template <bool SPS>
+__global__ __launch_bounds__(kBlock) void k_interface(PV pv, GridDev g, SimParams P, NbrList L, DevState* st, int n,
                                                      int csf_mode, double* heat_partials) {
   const int i = gtid();
   if (i == 0) {
@@ -158,7 +175,7 @@ __global__ __launch_bounds__(kBlock) void k_interface(PV pv, GridDev g, SimParam
   if (i < n) {
     const uint8_t ph = pv.s.phase[i];
     if (!is_fluid(ph)) {
-      if (P.sps) pv.t.nu_t[i] = 0.0f;
+      if (SPS) pv.t.nu_t[i] = 0.0f;
     } else {
       const V3 xi = ld3(pv.s.x[i]);
       const float ci = ph == kLiquid ? 1.0f : 0.0f;
@@ -167,7 +184,7 @@ __global__ __launch_bounds__(kBlock) void k_interface(PV pv, GridDev g, SimParam
       // SPS velocity gradient (navier_stokes.cu:71-119)
       const V3 vi = ld3(pv.s.v[i]);
       float Lm[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-      for_each_nbr(g, xi, [&](int j) {
+      for_each_pair(L, g, i, xi, [&](int j) {
         const V3 rij = xi - ld3(pv.s.x[j]);
         const float r2 = norm2(rij);
         if (r2 >= P.support2) return;
@@ -175,7 +192,7 @@ __global__ __launch_bounds__(kBlock) void k_interface(PV pv, GridDev g, SimParam
         const float w = kW(P.kernel, r);
         const uint8_t phj = pv.s.phase[j];
         const V3 gw = kgradW(P.kernel, rij, r);
-        if (P.sps && j != i && !(phj == kWall && !P.no_slip)) {
+        if (SPS && j != i && !(phj == kWall && !P.no_slip)) {
           V3 vj;
           float Vj;
           if (phj == kWall) {
@@ -208,7 +225,7 @@ __global__ __launch_bounds__(kBlock) void k_interface(PV pv, GridDev g, SimParam
         gradc += gw * (Vj * (cj - ci));
         if (phj == kLiquid) akn += gw * Vj;
       });
-      if (P.sps) {
+      if (SPS) {
         float ss = 0.0f;
         for (int a = 0; a < 3; ++a)
           for (int b = 0; b < 3; ++b) {
@@ -267,7 +284,7 @@ __global__ __launch_bounds__(kBlock) void k_interface(PV pv, GridDev g, SimParam
 }
 
 // surface_tension.cu:87-118 with neighbour smoothing on.
-__global__ __launch_bounds__(kBlock) void k_smooth_normals(PV pv, GridDev g, SimParams P, int n) {
+__global__ __launch_bounds__(kBlock) void k_smooth_normals(PV pv, GridDev g, SimParams P, NbrList L, int n) {
   const int i = gtid();
   if (i >= n) return;
   if (!is_fluid(pv.s.phase[i])) return;
@@ -279,7 +296,7 @@ __global__ __launch_bounds__(kBlock) void k_smooth_normals(PV pv, GridDev g, Sim
   }
   const V3 xi = ld3(pv.s.x[i]);
   V3 acc{0, 0, 0};
-  for_each_nbr(g, xi, [&](int j) {
+  for_each_pair(L, g, i, xi, [&](int j) {
     if (!is_fluid(pv.s.phase[j])) return;
     const V3 gj = ld3(pv.s.nrm[j]);
     const float gjm = norm(gj);
@@ -292,15 +309,12 @@ __global__ __launch_bounds__(kBlock) void k_smooth_normals(PV pv, GridDev g, Sim
 }
 
 // ------------------------------------------------------------------ forces
-struct ForceFlags {
-  int csf_mode;     // CSF with vapour present
-  int akinci;       // Akinci model
-  int continuity;   // continuity density rate
-  int thermal;      // conduction + heater
-};
-
-__global__ __launch_bounds__(kBlock) void k_forces(PV pv, GridDev g, SimParams P, DevState* st, int n,
-                                                  ForceFlags F) {
+// ST: 0 no tension pass, 1 CSF with vapour present, 2 Akinci.
+template <int ST, bool CONT, bool THERMAL>
+__global__ __launch_bounds__(kBlock) void k_forces(PV pv, GridDev g, SimParams P, NbrList L, DevState* st, int n) {
+  struct {
+    bool csf_mode, akinci, continuity, thermal;
+  } constexpr F{ST == 1, ST == 2, CONT, THERMAL};
   const int i = gtid();
   float m_v = 0.0f, m_a = 0.0f, m_tv = 0.0f, m_nu = 0.0f, m_sp = 0.0f;
   if (i < n) {
@@ -366,7 +380,7 @@ __global__ __launch_bounds__(kBlock) void k_forces(PV pv, GridDev g, SimParams P
       const bool iso = P.wall_bc == 1;
       float q = 0.0f;
 
-      for_each_nbr(g, xi, [&](int j) {
+      for_each_pair(L, g, i, xi, [&](int j) {
         if (j == i) return;
         const V3 xj = ld3(pv.s.x[j]);
         const V3 rij = xi - xj;
@@ -597,8 +611,8 @@ __global__ __launch_bounds__(kBlock) void k_probe(PV pv, GridDev g, SimParams P,
 }
 
 // SPH_kernels.cu:106-160 (delta+ shifting), pass 1 and pass 2.
-__global__ __launch_bounds__(kBlock) void k_shift_compute(PV pv, GridDev g, SimParams P, const DevState* st,
-                                                         int n) {
+__global__ __launch_bounds__(kBlock) void k_shift_compute(PV pv, GridDev g, SimParams P, NbrList L,
+                                                         const DevState* st, int n) {
   const int i = gtid();
   if (i >= n) return;
   pv.t.ns[i] = st3(V3{0, 0, 0});
@@ -611,7 +625,7 @@ __global__ __launch_bounds__(kBlock) void k_shift_compute(PV pv, GridDev g, SimP
   const float cap = 0.1f * P.dx;
   const V3 xi = ld3(pv.s.x[i]);
   V3 sum{0, 0, 0};
-  for_each_nbr(g, xi, [&](int j) {
+  for_each_pair(L, g, i, xi, [&](int j) {
     if (j == i) return;
     const uint8_t phj = pv.s.phase[j];
     if (phj == kDead) return;
@@ -648,40 +662,62 @@ __global__ __launch_bounds__(kBlock) void k_shift_apply(PV pv, int n) {
 
 }  // namespace
 
-void launch_density(const PV& pv, const GridDev& g, const SimParams& P, int n, hipStream_t s) {
-  if (n > 0) k_density<<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, n);
+void launch_density(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, int n, hipStream_t s) {
+  if (n > 0) k_density<<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, n);
 }
 void launch_normalize_masses(const PV& pv, const GridDev& g, const SimParams& P, int n, hipStream_t s) {
   if (n > 0) k_normalize_masses<<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, n);
 }
-void launch_wall_bc(const PV& pv, const GridDev& g, const SimParams& P, const DevState* st, int n, hipStream_t s) {
-  if (n > 0) k_wall_bc<<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, st, n);
+void launch_wall_bc(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, const DevState* st, int n,
+                    hipStream_t s) {
+  if (n > 0) k_wall_bc<<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, st, n);
 }
-int launch_interface(const PV& pv, const GridDev& g, const SimParams& P, DevState* st, int n, bool csf_mode,
-                     double* heat_partials, hipStream_t s) {
+int launch_interface(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, int n,
+                     bool csf_mode, double* heat_partials, hipStream_t s) {
   const int nb = blocks_for(n > 0 ? n : 1);
-  k_interface<<<nb, kBlock, 0, s>>>(pv, g, P, st, n, csf_mode ? 1 : 0, heat_partials);
+  if (P.sps) k_interface<true><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, n, csf_mode ? 1 : 0, heat_partials);
+  else k_interface<false><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, n, csf_mode ? 1 : 0, heat_partials);
   return nb;
 }
-void launch_smooth_normals(const PV& pv, const GridDev& g, const SimParams& P, int n, hipStream_t s) {
-  if (n > 0) k_smooth_normals<<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, n);
+void launch_smooth_normals(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, int n,
+                           hipStream_t s) {
+  if (n > 0) k_smooth_normals<<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, n);
 }
 void launch_heat_norm(DevState* st, const double* partials, int nb, bool active, hipStream_t s) {
   k_heat_norm<<<1, 1024, 0, s>>>(st, partials, nb, active ? 1 : 0);
 }
-void launch_forces(const PV& pv, const GridDev& g, const SimParams& P, DevState* st, int n, bool csf_mode,
-                   bool akinci, bool continuity, bool thermal, hipStream_t s) {
-  ForceFlags F{csf_mode ? 1 : 0, akinci ? 1 : 0, continuity ? 1 : 0, thermal ? 1 : 0};
-  k_forces<<<blocks_for(n > 0 ? n : 1), kBlock, 0, s>>>(pv, g, P, st, n, F);
+
+template <int ST, bool CONT, bool THERMAL>
+static void forces_variant(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, int n,
+                           hipStream_t s) {
+  k_forces<ST, CONT, THERMAL><<<blocks_for(n > 0 ? n : 1), kBlock, 0, s>>>(pv, g, P, L, st, n);
+}
+template <int ST>
+static void forces_st(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, int n,
+                      bool continuity, bool thermal, hipStream_t s) {
+  if (continuity) {
+    if (thermal) forces_variant<ST, true, true>(pv, g, P, L, st, n, s);
+    else forces_variant<ST, true, false>(pv, g, P, L, st, n, s);
+  } else {
+    if (thermal) forces_variant<ST, false, true>(pv, g, P, L, st, n, s);
+    else forces_variant<ST, false, false>(pv, g, P, L, st, n, s);
+  }
+}
+void launch_forces(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, int n,
+                   bool csf_mode, bool akinci, bool continuity, bool thermal, hipStream_t s) {
+  if (akinci) forces_st<2>(pv, g, P, L, st, n, continuity, thermal, s);
+  else if (csf_mode) forces_st<1>(pv, g, P, L, st, n, continuity, thermal, s);
+  else forces_st<0>(pv, g, P, L, st, n, continuity, thermal, s);
   k_publish_maxima<<<1, 1, 0, s>>>(st, P.has_vapor, P.sps);
 }
 void launch_probe(const PV& pv, const GridDev& g, const SimParams& P, const float* pts, float* out, int np,
                   hipStream_t s) {
   if (np > 0) k_probe<<<blocks_for(np), kBlock, 0, s>>>(pv, g, P, pts, out, np);
 }
-void launch_shifting(const PV& pv, const GridDev& g, const SimParams& P, const DevState* st, int n, hipStream_t s) {
+void launch_shifting(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, const DevState* st, int n,
+                     hipStream_t s) {
   if (n <= 0) return;
-  k_shift_compute<<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, st, n);
+  k_shift_compute<<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, st, n);
   k_shift_apply<<<blocks_for(n), kBlock, 0, s>>>(pv, n);
 }
 

@@ -186,8 +186,8 @@ __global__ __launch_bounds__(kBlock) void k_kick_sources(PV pv, SimParams P, Dev
 
 // phase_change.cu:139-172, plus the partials of sph_solver.cu:108-124
 // (evaporated, condensed, merges) over every particle.
-__global__ __launch_bounds__(kBlock) void k_pc_normalize(PV pv, GridDev g, SimParams P, int n, double* partials,
-                                                        int stride) {
+__global__ __launch_bounds__(kBlock) void k_pc_normalize(PV pv, GridDev g, SimParams P, NbrList L, int n,
+                                                        double* partials, int stride) {
   const int i = gtid();
   double acc[3] = {0.0, 0.0, 0.0};
   if (i < n) {
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(kBlock) void k_pc_normalize(PV pv, GridDev g, SimPa
       const uint8_t target = (fl & kTargetVapor) ? kVapor : kLiquid;
       const V3 xi = ld3(pv.s.x[i]);
       float wsum = 0.0f, wr_max = 0.0f;
-      for_each_nbr(g, xi, [&](int j) {
+      for_each_pair(L, g, i, xi, [&](int j) {
         if (j == i || pv.s.phase[j] != target || (pv.s.flags[j] & kDying)) return;
         const float r2 = norm2(xi - ld3(pv.s.x[j]));
         if (r2 >= P.support2) return;
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(kBlock) void k_pc_normalize(PV pv, GridDev g, SimPa
 }
 
 // phase_change.cu:174-215
-__global__ __launch_bounds__(kBlock) void k_pc_apply(PV pv, GridDev g, SimParams P, int n) {
+__global__ __launch_bounds__(kBlock) void k_pc_apply(PV pv, GridDev g, SimParams P, NbrList L, int n) {
   const int i = gtid();
   if (i >= n) return;
   const uint8_t ph = pv.s.phase[i];
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void k_pc_apply(PV pv, GridDev g, SimParams
   const V3 xi = ld3(pv.s.x[i]);
   float in_m = 0.0f, in_H = 0.0f;
   if (!(fl & kDying)) {
-    for_each_nbr(g, xi, [&](int j) {
+    for_each_pair(L, g, i, xi, [&](int j) {
       if (j == i) return;
       const float om = pv.t.out_m[j];
       if (!(om > 0.0f)) return;
@@ -267,7 +267,7 @@ __global__ __launch_bounds__(kBlock) void k_pc_apply(PV pv, GridDev g, SimParams
 }
 
 // phase_change.cu:217-249 (pass 1)
-__global__ __launch_bounds__(kBlock) void k_rebalance(PV pv, GridDev g, SimParams P, int n) {
+__global__ __launch_bounds__(kBlock) void k_rebalance(PV pv, GridDev g, SimParams P, NbrList L, int n) {
   const int i = gtid();
   if (i >= n) return;
   pv.t.out_m[i] = 0.0f;
@@ -281,7 +281,7 @@ __global__ __launch_bounds__(kBlock) void k_rebalance(PV pv, GridDev g, SimParam
   const V3 vi = ld3(pv.s.v[i]);
   float dm = 0.0f, dH = 0.0f;
   V3 dP{0, 0, 0};
-  for_each_nbr(g, xi, [&](int j) {
+  for_each_pair(L, g, i, xi, [&](int j) {
     if (j == i || pv.s.phase[j] != kVapor) return;
     const float r2 = norm2(xi - ld3(pv.s.x[j]));
     if (r2 >= P.support2) return;
@@ -617,17 +617,17 @@ void launch_kick_sources(const PV& pv, const SimParams& P, DevState* st, int n, 
   k_kick_sources<<<blocks_for(n > 0 ? n : 1), kBlock, 0, s>>>(pv, P, st, n, do_kick ? 1 : 0, thermal ? 1 : 0,
                                                               sources ? 1 : 0);
 }
-int launch_pc_normalize(const PV& pv, const GridDev& g, const SimParams& P, int n, double* partials, int stride,
-                        hipStream_t s) {
+int launch_pc_normalize(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, int n,
+                        double* partials, int stride, hipStream_t s) {
   const int nb = blocks_for(n > 0 ? n : 1);
-  k_pc_normalize<<<nb, kBlock, 0, s>>>(pv, g, P, n, partials, stride);
+  k_pc_normalize<<<nb, kBlock, 0, s>>>(pv, g, P, L, n, partials, stride);
   return nb;
 }
-void launch_pc_apply(const PV& pv, const GridDev& g, const SimParams& P, int n, hipStream_t s) {
-  if (n > 0) k_pc_apply<<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, n);
+void launch_pc_apply(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, int n, hipStream_t s) {
+  if (n > 0) k_pc_apply<<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, n);
 }
-void launch_rebalance(const PV& pv, const GridDev& g, const SimParams& P, int n, hipStream_t s) {
-  if (n > 0) k_rebalance<<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, n);
+void launch_rebalance(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, int n, hipStream_t s) {
+  if (n > 0) k_rebalance<<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, n);
 }
 void launch_rebalance_apply_mark(const PV& pv, const SimParams& P, int n, bool rebalance, uint32_t* flag,
                                  hipStream_t s) {

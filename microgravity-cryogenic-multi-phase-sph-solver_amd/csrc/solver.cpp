@@ -225,6 +225,10 @@ struct cryo_solver {
   // grid and scans
   DevBuf<uint32_t> keys_[2], vals_[2], tile_hist, split_flag, split_off, scan_sums, scan_total;
   DevBuf<int> cell_off;
+  // neighbour lists of the current step (NbrList in cryo_device.hpp)
+  static constexpr int kListK = 96;
+  DevBuf<int> nbr_idx, nbr_count;
+  bool lists_valid = false;
   const uint32_t* last_perm = nullptr;
   int n_sorted = 0;
   // reductions
@@ -264,6 +268,58 @@ struct cryo_solver {
   long long tele_step = 0;
   int tele_n = 0;
 
+  // Per-section HIP-event timing on the solver stream (cryo_solver_profile_*).
+  static constexpr int kSections = 10;
+  static constexpr const char* kSectionNames[kSections] = {
+      "begin_kick_keys", "cell_sort", "gather", "density_eos", "wall_bc", "interface", "forces", "kick_sources",
+      "phase_change", "end_step"};
+  unsigned prof_mask = 0;
+  struct Mark {
+    int sec;
+    hipEvent_t a, b;
+  };
+  std::vector<Mark> marks;
+  std::vector<hipEvent_t> event_pool;
+  hipEvent_t open_event[kSections] = {};
+  double prof_ms[kSections] = {};
+  long long prof_count[kSections] = {};
+
+  hipEvent_t take_event() {
+    if (!event_pool.empty()) {
+      hipEvent_t e = event_pool.back();
+      event_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    CRYO_HIP(hipEventCreate(&e));
+    return e;
+  }
+  void prof_begin(int sec) {
+    if (!(prof_mask & (1u << sec))) return;
+    open_event[sec] = take_event();
+    CRYO_HIP(hipEventRecord(open_event[sec], stream));
+  }
+  void prof_end(int sec) {
+    if (!(prof_mask & (1u << sec)) || !open_event[sec]) return;
+    hipEvent_t b = take_event();
+    CRYO_HIP(hipEventRecord(b, stream));
+    marks.push_back(Mark{sec, open_event[sec], b});
+    open_event[sec] = nullptr;
+  }
+  void prof_collect() {
+    if (marks.empty()) return;
+    CRYO_HIP(hipStreamSynchronize(stream));
+    for (const Mark& m : marks) {
+      float ms = 0;
+      CRYO_HIP(hipEventElapsedTime(&ms, m.a, m.b));
+      prof_ms[m.sec] += ms;
+      prof_count[m.sec] += 1;
+      event_pool.push_back(m.a);
+      event_pool.push_back(m.b);
+    }
+    marks.clear();
+  }
+
   cryo_solver(const cryo_config& c, const cryo_fluid& f) : cfg(c), fluid(f) {
     if (!(cfg.dx > 0)) throw std::invalid_argument("SolverConfig.dx must be > 0");
     if (cfg.contact_angle_deg < 0 || cfg.contact_angle_deg > 180)
@@ -288,6 +344,11 @@ struct cryo_solver {
     if (stream) (void)hipStreamSynchronize(stream);
     if (copy_stream) (void)hipStreamSynchronize(copy_stream);
     release_telemetry();
+    for (const Mark& m : marks) {
+      (void)hipEventDestroy(m.a);
+      (void)hipEventDestroy(m.b);
+    }
+    for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
     if (hs) (void)hipHostFree(hs);
     if (tele_computed) (void)hipEventDestroy(tele_computed);
     if (tele_landed) (void)hipEventDestroy(tele_landed);
@@ -327,6 +388,9 @@ struct cryo_solver {
     gd.off = cell_off.get();
     return gd;
   }
+  NbrList lists() {
+    return NbrList{nbr_idx.get(), nbr_count.get(), capacity, kListK, lists_valid ? 1 : 0};
+  }
 
   template <class F>
   void each_particle_array(F&& f) {
@@ -358,6 +422,9 @@ struct cryo_solver {
     split_off.resize(cap);
     scan_sums.resize(static_cast<std::size_t>((cap + 1023) / 1024) + 1);
     scan_total.resize(1);
+    nbr_idx.resize(static_cast<std::size_t>(kListK) * cap);
+    nbr_count.resize(cap);
+    lists_valid = false;
     stride = (cap + 255) / 256 + 1;
     partials.resize(static_cast<std::size_t>(std::max(diag_quantities(), 8)) * stride);
     diag_out.resize(diag_quantities());
@@ -616,19 +683,26 @@ struct cryo_solver {
   void grid_build(bool do_kick) {
     PV pv = view();
     const GridDev gd = grid();
+    prof_begin(0);
     launch_kick_drift_keys(pv, gd, state.get(), n, do_kick, cfg.density_mode == CRYO_DENSITY_CONTINUITY,
                            P.tvf != 0, keys_[0].get(), vals_[0].get(), stream);
+    prof_end(0);
+    prof_begin(1);
     bool alt = false;
     launch_radix_sort(keys_[0].get(), vals_[0].get(), keys_[1].get(), vals_[1].get(), n, key_bits, tile_hist.get(),
                       stream, &alt);
     const uint32_t* keys = alt ? keys_[1].get() : keys_[0].get();
     const uint32_t* perm = alt ? vals_[1].get() : vals_[0].get();
     launch_cell_offsets(keys, n, g.num_cells, cell_off.get(), state.get(), stream);
+    prof_end(1);
+    prof_begin(2);
     launch_gather(persist(1 - set), persist(set), perm, state.get(), n, stream);
+    prof_end(2);
     check_launch();
     set = 1 - set;
     last_perm = perm;
     n_sorted = n;
+    lists_valid = false;
     n -= pending_dead;  // dead particles sort last and are dropped
     pending_dead = 0;
   }
@@ -638,13 +712,25 @@ struct cryo_solver {
     const GridDev gd = grid();
     const bool csf_mode = cfg.surface_tension == CRYO_ST_CSF && has_vapor;
     const bool heat_sum = P.thermal && P.heat_power != 0.0f;
-    launch_density(pv, gd, P, n, stream);
-    launch_wall_bc(pv, gd, P, state.get(), n, stream);
-    const int nb = launch_interface(pv, gd, P, state.get(), n, csf_mode, heat_sum ? partials.get() : nullptr, stream);
-    if (csf_mode && P.csf_smooth) launch_smooth_normals(pv, gd, P, n, stream);
+    prof_begin(3);
+    lists_valid = false;
+    launch_density(pv, gd, P, lists(), n, stream);
+    lists_valid = true;
+    const NbrList L = lists();
+    prof_end(3);
+    prof_begin(4);
+    launch_wall_bc(pv, gd, P, L, state.get(), n, stream);
+    prof_end(4);
+    prof_begin(5);
+    const int nb =
+        launch_interface(pv, gd, P, L, state.get(), n, csf_mode, heat_sum ? partials.get() : nullptr, stream);
+    if (csf_mode && P.csf_smooth) launch_smooth_normals(pv, gd, P, L, n, stream);
     if (P.thermal) launch_heat_norm(state.get(), partials.get(), nb, heat_sum, stream);
-    launch_forces(pv, gd, P, state.get(), n, csf_mode, cfg.surface_tension == CRYO_ST_AKINCI,
+    prof_end(5);
+    prof_begin(6);
+    launch_forces(pv, gd, P, L, state.get(), n, csf_mode, cfg.surface_tension == CRYO_ST_AKINCI,
                   cfg.density_mode == CRYO_DENSITY_CONTINUITY, P.thermal != 0, stream);
+    prof_end(6);
     check_launch();
   }
 
@@ -673,10 +759,11 @@ struct cryo_solver {
   void phase_change_step(int& nb_pc, int& nb_vent) {  // sph_solver.cu:545-584
     const GridDev gd = grid();
     PV pv = view();
-    nb_pc = launch_pc_normalize(pv, gd, P, n, partials.get(), stride, stream);
-    launch_pc_apply(pv, gd, P, n, stream);
+    const NbrList L = lists();
+    nb_pc = launch_pc_normalize(pv, gd, P, L, n, partials.get(), stride, stream);
+    launch_pc_apply(pv, gd, P, L, n, stream);
     const bool rebalance = P.vapor_rebalance > 0.0f && has_vapor;
-    if (rebalance) launch_rebalance(pv, gd, P, n, stream);
+    if (rebalance) launch_rebalance(pv, gd, P, L, n, stream);
     if (static_cast<std::size_t>(n) * 4 > static_cast<std::size_t>(capacity) * 3) {
       grow(capacity * 3 / 2 + 1024);
       pv = view();
@@ -684,6 +771,7 @@ struct cryo_solver {
     launch_rebalance_apply_mark(pv, P, n, rebalance, split_flag.get(), stream);
     launch_exclusive_scan(split_flag.get(), split_off.get(), n, scan_sums.get(), scan_total.get(), stream);
     launch_split(pv, P, state.get(), n, split_flag.get(), split_off.get(), stream);
+    lists_valid = false;  // split moves parents and appends particles
     launch_split_commit(state.get(), n, capacity, scan_total.get(), stream);
     nb_vent = 0;
     if (cfg.thermo_mode == CRYO_VENTED && has_vapor)
@@ -696,18 +784,26 @@ struct cryo_solver {
     ensure_initialized();
     launch_begin_step(state.get(), P, L, man, stream);
     grid_build(true);
+    if (marks.size() > 4096) prof_collect();
     evaluate_rates();
     const bool shifting = P.shift_strength > 0.0f;
     const bool pc = cfg.phase_change != 0;
     PV pv = view();
+    prof_begin(7);
     launch_kick_sources(pv, P, state.get(), n, true, P.thermal != 0, pc && !shifting, stream);
     if (shifting) {
-      launch_shifting(pv, grid(), P, state.get(), n, stream);
+      launch_shifting(pv, grid(), P, lists(), state.get(), n, stream);
+      lists_valid = false;  // positions moved
       if (pc) launch_kick_sources(pv, P, state.get(), n, false, false, true, stream);
     }
+    prof_end(7);
     int nb_pc = 0, nb_vent = 0;
+    prof_begin(8);
     if (pc) phase_change_step(nb_pc, nb_vent);
+    prof_end(8);
+    prof_begin(9);
     end_step(nb_pc, nb_vent, true);
+    prof_end(9);
     if (pc) {
       sync_state();
       n = hs->n;
@@ -1348,6 +1444,38 @@ int cryo_hash_sort(int32_t n, const float* x, const uint8_t* phase, const float 
       CRYO_HIP(hipMemcpyAsync(perm, z.view().s.id, sizeof(uint32_t) * z.n, hipMemcpyDeviceToHost, z.stream));
     CRYO_HIP(hipStreamSynchronize(z.stream));
     if (counts) z.neighbor_counts(radius, counts, id_sums);
+  })
+}
+
+int cryo_solver_profile_enable(cryo_solver* s, uint32_t mask) {
+  CRYO_GUARD({
+    cryo_solver& z = S(s);
+    z.prof_collect();
+    z.prof_mask = mask;
+  })
+}
+int cryo_solver_profile_reset(cryo_solver* s) {
+  CRYO_GUARD({
+    cryo_solver& z = S(s);
+    z.prof_collect();
+    for (int k = 0; k < cryo_solver::kSections; ++k) {
+      z.prof_ms[k] = 0;
+      z.prof_count[k] = 0;
+    }
+  })
+}
+int cryo_solver_profile_section(cryo_solver* s, int32_t index, char* name, size_t len, double* total_ms,
+                                int64_t* count) {
+  CRYO_GUARD({
+    cryo_solver& z = S(s);
+    if (index < 0 || index >= cryo_solver::kSections) throw std::out_of_range("profile section index");
+    z.prof_collect();
+    if (name && len) {
+      std::strncpy(name, cryo_solver::kSectionNames[index], len - 1);
+      name[len - 1] = 0;
+    }
+    if (total_ms) *total_ms = z.prof_ms[index];
+    if (count) *count = z.prof_count[index];
   })
 }
 

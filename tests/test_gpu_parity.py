@@ -1,0 +1,312 @@
+"""Parity of the gfx950 path (libcryosph.so through its C ABI) with the
+reference: golden vectors written by the reference build, the oracle on the
+same seeded inputs, and size-independent properties at full scale.
+
+Bars: cell sort and neighbour sets bit-exact; per-field
+rel-L2 <= 1e-5 after 100 steps (fields aligned by particle id); mass error
+<= 2.5e-6; energy error < 1 % of the latent heat moved.
+"""
+import ctypes as C
+import math
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import cabi
+from cabi import FIELDS, WALL, CpuSolver, HipSolver, ParticleSet
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / "golden"
+REL_L2 = 1e-5
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    den = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / den) if den > 0 else float(np.linalg.norm(a - b))
+
+
+def by_id(s, f):
+    ids = s.field("id")
+    order = np.argsort(ids, kind="stable")
+    return ids[order], s.field(f)[order]
+
+
+def jittered(nodes, fluid, dx, fill, seed=0):
+    ps = cabi.hip_build_tank_particles(nodes, fluid, dx, fill)
+    rng = np.random.default_rng(seed)
+    fl = ps.phase != WALL
+    ps.x[fl] += rng.uniform(-0.05 * dx, 0.05 * dx, size=(int(fl.sum()), 3)).astype(np.float32)
+    return ps, ps.x.min(0) - np.float32(4 * dx), ps.x.max(0) + np.float32(4 * dx)
+
+
+def pair(cfg, fluid, ps, lo, hi):
+    h = HipSolver(cfg, fluid)
+    o = CpuSolver("orc", cfg, fluid, 32)
+    for s in (h, o):
+        s.set_particles(ps, lo, hi)
+    return h, o
+
+
+# ------------------------------------------------------------ spatial hash
+def test_hash_sort_matches_reference_golden():
+    say("test_hash_sort_matches_reference_golden")
+    g = np.load(GOLD / "hash_sort.npz")
+    lib = HipSolver.load()
+    x = np.ascontiguousarray(g["x"])
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    for tag in ("alive", "dead"):
+        ph = np.ascontiguousarray(g[f"{tag}_phase"])
+        n = len(x)
+        perm = np.zeros(n, np.uint32)
+        cnt = np.zeros(n, np.int32)
+        sums = np.zeros(n, np.uint64)
+        na = C.c_int32()
+        rc = lib.cryo_hash_sort(n, vp(x), vp(ph), vp(np.zeros(3, np.float32)), vp(np.ones(3, np.float32)),
+                                C.c_float(0.08), vp(perm), C.byref(na), C.c_float(0.08), vp(cnt), vp(sums))
+        assert rc == 0, lib.cryo_last_error()
+        k = na.value
+        assert k == int(g[f"{tag}_n_alive"])
+        assert np.array_equal(perm[:k], g[f"{tag}_perm"])
+        assert np.array_equal(cnt[:k], g[f"{tag}_counts"])
+        assert np.array_equal(sums[:k], g[f"{tag}_id_sums"])
+
+
+def test_c1_sort_cells_and_neighbour_sets_bit_exact():
+    say("test_c1_sort_cells_and_neighbour_sets_bit_exact")
+    g = np.load(GOLD / "c1_grid.npz")
+    n = len(g["in_x"])
+    fluid = cabi.hip_fluid_preset("LN2")
+    cfg = cabi.default_config(dx=0.004, gravity=(0, 0, -9.81e-4), heater_power=20.0, thermo_mode=1)
+    zeros = np.zeros((n, 3), np.float32)
+    ps = ParticleSet(g["in_x"], zeros, zeros, np.ones(n), np.full(n, 800.0), np.full(n, 77.36), g["in_phase"], None,
+                     None)
+    h, o = pair(cfg, fluid, ps, g["lo"], g["hi"])
+    h.next_dt()
+    o.next_dt()
+    assert np.array_equal(h.field("id"), g["sorted_id"])
+    cs, ce = h.grid_tables()
+    assert np.array_equal(cs, g["cell_start"]) and np.array_equal(ce, g["cell_end"])
+    support = float(g["support"])
+    cnt, sums = h.neighbor_stats(support)
+    assert np.array_equal(cnt, g["counts"]) and np.array_equal(sums, g["id_sums"])
+    off_h, idx_h = h.neighbor_csr(support)
+    off_o, idx_o = o.neighbor_csr(support)
+    assert np.array_equal(off_h, off_o) and np.array_equal(idx_h, idx_o)
+
+
+# ------------------------------------------------------------ fields
+def test_box_tank_matches_reference_golden():
+    say("test_box_tank_matches_reference_golden")
+    g = np.load(GOLD / "box_tank.npz")
+    ps = ParticleSet(*(g["in_" + k] for k in ("x", "v", "nrm", "m", "rho", "T", "phase", "flags", "id")))
+    fluid = cabi.hip_fluid_preset("LN2")
+    cfg = cabi.default_config(dx=0.004, gravity=(0, 0, -0.5), contact_angle_deg=10, velocity_scale=1.0,
+                              thermo_mode=1, lee_evaporation=5.0, lee_condensation=5.0)
+    h = HipSolver(cfg, fluid)
+    h.set_particles(ps, g["lo"], g["hi"])
+    h.next_dt()
+    inexact = []
+    for f in FIELDS:
+        a, b = h.field(f), g["init_" + f]
+        if not np.array_equal(a, b):
+            inexact.append(f)
+            assert f in ("ab",) and rel_l2(a, b) <= 1e-6, (f, rel_l2(a, b))
+    say("  start-up fields not bit-exact:", inexact)
+    h.run_steps(20)
+    for f in ("id", "phase"):
+        assert np.array_equal(h.field(f), g["step20_" + f])
+    for f in ("x", "v", "m", "T", "rho", "a", "p"):
+        assert rel_l2(h.field(f), g["step20_" + f]) <= 1e-6, f
+
+
+SCENARIOS = {
+    "closed_pc_maneuver": dict(cfg=dict(dx=0.004, gravity=(0, 0, -0.5), contact_angle_deg=10, velocity_scale=1.0,
+                                        thermo_mode=1, lee_evaporation=5.0, lee_condensation=5.0, split_ratio=1.05),
+                               geo=lambda: cabi.box(0.04, 0.04, 0.04), fill=dict(T_liquid=80.36), steps=40,
+                               maneuver=True),
+    "vented_heater": dict(cfg=dict(dx=0.004, gravity=(0, 0, -1e-4), contact_angle_deg=5, velocity_scale=0.5,
+                                   heater_power=5.0, lee_evaporation=0.5, lee_condensation=0.5, vent_relaxation=0.2),
+                          geo=lambda: cabi.cylinder(0.03, 0.06),
+                          fill=dict(T_liquid=77.86, heater_zmin=-0.02, heater_zmax=0.0), steps=30),
+    "continuity_sps": dict(cfg=dict(dx=0.005, density_mode=1, sps_turbulence=1, surface_tension=0, thermal=0,
+                                    phase_change=0, artificial_viscosity=0.05),
+                           geo=lambda: cabi.box(0.06, 0.04, 0.05), fill=dict(include_vapor=0), steps=30),
+    "akinci": dict(cfg=dict(dx=0.005, surface_tension=2, thermal=0, phase_change=0, gravity=(0, 0, -0.3),
+                            contact_angle_deg=60),
+                   geo=lambda: cabi.sphere(0.03), fill=dict(include_vapor=0, fill_fraction=0.6), steps=25),
+    "wetting_both_smooth": dict(cfg=dict(dx=0.004, wetting_model=2, csf_smooth_normals=1, csf_balanced=1,
+                                         contact_angle_deg=30, gravity=(0, 0, -0.2), velocity_scale=0.5,
+                                         wall_thermal_bc=1, thermo_mode=1),
+                                geo=lambda: cabi.capsule(0.02, 0.03), fill=dict(T_liquid=78.0), steps=25),
+    "shifting_tvf_stress": dict(cfg=dict(dx=0.005, particle_shifting=1, transport_stress=1, thermal=0,
+                                         phase_change=0, surface_tension=1, velocity_scale=1.0),
+                                geo=lambda: cabi.box(0.05, 0.05, 0.05), fill=dict(), steps=25),
+}
+
+
+@pytest.mark.parametrize("name", sorted(SCENARIOS))
+def test_scenario_matches_oracle(name):
+    say(f"test_scenario_matches_oracle[{name}]")
+    sc = SCENARIOS[name]
+    fluid = cabi.hip_fluid_preset("LN2")
+    cfg = cabi.default_config(**sc["cfg"])
+    ps, lo, hi = jittered(sc["geo"](), fluid, sc["cfg"]["dx"], cabi.default_fill(**sc["fill"]))
+    h, o = pair(cfg, fluid, ps, lo, hi)
+    h.next_dt()
+    o.next_dt()
+    for f in FIELDS:
+        a, b = h.field(f), o.field(f)
+        if f in ("phase", "flags", "id"):
+            assert np.array_equal(a, b), f
+        else:
+            assert rel_l2(a, b) <= 1e-6, (f, rel_l2(a, b))
+    if sc.get("maneuver"):
+        for s in (h, o):
+            s.set_maneuver([0.0, 0.01, 0.02], [[0, 0, 0], [0.5, 0, 0], [0.5, 0.2, 0]],
+                           [[0, 0, 0], [0, 0, 1], [0, 0, 2]], (0.0, 0.0, -0.01))
+    h.run_steps(sc["steps"])
+    o.run_steps(sc["steps"])
+    assert h.num_particles == o.num_particles
+    for f in ("x", "v", "m", "T", "rho"):
+        ih, a = by_id(h, f)
+        io, b = by_id(o, f)
+        assert np.array_equal(ih, io)
+        assert rel_l2(a, b) <= REL_L2, (f, rel_l2(a, b))
+    dh, do = h.diagnostics(), o.diagnostics()
+    assert dh.n_total == do.n_total and dh.splits == do.splits and dh.merges == do.merges
+    assert dh.mass_total == pytest.approx(do.mass_total, rel=1e-7)
+    assert dh.time == pytest.approx(do.time, rel=1e-12)
+
+
+# ------------------------------------------------ BASELINE config 1 (C1)
+def c1_setup():
+    fluid = cabi.hip_fluid_preset("LN2")
+    cfg = cabi.default_config(dx=0.004, gravity=(0, 0, -9.81e-4), heater_power=20.0, thermo_mode=1)
+    ps, lo, hi = jittered(cabi.cylinder(0.05, 0.12), fluid, 0.004, cabi.default_fill(T_liquid=fluid.T_sat + 2.0))
+    return fluid, cfg, ps, lo, hi
+
+
+def test_c1_100_steps_parity_mass_energy():
+    """LN2 cylinder r=0.05 h=0.12 dx=4 mm (26,360 particles), closed tank, heater 20 W, superheated liquid."""
+    say("test_c1_100_steps_parity_mass_energy")
+    fluid, cfg, ps, lo, hi = c1_setup()
+    h, o = pair(cfg, fluid, ps, lo, hi)
+    d0 = h.diagnostics()
+    h.run_steps(100)
+    o.run_steps(100)
+    worst = {}
+    for f in ("x", "v", "vt", "m", "rho", "T", "p", "a"):
+        ih, a = by_id(h, f)
+        io, b = by_id(o, f)
+        assert np.array_equal(ih, io)
+        worst[f] = rel_l2(a, b)
+    say("  rel-L2 after 100 steps:", {k: f"{v:.2e}" for k, v in worst.items()})
+    assert max(worst.values()) <= REL_L2, worst
+    d1, do = h.diagnostics(), o.diagnostics()
+    # mass bookkeeping: the closed tank conserves mass through evaporation
+    mass_err = abs(d1.mass_total - d0.mass_total) / d0.mass_total
+    assert mass_err <= 2.5e-6, mass_err
+    assert d1.evaporated_mass > 0
+    # energy: enthalpy change = heater input, within 1 % of the latent heat moved
+    latent = (d1.evaporated_mass + d1.condensed_mass) * fluid.h_fg
+    e_err = abs((d1.enthalpy - d0.enthalpy) - d1.heat_input)
+    roundoff = 1e-6 * d0.mass_total * fluid.cp_liquid * fluid.T_sat
+    assert e_err < 0.01 * latent + roundoff, (e_err, latent)
+    for k in ("evaporated_mass", "p_ullage", "T_sat", "heat_input", "mass_liquid", "mass_vapor"):
+        assert getattr(d1, k) == pytest.approx(getattr(do, k), rel=1e-5), k
+
+
+def test_determinism_bitwise_with_splitting():
+    """test_determinism.cpp:40-54: two identical 300-step runs with splitting are bitwise equal."""
+    say("test_determinism_bitwise_with_splitting")
+    fluid = cabi.hip_fluid_preset("LN2")
+    cfg = cabi.default_config(dx=0.004, gravity=(0, 0, -0.5), contact_angle_deg=10, velocity_scale=1.0,
+                              thermo_mode=1, lee_evaporation=5.0, lee_condensation=5.0, split_ratio=1.05)
+    runs = []
+    for _ in range(2):
+        s = HipSolver(cfg, fluid)
+        s.build_tank(cabi.box(0.04, 0.04, 0.04), cabi.default_fill(T_liquid=fluid.T_sat + 3.0))
+        s.run_steps(300)
+        runs.append((s.diagnostics(), {f: s.field(f) for f in ("x", "v", "m", "T", "id")}))
+    assert runs[0][0].splits > 0
+    for f in runs[0][1]:
+        assert np.array_equal(runs[0][1][f].view(np.uint8), runs[1][1][f].view(np.uint8)), f
+    assert runs[0][0].as_dict() == runs[1][0].as_dict()
+
+
+def test_relax_and_run_until_match_oracle():
+    say("test_relax_and_run_until_match_oracle")
+    fluid = cabi.hip_fluid_preset("LN2")
+    cfg = cabi.default_config(dx=0.005, gravity=(0, 0, 0), velocity_scale=1.0, thermo_mode=1, lee_evaporation=2.0)
+    ps, lo, hi = jittered(cabi.box(0.04, 0.04, 0.04), fluid, 0.005, cabi.default_fill(T_liquid=79.36))
+    h, o = pair(cfg, fluid, ps, lo, hi)
+    for s in (h, o):
+        s.relax(0.01)
+        s.run_until(0.004)
+        s.damp_velocities(0.5)
+        s.set_heater_power(3.0)
+        s.set_contact_angle(40.0)
+        s.run_steps(5)
+    assert h.step_count == o.step_count and h.num_particles == o.num_particles
+    assert h.time == pytest.approx(o.time, rel=1e-12)
+    for f in ("x", "v", "m", "T"):
+        assert rel_l2(h.field(f), o.field(f)) <= REL_L2, f
+
+
+def test_probe_pressure_matches_oracle():
+    say("test_probe_pressure_matches_oracle")
+    fluid, cfg, ps, lo, hi = c1_setup()
+    h, o = pair(cfg, fluid, ps, lo, hi)
+    h.run_steps(3)
+    o.run_steps(3)
+    pts = np.array([[0.0, 0.0, -0.05], [0.02, 0.0, -0.03], [0.0, 0.0, 0.05], [1.0, 1.0, 1.0]], np.float32)
+    ph = h.probe_pressure(pts)
+    po = np.empty(len(pts), np.float32)
+    o.lib.orc_probe_pressure(o.h, C.c_int(len(pts)), pts.ctypes.data_as(C.c_void_p), po.ctypes.data_as(C.c_void_p))
+    assert np.isnan(ph[3]) and np.isnan(po[3])
+    assert np.allclose(ph[:3], po[:3], rtol=1e-6, atol=0)
+
+
+# ------------------------------------------- full size, size-independent
+def test_c2_full_size_properties():
+    """BASELINE config 2 at full size (2,233,076 particles): the cell sort is a
+    stable sort by key, ids stay a permutation, total mass is invariant in
+    continuity mode, the walls do not move, and two runs agree bitwise."""
+    say("test_c2_full_size_properties")
+    sys.path.insert(0, str(Path(__file__).resolve().parent.parent))
+    import bench
+    water, cfg, ps, lo, hi = bench.c2_particles(cabi, 0.007)
+    assert len(ps) == 2233076
+    outs = []
+    for _ in range(2):
+        s = HipSolver(cfg, water)
+        s.set_particles(ps, lo, hi)
+        s.run_steps(3)
+        outs.append({f: s.field(f) for f in ("x", "v", "id", "m", "phase")})
+        d = s.diagnostics()
+        assert d.n_total == 2233076 and d.n_wall == 791350
+        assert d.mass_total == pytest.approx(float(ps.m[ps.phase != WALL].astype(np.float64).sum()), rel=1e-12)
+    a, b = outs
+    for f in a:
+        assert np.array_equal(a[f].view(np.uint8), b[f].view(np.uint8)), f
+    ids = a["id"]
+    assert np.array_equal(np.sort(ids), np.arange(len(ps), dtype=np.uint32))
+    x = a["x"]
+    dims, origin, cell, nc = s.grid_info()
+    c = np.clip(np.floor((x - np.asarray(origin, np.float32)) * np.float32(1.0 / np.float32(cell))).astype(np.int64),
+                0, np.asarray(dims) - 1)
+    key = (c[:, 2] * dims[1] + c[:, 1]) * dims[0] + c[:, 0]
+    # no pass moves particles between the last sort and the end of the step
+    # (kick changes velocities only), so the stored order is sorted by key
+    assert np.all(np.diff(key) >= 0)
+    walls = a["phase"] == WALL
+    assert np.array_equal(x[walls][np.argsort(ids[walls])], ps.x[ps.phase == WALL])
