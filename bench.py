@@ -1,8 +1,8 @@
 """Throughput benchmark of the CryoSPH step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--dx 0.007] [--cpu-steps 2] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--dx 0.007] [--cpu-steps 20] [--no-cpu]
 
-Workload (the plain form at scale): single-phase water
+Workload (the reference dam-break form at scale): single-phase water
 column collapse in a 3.0 x 1.0 x 0.8 m box with a 0.15 x 0.4 x 0.15 m block on
 the floor at x = 2.3 m; water column 1.0 x 1.0 x 0.5 m at x = 0; rho = 1000,
 mu = 1e-3, c0 = 10 sqrt(2 g H), continuity density with delta-SPH 0.1, alpha 0.02,
@@ -96,18 +96,38 @@ def cpu_baseline(cabi, water, cfg, ps, lo, hi, steps):
                       f"{sec:.1f} s, OMP threads={cores}"}
 
 
+def copy_calibration(mib):
+    """A plain device-to-device copy of known size: what FETCH_SIZE / WRITE_SIZE
+    report for it converts those counters into bytes for the other kernels."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "dispatches": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--dx", type=float, default=0.007)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (scales the HBM byte counters) and exit")
     a = ap.parse_args()
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
+        return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported (how the step would shard "
-              "across GPUs is described in DESIGN.md)", file=sys.stderr)
+              "across GPUs is described in the design notes)", file=sys.stderr)
         sys.exit(2)
 
     import cabi

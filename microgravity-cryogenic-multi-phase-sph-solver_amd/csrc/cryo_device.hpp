@@ -244,6 +244,7 @@ struct GridDev {
   float ox, oy, oz, inv_cell;
   int nx, ny, nz, num_cells;
   const int* off;
+  float cull_pad;  // cell units: rounding allowance of the cell-culled sweep
 };
 __host__ __device__ inline void cell_coords(const GridDev& g, V3 p, int& cx, int& cy, int& cz) {
   cx = clampv(static_cast<int>(floorf((p.x - g.ox) * g.inv_cell)), 0, g.nx - 1);
@@ -258,7 +259,7 @@ __host__ __device__ inline int cell_index(const GridDev& g, V3 p) {
 
 // Visits every particle of the 27 cells around p, in the reference order.
 template <class F>
-__device__ inline void for_each_nbr(const GridDev& g, V3 p, F&& f) {
+__host__ __device__ inline void for_each_nbr(const GridDev& g, V3 p, F&& f) {
   int cx, cy, cz;
   cell_coords(g, p, cx, cy, cz);
   const int z0 = cz > 0 ? cz - 1 : 0, z1 = cz < g.nz - 1 ? cz + 1 : cz;
@@ -266,6 +267,40 @@ __device__ inline void for_each_nbr(const GridDev& g, V3 p, F&& f) {
   const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx < g.nx - 1 ? cx + 1 : cx;
   for (int z = z0; z <= z1; ++z) {
     for (int y = y0; y <= y1; ++y) {
+      const int row = (z * g.ny + y) * g.nx;
+      const int s = g.off[row + x0], e = g.off[row + x1 + 1];
+      for (int j = s; j < e; ++j) f(j);
+    }
+  }
+}
+
+// The same sweep without the cells that provably hold nothing within `radius`
+// of p: a row (or the first / last cell of a row) is skipped when the gap
+// between p and that cell, measured in cell units from the same
+// (p - origin) * inv_cell that assigns cells, is at least radius plus a
+// rounding allowance.  Particles clamped into the outer cells lie beyond those
+// cells, so the gap stays a lower bound of their distance.  The visiting order
+// of what remains is unchanged (ascending sorted index).
+template <class F>
+__host__ __device__ inline void for_each_cand(const GridDev& g, V3 p, float radius, F&& f) {
+  const float ux = (p.x - g.ox) * g.inv_cell, uy = (p.y - g.oy) * g.inv_cell, uz = (p.z - g.oz) * g.inv_cell;
+  const int cx = clampv(static_cast<int>(floorf(ux)), 0, g.nx - 1);
+  const int cy = clampv(static_cast<int>(floorf(uy)), 0, g.ny - 1);
+  const int cz = clampv(static_cast<int>(floorf(uz)), 0, g.nz - 1);
+  const float fx = ux - static_cast<float>(cx), fy = uy - static_cast<float>(cy), fz = uz - static_cast<float>(cz);
+  const float rc = radius * g.inv_cell + g.cull_pad;
+  const float rr = rc * rc;
+  const float gxl = fx * fx, gxr = (1.0f - fx) * (1.0f - fx);
+  const int z0 = cz > 0 ? cz - 1 : 0, z1 = cz < g.nz - 1 ? cz + 1 : cz;
+  const int y0 = cy > 0 ? cy - 1 : 0, y1 = cy < g.ny - 1 ? cy + 1 : cy;
+  for (int z = z0; z <= z1; ++z) {
+    const float gz = z < cz ? fz : (z > cz ? 1.0f - fz : 0.0f);
+    for (int y = y0; y <= y1; ++y) {
+      const float gy = y < cy ? fy : (y > cy ? 1.0f - fy : 0.0f);
+      const float gyz = gy * gy + gz * gz;
+      if (gyz >= rr) continue;
+      const int x0 = (cx > 0 && gyz + gxl < rr) ? cx - 1 : cx;
+      const int x1 = (cx < g.nx - 1 && gyz + gxr < rr) ? cx + 1 : cx;
       const int row = (z * g.ny + y) * g.nx;
       const int s = g.off[row + x0], e = g.off[row + x1 + 1];
       for (int j = s; j < e; ++j) f(j);
@@ -294,10 +329,38 @@ __device__ inline void for_each_pair(const NbrList& L, const GridDev& g, int i, 
   const int c = L.valid ? L.count[i] : -1;
   if (c >= 0) {
     const int* p = L.idx + i;
-    for (int k = 0; k < c; ++k) f(p[static_cast<long long>(k) * L.stride]);
+    const long long stride = L.stride;
+    int jn = c > 0 ? p[0] : 0;
+    for (int k = 0; k < c; ++k) {  // the next index is in flight while f runs
+      const int j = jn;
+      if (k + 1 < c) jn = p[(k + 1) * stride];
+      f(j);
+    }
   } else {
     for_each_nbr(g, xi, f);
   }
+}
+
+// The list part of for_each_pair alone (count[i] >= 0 and valid lists).
+template <class F>
+__device__ inline void for_each_listed(const NbrList& L, int i, int c, F&& f) {
+  const int* p = L.idx + i;
+  const long long stride = L.stride;
+  int jn = c > 0 ? p[0] : 0;
+  for (int k = 0; k < c; ++k) {
+    const int j = jn;
+    if (k + 1 < c) jn = p[(k + 1) * stride];
+    f(j);
+  }
+}
+
+// Holds the listed values in registers here: every load issued before this
+// point completes here, together, instead of each at its first use.
+__device__ inline void settle1(float v) { asm volatile("" ::"v"(v)); }
+__device__ inline void settle1(int v) { asm volatile("" ::"v"(v)); }
+template <class... T>
+__device__ inline void settle(T... v) {
+  (settle1(v), ...);
 }
 
 // ------------------------------------------------------------- particle SoA
@@ -310,6 +373,12 @@ struct Persist {
 struct Scratch {
   float4 *a, *ab, *ns, *vw, *fw, *mw;
   float *p, *drho, *dTdt, *sigma, *kappa, *nu_t, *alpha, *wall_w, *heat_w, *out_m, *out_h, *w_norm, *p_wv, *rmin;
+  // Neighbour record of one rates evaluation, so a pair loop gathers one
+  // float4 instead of four scalars: {p, m, rho, T} for fluid particles
+  // (written by the density pass), {p, p_wv, rho_tvf, T} for walls (wall
+  // pass; rho_tvf is the wall density liquid targets use for the transport
+  // velocity).
+  float4* nrec;
 };
 struct PV {  // kernel view
   Persist s;

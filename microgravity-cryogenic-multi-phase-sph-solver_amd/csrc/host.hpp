@@ -9,7 +9,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/cryosph.h"
+#include "cryosph.h"
 #include "cryo_device.hpp"
 #include "kernels.hpp"
 

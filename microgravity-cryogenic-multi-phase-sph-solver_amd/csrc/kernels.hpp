@@ -28,7 +28,7 @@ void launch_kick_drift_keys(const PV& pv, const GridDev& g, DevState* st, int n,
                             bool tvf, uint32_t* keys, uint32_t* vals, hipStream_t s);
 int radix_tiles(int n);
 void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, int n,
-                       int key_bits, uint32_t* tile_hist, hipStream_t s, bool* result_in_alt);
+                       int key_bits, uint32_t* tile_hist, uint32_t* row_total, hipStream_t s, bool* result_in_alt);
 void launch_cell_offsets(const uint32_t* keys, int n, int num_cells, int* off, DevState* st, hipStream_t s);
 void launch_gather(const Persist& dst, const Persist& src, const uint32_t* perm, const DevState* st, int n_ub,
                    hipStream_t s);

@@ -87,16 +87,58 @@ __global__ __launch_bounds__(1024) void k_scan_one_block(uint32_t* data, int tot
   if (tid == 1023 && total_out) *total_out = part[1023];
 }
 
+// Exclusive scan of one digit's counts over all tiles (row d of the
+// digit-major histogram), in place; the row total goes to row_total[d].
+__global__ __launch_bounds__(256) void k_radix_scan_rows(uint32_t* tile_hist, int num_tiles, uint32_t* row_total) {
+  __shared__ uint32_t part[256];
+  uint32_t* row = tile_hist + static_cast<long long>(blockIdx.x) * num_tiles;
+  const int tid = threadIdx.x;
+  const int chunk = (num_tiles + 255) / 256;
+  const int lo = tid * chunk;
+  const int hi = lo + chunk < num_tiles ? lo + chunk : num_tiles;
+  uint32_t s = 0;
+  for (int k = lo; k < hi; ++k) s += row[k];
+  part[tid] = s;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const uint32_t v = tid >= off ? part[tid - off] : 0u;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  uint32_t run = tid > 0 ? part[tid - 1] : 0u;
+  for (int k = lo; k < hi; ++k) {
+    const uint32_t v = row[k];
+    row[k] = run;
+    run += v;
+  }
+  if (tid == 255) row_total[blockIdx.x] = part[255];
+}
+
 // Stable scatter: ranks inside a tile follow element order (round by round,
 // wave by wave, lane by lane), tiles follow tile order through tile_off.
 __global__ __launch_bounds__(kRsThreads) void k_radix_scatter(const uint32_t* keys_in, const uint32_t* vals_in,
                                                              uint32_t* keys_out, uint32_t* vals_out, int n,
-                                                             int shift, const uint32_t* tile_off, int num_tiles) {
+                                                             int shift, const uint32_t* tile_off, int num_tiles,
+                                                             const uint32_t* row_total) {
   __shared__ uint32_t running[256];
+  __shared__ uint32_t digit_base[256];
   __shared__ uint32_t wave_cnt[kRsThreads / 64][256];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
+  // exclusive scan of the 256 digit totals: where each digit's run starts
+  digit_base[tid] = row_total[tid];
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const uint32_t v = tid >= off ? digit_base[tid - off] : 0u;
+    __syncthreads();
+    digit_base[tid] += v;
+    __syncthreads();
+  }
+  const uint32_t incl = digit_base[tid];
+  __syncthreads();
+  digit_base[tid] = incl - row_total[tid];
   running[tid] = 0;
   for (int w = 0; w < kRsThreads / 64; ++w) wave_cnt[w][tid] = 0;
   __syncthreads();
@@ -122,7 +164,7 @@ __global__ __launch_bounds__(kRsThreads) void k_radix_scatter(const uint32_t* ke
     if (valid) {
       uint32_t pos = running[d] + rank;
       for (int w = 0; w < wave; ++w) pos += wave_cnt[w][d];
-      const uint32_t dest = tile_off[d * num_tiles + blockIdx.x] + pos;
+      const uint32_t dest = digit_base[d] + tile_off[d * num_tiles + blockIdx.x] + pos;
       keys_out[dest] = key;
       vals_out[dest] = vals_in[i];
     }
@@ -221,15 +263,15 @@ void launch_kick_drift_keys(const PV& pv, const GridDev& g, DevState* st, int n,
 int radix_tiles(int n) { return (n + kRsTile - 1) / kRsTile; }
 
 void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, int n,
-                       int key_bits, uint32_t* tile_hist, hipStream_t s, bool* result_in_alt) {
+                       int key_bits, uint32_t* tile_hist, uint32_t* row_total, hipStream_t s, bool* result_in_alt) {
   *result_in_alt = false;
   if (n <= 0) return;
   const int tiles = radix_tiles(n);
   uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt;
   for (int shift = 0; shift < key_bits; shift += 8) {
     k_radix_hist<<<tiles, kRsThreads, 0, s>>>(ki, n, shift, tile_hist, tiles);
-    k_scan_one_block<<<1, 1024, 0, s>>>(tile_hist, 256 * tiles, nullptr);
-    k_radix_scatter<<<tiles, kRsThreads, 0, s>>>(ki, vi, ko, vo, n, shift, tile_hist, tiles);
+    k_radix_scan_rows<<<256, 256, 0, s>>>(tile_hist, tiles, row_total);
+    k_radix_scatter<<<tiles, kRsThreads, 0, s>>>(ki, vi, ko, vo, n, shift, tile_hist, tiles, row_total);
     uint32_t* t = ki;
     ki = ko;
     ko = t;

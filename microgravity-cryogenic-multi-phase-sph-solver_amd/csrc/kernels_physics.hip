@@ -54,22 +54,23 @@ __device__ inline void atomic_max_nonneg(unsigned* addr, float v) {
 __device__ inline float powf_cr(float x, float y) { return static_cast<float>(pow(static_cast<double>(x), static_cast<double>(y))); }
 
 // ------------------------------------------------------- density + EOS
-// Also builds the step's neighbour lists from the same sweep.
+// Also builds the step's neighbour lists from the same (cell-culled) sweep.
 __global__ __launch_bounds__(kBlock) void k_density(PV pv, GridDev g, SimParams P, NbrList L, int n) {
   const int i = gtid();
   if (i >= n) return;
   const uint8_t ph = pv.s.phase[i];
   const V3 xi = ld3(pv.s.x[i]);
   int* out = L.idx + i;
+  const long long stride = L.stride;
   int c = 0;
   if (!is_fluid(ph)) {
     pv.t.sigma[i] = 1.0f / P.vol0;
     pv.t.rmin[i] = P.kernel.support;
     if (ph == kWall) {  // wall targets only ever need their fluid neighbours
-      for_each_nbr(g, xi, [&](int j) {
+      for_each_cand(g, xi, P.kernel.support, [&](int j) {
         if (!is_fluid(pv.s.phase[j])) return;
         if (norm2(xi - ld3(pv.s.x[j])) >= P.support2) return;
-        if (c < L.K) out[static_cast<long long>(c) * L.stride] = j;
+        if (c < L.K) out[c * stride] = j;
         ++c;
       });
     }
@@ -77,23 +78,27 @@ __global__ __launch_bounds__(kBlock) void k_density(PV pv, GridDev g, SimParams 
     return;
   }
   float sum = 0.0f, r2min = P.support2;
-  for_each_nbr(g, xi, [&](int j) {
+  for_each_cand(g, xi, P.kernel.support, [&](int j) {
     const float r2 = norm2(xi - ld3(pv.s.x[j]));
     if (r2 >= P.support2) return;
     sum += kW(P.kernel, sqrtf(r2));
     if (j != i && r2 < r2min) r2min = r2;
-    if (c < L.K) out[static_cast<long long>(c) * L.stride] = j;
+    if (c < L.K) out[c * stride] = j;
     ++c;
   });
   L.count[i] = c <= L.K ? c : -1;
   pv.t.sigma[i] = sum;
   pv.t.rmin[i] = sqrtf(r2min);
   float rho = pv.s.rho[i];
+  const float mi = pv.s.m[i];
   if (P.density_mode == 0) {
-    rho = (ph == kLiquid ? pv.s.m0[i] : pv.s.m[i]) * sum;
+    rho = (ph == kLiquid ? pv.s.m0[i] : mi) * sum;
     pv.s.rho[i] = rho;
   }
-  pv.t.p[i] = eos_pressure(P, ph, rho, pv.s.T[i]);
+  const float Ti = pv.s.T[i];
+  const float p = eos_pressure(P, ph, rho, Ti);
+  pv.t.p[i] = p;
+  pv.t.nrec[i] = make_float4(p, mi, rho, Ti);
 }
 
 // SPH_kernels.cu:36-54 (start-up only).
@@ -151,13 +156,23 @@ __global__ __launch_bounds__(kBlock) void k_wall_bc(PV pv, GridDev g, SimParams 
   pv.t.p_wv[i] = pw[1];
   pv.t.alpha[i] = (sw[0] + sw[1]) > 0.0f ? sw[0] / (sw[0] + sw[1]) : 0.0f;
   const float swt = sw[0] + sw[1];
+  float rho_w;
   if (swt > 0.0f) {
-    pv.s.rho[i] = srho / swt;
+    rho_w = srho / swt;
     pv.t.vw[i] = st3(sv * (-1.0f / swt));
   } else {
-    pv.s.rho[i] = P.rho0[0];
+    rho_w = P.rho0[0];
     pv.t.vw[i] = st3(V3{0, 0, 0});
   }
+  pv.s.rho[i] = rho_w;
+  // Liquid targets' transport-velocity wall density (SPH_kernels.cu:56-104)
+  // depends on this wall only: evaluated once here, not once per pair.
+  float rho_tvf = rho_w;
+  if (P.tvf) {
+    const float ratio = (pw[0] - P.p_bg) / P.tait_B + 1.0f;
+    rho_tvf = P.rho0[0] * powf_cr(ratio > 0.5f ? ratio : 0.5f, 1.0f / P.tait_gamma);
+  }
+  pv.t.nrec[i] = make_float4(pw[0], pw[1], rho_tvf, pv.s.T[i]);
   pv.t.a[i] = st3(V3{0, 0, 0});
 }
 
@@ -217,7 +232,8 @@ __global__ __launch_bounds__(kBlock) void k_interface(PV pv, GridDev g, SimParam
           if (P.wall_colour) gradc += gw * (P.vol0 * (pv.t.alpha[j] - ci));
           return;
         }
-        const float Vj = pv.s.m[j] / pv.s.rho[j];
+        const float4 rec = pv.t.nrec[j];  // {p, m, rho, T}
+        const float Vj = rec.y / rec.z;
         s_all += Vj * w;
         if (phj != ph) s_other += Vj * w;
         if (j == i) return;
@@ -310,14 +326,19 @@ __global__ __launch_bounds__(kBlock) void k_smooth_normals(PV pv, GridDev g, Sim
 
 // ------------------------------------------------------------------ forces
 // ST: 0 no tension pass, 1 CSF with vapour present, 2 Akinci.
-template <int ST, bool CONT, bool THERMAL>
+// SWEEP = false handles the particles whose neighbour list is complete
+// (count >= 0); SWEEP = true the overflowed ones (count < 0) by the cell sweep,
+// which visits the same neighbours in the same order.  Two instances keep the
+// sweep's code and registers out of the list loop.
+template <int ST, bool CONT, bool THERMAL, bool SWEEP>
 __global__ __launch_bounds__(kBlock) void k_forces(PV pv, GridDev g, SimParams P, NbrList L, DevState* st, int n) {
   struct {
     bool csf_mode, akinci, continuity, thermal;
   } constexpr F{ST == 1, ST == 2, CONT, THERMAL};
   const int i = gtid();
   float m_v = 0.0f, m_a = 0.0f, m_tv = 0.0f, m_nu = 0.0f, m_sp = 0.0f;
-  if (i < n) {
+  const int cnt = i < n ? L.count[i] : 0;
+  if (i < n && (cnt < 0) == SWEEP) {
     const uint8_t ph = pv.s.phase[i];
     if (!is_fluid(ph)) {
       pv.t.ab[i] = st3(V3{0, 0, 0});
@@ -380,9 +401,20 @@ __global__ __launch_bounds__(kBlock) void k_forces(PV pv, GridDev g, SimParams P
       const bool iso = P.wall_bc == 1;
       float q = 0.0f;
 
-      for_each_pair(L, g, i, xi, [&](int j) {
+      const bool need_vt = tvf_stress || (F.continuity && P.tvf);
+      auto pair = [&](int j) {
         if (j == i) return;
-        const V3 xj = ld3(pv.s.x[j]);
+        // every gather of the pair, issued together
+        const float4 xj4 = pv.s.x[j];
+        const uint8_t phj = pv.s.phase[j];
+        const float4 rec = pv.t.nrec[j];  // {p, m | p_wv, rho, T}
+        const float4 vj4 = pv.s.v[j];
+        const float4 vwj4 = P.no_slip ? pv.t.vw[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const float4 vtj4 = need_vt ? pv.s.vt[j] : vj4;
+        settle(xj4.x, xj4.y, xj4.z, static_cast<int>(phj), rec.x, rec.y, rec.z, rec.w, vj4.x, vj4.y, vj4.z);
+        if (P.no_slip) settle(vwj4.x, vwj4.y, vwj4.z);
+        if (need_vt) settle(vtj4.x, vtj4.y, vtj4.z);
+        const V3 xj = ld3(xj4);
         const V3 rij = xi - xj;
         const float r2 = norm2(rij);
         if (r2 >= P.support2) return;
@@ -390,15 +422,15 @@ __global__ __launch_bounds__(kBlock) void k_forces(PV pv, GridDev g, SimParams P
         const float dW = kdW(P.kernel, r);
         const bool separated = r > 1e-12f * h;
         const V3 gw = separated ? rij * (dW / r) : V3{0, 0, 0};
-        const uint8_t phj = pv.s.phase[j];
+        const float Tj = rec.w;
         if (phj == kWall) {
+          const float pj = rec.x, pwv = rec.y;
           if (tvf_on) {
             float rho_w;
             if (ph == kVapor) {
-              rho_w = (pv.t.p_wv[j] - (P.p_bg - P.vapor_K)) / (P.gas_R * Ti);
+              rho_w = (pwv - (P.p_bg - P.vapor_K)) / (P.gas_R * Ti);
             } else {
-              const float ratio = (pv.t.p[j] - P.p_bg) / P.tait_B + 1.0f;
-              rho_w = P.rho0[0] * powf_cr(ratio > 0.5f ? ratio : 0.5f, inv_gamma);
+              rho_w = rec.z;  // the wall pass stored rho0 * max(ratio, 0.5)^(1/gamma) here
             }
             const float Vj = rho_w > 0.0f ? Vt * rhoi / rho_w : Vt;
             sum_t += gw * (Vt2 + Vj * Vj);
@@ -407,7 +439,7 @@ __global__ __launch_bounds__(kBlock) void k_forces(PV pv, GridDev g, SimParams P
             const V3 f_before = force, a_before = acc_av;
             const float Vj = P.vol0;
             const float vsum = Vi2 + Vj * Vj;
-            const float pw = ph == kVapor ? pv.t.p_wv[j] : pv.t.p[j];
+            const float pw = ph == kVapor ? pwv : pj;
             force -= gw * (vsum * 0.5f * (pi + pw));
             if (tvf_stress) force += vi * (0.5f * vsum * rhoi * dot(dvi, gw));
             if (r < r_guard && separated) {
@@ -418,7 +450,7 @@ __global__ __launch_bounds__(kBlock) void k_forces(PV pv, GridDev g, SimParams P
               if (vn < 0.0f && dt_now > 0.0f) acc_av -= nn * (qg * vn / dt_now);
             }
             if (P.no_slip) {
-              const V3 vij = vi - ld3(pv.t.vw[j]);
+              const V3 vij = vi - ld3(vwj4);
               if (separated) force += vij * (vsum * etai * dW / r);
               const float vr = dot(vij, rij);
               if (alpha_i > 0.0f && vr < 0.0f) {
@@ -441,24 +473,23 @@ __global__ __launch_bounds__(kBlock) void k_forces(PV pv, GridDev g, SimParams P
           if (F.continuity) div += P.vol0 * dot(vci, gw);
           if (F.thermal && iso) {
             const float rgw = r * dW;
-            q += Vi * P.vol0 * (2.0f * ki) * (Ti - pv.s.T[j]) * rgw / (r2 + eta2);
+            q += Vi * P.vol0 * (2.0f * ki) * (Ti - Tj) * rgw / (r2 + eta2);
           }
           return;
         }
         // fluid neighbour
-        const float mj = pv.s.m[j];
-        const float rhoj = pv.s.rho[j];
+        const float pj = rec.x, mj = rec.y, rhoj = rec.z;
         if (tvf_on) {
           const float Vj = 1.0f / pv.t.sigma[j];
           sum_t += gw * (Vt2 + Vj * Vj);
         }
         const float Vj = mj / rhoj;
         const float vsum = Vi2 + Vj * Vj;
-        const float pt = (rhoj * pi + rhoi * pv.t.p[j]) / (rhoi + rhoj);
+        const float pt = (rhoj * pi + rhoi * pj) / (rhoi + rhoj);
         force -= gw * (vsum * pt);
-        const V3 vj = ld3(pv.s.v[j]);
+        const V3 vj = ld3(vj4);
         if (tvf_stress) {
-          const V3 Ag = vi * (rhoi * dot(dvi, gw)) + vj * (rhoj * dot(ld3(pv.s.vt[j]) - vj, gw));
+          const V3 Ag = vi * (rhoi * dot(dvi, gw)) + vj * (rhoj * dot(ld3(vtj4) - vj, gw));
           force += Ag * (0.5f * vsum);
         }
         const V3 vij = vi - vj;
@@ -468,7 +499,6 @@ __global__ __launch_bounds__(kBlock) void k_forces(PV pv, GridDev g, SimParams P
           const float eta_ij = 2.0f * etai * etaj / eta_sum;
           force += vij * (vsum * eta_ij * dW / r);
         }
-        const float Tj = pv.s.T[j];
         if (alpha_i > 0.0f && phj == ph) {
           const float vr = dot(vij, rij);
           if (vr < 0.0f) {
@@ -498,7 +528,7 @@ __global__ __launch_bounds__(kBlock) void k_forces(PV pv, GridDev g, SimParams P
           acc_ak += (f_coh + f_curv) * (K / mi);
         }
         if (F.continuity) {
-          div += Vj * dot(vci - ld3(vel[j]), gw);
+          div += Vj * dot(vci - (P.tvf ? ld3(vtj4) : vj), gw);
           if (phj == ph) diff += Vj * 2.0f * (rhoj - rhoi) * dot(-rij, gw) / (r2 + eta2);
         }
         if (F.thermal) {
@@ -507,7 +537,9 @@ __global__ __launch_bounds__(kBlock) void k_forces(PV pv, GridDev g, SimParams P
           const float rgw = r * dW;
           q += Vi * Vj * kij * (Ti - Tj) * rgw / (r2 + eta2);
         }
-      });
+      };
+      if (SWEEP) for_each_nbr(g, xi, pair);
+      else for_each_listed(L, i, cnt, pair);
 
       const V3 ab = tvf_on ? sum_t * (-pb / mt) : V3{0, 0, 0};
       pv.t.ab[i] = st3(ab);
@@ -690,7 +722,9 @@ void launch_heat_norm(DevState* st, const double* partials, int nb, bool active,
 template <int ST, bool CONT, bool THERMAL>
 static void forces_variant(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, int n,
                            hipStream_t s) {
-  k_forces<ST, CONT, THERMAL><<<blocks_for(n > 0 ? n : 1), kBlock, 0, s>>>(pv, g, P, L, st, n);
+  if (!L.valid) throw std::logic_error("forces pass: neighbour lists not built");
+  k_forces<ST, CONT, THERMAL, false><<<blocks_for(n > 0 ? n : 1), kBlock, 0, s>>>(pv, g, P, L, st, n);
+  k_forces<ST, CONT, THERMAL, true><<<blocks_for(n > 0 ? n : 1), kBlock, 0, s>>>(pv, g, P, L, st, n);
 }
 template <int ST>
 static void forces_st(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, int n,

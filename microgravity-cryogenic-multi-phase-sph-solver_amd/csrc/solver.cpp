@@ -220,10 +220,10 @@ struct cryo_solver {
   DevBuf<uint8_t> phase_[2], flags_[2];
   DevBuf<uint32_t> id_[2];
   int set = 0;
-  DevBuf<float4> a, ab, ns, vw, fw, mw;
+  DevBuf<float4> a, ab, ns, vw, fw, mw, nrec;
   DevBuf<float> p, drho, dTdt, sigma, kappa, nu_t, alpha, wall_w, heat_w, out_m, out_h, w_norm, p_wv, rmin;
   // grid and scans
-  DevBuf<uint32_t> keys_[2], vals_[2], tile_hist, split_flag, split_off, scan_sums, scan_total;
+  DevBuf<uint32_t> keys_[2], vals_[2], tile_hist, row_total, split_flag, split_off, scan_sums, scan_total;
   DevBuf<int> cell_off;
   // neighbour lists of the current step (NbrList in cryo_device.hpp)
   static constexpr int kListK = 96;
@@ -375,7 +375,7 @@ struct cryo_solver {
                    rho_[k].get(), T_[k].get(), phase_[k].get(), flags_[k].get(), id_[k].get()};
     pv.t = Scratch{a.get(), ab.get(), ns.get(), vw.get(), fw.get(), mw.get(), p.get(), drho.get(), dTdt.get(),
                    sigma.get(), kappa.get(), nu_t.get(), alpha.get(), wall_w.get(), heat_w.get(), out_m.get(),
-                   out_h.get(), w_norm.get(), p_wv.get(), rmin.get()};
+                   out_h.get(), w_norm.get(), p_wv.get(), rmin.get(), nrec.get()};
     pv.capacity = capacity;
     return pv;
   }
@@ -407,7 +407,7 @@ struct cryo_solver {
       f(flags_[k], k == set);
       f(id_[k], k == set);
     }
-    f(a, true); f(ab, true); f(ns, true); f(vw, true); f(fw, true); f(mw, true);
+    f(a, true); f(ab, true); f(ns, true); f(vw, true); f(fw, true); f(mw, true); f(nrec, false);
     f(p, true); f(drho, true); f(dTdt, true); f(sigma, true); f(kappa, true); f(nu_t, true); f(alpha, true);
     f(wall_w, true); f(heat_w, true); f(out_m, true); f(out_h, true); f(w_norm, true); f(p_wv, true); f(rmin, true);
   }
@@ -418,6 +418,7 @@ struct cryo_solver {
       vals_[k].resize(cap);
     }
     tile_hist.resize(static_cast<std::size_t>(256) * radix_tiles(cap));
+    row_total.resize(256);
     split_flag.resize(cap);
     split_off.resize(cap);
     scan_sums.resize(static_cast<std::size_t>((cap + 1023) / 1024) + 1);
@@ -671,6 +672,8 @@ struct cryo_solver {
     g.ny = ny;
     g.nz = nz;
     g.num_cells = static_cast<int>(cells);
+    // |error| of (p - origin) * inv_cell is a few ulp of the largest index
+    g.cull_pad = 1e-3f + 5e-7f * static_cast<float>(std::max(nx, std::max(ny, nz)));
     cell_off.resize(static_cast<std::size_t>(cells) + 2);
     key_bits = 1;
     while ((1ll << key_bits) <= cells) ++key_bits;
@@ -690,7 +693,7 @@ struct cryo_solver {
     prof_begin(1);
     bool alt = false;
     launch_radix_sort(keys_[0].get(), vals_[0].get(), keys_[1].get(), vals_[1].get(), n, key_bits, tile_hist.get(),
-                      stream, &alt);
+                      row_total.get(), stream, &alt);
     const uint32_t* keys = alt ? keys_[1].get() : keys_[0].get();
     const uint32_t* perm = alt ? vals_[1].get() : vals_[0].get();
     launch_cell_offsets(keys, n, g.num_cells, cell_off.get(), state.get(), stream);

@@ -93,7 +93,7 @@ def test_lattice_generator_matches_reference(name):
 
 
 def test_c1_lattice_count():
-    """BASELINE config 1 tank: 26,360 particles = 14,520 fluid + 11,840 wall."""
+    """LN2 cylinder tank r=0.05 h=0.12 dx=4 mm: 26,360 particles = 14,520 fluid + 11,840 wall."""
     ps = cabi.hip_build_tank_particles(cabi.cylinder(0.05, 0.12), cabi.hip_fluid_preset("LN2"), 0.004,
                                        cabi.default_fill())
     assert len(ps) == 26360 and int((ps.phase == cabi.WALL).sum()) == 11840
