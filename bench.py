@@ -67,6 +67,74 @@ def c2_particles(cabi, dx):
     return water, cfg, ps, lo, hi
 
 
+def c4_tanks(cabi, count=64, seed=0):
+    """The batched-ensemble workload: `count` LN2 / LOX cylinder and capsule tanks
+    (r = 0.1 m; cylinder h = 0.3 m, capsule straight length 0.2 m), particle
+    counts drawn log-uniform in [20k, 400k] (dx solved from the tank volume),
+    fill U(0.2, 0.8), lateral acceleration U(0.01, 0.1) m/s^2, Lee coefficient
+    U(0.1, 20) 1/s for evaporation and condensation, liquid superheat U(0, 5) K;
+    microgravity 1e-4 g, closed tanks, thermal and phase change on."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        name = ("LN2", "LOX")[int(rng.integers(2))]
+        shape = ("cylinder", "capsule")[int(rng.integers(2))]
+        n_target = float(np.exp(rng.uniform(np.log(20e3), np.log(400e3))))
+        fill = float(rng.uniform(0.2, 0.8))
+        a_lat = float(rng.uniform(0.01, 0.1))
+        lee = float(rng.uniform(0.1, 20.0))
+        superheat = float(rng.uniform(0.0, 5.0))
+        r = 0.1
+        if shape == "cylinder":
+            h = 0.3
+            vol = lambda d: math.pi * (r + 3 * d) ** 2 * (h + 6 * d)  # noqa: E731  (tank + 3 wall layers)
+            nodes = cabi.cylinder(r, h)
+        else:
+            length = 0.2
+            vol = lambda d: math.pi * (r + 3 * d) ** 2 * length + 4.0 / 3.0 * math.pi * (r + 3 * d) ** 3  # noqa: E731
+            nodes = cabi.capsule(r, length)
+        dx = 0.01
+        for _ in range(30):
+            dx = (vol(dx) / n_target) ** (1.0 / 3.0)
+        fluid = cabi.hip_fluid_preset(name)
+        dx = float(np.float32(dx))
+        ps = cabi.hip_build_tank_particles(nodes, fluid, dx, cabi.default_fill(fill_fraction=fill,
+                                                                              T_liquid=fluid.T_sat + superheat))
+        cfg = cabi.default_config(dx=dx, gravity=(0.0, 0.0, -9.81e-4), thermo_mode=1, lee_evaporation=lee,
+                                  lee_condensation=lee)
+        pad = np.float32(4 * dx)
+        out.append(dict(fluid=fluid, cfg=cfg, ps=ps, lo=ps.x.min(0) - pad, hi=ps.x.max(0) + pad, a_lat=a_lat,
+                        desc=f"{name} {shape} dx={dx:.4g} N={len(ps)}"))
+    return out
+
+
+def ensemble_measure(cabi, tanks, steps):
+    """The same tanks advanced `steps` steps by the per-tank loop of Solvers and
+    then by cryo_ensemble_run_steps (bitwise the same results; the GPU test
+    checks that), wall-clock per ensemble step each way."""
+    solvers = []
+    for t in tanks:
+        s = cabi.HipSolver(t["cfg"], t["fluid"])
+        s.set_particles(t["ps"], t["lo"], t["hi"])
+        s.set_frame_motion(acc=(t["a_lat"], 0.0, 0.0))
+        solvers.append(s)
+    for s in solvers:  # start-up evaluation + one step each, untimed
+        s.run_steps(1)
+    t0 = time.perf_counter()
+    for s in solvers:
+        s.run_steps(steps)
+    loop = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    cabi.hip_ensemble_run_steps(solvers, steps)
+    ens = time.perf_counter() - t0
+    n = sum(s.num_particles for s in solvers)
+    del solvers
+    return {"tanks": len(tanks), "particles": n, "steps": steps,
+            "per_tank_loop_ms_per_step": 1e3 * loop / steps, "ensemble_ms_per_step": 1e3 * ens / steps,
+            "speedup": loop / ens,
+            "ensemble_particle_steps_per_s": n * steps / ens}
+
+
 def profile_section(lib, handle, index):
     name = C.create_string_buffer(64)
     ms = C.c_double()
@@ -119,6 +187,7 @@ def main():
     ap.add_argument("--dx", type=float, default=0.007)
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=10, help="0 skips the 64-tank ensemble leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (scales the HBM byte counters) and exit")
     a = ap.parse_args()
@@ -176,6 +245,9 @@ def main():
                           "achieved": step_bytes / (ms_step * 1e-3) / 1e9,
                           "frac": step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
     }
+    del s
+    if a.ensemble_steps > 0:
+        out["ensemble"] = ensemble_measure(cabi, c4_tanks(cabi), a.ensemble_steps)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cabi, water, cfg, ps, lo, hi, a.cpu_steps)
     print(json.dumps(out))

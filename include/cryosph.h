@@ -80,6 +80,11 @@ typedef struct cryo_config {
   float vent_relaxation, merge_ratio, split_ratio, max_transfer_fraction, phase_change_density_limit,
       vapor_mass_rebalance;
   float cfl, viscous_number, force_number, capillary_number, thermal_number, dt_max, dt_min, fixed_dt;
+  /* Extension (no SolverConfig counterpart): 1 = precision form.  Neighbour
+   * sums use compensated (Kahan-Babuska) FP32 accumulation instead of the
+   * reference's plain running sums; results then differ from the reference in
+   * the last bits and are compared with the FP64 build instead.  Default 0. */
+  int32_t compensated_sums;
 } cryo_config;
 
 /* FluidProperties, fluid_properties.hpp:11-38. */
@@ -163,6 +168,15 @@ int cryo_solver_relax(cryo_solver* s, double duration, float damping);
 int cryo_solver_next_dt(cryo_solver* s, double* dt);
 /* Solver::damp_velocities (sph_solver.cu:701-705). */
 int cryo_solver_damp_velocities(cryo_solver* s, float factor);
+
+/* Extension (no counterpart in the reference, whose ensembles are a loop of
+ * independent Solvers): advances every solver of an ensemble `steps` steps.
+ * Each member's step is queued on its own stream before any member waits for
+ * its host bookkeeping, so independent tanks run side by side on the GPU.
+ * Results are those of calling cryo_solver_run_steps on each member alone.
+ * Members must be distinct; a solver is not thread-safe, so no other thread
+ * may use a member during the call. */
+int cryo_ensemble_run_steps(cryo_solver* const* solvers, int32_t count, int64_t steps);
 
 /* Solver::time / step_count / last_dt / num_particles (sph_solver.hpp:153-163). */
 int cryo_solver_time(cryo_solver* s, double* t);

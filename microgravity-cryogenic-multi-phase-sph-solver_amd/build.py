@@ -29,8 +29,11 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 LIB_SOURCES = ["kernels_grid.hip", "kernels_physics.hip", "kernels_step.hip", "solver.cpp", "geometry.cpp"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wno-unknown-pragmas",
           f"-I{ROOT / 'include'}", f"-I{CSRC}"]
+# -simplifycfg-sink-common=false: merging the identical tails of two branches
+# (conditional accumulations in the pair loops) makes LLVM select between the
+# accumulators' addresses, which forces them out of registers into scratch.
 DEVICE = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          "-fno-gpu-rdc"]
+          "-fno-gpu-rdc", "-mllvm", "-simplifycfg-sink-common=false"]
 
 
 def run(cmd):

@@ -567,6 +567,7 @@ PYBIND11_MODULE(CryoSPH, m) {
   RWF(max_transfer_fraction); RWF(phase_change_density_limit); RWF(vapor_mass_rebalance); RWF(cfl);
   RWF(viscous_number); RWF(force_number); RWF(capillary_number); RWF(thermal_number); RWF(dt_max); RWF(dt_min);
   RWF(fixed_dt);
+  RWB(compensated_sums);
 #undef RWF
 #undef RWB
 #undef RWE

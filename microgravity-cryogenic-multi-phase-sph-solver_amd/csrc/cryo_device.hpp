@@ -52,6 +52,56 @@ __host__ __device__ inline V3& operator*=(V3& a, float s) {
   a.z *= s;
   return a;
 }
+// Neighbour-sum accumulators.  KS = false: a plain FP32 running sum, the
+// reference's rounding.  KS = true: the compensated (Kahan-Babuska /
+// Neumaier) sum, whose rounding error stays O(eps) instead of growing with
+// the number of neighbours; the opt-in precision form.
+template <bool KS>
+struct Acc;
+template <>
+struct Acc<false> {
+  float s = 0.0f;
+  __host__ __device__ Acc& operator+=(float x) {
+    s += x;
+    return *this;
+  }
+  __host__ __device__ Acc& operator-=(float x) {
+    s -= x;
+    return *this;
+  }
+  __host__ __device__ operator float() const { return s; }
+};
+template <>
+struct Acc<true> {
+  float s = 0.0f, c = 0.0f;
+  __host__ __device__ Acc& operator+=(float x) {
+    const float t = s + x;
+    const float lost = fabsf(s) >= fabsf(x) ? (s - t) + x : (x - t) + s;
+    c += lost;
+    s = t;
+    return *this;
+  }
+  __host__ __device__ Acc& operator-=(float x) { return *this += -x; }
+  __host__ __device__ operator float() const { return s + c; }
+};
+template <bool KS>
+struct AccV3 {
+  Acc<KS> x, y, z;
+  __host__ __device__ AccV3& operator+=(V3 a) {
+    x += a.x;
+    y += a.y;
+    z += a.z;
+    return *this;
+  }
+  __host__ __device__ AccV3& operator-=(V3 a) {
+    x -= a.x;
+    y -= a.y;
+    z -= a.z;
+    return *this;
+  }
+  __host__ __device__ operator V3() const { return V3{float(x), float(y), float(z)}; }
+};
+
 __host__ __device__ inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __host__ __device__ inline V3 cross(V3 a, V3 b) {
   return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
@@ -114,6 +164,13 @@ __host__ __device__ inline float akinci_coh(float r, float c) {
   return coeff * (2.0f * base - c3 * c3 / 64.0f);
 }
 
+// Element k (0 or 1) of a two-entry table without indexing it: a dynamic
+// index into a by-value parameter struct would put the struct in scratch.
+template <class T>
+__host__ __device__ inline T sel2(const T (&a)[2], int k) {
+  return k ? a[1] : a[0];
+}
+
 // ---------------------------------------------------------- static params
 // Everything init_params derives once (sph_solver.cu:292-405).  Passed by
 // value to every kernel; values that change during a run live in DevState.
@@ -136,6 +193,7 @@ struct SimParams {
   float T_wall, T_ref, h_fg, lee_evap, lee_cond, max_transfer, vapor_rebalance, pc_density_limit, merge_ratio,
       split_ratio, heat_power;
   int has_vapor;
+  int compensated;  // compensated neighbour sums (precision form)
 };
 
 // ------------------------------------------------------- device state block
@@ -308,49 +366,118 @@ __host__ __device__ inline void for_each_cand(const GridDev& g, V3 p, float radi
   }
 }
 
+// for_each_cand with the candidates' positions gathered eight at a time
+// (one wait per eight loads instead of one per candidate): f(j, x[j]).
+template <class F>
+__device__ inline void for_each_cand_x(const GridDev& g, V3 p, float radius, const float4* __restrict__ xs, F&& f) {
+  const float ux = (p.x - g.ox) * g.inv_cell, uy = (p.y - g.oy) * g.inv_cell, uz = (p.z - g.oz) * g.inv_cell;
+  const int cx = clampv(static_cast<int>(floorf(ux)), 0, g.nx - 1);
+  const int cy = clampv(static_cast<int>(floorf(uy)), 0, g.ny - 1);
+  const int cz = clampv(static_cast<int>(floorf(uz)), 0, g.nz - 1);
+  const float fx = ux - static_cast<float>(cx), fy = uy - static_cast<float>(cy), fz = uz - static_cast<float>(cz);
+  const float rc = radius * g.inv_cell + g.cull_pad;
+  const float rr = rc * rc;
+  const float gxl = fx * fx, gxr = (1.0f - fx) * (1.0f - fx);
+  const int z0 = cz > 0 ? cz - 1 : 0, z1 = cz < g.nz - 1 ? cz + 1 : cz;
+  const int y0 = cy > 0 ? cy - 1 : 0, y1 = cy < g.ny - 1 ? cy + 1 : cy;
+  for (int z = z0; z <= z1; ++z) {
+    const float gz = z < cz ? fz : (z > cz ? 1.0f - fz : 0.0f);
+    for (int y = y0; y <= y1; ++y) {
+      const float gy = y < cy ? fy : (y > cy ? 1.0f - fy : 0.0f);
+      const float gyz = gy * gy + gz * gz;
+      if (gyz >= rr) continue;
+      const int x0 = (cx > 0 && gyz + gxl < rr) ? cx - 1 : cx;
+      const int x1 = (cx < g.nx - 1 && gyz + gxr < rr) ? cx + 1 : cx;
+      const int row = (z * g.ny + y) * g.nx;
+      const int s = g.off[row + x0], e = g.off[row + x1 + 1];
+      int j = s;
+      for (; j + 8 <= e; j += 8) {
+        float4 b[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) b[u] = xs[j + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) f(j + u, b[u]);
+      }
+      for (; j < e; ++j) f(j, xs[j]);
+    }
+  }
+}
+
 // ---------------------------------------------------------- neighbour lists
 // Built once per step by the density pass from the same 27-cell sweep, in
 // the same ascending order, holding every j with r^2 < (2h)^2 (fluid targets:
 // all particles, self included; wall targets: fluid particles only).
-// Transposed layout idx[k * stride + i], so lane i of a wave reading entry k
-// touches consecutive addresses.  count[i] = -1 marks an overflow (more than
-// K neighbours): that particle's passes fall back to the cell sweep, which
-// visits the same j in the same order.
+// Particle i owns the K consecutive entries idx[i * K, i * K + K), K a
+// multiple of 4, so its list is read and written in 16-byte chunks: a pair
+// loop waits for memory once per four neighbours.  count[i] = -1 marks an
+// overflow (more than K neighbours): that particle's passes fall back to the
+// cell sweep, which visits the same j in the same order.
 struct NbrList {
   int* idx;
   int* count;
-  int stride;
+  int stride;  // unused (layout is particle-major)
   int K;
   int valid;  // 0: positions or cells changed since the lists were built
 };
+
+// Appends one entry of particle i's list, staging four in registers.
+struct ListWriter {
+  int4* out;
+  int4 buf;
+  int c = 0;
+  int K;
+  __device__ ListWriter(const NbrList& L, int i) : out(reinterpret_cast<int4*>(L.idx + static_cast<long long>(i) * L.K)), K(L.K) {}
+  __device__ inline void push(int j) {
+    switch (c & 3) {
+      case 0: buf.x = j; break;
+      case 1: buf.y = j; break;
+      case 2: buf.z = j; break;
+      default:
+        buf.w = j;
+        if (c < K) out[c >> 2] = buf;
+    }
+    ++c;
+  }
+  // Writes the last partial chunk; returns the count to store (-1: overflow).
+  __device__ inline int finish() {
+    if ((c & 3) != 0 && c < K) out[c >> 2] = buf;
+    return c <= K ? c : -1;
+  }
+};
+
+// Visits particle i's c (>= 0) listed neighbours in order; the next chunk
+// of four is in flight while the current one is processed.
+template <class F>
+__device__ inline void for_each_listed(const NbrList& L, int i, int c, F&& f) {
+  const int4* p = reinterpret_cast<const int4*>(L.idx + static_cast<long long>(i) * L.K);
+  const int nq = (c + 3) >> 2;
+  int j0 = 0, j1 = 0, j2 = 0, j3 = 0;
+  if (nq > 0) {
+    const int4 t = p[0];
+    j0 = t.x, j1 = t.y, j2 = t.z, j3 = t.w;
+  }
+  for (int q = 0; q < nq; ++q) {
+    int n0 = j0, n1 = j1, n2 = j2, n3 = j3;
+    if (q + 1 < nq) {
+      const int4 t = p[q + 1];
+      n0 = t.x, n1 = t.y, n2 = t.z, n3 = t.w;
+    }
+    const int m = c - (q << 2) < 4 ? c - (q << 2) : 4;
+#pragma unroll 1
+    for (int u = 0; u < m; ++u) {  // one copy of f; the entry picked by selects
+      f(u == 0 ? j0 : (u == 1 ? j1 : (u == 2 ? j2 : j3)));
+    }
+    j0 = n0, j1 = n1, j2 = n2, j3 = n3;
+  }
+}
 
 template <class F>
 __device__ inline void for_each_pair(const NbrList& L, const GridDev& g, int i, V3 xi, F&& f) {
   const int c = L.valid ? L.count[i] : -1;
   if (c >= 0) {
-    const int* p = L.idx + i;
-    const long long stride = L.stride;
-    int jn = c > 0 ? p[0] : 0;
-    for (int k = 0; k < c; ++k) {  // the next index is in flight while f runs
-      const int j = jn;
-      if (k + 1 < c) jn = p[(k + 1) * stride];
-      f(j);
-    }
+    for_each_listed(L, i, c, f);
   } else {
     for_each_nbr(g, xi, f);
-  }
-}
-
-// The list part of for_each_pair alone (count[i] >= 0 and valid lists).
-template <class F>
-__device__ inline void for_each_listed(const NbrList& L, int i, int c, F&& f) {
-  const int* p = L.idx + i;
-  const long long stride = L.stride;
-  int jn = c > 0 ? p[0] : 0;
-  for (int k = 0; k < c; ++k) {
-    const int j = jn;
-    if (k + 1 < c) jn = p[(k + 1) * stride];
-    f(j);
   }
 }
 

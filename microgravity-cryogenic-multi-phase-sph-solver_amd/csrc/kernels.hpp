@@ -49,7 +49,7 @@ void launch_smooth_normals(const PV& pv, const GridDev& g, const SimParams& P, c
                            hipStream_t s);
 void launch_heat_norm(DevState* st, const double* partials, int nb, bool active, hipStream_t s);
 void launch_forces(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, int n,
-                   bool csf_mode, bool akinci, bool continuity, bool thermal, hipStream_t s);
+                   bool csf_mode, bool akinci, bool continuity, bool thermal, bool tiled, hipStream_t s);
 void launch_probe(const PV& pv, const GridDev& g, const SimParams& P, const float* pts, float* out, int np,
                   hipStream_t s);
 void launch_shifting(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, const DevState* st, int n,

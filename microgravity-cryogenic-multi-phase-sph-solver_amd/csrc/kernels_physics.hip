@@ -55,44 +55,42 @@ __device__ inline float powf_cr(float x, float y) { return static_cast<float>(po
 
 // ------------------------------------------------------- density + EOS
 // Also builds the step's neighbour lists from the same (cell-culled) sweep.
+template <bool KS>
 __global__ __launch_bounds__(kBlock) void k_density(PV pv, GridDev g, SimParams P, NbrList L, int n) {
   const int i = gtid();
   if (i >= n) return;
   const uint8_t ph = pv.s.phase[i];
   const V3 xi = ld3(pv.s.x[i]);
-  int* out = L.idx + i;
-  const long long stride = L.stride;
-  int c = 0;
+  ListWriter lw(L, i);
   if (!is_fluid(ph)) {
     pv.t.sigma[i] = 1.0f / P.vol0;
     pv.t.rmin[i] = P.kernel.support;
     if (ph == kWall) {  // wall targets only ever need their fluid neighbours
-      for_each_cand(g, xi, P.kernel.support, [&](int j) {
+      for_each_cand_x(g, xi, P.kernel.support, pv.s.x, [&](int j, float4 xj) {
+        if (norm2(xi - ld3(xj)) >= P.support2) return;
         if (!is_fluid(pv.s.phase[j])) return;
-        if (norm2(xi - ld3(pv.s.x[j])) >= P.support2) return;
-        if (c < L.K) out[c * stride] = j;
-        ++c;
+        lw.push(j);
       });
     }
-    L.count[i] = c <= L.K ? c : -1;
+    L.count[i] = lw.finish();
     return;
   }
-  float sum = 0.0f, r2min = P.support2;
-  for_each_cand(g, xi, P.kernel.support, [&](int j) {
-    const float r2 = norm2(xi - ld3(pv.s.x[j]));
+  Acc<KS> sum;
+  float r2min = P.support2;
+  for_each_cand_x(g, xi, P.kernel.support, pv.s.x, [&](int j, float4 xj) {
+    const float r2 = norm2(xi - ld3(xj));
     if (r2 >= P.support2) return;
     sum += kW(P.kernel, sqrtf(r2));
     if (j != i && r2 < r2min) r2min = r2;
-    if (c < L.K) out[c * stride] = j;
-    ++c;
+    lw.push(j);
   });
-  L.count[i] = c <= L.K ? c : -1;
-  pv.t.sigma[i] = sum;
+  L.count[i] = lw.finish();
+  pv.t.sigma[i] = float(sum);
   pv.t.rmin[i] = sqrtf(r2min);
   float rho = pv.s.rho[i];
   const float mi = pv.s.m[i];
   if (P.density_mode == 0) {
-    rho = (ph == kLiquid ? pv.s.m0[i] : mi) * sum;
+    rho = (ph == kLiquid ? pv.s.m0[i] : mi) * float(sum);
     pv.s.rho[i] = rho;
   }
   const float Ti = pv.s.T[i];
@@ -120,6 +118,7 @@ __global__ __launch_bounds__(kBlock) void k_normalize_masses(PV pv, GridDev g, S
 }
 
 // --------------------------------------------------------------- wall BC
+template <bool KS>
 __global__ __launch_bounds__(kBlock) void k_wall_bc(PV pv, GridDev g, SimParams P, NbrList L, const DevState* st,
                                                    int n) {
   const int i = gtid();
@@ -127,9 +126,8 @@ __global__ __launch_bounds__(kBlock) void k_wall_bc(PV pv, GridDev g, SimParams 
   if (pv.s.phase[i] != kWall) return;
   const V3 grav = st->gravity;
   const V3 xi = ld3(pv.s.x[i]);
-  float sw[2] = {0.0f, 0.0f}, sp[2] = {0.0f, 0.0f}, sgr[2] = {0.0f, 0.0f};
-  float srho = 0.0f;
-  V3 sv{0, 0, 0};
+  Acc<KS> asw[2], asp[2], asgr[2], srho;
+  AccV3<KS> sv;
   for_each_pair(L, g, i, xi, [&](int j) {
     const uint8_t phj = pv.s.phase[j];
     if (!is_fluid(phj)) return;
@@ -139,12 +137,17 @@ __global__ __launch_bounds__(kBlock) void k_wall_bc(PV pv, GridDev g, SimParams 
     const float w = kW(P.kernel, sqrtf(r2));
     const float rhoj = pv.s.rho[j];
     const int k = pidx(phj);
-    sw[k] += w;
-    sp[k] += pv.t.p[j] * w;
-    sgr[k] += dot(grav, rwf) * rhoj * w;
+    for (int q = 0; q < 2; ++q) {  // constant indices: the sums stay in registers
+      if (q != k) continue;
+      asw[q] += w;
+      asp[q] += pv.t.p[j] * w;
+      asgr[q] += dot(grav, rwf) * rhoj * w;
+    }
     srho += rhoj * w;
     sv += ld3(pv.s.v[j]) * w;
   });
+  const float sw[2] = {float(asw[0]), float(asw[1])};
+  const float sp[2] = {float(asp[0]), float(asp[1])}, sgr[2] = {float(asgr[0]), float(asgr[1])};
   float pw[2];
   for (int k = 0; k < 2; ++k) {
     pw[k] = sw[k] > 0.0f ? (sp[k] + sgr[k]) / sw[k] : P.p_bg;
@@ -158,8 +161,8 @@ __global__ __launch_bounds__(kBlock) void k_wall_bc(PV pv, GridDev g, SimParams 
   const float swt = sw[0] + sw[1];
   float rho_w;
   if (swt > 0.0f) {
-    rho_w = srho / swt;
-    pv.t.vw[i] = st3(sv * (-1.0f / swt));
+    rho_w = float(srho) / swt;
+    pv.t.vw[i] = st3(V3(sv) * (-1.0f / swt));
   } else {
     rho_w = P.rho0[0];
     pv.t.vw[i] = st3(V3{0, 0, 0});
@@ -179,7 +182,7 @@ __global__ __launch_bounds__(kBlock) void k_wall_bc(PV pv, GridDev g, SimParams 
 // ------------------------------------------------------- interface fields
 // Also resets the dt maxima for k_forces and writes FP64 block partials of
 // sum_fluid heat_w (sph_solver.cu:101-104) when heat_partials != nullptr.
-template <bool SPS>
+template <bool SPS, bool KS>
 __global__ __launch_bounds__(kBlock) void k_interface(PV pv, GridDev g, SimParams P, NbrList L, DevState* st, int n,
                                                      int csf_mode, double* heat_partials) {
   const int i = gtid();
@@ -194,11 +197,11 @@ __global__ __launch_bounds__(kBlock) void k_interface(PV pv, GridDev g, SimParam
     } else {
       const V3 xi = ld3(pv.s.x[i]);
       const float ci = ph == kLiquid ? 1.0f : 0.0f;
-      V3 gradc{0, 0, 0}, wall_n{0, 0, 0}, akn{0, 0, 0}, gradphi{0, 0, 0};
-      float s_all = 0.0f, s_other = 0.0f, ww = 0.0f, hw = 0.0f;
+      AccV3<KS> agradc, awall_n, aakn, agradphi;
+      Acc<KS> as_all, as_other, aww, ahw;
       // SPS velocity gradient (navier_stokes.cu:71-119)
       const V3 vi = ld3(pv.s.v[i]);
-      float Lm[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      Acc<KS> Lm[3][3];
       for_each_pair(L, g, i, xi, [&](int j) {
         const V3 rij = xi - ld3(pv.s.x[j]);
         const float r2 = norm2(rij);
@@ -225,27 +228,30 @@ __global__ __launch_bounds__(kBlock) void k_interface(PV pv, GridDev g, SimParam
         }
         if (phj == kWall) {
           const float vw = P.vol0 * w;
-          ww += vw;
-          if (pv.s.flags[j] & kHeated) hw += vw;
-          wall_n += ld3(pv.s.nrm[j]) * vw;
-          gradphi += gw * P.vol0;
-          if (P.wall_colour) gradc += gw * (P.vol0 * (pv.t.alpha[j] - ci));
+          aww += vw;
+          if (pv.s.flags[j] & kHeated) ahw += vw;
+          awall_n += ld3(pv.s.nrm[j]) * vw;
+          agradphi += gw * P.vol0;
+          if (P.wall_colour) agradc += gw * (P.vol0 * (pv.t.alpha[j] - ci));
           return;
         }
         const float4 rec = pv.t.nrec[j];  // {p, m, rho, T}
         const float Vj = rec.y / rec.z;
-        s_all += Vj * w;
-        if (phj != ph) s_other += Vj * w;
+        as_all += Vj * w;
+        if (phj != ph) as_other += Vj * w;
         if (j == i) return;
         const float cj = phj == kLiquid ? 1.0f : 0.0f;
-        gradc += gw * (Vj * (cj - ci));
-        if (phj == kLiquid) akn += gw * Vj;
+        agradc += gw * (Vj * (cj - ci));
+        if (phj == kLiquid) aakn += gw * Vj;
       });
+      V3 gradc = agradc;
+      const V3 wall_n = awall_n, akn = aakn, gradphi = agradphi;
+      const float s_all = as_all, s_other = as_other, ww = aww, hw = ahw;
       if (SPS) {
         float ss = 0.0f;
         for (int a = 0; a < 3; ++a)
           for (int b = 0; b < 3; ++b) {
-            const float sab = 0.5f * (Lm[a][b] + Lm[b][a]);
+            const float sab = 0.5f * (float(Lm[a][b]) + float(Lm[b][a]));
             ss += sab * sab;
           }
         const float smag = sqrtf(2.0f * ss);
@@ -300,6 +306,7 @@ __global__ __launch_bounds__(kBlock) void k_interface(PV pv, GridDev g, SimParam
 }
 
 // surface_tension.cu:87-118 with neighbour smoothing on.
+template <bool KS>
 __global__ __launch_bounds__(kBlock) void k_smooth_normals(PV pv, GridDev g, SimParams P, NbrList L, int n) {
   const int i = gtid();
   if (i >= n) return;
@@ -311,7 +318,7 @@ __global__ __launch_bounds__(kBlock) void k_smooth_normals(PV pv, GridDev g, Sim
     return;
   }
   const V3 xi = ld3(pv.s.x[i]);
-  V3 acc{0, 0, 0};
+  AccV3<KS> acc;
   for_each_pair(L, g, i, xi, [&](int j) {
     if (!is_fluid(pv.s.phase[j])) return;
     const V3 gj = ld3(pv.s.nrm[j]);
@@ -321,24 +328,58 @@ __global__ __launch_bounds__(kBlock) void k_smooth_normals(PV pv, GridDev g, Sim
     if (r2 >= P.support2) return;
     acc += gj * ((pv.s.m[j] / pv.s.rho[j]) * kW(P.kernel, sqrtf(r2)) / gjm);
   });
-  pv.t.ns[i] = st3(normalized(acc));
+  pv.t.ns[i] = st3(normalized(V3(acc)));
 }
 
 // ------------------------------------------------------------------ forces
+// What one neighbour contributes to the forces pass.  For a wall neighbour
+// `v` holds its no-slip ghost velocity vw, for a fluid one its velocity; `vt`
+// is the fluid transport velocity (read only when the pass needs it).
+struct NbrData {
+  float4 x;
+  int ph;
+  float4 rec;  // {p, m, rho, T} fluid; {p, p_wv, rho_tvf, T} wall
+  float4 v;
+  float4 vt;
+};
+
+__device__ inline bool forces_need_vt(const SimParams& P, bool continuity) {
+  return (P.tvf && P.tvf_stress) || (continuity && P.tvf);
+}
+
+// Neighbour data straight from the particle arrays.
+__device__ inline NbrData load_nbr(const PV& pv, const SimParams& P, int j, bool need_vt) {
+  NbrData d;
+  d.x = pv.s.x[j];
+  d.ph = pv.s.phase[j];
+  d.rec = pv.t.nrec[j];
+  const float4 v = pv.s.v[j];
+  float4 vw = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (P.no_slip) vw = pv.t.vw[j];
+  d.vt = v;
+  if (need_vt) d.vt = pv.s.vt[j];
+  // every gather of the pair in flight together, then one wait
+  settle(d.x.x, d.x.y, d.x.z, d.ph, d.rec.x, d.rec.y, d.rec.z, d.rec.w, v.x, v.y, v.z);
+  if (P.no_slip) settle(vw.x, vw.y, vw.z);
+  if (need_vt) settle(d.vt.x, d.vt.y, d.vt.z);
+  d.v = d.ph == kWall ? vw : v;
+  return d;
+}
+
+// Momentum, wall loads, transport velocity, CSF / Akinci, continuity and
+// conduction rates of particle i, and its dt maxima.  `visit(f)` calls
+// f(j, NbrData) for every neighbour in the reference order.
 // ST: 0 no tension pass, 1 CSF with vapour present, 2 Akinci.
-// SWEEP = false handles the particles whose neighbour list is complete
-// (count >= 0); SWEEP = true the overflowed ones (count < 0) by the cell sweep,
-// which visits the same neighbours in the same order.  Two instances keep the
-// sweep's code and registers out of the list loop.
-template <int ST, bool CONT, bool THERMAL, bool SWEEP>
-__global__ __launch_bounds__(kBlock) void k_forces(PV pv, GridDev g, SimParams P, NbrList L, DevState* st, int n) {
+struct Maxima {
+  float v = 0.0f, a = 0.0f, tv = 0.0f, nu = 0.0f, sp = 0.0f;
+};
+template <int ST, bool CONT, bool THERMAL, bool KS, class Visit>
+__device__ inline void forces_particle(const PV& pv, const SimParams& P, DevState* st, int i, Visit&& visit,
+                                       Maxima& mx) {
   struct {
     bool csf_mode, akinci, continuity, thermal;
   } constexpr F{ST == 1, ST == 2, CONT, THERMAL};
-  const int i = gtid();
-  float m_v = 0.0f, m_a = 0.0f, m_tv = 0.0f, m_nu = 0.0f, m_sp = 0.0f;
-  const int cnt = i < n ? L.count[i] : 0;
-  if (i < n && (cnt < 0) == SWEEP) {
+  {
     const uint8_t ph = pv.s.phase[i];
     if (!is_fluid(ph)) {
       pv.t.ab[i] = st3(V3{0, 0, 0});
@@ -369,52 +410,43 @@ __global__ __launch_bounds__(kBlock) void k_forces(PV pv, GridDev g, SimParams P
       const float Vt = 1.0f / sigmai;
       const float Vt2 = Vt * Vt;
       const float mt = rhoi * Vt;
-      const float pb = P.tvf_factor * P.rho0[pi_idx] * ci * ci;
+      const float pb = P.tvf_factor * sel2(P.rho0, pi_idx) * ci * ci;
       const float inv_gamma = 1.0f / P.tait_gamma;
-      V3 sum_t{0, 0, 0};
+      AccV3<KS> sum_t;
       // momentum (navier_stokes.cu:121-245)
-      const float etai = P.mu[pi_idx] + (P.sps ? rhoi * pv.t.nu_t[i] : 0.0f);
+      const float etai = sel2(P.mu, pi_idx) + (P.sps ? rhoi * pv.t.nu_t[i] : 0.0f);
       const float alpha_i = ph == kVapor ? P.alpha_av_vap : P.alpha_av;
-      V3 force{0, 0, 0}, acc_av{0, 0, 0};
+      AccV3<KS> force, acc_av;
       const float r_guard = 0.6f * P.dx;
       const bool tvf_stress = P.tvf && P.tvf_stress;
       const V3 dvi = ld3(pv.s.vt[i]) - vi;
-      V3 fwall{0, 0, 0}, mwall{0, 0, 0};
+      AccV3<KS> fwall, mwall;
       const V3 origin = st->origin;
       // CSF (surface_tension.cu:120-177)
       const V3 ni = F.csf_mode ? ld3(pv.t.ns[i]) : V3{0, 0, 0};
       const bool csf_on = F.csf_mode && norm2(ni) > 0.25f;
       const float cci = ph == kLiquid ? 1.0f : 0.0f;
-      float num = 0.0f, den = 0.0f;
-      V3 G{0, 0, 0};
+      Acc<KS> num, den;
+      AccV3<KS> G;
       // Akinci (surface_tension.cu:179-213)
       const bool ak_on = F.akinci && ph == kLiquid;
       const V3 nak = ak_on ? ld3(pv.s.nrm[i]) : V3{0, 0, 0};
       const float psi = P.rho0[0] * P.vol0;
-      V3 acc_ak{0, 0, 0};
+      AccV3<KS> acc_ak;
       // continuity (SPH_kernels.cu:162-203)
       const float4* vel = P.tvf ? pv.s.vt : pv.s.v;
       const V3 vci = F.continuity ? ld3(vel[i]) : V3{0, 0, 0};
-      float div = 0.0f, diff = 0.0f;
+      Acc<KS> div, diff;
       // conduction (phase_change.cu:31-75)
-      const float ki = P.k[pi_idx];
+      const float ki = sel2(P.k, pi_idx);
       const bool iso = P.wall_bc == 1;
-      float q = 0.0f;
+      Acc<KS> q;
 
-      const bool need_vt = tvf_stress || (F.continuity && P.tvf);
-      auto pair = [&](int j) {
+      visit([&](int j, NbrData d) {
         if (j == i) return;
-        // every gather of the pair, issued together
-        const float4 xj4 = pv.s.x[j];
-        const uint8_t phj = pv.s.phase[j];
-        const float4 rec = pv.t.nrec[j];  // {p, m | p_wv, rho, T}
-        const float4 vj4 = pv.s.v[j];
-        const float4 vwj4 = P.no_slip ? pv.t.vw[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        const float4 vtj4 = need_vt ? pv.s.vt[j] : vj4;
-        settle(xj4.x, xj4.y, xj4.z, static_cast<int>(phj), rec.x, rec.y, rec.z, rec.w, vj4.x, vj4.y, vj4.z);
-        if (P.no_slip) settle(vwj4.x, vwj4.y, vwj4.z);
-        if (need_vt) settle(vtj4.x, vtj4.y, vtj4.z);
-        const V3 xj = ld3(xj4);
+        const V3 xj = ld3(d.x);
+        const int phj = d.ph;
+        const float4 rec = d.rec;
         const V3 rij = xi - xj;
         const float r2 = norm2(rij);
         if (r2 >= P.support2) return;
@@ -436,30 +468,42 @@ __global__ __launch_bounds__(kBlock) void k_forces(PV pv, GridDev g, SimParams P
             sum_t += gw * (Vt2 + Vj * Vj);
           }
           {
-            const V3 f_before = force, a_before = acc_av;
             const float Vj = P.vol0;
             const float vsum = Vi2 + Vj * Vj;
             const float pw = ph == kVapor ? pwv : pj;
-            force -= gw * (vsum * 0.5f * (pi + pw));
-            if (tvf_stress) force += vi * (0.5f * vsum * rhoi * dot(dvi, gw));
-            if (r < r_guard && separated) {
-              const float qg = 1.0f - r / r_guard;
-              const V3 nn = rij / r;
-              acc_av += nn * (4.0f * ci * ci * qg * qg / r_guard);
-              const float vn = dot(vi, nn);
-              if (vn < 0.0f && dt_now > 0.0f) acc_av -= nn * (qg * vn / dt_now);
-            }
-            if (P.no_slip) {
-              const V3 vij = vi - ld3(vwj4);
-              if (separated) force += vij * (vsum * etai * dW / r);
-              const float vr = dot(vij, rij);
-              if (alpha_i > 0.0f && vr < 0.0f) {
-                const float mu_ij = h * vr / (r2 + eta2);
-                const float Pi = -alpha_i * ci * mu_ij / rhoi;
-                acc_av -= gw * (rhoi * P.vol0 * Pi);
+            auto wall_terms = [&](auto& fo, auto& ac) {
+              fo -= gw * (vsum * 0.5f * (pi + pw));
+              if (tvf_stress) fo += vi * (0.5f * vsum * rhoi * dot(dvi, gw));
+              if (r < r_guard && separated) {
+                const float qg = 1.0f - r / r_guard;
+                const V3 nn = rij / r;
+                ac += nn * (4.0f * ci * ci * qg * qg / r_guard);
+                const float vn = dot(vi, nn);
+                if (vn < 0.0f && dt_now > 0.0f) ac -= nn * (qg * vn / dt_now);
               }
+              if (P.no_slip) {
+                const V3 vij = vi - ld3(d.v);
+                if (separated) fo += vij * (vsum * etai * dW / r);
+                const float vr = dot(vij, rij);
+                if (alpha_i > 0.0f && vr < 0.0f) {
+                  const float mu_ij = h * vr / (r2 + eta2);
+                  const float Pi = -alpha_i * ci * mu_ij / rhoi;
+                  ac -= gw * (rhoi * P.vol0 * Pi);
+                }
+              }
+            };
+            V3 dF;
+            if constexpr (KS) {  // the pair's own terms, added once to each sum
+              V3 df{0, 0, 0}, da{0, 0, 0};
+              wall_terms(df, da);
+              force += df;
+              acc_av += da;
+              dF = df + da * mi;
+            } else {  // the reference's wall load: what the pair changed
+              const V3 f_before = force, a_before = acc_av;
+              wall_terms(force, acc_av);
+              dF = (V3(force) - f_before) + (V3(acc_av) - a_before) * mi;
             }
-            const V3 dF = (force - f_before) + (acc_av - a_before) * mi;
             fwall += dF;
             mwall += cross(xj - origin, dF);
           }
@@ -487,13 +531,13 @@ __global__ __launch_bounds__(kBlock) void k_forces(PV pv, GridDev g, SimParams P
         const float vsum = Vi2 + Vj * Vj;
         const float pt = (rhoj * pi + rhoi * pj) / (rhoi + rhoj);
         force -= gw * (vsum * pt);
-        const V3 vj = ld3(vj4);
+        const V3 vj = ld3(d.v);
         if (tvf_stress) {
-          const V3 Ag = vi * (rhoi * dot(dvi, gw)) + vj * (rhoj * dot(ld3(vtj4) - vj, gw));
+          const V3 Ag = vi * (rhoi * dot(dvi, gw)) + vj * (rhoj * dot(ld3(d.vt) - vj, gw));
           force += Ag * (0.5f * vsum);
         }
         const V3 vij = vi - vj;
-        const float etaj = P.mu[pidx(phj)] + (P.sps ? rhoj * pv.t.nu_t[j] : 0.0f);
+        const float etaj = sel2(P.mu, pidx(phj)) + (P.sps ? rhoj * pv.t.nu_t[j] : 0.0f);
         const float eta_sum = etai + etaj;
         if (eta_sum > 0.0f && separated) {
           const float eta_ij = 2.0f * etai * etaj / eta_sum;
@@ -528,22 +572,20 @@ __global__ __launch_bounds__(kBlock) void k_forces(PV pv, GridDev g, SimParams P
           acc_ak += (f_coh + f_curv) * (K / mi);
         }
         if (F.continuity) {
-          div += Vj * dot(vci - (P.tvf ? ld3(vtj4) : vj), gw);
+          div += Vj * dot(vci - (P.tvf ? ld3(d.vt) : vj), gw);
           if (phj == ph) diff += Vj * 2.0f * (rhoj - rhoi) * dot(-rij, gw) / (r2 + eta2);
         }
         if (F.thermal) {
-          const float kj = P.k[pidx(phj)];
+          const float kj = sel2(P.k, pidx(phj));
           const float kij = 4.0f * ki * kj / (ki + kj);
           const float rgw = r * dW;
           q += Vi * Vj * kij * (Ti - Tj) * rgw / (r2 + eta2);
         }
-      };
-      if (SWEEP) for_each_nbr(g, xi, pair);
-      else for_each_listed(L, i, cnt, pair);
+      });
 
-      const V3 ab = tvf_on ? sum_t * (-pb / mt) : V3{0, 0, 0};
+      const V3 ab = tvf_on ? V3(sum_t) * (-pb / mt) : V3{0, 0, 0};
       pv.t.ab[i] = st3(ab);
-      V3 acc = grav + force / mi + acc_av;
+      V3 acc = grav + V3(force) / mi + V3(acc_av);
       if (st->frame_rot) {
         const V3 om = st->omega, omd = st->omega_dot;
         const V3 rr = xi - origin;
@@ -551,11 +593,11 @@ __global__ __launch_bounds__(kBlock) void k_forces(PV pv, GridDev g, SimParams P
       }
       if (F.csf_mode) {
         if (csf_on) {
-          const float kappa = den > 0.0f ? 3.0f * num / den : 0.0f;
+          const float kappa = float(den) > 0.0f ? 3.0f * float(num) / float(den) : 0.0f;
           pv.t.kappa[i] = kappa;
           if (P.wetting != 0) acc += ld3(pv.t.vw[i]);
           if (P.csf_balanced) {
-            acc += G * (P.sigma * kappa / mi);
+            acc += V3(G) * (P.sigma * kappa / mi);
           } else {
             const float coeff = 2.0f * P.sigma / (P.rho0[0] + P.rho0[1]);
             acc += ld3(pv.s.nrm[i]) * (coeff * kappa);
@@ -564,37 +606,187 @@ __global__ __launch_bounds__(kBlock) void k_forces(PV pv, GridDev g, SimParams P
           pv.t.kappa[i] = 0.0f;
         }
       }
-      if (ak_on) acc += acc_ak;
+      if (ak_on) acc += V3(acc_ak);
       pv.t.a[i] = st3(acc);
-      pv.t.fw[i] = st3(fwall);
-      pv.t.mw[i] = st3(mwall);
-      if (F.continuity) pv.t.drho[i] = rhoi * div + P.delta_sph * h * P.c0 * diff;
+      pv.t.fw[i] = st3(V3(fwall));
+      pv.t.mw[i] = st3(V3(mwall));
+      if (F.continuity) pv.t.drho[i] = rhoi * float(div) + P.delta_sph * h * P.c0 * float(diff);
       if (F.thermal) {
-        q += P.heat_power * pv.t.heat_w[i] * st->heat_inv_norm;
-        pv.t.dTdt[i] = q / (mi * P.cp[pi_idx]);
+        const float qt = float(q) + P.heat_power * pv.t.heat_w[i] * st->heat_inv_norm;
+        pv.t.dTdt[i] = qt / (mi * sel2(P.cp, pi_idx));
       }
       // evaluate_rates maxima (sph_solver.cu:62-99)
-      m_v = norm(vi);
+      mx.v = norm(vi);
       const float na = norm(acc), nb = norm(ab);
-      m_a = na > nb ? na : nb;
-      if (ph == kVapor) m_tv = Ti;
-      if (P.sps) m_nu = pv.t.nu_t[i];
+      mx.a = na > nb ? na : nb;
+      if (ph == kVapor) mx.tv = Ti;
+      if (P.sps) mx.nu = pv.t.nu_t[i];
       const float rm = pv.t.rmin[i] > 0.05f * P.dx ? pv.t.rmin[i] : 0.05f * P.dx;
-      m_sp = 0.5f * P.dx / rm;
+      mx.sp = 0.5f * P.dx / rm;
     }
   }
-  m_v = wave_max(m_v);
-  m_a = wave_max(m_a);
-  m_tv = wave_max(m_tv);
-  m_nu = wave_max(m_nu);
-  m_sp = wave_max(m_sp);
+}
+
+__device__ inline void publish_maxima(DevState* st, Maxima mx) {
+  mx.v = wave_max(mx.v);
+  mx.a = wave_max(mx.a);
+  mx.tv = wave_max(mx.tv);
+  mx.nu = wave_max(mx.nu);
+  mx.sp = wave_max(mx.sp);
   if ((threadIdx.x & 63) == 0) {
-    atomic_max_nonneg(&st->max_bits[0], m_v);
-    atomic_max_nonneg(&st->max_bits[1], m_a);
-    atomic_max_nonneg(&st->max_bits[2], m_tv);
-    atomic_max_nonneg(&st->max_bits[3], m_nu);
-    atomic_max_nonneg(&st->max_bits[4], m_sp);
+    atomic_max_nonneg(&st->max_bits[0], mx.v);
+    atomic_max_nonneg(&st->max_bits[1], mx.a);
+    atomic_max_nonneg(&st->max_bits[2], mx.tv);
+    atomic_max_nonneg(&st->max_bits[3], mx.nu);
+    atomic_max_nonneg(&st->max_bits[4], mx.sp);
   }
+}
+
+// SWEEP = false: particles whose neighbour list is complete (count >= 0),
+// neighbour data gathered from the particle arrays.  SWEEP = true: the
+// overflowed ones (count < 0) by the cell sweep, which visits the same
+// neighbours in the same order.  Two instances keep the sweep's code and
+// registers out of the list loop.
+template <int ST, bool CONT, bool THERMAL, bool KS, bool SWEEP>
+__global__ __launch_bounds__(kBlock) void k_forces(PV pv, GridDev g, SimParams P, NbrList L, DevState* st, int n) {
+  const int i = gtid();
+  Maxima mx;
+  const int cnt = i < n ? L.count[i] : 0;
+  if (i < n && (cnt < 0) == SWEEP) {
+    const bool need_vt = forces_need_vt(P, CONT);
+    const V3 xi = ld3(pv.s.x[i]);
+    auto gathered = [&](auto&& f) {
+      auto one = [&](int j) { f(j, load_nbr(pv, P, j, need_vt)); };
+      if (SWEEP) for_each_nbr(g, xi, one);
+      else for_each_listed(L, i, cnt, one);
+    };
+    forces_particle<ST, CONT, THERMAL, KS>(pv, P, st, i, gathered, mx);
+  }
+  publish_maxima(st, mx);
+}
+
+// ------------------------------------------------ forces, LDS-staged tiles
+// One workgroup per tile of 3 x 3 x 3 cells.  Every neighbour of the tile's
+// particles lies in the 5 x 5 x 5 cells around it: up to 25 (z, y) rows, each
+// one contiguous index range of the cell-sorted arrays.  The workgroup copies
+// those rows' neighbour data into LDS once (coalesced), then each thread
+// walks its particle's neighbour list reading LDS.  List entries ascend, so a
+// forward-moving row cursor maps index j to its LDS slot.  A tile whose rows
+// hold more than kTileCap particles gathers from global memory instead;
+// overflowed lists (count < 0) are left to the sweep instance of k_forces.
+constexpr int kTileCells = 3;
+constexpr int kTileThreads = 512;
+constexpr int kTileCap = 2304;
+constexpr int kTileRows = (kTileCells + 2) * (kTileCells + 2);
+constexpr int kTileTargetRows = kTileCells * kTileCells;
+
+struct TileRows {
+  int start[kTileRows];     // first sorted index of each staged row
+  int base[kTileRows + 1];  // its first LDS slot (exclusive prefix of row lengths)
+  int nrows;
+  int tstart[kTileTargetRows];  // the tile's own rows (targets)
+  int tbase[kTileTargetRows + 1];
+  int ntrows;
+};
+
+// Row tables of tile `b`; called by the whole block, ends with a barrier.
+__device__ inline void tile_rows(const GridDev& g, int b, TileRows& T) {
+  const int tiles_x = (g.nx + kTileCells - 1) / kTileCells;
+  const int tiles_y = (g.ny + kTileCells - 1) / kTileCells;
+  const int cx0 = (b % tiles_x) * kTileCells;
+  const int cy0 = ((b / tiles_x) % tiles_y) * kTileCells;
+  const int cz0 = (b / (tiles_x * tiles_y)) * kTileCells;
+  const int cx1 = min(cx0 + kTileCells, g.nx), cy1 = min(cy0 + kTileCells, g.ny), cz1 = min(cz0 + kTileCells, g.nz);
+  const int sx0 = max(cx0 - 1, 0), sx1 = min(cx1, g.nx - 1);
+  const int sy0 = max(cy0 - 1, 0), sy1 = min(cy1, g.ny - 1);
+  const int sz0 = max(cz0 - 1, 0), sz1 = min(cz1, g.nz - 1);
+  const int nyr = sy1 - sy0 + 1;
+  const int nrows = (sz1 - sz0 + 1) * nyr;
+  const int tyr = cy1 - cy0;
+  const int ntrows = (cz1 - cz0) * tyr;
+  const int t = threadIdx.x;
+  // row lengths in parallel: threads [0, 25) staged rows, [32, 41) target rows
+  if (t < nrows) {
+    const int row = ((sz0 + t / nyr) * g.ny + (sy0 + t % nyr)) * g.nx;
+    const int s = g.off[row + sx0];
+    T.start[t] = s;
+    T.base[t + 1] = g.off[row + sx1 + 1] - s;
+  } else if (t >= 32 && t < 32 + ntrows) {
+    const int k = t - 32;
+    const int row = ((cz0 + k / tyr) * g.ny + (cy0 + k % tyr)) * g.nx;
+    const int s = g.off[row + cx0];
+    T.tstart[k] = s;
+    T.tbase[k + 1] = g.off[row + cx1] - s;
+  }
+  __syncthreads();
+  if (t == 0) {
+    T.base[0] = 0;
+    for (int r = 0; r < nrows; ++r) T.base[r + 1] += T.base[r];
+    T.nrows = nrows;
+  } else if (t == 64) {
+    T.tbase[0] = 0;
+    for (int k = 0; k < ntrows; ++k) T.tbase[k + 1] += T.tbase[k];
+    T.ntrows = ntrows;
+  }
+  __syncthreads();
+}
+
+template <int ST, bool CONT, bool THERMAL, bool KS>
+__global__ __launch_bounds__(kTileThreads) void k_forces_tiled(PV pv, GridDev g, SimParams P, NbrList L,
+                                                             DevState* st) {
+  __shared__ TileRows T;
+  __shared__ float4 s_x[kTileCap], s_rec[kTileCap], s_v[kTileCap], s_vt[kTileCap];
+  tile_rows(g, blockIdx.x, T);
+  const int ntarget = T.tbase[T.ntrows];
+  if (ntarget == 0) return;  // uniform over the block
+  const int M = T.base[T.nrows];
+  const bool staged = M <= kTileCap;
+  const bool need_vt = forces_need_vt(P, CONT);
+  if (staged) {
+    for (int s = threadIdx.x; s < M; s += kTileThreads) {
+      int lo = 0, hi = T.nrows - 1;  // last row with base <= s
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (T.base[mid] <= s) lo = mid;
+        else hi = mid - 1;
+      }
+      const int j = T.start[lo] + (s - T.base[lo]);
+      const NbrData d = load_nbr(pv, P, j, need_vt);
+      s_x[s] = make_float4(d.x.x, d.x.y, d.x.z, __int_as_float(d.ph));
+      s_rec[s] = d.rec;
+      s_v[s] = d.v;
+      if (need_vt) s_vt[s] = d.vt;
+    }
+  }
+  __syncthreads();
+  Maxima mx;
+  for (int t = threadIdx.x; t < ntarget; t += kTileThreads) {
+    int k = 0;
+    while (t >= T.tbase[k + 1]) ++k;
+    const int i = T.tstart[k] + (t - T.tbase[k]);
+    const int cnt = L.count[i];
+    if (cnt < 0) continue;  // the sweep instance handles it
+    if (staged) {
+      auto from_lds = [&](auto&& f) {
+        int r = 0;
+        for_each_listed(L, i, cnt, [&](int j) {
+          while (r < T.nrows - 1 && j >= T.start[r] + (T.base[r + 1] - T.base[r])) ++r;
+          const int s = T.base[r] + (j - T.start[r]);
+          const float4 xs = s_x[s], rs = s_rec[s], vs = s_v[s];
+          float4 vts = vs;
+          if (need_vt) vts = s_vt[s];
+          f(j, NbrData{xs, __float_as_int(xs.w), rs, vs, vts});
+        });
+      };
+      forces_particle<ST, CONT, THERMAL, KS>(pv, P, st, i, from_lds, mx);
+    } else {
+      auto gathered = [&](auto&& f) {
+        for_each_listed(L, i, cnt, [&](int j) { f(j, load_nbr(pv, P, j, need_vt)); });
+      };
+      forces_particle<ST, CONT, THERMAL, KS>(pv, P, st, i, gathered, mx);
+    }
+  }
+  publish_maxima(st, mx);
 }
 
 // Publishes the maxima (spacing ratio starts from 1, sph_solver.cu:62-71).
@@ -643,6 +835,7 @@ __global__ __launch_bounds__(kBlock) void k_probe(PV pv, GridDev g, SimParams P,
 }
 
 // SPH_kernels.cu:106-160 (delta+ shifting), pass 1 and pass 2.
+template <bool KS>
 __global__ __launch_bounds__(kBlock) void k_shift_compute(PV pv, GridDev g, SimParams P, NbrList L,
                                                          const DevState* st, int n) {
   const int i = gtid();
@@ -656,7 +849,7 @@ __global__ __launch_bounds__(kBlock) void k_shift_compute(PV pv, GridDev g, SimP
   const float coeff = P.shift_strength * mach * 4.0f * h * h;
   const float cap = 0.1f * P.dx;
   const V3 xi = ld3(pv.s.x[i]);
-  V3 sum{0, 0, 0};
+  AccV3<KS> sum;
   for_each_pair(L, g, i, xi, [&](int j) {
     if (j == i) return;
     const uint8_t phj = pv.s.phase[j];
@@ -670,7 +863,7 @@ __global__ __launch_bounds__(kBlock) void k_shift_compute(PV pv, GridDev g, SimP
     const float q2 = qq * qq;
     sum += kgradW(P.kernel, rij, r) * (Vj * (1.0f + 0.2f * q2 * q2));
   });
-  V3 dr = sum * (-coeff);
+  V3 dr = V3(sum) * (-coeff);
   if (pv.t.alpha[i] > 0.0f) {
     const V3 gn = ld3(pv.s.nrm[i]);
     const float gm = norm(gn);
@@ -695,53 +888,78 @@ __global__ __launch_bounds__(kBlock) void k_shift_apply(PV pv, int n) {
 }  // namespace
 
 void launch_density(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, int n, hipStream_t s) {
-  if (n > 0) k_density<<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, n);
+  if (n <= 0) return;
+  if (P.compensated) k_density<true><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, n);
+  else k_density<false><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, n);
 }
 void launch_normalize_masses(const PV& pv, const GridDev& g, const SimParams& P, int n, hipStream_t s) {
   if (n > 0) k_normalize_masses<<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, n);
 }
 void launch_wall_bc(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, const DevState* st, int n,
                     hipStream_t s) {
-  if (n > 0) k_wall_bc<<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, st, n);
+  if (n <= 0) return;
+  if (P.compensated) k_wall_bc<true><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, st, n);
+  else k_wall_bc<false><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, st, n);
 }
 int launch_interface(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, int n,
                      bool csf_mode, double* heat_partials, hipStream_t s) {
   const int nb = blocks_for(n > 0 ? n : 1);
-  if (P.sps) k_interface<true><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, n, csf_mode ? 1 : 0, heat_partials);
-  else k_interface<false><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, n, csf_mode ? 1 : 0, heat_partials);
+  const int cm = csf_mode ? 1 : 0;
+  if (P.compensated) {
+    if (P.sps) k_interface<true, true><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, n, cm, heat_partials);
+    else k_interface<false, true><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, n, cm, heat_partials);
+  } else {
+    if (P.sps) k_interface<true, false><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, n, cm, heat_partials);
+    else k_interface<false, false><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, n, cm, heat_partials);
+  }
   return nb;
 }
 void launch_smooth_normals(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, int n,
                            hipStream_t s) {
-  if (n > 0) k_smooth_normals<<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, n);
+  if (n <= 0) return;
+  if (P.compensated) k_smooth_normals<true><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, n);
+  else k_smooth_normals<false><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, n);
 }
 void launch_heat_norm(DevState* st, const double* partials, int nb, bool active, hipStream_t s) {
   k_heat_norm<<<1, 1024, 0, s>>>(st, partials, nb, active ? 1 : 0);
 }
 
-template <int ST, bool CONT, bool THERMAL>
+template <int ST, bool CONT, bool THERMAL, bool KS>
 static void forces_variant(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, int n,
-                           hipStream_t s) {
+                           bool tiled, hipStream_t s) {
   if (!L.valid) throw std::logic_error("forces pass: neighbour lists not built");
-  k_forces<ST, CONT, THERMAL, false><<<blocks_for(n > 0 ? n : 1), kBlock, 0, s>>>(pv, g, P, L, st, n);
-  k_forces<ST, CONT, THERMAL, true><<<blocks_for(n > 0 ? n : 1), kBlock, 0, s>>>(pv, g, P, L, st, n);
-}
-template <int ST>
-static void forces_st(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, int n,
-                      bool continuity, bool thermal, hipStream_t s) {
-  if (continuity) {
-    if (thermal) forces_variant<ST, true, true>(pv, g, P, L, st, n, s);
-    else forces_variant<ST, true, false>(pv, g, P, L, st, n, s);
+  if (tiled) {
+    const long long tiles = static_cast<long long>((g.nx + kTileCells - 1) / kTileCells) *
+                            ((g.ny + kTileCells - 1) / kTileCells) * ((g.nz + kTileCells - 1) / kTileCells);
+    if (tiles > 0x7fffffffLL) throw std::runtime_error("forces pass: too many cell tiles");
+    k_forces_tiled<ST, CONT, THERMAL, KS><<<static_cast<int>(tiles), kTileThreads, 0, s>>>(pv, g, P, L, st);
   } else {
-    if (thermal) forces_variant<ST, false, true>(pv, g, P, L, st, n, s);
-    else forces_variant<ST, false, false>(pv, g, P, L, st, n, s);
+    k_forces<ST, CONT, THERMAL, KS, false><<<blocks_for(n > 0 ? n : 1), kBlock, 0, s>>>(pv, g, P, L, st, n);
+  }
+  k_forces<ST, CONT, THERMAL, KS, true><<<blocks_for(n > 0 ? n : 1), kBlock, 0, s>>>(pv, g, P, L, st, n);
+}
+template <int ST, bool KS>
+static void forces_st(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, int n,
+                      bool continuity, bool thermal, bool tiled, hipStream_t s) {
+  if (continuity) {
+    if (thermal) forces_variant<ST, true, true, KS>(pv, g, P, L, st, n, tiled, s);
+    else forces_variant<ST, true, false, KS>(pv, g, P, L, st, n, tiled, s);
+  } else {
+    if (thermal) forces_variant<ST, false, true, KS>(pv, g, P, L, st, n, tiled, s);
+    else forces_variant<ST, false, false, KS>(pv, g, P, L, st, n, tiled, s);
   }
 }
+template <bool KS>
+static void forces_ks(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, int n,
+                      bool csf_mode, bool akinci, bool continuity, bool thermal, bool tiled, hipStream_t s) {
+  if (akinci) forces_st<2, KS>(pv, g, P, L, st, n, continuity, thermal, tiled, s);
+  else if (csf_mode) forces_st<1, KS>(pv, g, P, L, st, n, continuity, thermal, tiled, s);
+  else forces_st<0, KS>(pv, g, P, L, st, n, continuity, thermal, tiled, s);
+}
 void launch_forces(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, int n,
-                   bool csf_mode, bool akinci, bool continuity, bool thermal, hipStream_t s) {
-  if (akinci) forces_st<2>(pv, g, P, L, st, n, continuity, thermal, s);
-  else if (csf_mode) forces_st<1>(pv, g, P, L, st, n, continuity, thermal, s);
-  else forces_st<0>(pv, g, P, L, st, n, continuity, thermal, s);
+                   bool csf_mode, bool akinci, bool continuity, bool thermal, bool tiled, hipStream_t s) {
+  if (P.compensated) forces_ks<true>(pv, g, P, L, st, n, csf_mode, akinci, continuity, thermal, tiled, s);
+  else forces_ks<false>(pv, g, P, L, st, n, csf_mode, akinci, continuity, thermal, tiled, s);
   k_publish_maxima<<<1, 1, 0, s>>>(st, P.has_vapor, P.sps);
 }
 void launch_probe(const PV& pv, const GridDev& g, const SimParams& P, const float* pts, float* out, int np,
@@ -751,7 +969,8 @@ void launch_probe(const PV& pv, const GridDev& g, const SimParams& P, const floa
 void launch_shifting(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, const DevState* st, int n,
                      hipStream_t s) {
   if (n <= 0) return;
-  k_shift_compute<<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, st, n);
+  if (P.compensated) k_shift_compute<true><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, st, n);
+  else k_shift_compute<false><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, st, n);
   k_shift_apply<<<blocks_for(n), kBlock, 0, s>>>(pv, n);
 }
 

@@ -144,7 +144,7 @@ __global__ __launch_bounds__(kBlock) void k_kick_sources(PV pv, SimParams P, Dev
   const float m = pv.s.m[i];
   const float T = pv.s.T[i];
   const float T_sat = st->T_sat;
-  if (m < P.merge_ratio * P.mref[ii]) {
+  if (m < P.merge_ratio * sel2(P.mref, ii)) {
     pv.t.out_m[i] = m;
     pv.t.out_h[i] = enthalpy(P, ph, T);
     fl |= kSource | kDying;
@@ -186,6 +186,7 @@ __global__ __launch_bounds__(kBlock) void k_kick_sources(PV pv, SimParams P, Dev
 
 // phase_change.cu:139-172, plus the partials of sph_solver.cu:108-124
 // (evaporated, condensed, merges) over every particle.
+template <bool KS>
 __global__ __launch_bounds__(kBlock) void k_pc_normalize(PV pv, GridDev g, SimParams P, NbrList L, int n,
                                                         double* partials, int stride) {
   const int i = gtid();
@@ -195,17 +196,19 @@ __global__ __launch_bounds__(kBlock) void k_pc_normalize(PV pv, GridDev g, SimPa
     if (fl & kSource) {
       const uint8_t target = (fl & kTargetVapor) ? kVapor : kLiquid;
       const V3 xi = ld3(pv.s.x[i]);
-      float wsum = 0.0f, wr_max = 0.0f;
+      Acc<KS> awsum;
+      float wr_max = 0.0f;
       for_each_pair(L, g, i, xi, [&](int j) {
         if (j == i || pv.s.phase[j] != target || (pv.s.flags[j] & kDying)) return;
         const float r2 = norm2(xi - ld3(pv.s.x[j]));
         if (r2 >= P.support2) return;
         const float w = kW(P.kernel, sqrtf(r2));
         const float rhoj = pv.s.rho[j];
-        wsum += (pv.s.m[j] / rhoj) * w;
+        awsum += (pv.s.m[j] / rhoj) * w;
         const float wr = w / rhoj;
         wr_max = wr > wr_max ? wr : wr_max;
       });
+      const float wsum = awsum;
       pv.t.w_norm[i] = wsum;
       float om = pv.t.out_m[i];
       if (!(wsum > 0.0f)) {
@@ -229,6 +232,7 @@ __global__ __launch_bounds__(kBlock) void k_pc_normalize(PV pv, GridDev g, SimPa
 }
 
 // phase_change.cu:174-215
+template <bool KS>
 __global__ __launch_bounds__(kBlock) void k_pc_apply(PV pv, GridDev g, SimParams P, NbrList L, int n) {
   const int i = gtid();
   if (i >= n) return;
@@ -238,7 +242,7 @@ __global__ __launch_bounds__(kBlock) void k_pc_apply(PV pv, GridDev g, SimParams
   const float mi = pv.s.m[i];
   const float Vi = mi / pv.s.rho[i];
   const V3 xi = ld3(pv.s.x[i]);
-  float in_m = 0.0f, in_H = 0.0f;
+  Acc<KS> ain_m, ain_H;
   if (!(fl & kDying)) {
     for_each_pair(L, g, i, xi, [&](int j) {
       if (j == i) return;
@@ -249,10 +253,11 @@ __global__ __launch_bounds__(kBlock) void k_pc_apply(PV pv, GridDev g, SimParams
       const float r2 = norm2(xi - ld3(pv.s.x[j]));
       if (r2 >= P.support2) return;
       const float share = om * Vi * kW(P.kernel, sqrtf(r2)) / pv.t.w_norm[j];
-      in_m += share;
-      in_H += share * pv.t.out_h[j];
+      ain_m += share;
+      ain_H += share * pv.t.out_h[j];
     });
   }
+  const float in_m = ain_m, in_H = ain_H;
   const float om_i = pv.t.out_m[i];
   if ((fl & kDying) && om_i > 0.0f) {
     pv.s.m[i] = 0.0f;
@@ -267,6 +272,7 @@ __global__ __launch_bounds__(kBlock) void k_pc_apply(PV pv, GridDev g, SimParams
 }
 
 // phase_change.cu:217-249 (pass 1)
+template <bool KS>
 __global__ __launch_bounds__(kBlock) void k_rebalance(PV pv, GridDev g, SimParams P, NbrList L, int n) {
   const int i = gtid();
   if (i >= n) return;
@@ -279,8 +285,8 @@ __global__ __launch_bounds__(kBlock) void k_rebalance(PV pv, GridDev g, SimParam
   const float mi = pv.s.m[i];
   const float hi = enthalpy(P, kVapor, pv.s.T[i]);
   const V3 vi = ld3(pv.s.v[i]);
-  float dm = 0.0f, dH = 0.0f;
-  V3 dP{0, 0, 0};
+  Acc<KS> dm, dH;
+  AccV3<KS> dP;
   for_each_pair(L, g, i, xi, [&](int j) {
     if (j == i || pv.s.phase[j] != kVapor) return;
     const float r2 = norm2(xi - ld3(pv.s.x[j]));
@@ -291,9 +297,9 @@ __global__ __launch_bounds__(kBlock) void k_rebalance(PV pv, GridDev g, SimParam
     dH += flow * (from_j ? enthalpy(P, kVapor, pv.s.T[j]) : hi);
     dP += (from_j ? ld3(pv.s.v[j]) : vi) * flow;
   });
-  pv.t.out_m[i] = dm;
-  pv.t.out_h[i] = dH;
-  pv.t.ns[i] = st3(dP);
+  pv.t.out_m[i] = float(dm);
+  pv.t.out_h[i] = float(dH);
+  pv.t.ns[i] = st3(V3(dP));
 }
 
 // phase_change.cu:250-268 (pass 2) fused with mark_heavy_particles
@@ -319,7 +325,7 @@ __global__ __launch_bounds__(kBlock) void k_rebalance_apply_mark(PV pv, SimParam
       m = m_new;
     }
   }
-  flag[i] = (is_fluid(ph) && m > P.split_ratio * P.mref[pidx(ph)]) ? 1u : 0u;
+  flag[i] = (is_fluid(ph) && m > P.split_ratio * sel2(P.mref, pidx(ph))) ? 1u : 0u;
 }
 
 // Exclusive scan of the split flags: per-block scan, block totals scanned by
@@ -620,14 +626,19 @@ void launch_kick_sources(const PV& pv, const SimParams& P, DevState* st, int n, 
 int launch_pc_normalize(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, int n,
                         double* partials, int stride, hipStream_t s) {
   const int nb = blocks_for(n > 0 ? n : 1);
-  k_pc_normalize<<<nb, kBlock, 0, s>>>(pv, g, P, L, n, partials, stride);
+  if (P.compensated) k_pc_normalize<true><<<nb, kBlock, 0, s>>>(pv, g, P, L, n, partials, stride);
+  else k_pc_normalize<false><<<nb, kBlock, 0, s>>>(pv, g, P, L, n, partials, stride);
   return nb;
 }
 void launch_pc_apply(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, int n, hipStream_t s) {
-  if (n > 0) k_pc_apply<<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, n);
+  if (n <= 0) return;
+  if (P.compensated) k_pc_apply<true><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, n);
+  else k_pc_apply<false><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, n);
 }
 void launch_rebalance(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, int n, hipStream_t s) {
-  if (n > 0) k_rebalance<<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, n);
+  if (n <= 0) return;
+  if (P.compensated) k_rebalance<true><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, n);
+  else k_rebalance<false><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, n);
 }
 void launch_rebalance_apply_mark(const PV& pv, const SimParams& P, int n, bool rebalance, uint32_t* flag,
                                  hipStream_t s) {

@@ -141,6 +141,7 @@ void config_default(cryo_config* c) {  // sph_solver.hpp:20-91
   c->dt_max = 1e-2f;
   c->dt_min = 1e-9f;
   c->fixed_dt = 0.0f;
+  c->compensated_sums = 0;
 }
 
 V3 v3(cryo_vec3 a) { return V3{a.x, a.y, a.z}; }
@@ -229,6 +230,10 @@ struct cryo_solver {
   static constexpr int kListK = 96;
   DevBuf<int> nbr_idx, nbr_count;
   bool lists_valid = false;
+  // Every particle [0, n) sits in the cell it was sorted into (no move or
+  // append since the last sort): the cell-tile passes may be used.
+  bool tiles_ok = false;
+  bool use_tiles = true;
   const uint32_t* last_perm = nullptr;
   int n_sorted = 0;
   // reductions
@@ -554,6 +559,7 @@ struct cryo_solver {
     Q.split_ratio = c.split_ratio;
     Q.heat_power = c.heater_power;
     Q.has_vapor = has_vapor ? 1 : 0;
+    Q.compensated = c.compensated_sums ? 1 : 0;
     L = DtLimits{c.cfl, c.viscous_number, c.force_number, c.capillary_number, c.thermal_number,
                  c.dt_max, c.dt_min, c.fixed_dt};
   }
@@ -679,6 +685,7 @@ struct cryo_solver {
     while ((1ll << key_bits) <= cells) ++key_bits;
     last_perm = nullptr;
     n_sorted = 0;
+    tiles_ok = false;
   }
 
   // ----------------------------------------------------------- the step
@@ -706,6 +713,7 @@ struct cryo_solver {
     last_perm = perm;
     n_sorted = n;
     lists_valid = false;
+    tiles_ok = true;
     n -= pending_dead;  // dead particles sort last and are dropped
     pending_dead = 0;
   }
@@ -732,7 +740,7 @@ struct cryo_solver {
     prof_end(5);
     prof_begin(6);
     launch_forces(pv, gd, P, L, state.get(), n, csf_mode, cfg.surface_tension == CRYO_ST_AKINCI,
-                  cfg.density_mode == CRYO_DENSITY_CONTINUITY, P.thermal != 0, stream);
+                  cfg.density_mode == CRYO_DENSITY_CONTINUITY, P.thermal != 0, tiles_ok && use_tiles, stream);
     prof_end(6);
     check_launch();
   }
@@ -775,6 +783,7 @@ struct cryo_solver {
     launch_exclusive_scan(split_flag.get(), split_off.get(), n, scan_sums.get(), scan_total.get(), stream);
     launch_split(pv, P, state.get(), n, split_flag.get(), split_off.get(), stream);
     lists_valid = false;  // split moves parents and appends particles
+    tiles_ok = false;
     launch_split_commit(state.get(), n, capacity, scan_total.get(), stream);
     nb_vent = 0;
     if (cfg.thermo_mode == CRYO_VENTED && has_vapor)
@@ -784,6 +793,12 @@ struct cryo_solver {
   }
 
   void step() {  // sph_solver.cu:511-533
+    step_issue();
+    step_finish();
+  }
+  // Everything of one step that the device can do without the host: queued
+  // on the solver stream, returns at once.
+  void step_issue() {
     ensure_initialized();
     launch_begin_step(state.get(), P, L, man, stream);
     grid_build(true);
@@ -797,6 +812,7 @@ struct cryo_solver {
     if (shifting) {
       launch_shifting(pv, grid(), P, lists(), state.get(), n, stream);
       lists_valid = false;  // positions moved
+      tiles_ok = false;
       if (pc) launch_kick_sources(pv, P, state.get(), n, false, false, true, stream);
     }
     prof_end(7);
@@ -807,7 +823,12 @@ struct cryo_solver {
     prof_begin(9);
     end_step(nb_pc, nb_vent, true);
     prof_end(9);
-    if (pc) {
+  }
+  // Host bookkeeping the next step needs: with phase change the particle
+  // count (splits append, merges drop at the next sort) comes back from the
+  // device, which waits for this solver's stream.
+  void step_finish() {
+    if (cfg.phase_change != 0) {
       sync_state();
       n = hs->n;
       pending_dead = hs->merges - merges_seen;
@@ -1230,6 +1251,25 @@ int cryo_solver_run_steps(cryo_solver* s, int64_t k) {
   CRYO_GUARD({
     for (int64_t q = 0; q < k; ++q) S(s).step();
     S(s).sync_state();
+  })
+}
+int cryo_ensemble_run_steps(cryo_solver* const* solvers, int32_t count, int64_t steps) {
+  CRYO_GUARD({
+    if (count < 0 || steps < 0) throw std::invalid_argument("cryo_ensemble_run_steps: negative count or steps");
+    if (count > 0 && !solvers) throw std::invalid_argument("cryo_ensemble_run_steps: null solver array");
+    for (int32_t t = 0; t < count; ++t) {
+      if (!solvers[t]) throw std::invalid_argument("cryo_ensemble_run_steps: null solver");
+      for (int32_t u = 0; u < t; ++u)
+        if (solvers[u] == solvers[t]) throw std::invalid_argument("cryo_ensemble_run_steps: a solver appears twice");
+    }
+    // Every member's step is queued on its own stream before any member
+    // waits, so the device runs the tanks' kernels side by side and a
+    // member's host round trip overlaps the others' device work.
+    for (int64_t q = 0; q < steps; ++q) {
+      for (int32_t t = 0; t < count; ++t) S(solvers[t]).step_issue();
+      for (int32_t t = 0; t < count; ++t) S(solvers[t]).step_finish();
+    }
+    for (int32_t t = 0; t < count; ++t) S(solvers[t]).sync_state();
   })
 }
 int cryo_solver_run_until(cryo_solver* s, double t_end) {

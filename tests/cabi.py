@@ -56,6 +56,7 @@ _CONFIG_FIELDS = [
     ("cfl", C.c_float), ("viscous_number", C.c_float), ("force_number", C.c_float),
     ("capillary_number", C.c_float), ("thermal_number", C.c_float), ("dt_max", C.c_float),
     ("dt_min", C.c_float), ("fixed_dt", C.c_float),
+    ("compensated_sums", C.c_int32),
 ]
 
 
@@ -140,7 +141,8 @@ def default_config(**kw) -> Config:
         wall_thermal_bc=0, heater_power=0.0, conductivity_scale=1.0, lee_evaporation=10.0, lee_condensation=10.0,
         thermo_mode=0, vent_relaxation=0.02, merge_ratio=0.3, split_ratio=2.0, max_transfer_fraction=0.25,
         phase_change_density_limit=0.01, vapor_mass_rebalance=1.0, cfl=0.25, viscous_number=0.125,
-        force_number=0.25, capillary_number=0.25, thermal_number=0.1, dt_max=1e-2, dt_min=1e-9, fixed_dt=0.0)
+        force_number=0.25, capillary_number=0.25, thermal_number=0.1, dt_max=1e-2, dt_min=1e-9, fixed_dt=0.0,
+        compensated_sums=0)
     for k, v in kw.items():
         if k == "gravity":
             v = Vec3(*v)
@@ -607,6 +609,15 @@ class HipSolver:
         out = np.empty(len(p), np.float32)
         self._call("solver_probe_pressure", self.h, C.c_int32(len(p)), _f32p(p), _f32p(out))
         return out
+
+
+def hip_ensemble_run_steps(solvers, steps):
+    """cryo_ensemble_run_steps over HipSolver objects (their streams run side by side)."""
+    lib = HipSolver.load()
+    arr = (C.c_void_p * len(solvers))(*[s.h.value for s in solvers])
+    rc = lib.cryo_ensemble_run_steps(arr, C.c_int32(len(solvers)), C.c_int64(steps))
+    if rc != 0:
+        raise RuntimeError(lib.cryo_last_error().decode())
 
 
 def hip_build_tank_particles(nodes, fluid: Fluid, dx, fill: Fill) -> ParticleSet:

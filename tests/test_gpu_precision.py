@@ -1,0 +1,108 @@
+"""Precision form: compensated FP32 neighbour sums against the FP64 oracle.
+
+Workload: the reference's two-phase manoeuvre setting at test size, an LH2
+sphere (r = 0.1 m, dx = 8 mm), half full, contact angle 5 degrees, gravity
+1e-4 g, a lateral 0.05 m/s^2 acceleration pulse, closed tank with thermal and
+phase change on.  The same input bits go to three runs over the same fixed
+time steps: the plain FP32 GPU step (the reference's rounding), the
+compensated FP32 GPU step (cryo_config.compensated_sums = 1) and the FP64
+oracle (the reference's algorithm with real = double).  For every result the
+relative error of both FP32 runs against FP64 is stated; the bookkeeping
+(masses, enthalpy, heat input) is reduced in FP64 on the GPU in both forms.
+
+The table is written to $CRYO_REPORT_DIR/precision.json when that is set.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import cabi
+from cabi import WALL, CpuSolver, HipSolver
+
+pytestmark = pytest.mark.gpu
+STEPS = 60
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def lh2_sphere(compensated):
+    fluid = cabi.hip_fluid_preset("LH2")
+    cfg = cabi.default_config(dx=0.008, gravity=(0.0, 0.0, -9.81e-4), thermo_mode=1, contact_angle_deg=5.0,
+                              heater_power=2.0, compensated_sums=compensated)
+    fill = cabi.default_fill(T_liquid=fluid.T_sat + 1.0)
+    ps = cabi.hip_build_tank_particles(cabi.sphere(0.1), fluid, 0.008, fill)
+    rng = np.random.default_rng(3)
+    fl = ps.phase != WALL
+    ps.x[fl] += rng.uniform(-0.0004, 0.0004, size=(int(fl.sum()), 3)).astype(np.float32)
+    lo = ps.x.min(0) - np.float32(0.032)
+    hi = ps.x.max(0) + np.float32(0.032)
+    return fluid, cfg, ps, lo, hi
+
+
+MANEUVER = dict(time=[0.0, 0.01, 2.0, 2.01], acc=[[0, 0, 0], [0.05, 0, 0], [0.05, 0, 0], [0, 0, 0]])
+
+
+def run(solver, dt):
+    solver.set_maneuver(**MANEUVER)
+    solver.run_steps(STEPS)
+    d = solver.diagnostics()
+    ids = solver.field("id")
+    order = np.argsort(ids, kind="stable")
+    fields = {f: np.asarray(solver.field(f), np.float64)[order] for f in ("x", "v", "m", "T")}
+    return d, ids[order], fields
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    den = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / den) if den > 0 else float(np.linalg.norm(a - b))
+
+
+def test_compensated_sums_against_fp64_oracle():
+    say("test_compensated_sums_against_fp64_oracle")
+    fluid, cfg, ps, lo, hi = lh2_sphere(0)
+    probe = HipSolver(cfg, fluid)
+    probe.set_particles(ps, lo, hi)
+    dt = float(np.float32(probe.next_dt()))  # the start-up dt, then held fixed for all runs
+    results = {}
+    for name, kind, comp in (("fp32", "hip", 0), ("fp32_compensated", "hip", 1), ("fp64", "orc", 0)):
+        fluid, cfg, ps, lo, hi = lh2_sphere(comp)
+        cfg.fixed_dt = dt
+        s = HipSolver(cfg, fluid) if kind == "hip" else CpuSolver("orc", cfg, fluid, 64)
+        s.set_particles(ps, lo, hi)
+        say(name, "N =", s.num_particles)
+        results[name] = run(s, dt)
+    d64, ids64, f64 = results["fp64"]
+    table = {"workload": f"LH2 sphere r=0.1 m dx=8 mm, {len(ps)} particles, {STEPS} steps of dt={dt:.3e} s",
+             "errors": {}}
+    for name in ("fp32", "fp32_compensated"):
+        d, ids, f = results[name]
+        e = {}
+        for q in ("slosh_force", "slosh_moment", "liquid_com", "liquid_momentum"):
+            e[q] = rel(getattr(d, q).tup(), getattr(d64, q).tup())
+        for q in ("mass_liquid", "mass_vapor", "mass_total", "enthalpy", "kinetic_energy", "evaporated_mass",
+                  "condensed_mass", "p_ullage", "T_liquid_mean"):
+            e[q] = rel(getattr(d, q), getattr(d64, q))
+        if np.array_equal(ids, ids64):
+            for q in ("x", "v", "m", "T"):
+                e["field_" + q] = rel(f[q], f64[q])
+        table["errors"][name] = e
+    say(json.dumps(table, indent=1))
+    out = os.environ.get("CRYO_REPORT_DIR")
+    if out:
+        with open(os.path.join(out, "precision.json"), "w") as fh:
+            json.dump(table, fh, indent=1)
+    plain, comp = table["errors"]["fp32"], table["errors"]["fp32_compensated"]
+    # bookkeeping is FP64 in both forms: mass closure against FP64 stays tight
+    assert plain["mass_total"] <= 2.5e-6 and comp["mass_total"] <= 2.5e-6
+    # the trajectories stay close to the FP64 one (bounded divergence)
+    for q in ("field_x", "field_v"):
+        if q in plain:
+            assert plain[q] <= 1e-3 and comp[q] <= 1e-3, (q, plain[q], comp[q])
