@@ -423,19 +423,17 @@ struct NbrList {
 // Appends one entry of particle i's list, staging four in registers.
 struct ListWriter {
   int4* out;
-  int4 buf;
+  int4 buf = make_int4(0, 0, 0, 0);
   int c = 0;
   int K;
   __device__ ListWriter(const NbrList& L, int i) : out(reinterpret_cast<int4*>(L.idx + static_cast<long long>(i) * L.K)), K(L.K) {}
   __device__ inline void push(int j) {
-    switch (c & 3) {
-      case 0: buf.x = j; break;
-      case 1: buf.y = j; break;
-      case 2: buf.z = j; break;
-      default:
-        buf.w = j;
-        if (c < K) out[c >> 2] = buf;
-    }
+    const int sl = c & 3;  // selects, not a switch: no divergent branches
+    buf.x = sl == 0 ? j : buf.x;
+    buf.y = sl == 1 ? j : buf.y;
+    buf.z = sl == 2 ? j : buf.z;
+    buf.w = sl == 3 ? j : buf.w;
+    if (sl == 3 && c < K) out[c >> 2] = buf;
     ++c;
   }
   // Writes the last partial chunk; returns the count to store (-1: overflow).

@@ -44,7 +44,7 @@ void launch_normalize_masses(const PV& pv, const GridDev& g, const SimParams& P,
 void launch_wall_bc(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, const DevState* st, int n,
                     hipStream_t s);
 int launch_interface(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, int n,
-                     bool csf_mode, double* heat_partials, hipStream_t s);
+                     bool csf_mode, double* heat_partials, bool tiled, hipStream_t s);
 void launch_smooth_normals(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, int n,
                            hipStream_t s);
 void launch_heat_norm(DevState* st, const double* partials, int nb, bool active, hipStream_t s);

@@ -180,127 +180,163 @@ __global__ __launch_bounds__(kBlock) void k_wall_bc(PV pv, GridDev g, SimParams 
 }
 
 // ------------------------------------------------------- interface fields
-// Also resets the dt maxima for k_forces and writes FP64 block partials of
-// sum_fluid heat_w (sph_solver.cu:101-104) when heat_partials != nullptr.
+// What one neighbour contributes to the interface pass.  x.w holds the
+// phase in bits 0-7 and the flags in bits 8-15.  Fluid: r.x = m / rho.
+// Wall: r = {nrm, alpha} (alpha only when walls carry a colour).  v: the
+// fluid velocity or the wall's ghost velocity (SPS only).
+struct IfaceNbr {
+  float4 x, r, v;
+};
+
+template <bool SPS>
+__device__ inline IfaceNbr load_iface_nbr(const PV& pv, const SimParams& P, int j) {
+  IfaceNbr d;
+  const float4 x = pv.s.x[j];
+  const int ph = pv.s.phase[j];
+  const int fl = pv.s.flags[j];
+  d.x = make_float4(x.x, x.y, x.z, __int_as_float(ph | (fl << 8)));
+  d.v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (ph == kWall) {
+    const float4 nw = pv.s.nrm[j];
+    // (fluid alpha is being written by this pass: walls' only)
+    d.r = make_float4(nw.x, nw.y, nw.z, P.wall_colour ? pv.t.alpha[j] : 0.0f);
+    if (SPS && P.no_slip) d.v = pv.t.vw[j];
+  } else {
+    const float4 rec = pv.t.nrec[j];
+    d.r = make_float4(rec.y / rec.z, 0.0f, 0.0f, 0.0f);
+    if (SPS) d.v = pv.s.v[j];
+  }
+  return d;
+}
+
+// compute_interface_fields (surface_tension.cu:5-85) + SPS (navier_stokes.cu:
+// 71-119) + normals without smoothing (surface_tension.cu:87-118) for particle
+// i; returns its heat weight (0 unless fluid).  visit(f): f(j, IfaceNbr) over
+// the neighbours in the reference order.
+template <bool SPS, bool KS, class Visit>
+__device__ inline float interface_particle(const PV& pv, const SimParams& P, int i, int csf_mode, Visit&& visit) {
+  const uint8_t ph = pv.s.phase[i];
+  if (!is_fluid(ph)) {
+    if (SPS) pv.t.nu_t[i] = 0.0f;
+    return 0.0f;
+  }
+  const V3 xi = ld3(pv.s.x[i]);
+  const float ci = ph == kLiquid ? 1.0f : 0.0f;
+  AccV3<KS> agradc, awall_n, aakn, agradphi;
+  Acc<KS> as_all, as_other, aww, ahw;
+  const V3 vi = ld3(pv.s.v[i]);
+  Acc<KS> Lm[3][3];
+  visit([&](int j, const IfaceNbr d) {
+    const V3 rij = xi - ld3(d.x);
+    const float r2 = norm2(rij);
+    if (r2 >= P.support2) return;
+    const float r = sqrtf(r2);
+    const float w = kW(P.kernel, r);
+    const int bits = __float_as_int(d.x.w);
+    const int phj = bits & 0xff;
+    const V3 gw = kgradW(P.kernel, rij, r);
+    if (SPS && j != i && !(phj == kWall && !P.no_slip)) {
+      const V3 vj = ld3(d.v);
+      const float Vj = phj == kWall ? P.vol0 : d.r.x;
+      const V3 dv = (vj - vi) * Vj;
+      const float dvc[3] = {dv.x, dv.y, dv.z};
+      const float gwc[3] = {gw.x, gw.y, gw.z};
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) Lm[a][b] += dvc[a] * gwc[b];
+    }
+    if (phj == kWall) {
+      const float vw = P.vol0 * w;
+      aww += vw;
+      if ((bits >> 8) & kHeated) ahw += vw;
+      awall_n += ld3(d.r) * vw;
+      agradphi += gw * P.vol0;
+      if (P.wall_colour) agradc += gw * (P.vol0 * (d.r.w - ci));
+      return;
+    }
+    const float Vj = d.r.x;
+    as_all += Vj * w;
+    if (phj != ph) as_other += Vj * w;
+    if (j == i) return;
+    const float cj = phj == kLiquid ? 1.0f : 0.0f;
+    agradc += gw * (Vj * (cj - ci));
+    if (phj == kLiquid) aakn += gw * Vj;
+  });
+  V3 gradc = agradc;
+  const V3 wall_n = awall_n, akn = aakn, gradphi = agradphi;
+  const float s_all = as_all, s_other = as_other, ww = aww, hw = ahw;
+  if (SPS) {
+    float ss = 0.0f;
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) {
+        const float sab = 0.5f * (float(Lm[a][b]) + float(Lm[b][a]));
+        ss += sab * sab;
+      }
+    const float smag = sqrtf(2.0f * ss);
+    const float l = P.smag_cs * P.dx;
+    pv.t.nu_t[i] = l * l * smag;
+  }
+  pv.t.alpha[i] = s_all > 0.0f ? s_other / s_all : 0.0f;
+  pv.t.wall_w[i] = ww;
+  pv.t.heat_w[i] = hw;
+  if (P.st_model == 2) {
+    pv.s.nrm[i] = st3(ph == kLiquid ? akn * (-P.kernel.h) : V3{0, 0, 0});
+  } else {
+    const float gmag = norm(gradc);
+    V3 vwi{0, 0, 0};
+    if (P.wetting != 0 && ww > 0.0f && gmag > P.normal_eps && norm2(wall_n) > 0.0f) {
+      const V3 nw = normalized(wall_n);
+      const V3 nh = gradc / gmag;
+      const V3 tan_in = nh - nw * dot(nh, nw);
+      const float tm = norm(tan_in);
+      if (tm > 1e-6f) {
+        const float cos_d = -dot(nh, nw);
+        const float delta_w = 2.0f * norm(gradphi);
+        const float coeff = 2.0f * P.sigma / (P.rho0[0] + P.rho0[1]);
+        vwi = tan_in * (-(P.cos_theta - cos_d) * coeff * gmag * delta_w / tm);
+      }
+    }
+    pv.t.vw[i] = st3(vwi);
+    if (P.wetting != 1 && ww > 0.0f && gmag > P.normal_eps) {
+      const V3 nw = normalized(wall_n);
+      const V3 nh = gradc / gmag;
+      V3 t = nh - nw * dot(nh, nw);
+      const float tmag = norm(t);
+      if (norm2(nw) > 0.0f && tmag > 1e-6f) {
+        t = t / tmag;
+        const V3 target = nw * (-P.cos_theta) + t * P.sin_theta;
+        const float beta = clampv(ww / P.wall_adh_w, 0.0f, 1.0f);
+        const V3 blended = normalized(nh * (1.0f - beta) + target * beta);
+        gradc = blended * gmag;
+      }
+    }
+    pv.s.nrm[i] = st3(gradc);
+    if (csf_mode && !P.csf_smooth) {
+      const float gm = norm(gradc);
+      pv.t.ns[i] = st3(!(gm > P.normal_eps) ? V3{0, 0, 0} : gradc / gm);
+    }
+  }
+  return hw;
+}
+
+// Global-memory instance.  only_overflow = 0: every particle, resets the dt
+// maxima for k_forces and writes FP64 block partials of sum_fluid heat_w
+// (sph_solver.cu:101-104) when heat_partials != nullptr.  only_overflow = 1:
+// just the particles whose list overflowed (the cell-tile instance does the
+// rest).
 template <bool SPS, bool KS>
 __global__ __launch_bounds__(kBlock) void k_interface(PV pv, GridDev g, SimParams P, NbrList L, DevState* st, int n,
-                                                     int csf_mode, double* heat_partials) {
+                                                     int csf_mode, double* heat_partials, int only_overflow) {
   const int i = gtid();
-  if (i == 0) {
+  if (i == 0 && !only_overflow) {
     for (int k = 0; k < 5; ++k) st->max_bits[k] = 0u;
   }
   double heat_contrib = 0.0;
-  if (i < n) {
-    const uint8_t ph = pv.s.phase[i];
-    if (!is_fluid(ph)) {
-      if (SPS) pv.t.nu_t[i] = 0.0f;
-    } else {
-      const V3 xi = ld3(pv.s.x[i]);
-      const float ci = ph == kLiquid ? 1.0f : 0.0f;
-      AccV3<KS> agradc, awall_n, aakn, agradphi;
-      Acc<KS> as_all, as_other, aww, ahw;
-      // SPS velocity gradient (navier_stokes.cu:71-119)
-      const V3 vi = ld3(pv.s.v[i]);
-      Acc<KS> Lm[3][3];
-      for_each_pair(L, g, i, xi, [&](int j) {
-        const V3 rij = xi - ld3(pv.s.x[j]);
-        const float r2 = norm2(rij);
-        if (r2 >= P.support2) return;
-        const float r = sqrtf(r2);
-        const float w = kW(P.kernel, r);
-        const uint8_t phj = pv.s.phase[j];
-        const V3 gw = kgradW(P.kernel, rij, r);
-        if (SPS && j != i && !(phj == kWall && !P.no_slip)) {
-          V3 vj;
-          float Vj;
-          if (phj == kWall) {
-            vj = ld3(pv.t.vw[j]);
-            Vj = P.vol0;
-          } else {
-            vj = ld3(pv.s.v[j]);
-            Vj = pv.s.m[j] / pv.s.rho[j];
-          }
-          const V3 dv = (vj - vi) * Vj;
-          const float dvc[3] = {dv.x, dv.y, dv.z};
-          const float gwc[3] = {gw.x, gw.y, gw.z};
-          for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b) Lm[a][b] += dvc[a] * gwc[b];
-        }
-        if (phj == kWall) {
-          const float vw = P.vol0 * w;
-          aww += vw;
-          if (pv.s.flags[j] & kHeated) ahw += vw;
-          awall_n += ld3(pv.s.nrm[j]) * vw;
-          agradphi += gw * P.vol0;
-          if (P.wall_colour) agradc += gw * (P.vol0 * (pv.t.alpha[j] - ci));
-          return;
-        }
-        const float4 rec = pv.t.nrec[j];  // {p, m, rho, T}
-        const float Vj = rec.y / rec.z;
-        as_all += Vj * w;
-        if (phj != ph) as_other += Vj * w;
-        if (j == i) return;
-        const float cj = phj == kLiquid ? 1.0f : 0.0f;
-        agradc += gw * (Vj * (cj - ci));
-        if (phj == kLiquid) aakn += gw * Vj;
-      });
-      V3 gradc = agradc;
-      const V3 wall_n = awall_n, akn = aakn, gradphi = agradphi;
-      const float s_all = as_all, s_other = as_other, ww = aww, hw = ahw;
-      if (SPS) {
-        float ss = 0.0f;
-        for (int a = 0; a < 3; ++a)
-          for (int b = 0; b < 3; ++b) {
-            const float sab = 0.5f * (float(Lm[a][b]) + float(Lm[b][a]));
-            ss += sab * sab;
-          }
-        const float smag = sqrtf(2.0f * ss);
-        const float l = P.smag_cs * P.dx;
-        pv.t.nu_t[i] = l * l * smag;
-      }
-      pv.t.alpha[i] = s_all > 0.0f ? s_other / s_all : 0.0f;
-      pv.t.wall_w[i] = ww;
-      pv.t.heat_w[i] = hw;
-      heat_contrib = static_cast<double>(hw);
-      if (P.st_model == 2) {
-        pv.s.nrm[i] = st3(ph == kLiquid ? akn * (-P.kernel.h) : V3{0, 0, 0});
-      } else {
-        const float gmag = norm(gradc);
-        V3 vwi{0, 0, 0};
-        if (P.wetting != 0 && ww > 0.0f && gmag > P.normal_eps && norm2(wall_n) > 0.0f) {
-          const V3 nw = normalized(wall_n);
-          const V3 nh = gradc / gmag;
-          const V3 tan_in = nh - nw * dot(nh, nw);
-          const float tm = norm(tan_in);
-          if (tm > 1e-6f) {
-            const float cos_d = -dot(nh, nw);
-            const float delta_w = 2.0f * norm(gradphi);
-            const float coeff = 2.0f * P.sigma / (P.rho0[0] + P.rho0[1]);
-            vwi = tan_in * (-(P.cos_theta - cos_d) * coeff * gmag * delta_w / tm);
-          }
-        }
-        pv.t.vw[i] = st3(vwi);
-        if (P.wetting != 1 && ww > 0.0f && gmag > P.normal_eps) {
-          const V3 nw = normalized(wall_n);
-          const V3 nh = gradc / gmag;
-          V3 t = nh - nw * dot(nh, nw);
-          const float tmag = norm(t);
-          if (norm2(nw) > 0.0f && tmag > 1e-6f) {
-            t = t / tmag;
-            const V3 target = nw * (-P.cos_theta) + t * P.sin_theta;
-            const float beta = clampv(ww / P.wall_adh_w, 0.0f, 1.0f);
-            const V3 blended = normalized(nh * (1.0f - beta) + target * beta);
-            gradc = blended * gmag;
-          }
-        }
-        pv.s.nrm[i] = st3(gradc);
-        // smooth_interface_normals without smoothing (surface_tension.cu:87-118)
-        if (csf_mode && !P.csf_smooth) {
-          const float gm = norm(gradc);
-          pv.t.ns[i] = st3(!(gm > P.normal_eps) ? V3{0, 0, 0} : gradc / gm);
-        }
-      }
-    }
+  if (i < n && (!only_overflow || (L.valid && L.count[i] < 0))) {
+    const V3 xi = ld3(pv.s.x[i]);
+    auto gathered = [&](auto&& f) {
+      for_each_pair(L, g, i, xi, [&](int j) { f(j, load_iface_nbr<SPS>(pv, P, j)); });
+    };
+    heat_contrib = static_cast<double>(interface_particle<SPS, KS>(pv, P, i, csf_mode, gathered));
   }
   if (heat_partials) block_sum_store(heat_contrib, heat_partials + blockIdx.x);
 }
@@ -767,16 +803,37 @@ __global__ __launch_bounds__(kTileThreads) void k_forces_tiled(PV pv, GridDev g,
     const int cnt = L.count[i];
     if (cnt < 0) continue;  // the sweep instance handles it
     if (staged) {
+      // Four neighbours at a time: their LDS records are read into registers
+      // (occupancy is set by LDS here, so registers are free) before the four
+      // pair bodies run, and the next four list entries are in flight.
       auto from_lds = [&](auto&& f) {
         int r = 0;
-        for_each_listed(L, i, cnt, [&](int j) {
-          while (r < T.nrows - 1 && j >= T.start[r] + (T.base[r + 1] - T.base[r])) ++r;
-          const int s = T.base[r] + (j - T.start[r]);
-          const float4 xs = s_x[s], rs = s_rec[s], vs = s_v[s];
-          float4 vts = vs;
-          if (need_vt) vts = s_vt[s];
-          f(j, NbrData{xs, __float_as_int(xs.w), rs, vs, vts});
-        });
+        const int4* lp = reinterpret_cast<const int4*>(L.idx + static_cast<long long>(i) * L.K);
+        const int nq = (cnt + 3) >> 2;
+        int4 cur = nq > 0 ? lp[0] : make_int4(0, 0, 0, 0);
+        for (int q = 0; q < nq; ++q) {
+          int4 nxt = cur;
+          if (q + 1 < nq) nxt = lp[q + 1];
+          const int m = cnt - (q << 2);
+          const int js[4] = {cur.x, cur.y, cur.z, cur.w};
+          NbrData d[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            if (u < m) {
+              const int j = js[u];
+              while (r < T.nrows - 1 && j >= T.start[r] + (T.base[r + 1] - T.base[r])) ++r;
+              const int s = T.base[r] + (j - T.start[r]);
+              const float4 xs = s_x[s], rs = s_rec[s], vs = s_v[s];
+              float4 vts = vs;
+              if (need_vt) vts = s_vt[s];
+              d[u] = NbrData{xs, __float_as_int(xs.w), rs, vs, vts};
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (u < m) f(js[u], d[u]);
+          cur = nxt;
+        }
       };
       forces_particle<ST, CONT, THERMAL, KS>(pv, P, st, i, from_lds, mx);
     } else {
@@ -787,6 +844,76 @@ __global__ __launch_bounds__(kTileThreads) void k_forces_tiled(PV pv, GridDev g,
     }
   }
   publish_maxima(st, mx);
+}
+
+// ---------------------------------------------- interface, LDS-staged tiles
+// Same tiles and row tables as k_forces_tiled.  Staged per particle: x with
+// phase and flags bits, and {m/rho} (fluid) or {nrm, alpha} (wall): 32 B, so
+// two workgroups fit a CU (48 B and one workgroup with SPS).  Wall normals
+// and colours are staged for walls only: the pass writes the fluid ones.
+// Writes heat_w, not its sum (k_heat_partials sums it).
+template <bool SPS, bool KS>
+__global__ __launch_bounds__(kTileThreads) void k_interface_tiled(PV pv, GridDev g, SimParams P, NbrList L,
+                                                                DevState* st, int csf_mode) {
+  __shared__ TileRows T;
+  __shared__ float4 s_x[kTileCap], s_r[kTileCap], s_v[SPS ? kTileCap : 1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    for (int k = 0; k < 5; ++k) st->max_bits[k] = 0u;
+  }
+  tile_rows(g, blockIdx.x, T);
+  const int ntarget = T.tbase[T.ntrows];
+  if (ntarget == 0) return;  // uniform over the block
+  const int M = T.base[T.nrows];
+  const bool staged = M <= kTileCap;
+  if (staged) {
+    for (int s = threadIdx.x; s < M; s += kTileThreads) {
+      int lo = 0, hi = T.nrows - 1;  // last row with base <= s
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (T.base[mid] <= s) lo = mid;
+        else hi = mid - 1;
+      }
+      const IfaceNbr d = load_iface_nbr<SPS>(pv, P, T.start[lo] + (s - T.base[lo]));
+      s_x[s] = d.x;
+      s_r[s] = d.r;
+      if (SPS) s_v[s] = d.v;
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < ntarget; t += kTileThreads) {
+    int k = 0;
+    while (t >= T.tbase[k + 1]) ++k;
+    const int i = T.tstart[k] + (t - T.tbase[k]);
+    const int cnt = L.count[i];
+    if (cnt < 0) continue;  // the overflow instance of k_interface handles it
+    if (staged) {
+      auto from_lds = [&](auto&& f) {
+        int r = 0;
+        for_each_listed(L, i, cnt, [&](int j) {
+          while (r < T.nrows - 1 && j >= T.start[r] + (T.base[r + 1] - T.base[r])) ++r;
+          const int s = T.base[r] + (j - T.start[r]);
+          const float4 xs = s_x[s], rs = s_r[s];
+          float4 vs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          if (SPS) vs = s_v[s];
+          f(j, IfaceNbr{xs, rs, vs});
+        });
+      };
+      interface_particle<SPS, KS>(pv, P, i, csf_mode, from_lds);
+    } else {
+      auto gathered = [&](auto&& f) {
+        for_each_listed(L, i, cnt, [&](int j) { f(j, load_iface_nbr<SPS>(pv, P, j)); });
+      };
+      interface_particle<SPS, KS>(pv, P, i, csf_mode, gathered);
+    }
+  }
+}
+
+// FP64 block partials of sum_fluid heat_w (sph_solver.cu:101-104).
+__global__ __launch_bounds__(kBlock) void k_heat_partials(PV pv, int n, double* partials) {
+  const int i = gtid();
+  double v = 0.0;
+  if (i < n && is_fluid(pv.s.phase[i])) v = static_cast<double>(pv.t.heat_w[i]);
+  block_sum_store(v, partials + blockIdx.x);
 }
 
 // Publishes the maxima (spacing ratio starts from 1, sph_solver.cu:62-71).
@@ -901,16 +1028,32 @@ void launch_wall_bc(const PV& pv, const GridDev& g, const SimParams& P, const Nb
   if (P.compensated) k_wall_bc<true><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, st, n);
   else k_wall_bc<false><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, st, n);
 }
+template <bool SPS, bool KS>
+static void interface_variant(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st,
+                              int n, int cm, double* heat_partials, bool tiled, hipStream_t s) {
+  const int nb = blocks_for(n > 0 ? n : 1);
+  if (!tiled) {
+    k_interface<SPS, KS><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, n, cm, heat_partials, 0);
+    return;
+  }
+  const long long tiles = static_cast<long long>((g.nx + kTileCells - 1) / kTileCells) *
+                          ((g.ny + kTileCells - 1) / kTileCells) * ((g.nz + kTileCells - 1) / kTileCells);
+  if (tiles > 0x7fffffffLL) throw std::runtime_error("interface pass: too many cell tiles");
+  k_interface_tiled<SPS, KS><<<static_cast<int>(tiles), kTileThreads, 0, s>>>(pv, g, P, L, st, cm);
+  k_interface<SPS, KS><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, n, cm, nullptr, 1);
+  if (heat_partials) k_heat_partials<<<nb, kBlock, 0, s>>>(pv, n, heat_partials);
+}
 int launch_interface(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, int n,
-                     bool csf_mode, double* heat_partials, hipStream_t s) {
+                     bool csf_mode, double* heat_partials, bool tiled, hipStream_t s) {
   const int nb = blocks_for(n > 0 ? n : 1);
   const int cm = csf_mode ? 1 : 0;
+  tiled = tiled && L.valid;
   if (P.compensated) {
-    if (P.sps) k_interface<true, true><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, n, cm, heat_partials);
-    else k_interface<false, true><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, n, cm, heat_partials);
+    if (P.sps) interface_variant<true, true>(pv, g, P, L, st, n, cm, heat_partials, tiled, s);
+    else interface_variant<false, true>(pv, g, P, L, st, n, cm, heat_partials, tiled, s);
   } else {
-    if (P.sps) k_interface<true, false><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, n, cm, heat_partials);
-    else k_interface<false, false><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, n, cm, heat_partials);
+    if (P.sps) interface_variant<true, false>(pv, g, P, L, st, n, cm, heat_partials, tiled, s);
+    else interface_variant<false, false>(pv, g, P, L, st, n, cm, heat_partials, tiled, s);
   }
   return nb;
 }

@@ -734,7 +734,8 @@ struct cryo_solver {
     prof_end(4);
     prof_begin(5);
     const int nb =
-        launch_interface(pv, gd, P, L, state.get(), n, csf_mode, heat_sum ? partials.get() : nullptr, stream);
+        launch_interface(pv, gd, P, L, state.get(), n, csf_mode, heat_sum ? partials.get() : nullptr,
+                         tiles_ok && use_tiles, stream);
     if (csf_mode && P.csf_smooth) launch_smooth_normals(pv, gd, P, L, n, stream);
     if (P.thermal) launch_heat_norm(state.get(), partials.get(), nb, heat_sum, stream);
     prof_end(5);
