@@ -767,11 +767,48 @@ __device__ inline void tile_rows(const GridDev& g, int b, TileRows& T) {
   __syncthreads();
 }
 
+// One round of a tile's targets (round * kTileThreads + threadIdx.x): the
+// fluid particles with a complete list go, in target order, to
+// work[0, returned count); `other(i)` is called for the rest with a complete
+// list (non-fluid: their zero outputs).  Overflowed lists are skipped (their
+// own instance handles them).  Block-wide: every thread calls it.
+template <class Other>
+__device__ inline int tile_round_fluid(const PV& pv, const NbrList& L, const TileRows& T, int ntarget, int round,
+                                       int* work, int* wave_tot, Other&& other) {
+  const int t = round * kTileThreads + static_cast<int>(threadIdx.x);
+  int i = -1;
+  bool keep = false;
+  if (t < ntarget) {
+    int k = 0;
+    while (t >= T.tbase[k + 1]) ++k;
+    i = T.tstart[k] + (t - T.tbase[k]);
+    if (L.count[i] >= 0) {
+      if (is_fluid(pv.s.phase[i])) keep = true;
+      else other(i);
+    }
+  }
+  const unsigned long long mask = __ballot(keep);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  if (lane == 0) wave_tot[wave] = __popcll(mask);
+  __syncthreads();
+  int base = 0, total = 0;
+  for (int w = 0; w < kTileThreads / 64; ++w) {
+    const int c = wave_tot[w];
+    base += w < wave ? c : 0;
+    total += c;
+  }
+  if (keep) work[base + __popcll(mask & lt)] = i;
+  __syncthreads();
+  return total;
+}
+
 template <int ST, bool CONT, bool THERMAL, bool KS>
 __global__ __launch_bounds__(kTileThreads) void k_forces_tiled(PV pv, GridDev g, SimParams P, NbrList L,
                                                              DevState* st) {
   __shared__ TileRows T;
   __shared__ float4 s_x[kTileCap], s_rec[kTileCap], s_v[kTileCap], s_vt[kTileCap];
+  __shared__ int s_work[kTileThreads], s_wave[kTileThreads / 64];
   tile_rows(g, blockIdx.x, T);
   const int ntarget = T.tbase[T.ntrows];
   if (ntarget == 0) return;  // uniform over the block
@@ -796,12 +833,16 @@ __global__ __launch_bounds__(kTileThreads) void k_forces_tiled(PV pv, GridDev g,
   }
   __syncthreads();
   Maxima mx;
-  for (int t = threadIdx.x; t < ntarget; t += kTileThreads) {
-    int k = 0;
-    while (t >= T.tbase[k + 1]) ++k;
-    const int i = T.tstart[k] + (t - T.tbase[k]);
+  // Fluid targets packed onto consecutive threads (walls only write zeros):
+  // no lane idles through a pair loop for a wall.
+  auto none = [](auto&&) {};
+  for (int round = 0; round * kTileThreads < ntarget; ++round) {
+    const int nfl = tile_round_fluid(pv, L, T, ntarget, round, s_work, s_wave, [&](int i) {
+      forces_particle<ST, CONT, THERMAL, KS>(pv, P, st, i, none, mx);
+    });
+    if (static_cast<int>(threadIdx.x) >= nfl) continue;
+    const int i = s_work[threadIdx.x];
     const int cnt = L.count[i];
-    if (cnt < 0) continue;  // the sweep instance handles it
     if (staged) {
       // Four neighbours at a time: their LDS records are read into registers
       // (occupancy is set by LDS here, so registers are free) before the four
@@ -857,6 +898,7 @@ __global__ __launch_bounds__(kTileThreads) void k_interface_tiled(PV pv, GridDev
                                                                 DevState* st, int csf_mode) {
   __shared__ TileRows T;
   __shared__ float4 s_x[kTileCap], s_r[kTileCap], s_v[SPS ? kTileCap : 1];
+  __shared__ int s_work[kTileThreads], s_wave[kTileThreads / 64];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     for (int k = 0; k < 5; ++k) st->max_bits[k] = 0u;
   }
@@ -880,12 +922,14 @@ __global__ __launch_bounds__(kTileThreads) void k_interface_tiled(PV pv, GridDev
     }
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < ntarget; t += kTileThreads) {
-    int k = 0;
-    while (t >= T.tbase[k + 1]) ++k;
-    const int i = T.tstart[k] + (t - T.tbase[k]);
+  auto none = [](auto&&) {};
+  for (int round = 0; round * kTileThreads < ntarget; ++round) {
+    const int nfl = tile_round_fluid(pv, L, T, ntarget, round, s_work, s_wave, [&](int i) {
+      interface_particle<SPS, KS>(pv, P, i, csf_mode, none);
+    });
+    if (static_cast<int>(threadIdx.x) >= nfl) continue;
+    const int i = s_work[threadIdx.x];
     const int cnt = L.count[i];
-    if (cnt < 0) continue;  // the overflow instance of k_interface handles it
     if (staged) {
       auto from_lds = [&](auto&& f) {
         int r = 0;
