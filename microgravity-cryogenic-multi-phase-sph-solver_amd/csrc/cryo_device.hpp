@@ -407,18 +407,28 @@ __device__ inline void for_each_cand_x(const GridDev& g, V3 p, float radius, con
 // Built once per step by the density pass from the same 27-cell sweep, in
 // the same ascending order, holding every j with r^2 < (2h)^2 (fluid targets:
 // all particles, self included; wall targets: fluid particles only).
-// Particle i owns the K consecutive entries idx[i * K, i * K + K), K a
-// multiple of 4, so its list is read and written in 16-byte chunks: a pair
-// loop waits for memory once per four neighbours.  count[i] = -1 marks an
-// overflow (more than K neighbours): that particle's passes fall back to the
-// cell sweep, which visits the same j in the same order.
+// Layout: 16-byte chunks of four entries, interleaved over groups of 64
+// particles: chunk q of particle i sits at int4 index
+// ((i / 64) * (K / 4) + q) * 64 + i % 64, so when the 64 lanes of a wave
+// read (or write) chunk q of 64 neighbouring particles they touch one
+// contiguous kilobyte, and a pair loop waits for memory once per four
+// neighbours.  K is a multiple of 4; the buffer holds capacity rounded up to
+// a multiple of 64 particles.  count[i] = -1 marks an overflow (more than K
+// neighbours): that particle's passes fall back to the cell sweep, which
+// visits the same j in the same order.
 struct NbrList {
   int* idx;
   int* count;
-  int stride;  // unused (layout is particle-major)
+  int stride;  // unused (the layout above needs only K)
   int K;
   int valid;  // 0: positions or cells changed since the lists were built
 };
+constexpr int kListGroup = 64;  // particles whose chunks interleave
+// Chunk 0 of particle i; chunk q is at [q * kListGroup].
+__host__ __device__ inline int4* list_chunks(const NbrList& L, int i) {
+  const long long g = i / kListGroup;
+  return reinterpret_cast<int4*>(L.idx) + (g * (L.K / 4)) * kListGroup + (i % kListGroup);
+}
 
 // Appends one entry of particle i's list, staging four in registers.
 struct ListWriter {
@@ -426,19 +436,19 @@ struct ListWriter {
   int4 buf = make_int4(0, 0, 0, 0);
   int c = 0;
   int K;
-  __device__ ListWriter(const NbrList& L, int i) : out(reinterpret_cast<int4*>(L.idx + static_cast<long long>(i) * L.K)), K(L.K) {}
+  __device__ ListWriter(const NbrList& L, int i) : out(list_chunks(L, i)), K(L.K) {}
   __device__ inline void push(int j) {
     const int sl = c & 3;  // selects, not a switch: no divergent branches
     buf.x = sl == 0 ? j : buf.x;
     buf.y = sl == 1 ? j : buf.y;
     buf.z = sl == 2 ? j : buf.z;
     buf.w = sl == 3 ? j : buf.w;
-    if (sl == 3 && c < K) out[c >> 2] = buf;
+    if (sl == 3 && c < K) out[(c >> 2) * kListGroup] = buf;
     ++c;
   }
   // Writes the last partial chunk; returns the count to store (-1: overflow).
   __device__ inline int finish() {
-    if ((c & 3) != 0 && c < K) out[c >> 2] = buf;
+    if ((c & 3) != 0 && c < K) out[(c >> 2) * kListGroup] = buf;
     return c <= K ? c : -1;
   }
 };
@@ -447,7 +457,7 @@ struct ListWriter {
 // of four is in flight while the current one is processed.
 template <class F>
 __device__ inline void for_each_listed(const NbrList& L, int i, int c, F&& f) {
-  const int4* p = reinterpret_cast<const int4*>(L.idx + static_cast<long long>(i) * L.K);
+  const int4* p = list_chunks(L, i);
   const int nq = (c + 3) >> 2;
   int j0 = 0, j1 = 0, j2 = 0, j3 = 0;
   if (nq > 0) {
@@ -457,7 +467,7 @@ __device__ inline void for_each_listed(const NbrList& L, int i, int c, F&& f) {
   for (int q = 0; q < nq; ++q) {
     int n0 = j0, n1 = j1, n2 = j2, n3 = j3;
     if (q + 1 < nq) {
-      const int4 t = p[q + 1];
+      const int4 t = p[(q + 1) * kListGroup];
       n0 = t.x, n1 = t.y, n2 = t.z, n3 = t.w;
     }
     const int m = c - (q << 2) < 4 ? c - (q << 2) : 4;

@@ -767,6 +767,20 @@ __device__ inline void tile_rows(const GridDev& g, int b, TileRows& T) {
   __syncthreads();
 }
 
+// Sorted index of LDS slot s of a tile (s < the tile's staged count).
+__device__ inline int tile_slot_index(const TileRows& T, int s) {
+  int lo = 0, hi = T.nrows - 1;  // last row with base <= s
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (T.base[mid] <= s) lo = mid;
+    else hi = mid - 1;
+  }
+  return T.start[lo] + (s - T.base[lo]);
+}
+// Slots a thread stages: every load of all of them is issued before the
+// first one is waited for (one memory round trip per staging pass).
+constexpr int kStageSlots = (kTileCap + kTileThreads - 1) / kTileThreads;
+
 // One round of a tile's targets (round * kTileThreads + threadIdx.x): the
 // fluid particles with a complete list go, in target order, to
 // work[0, returned count); `other(i)` is called for the rest with a complete
@@ -815,31 +829,58 @@ __global__ __launch_bounds__(kTileThreads) void k_forces_tiled(PV pv, GridDev g,
   const int M = T.base[T.nrows];
   const bool staged = M <= kTileCap;
   const bool need_vt = forces_need_vt(P, CONT);
-  if (staged) {
-    for (int s = threadIdx.x; s < M; s += kTileThreads) {
-      int lo = 0, hi = T.nrows - 1;  // last row with base <= s
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (T.base[mid] <= s) lo = mid;
-        else hi = mid - 1;
-      }
-      const int j = T.start[lo] + (s - T.base[lo]);
-      const NbrData d = load_nbr(pv, P, j, need_vt);
-      s_x[s] = make_float4(d.x.x, d.x.y, d.x.z, __int_as_float(d.ph));
-      s_rec[s] = d.rec;
-      s_v[s] = d.v;
-      if (need_vt) s_vt[s] = d.vt;
-    }
-  }
-  __syncthreads();
   Maxima mx;
   // Fluid targets packed onto consecutive threads (walls only write zeros):
-  // no lane idles through a pair loop for a wall.
+  // no lane idles through a pair loop for a wall.  The first round is
+  // packed before staging, so a tile of walls alone stages nothing.
   auto none = [](auto&&) {};
+  auto other = [&](int i) { forces_particle<ST, CONT, THERMAL, KS>(pv, P, st, i, none, mx); };
+  const int nfl0 = tile_round_fluid(pv, L, T, ntarget, 0, s_work, s_wave, other);
+  if (nfl0 == 0 && ntarget <= kTileThreads) return;  // uniform over the block
+  if (staged) {
+    // Named slots, not arrays (arrays of them are not kept in registers).
+    struct Slot {
+      float4 x, r, v, t;
+      int ph;
+    };
+    auto load = [&](int u) {
+      Slot q{};
+      const int s = threadIdx.x + u * kTileThreads;
+      if (s < M) {
+        const int j = tile_slot_index(T, s);
+        q.x = pv.s.x[j];
+        q.ph = pv.s.phase[j];
+        q.r = pv.t.nrec[j];
+        const float4 v = pv.s.v[j];
+        float4 w = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (P.no_slip) w = pv.t.vw[j];
+        float4 t = v;
+        if (need_vt) t = pv.s.vt[j];
+        q.v = q.ph == kWall ? w : v;
+        q.t = t;
+      }
+      return q;
+    };
+    auto store = [&](int u, const Slot q) {
+      const int s = threadIdx.x + u * kTileThreads;
+      if (s < M) {
+        s_x[s] = make_float4(q.x.x, q.x.y, q.x.z, __int_as_float(q.ph));
+        s_rec[s] = q.r;
+        s_v[s] = q.v;
+        if (need_vt) s_vt[s] = q.t;
+      }
+    };
+    static_assert(kStageSlots == 5, "staging is written out for five slots per thread");
+    const Slot q0 = load(0), q1 = load(1), q2 = load(2), q3 = load(3), q4 = load(4);
+    store(0, q0);
+    store(1, q1);
+    store(2, q2);
+    store(3, q3);
+    store(4, q4);
+  }
+  __syncthreads();
   for (int round = 0; round * kTileThreads < ntarget; ++round) {
-    const int nfl = tile_round_fluid(pv, L, T, ntarget, round, s_work, s_wave, [&](int i) {
-      forces_particle<ST, CONT, THERMAL, KS>(pv, P, st, i, none, mx);
-    });
+    const int nfl = round == 0 ? nfl0 : tile_round_fluid(pv, L, T, ntarget, round, s_work, s_wave, other);
     if (static_cast<int>(threadIdx.x) >= nfl) continue;
     const int i = s_work[threadIdx.x];
     const int cnt = L.count[i];
@@ -849,12 +890,12 @@ __global__ __launch_bounds__(kTileThreads) void k_forces_tiled(PV pv, GridDev g,
       // pair bodies run, and the next four list entries are in flight.
       auto from_lds = [&](auto&& f) {
         int r = 0;
-        const int4* lp = reinterpret_cast<const int4*>(L.idx + static_cast<long long>(i) * L.K);
+        const int4* lp = list_chunks(L, i);
         const int nq = (cnt + 3) >> 2;
         int4 cur = nq > 0 ? lp[0] : make_int4(0, 0, 0, 0);
         for (int q = 0; q < nq; ++q) {
           int4 nxt = cur;
-          if (q + 1 < nq) nxt = lp[q + 1];
+          if (q + 1 < nq) nxt = lp[(q + 1) * kListGroup];
           const int m = cnt - (q << 2);
           const int js[4] = {cur.x, cur.y, cur.z, cur.w};
           NbrData d[4];
@@ -907,26 +948,69 @@ __global__ __launch_bounds__(kTileThreads) void k_interface_tiled(PV pv, GridDev
   if (ntarget == 0) return;  // uniform over the block
   const int M = T.base[T.nrows];
   const bool staged = M <= kTileCap;
+  // the first round's fluid targets are packed before staging: a tile of
+  // walls alone stages nothing
+  auto none = [](auto&&) {};
+  auto other = [&](int i) { interface_particle<SPS, KS>(pv, P, i, csf_mode, none); };
+  const int nfl0 = tile_round_fluid(pv, L, T, ntarget, 0, s_work, s_wave, other);
+  if (nfl0 == 0 && ntarget <= kTileThreads) return;  // uniform over the block
   if (staged) {
-    for (int s = threadIdx.x; s < M; s += kTileThreads) {
-      int lo = 0, hi = T.nrows - 1;  // last row with base <= s
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (T.base[mid] <= s) lo = mid;
-        else hi = mid - 1;
+    // Named slots, not arrays (arrays of them are not kept in registers).
+    // First pass: position, phase, flags; second: the record the phase
+    // selects (a wall's normal and colour, a fluid particle's m and rho).
+    struct Slot {
+      float4 x, r, v;
+      int j, bits;
+      float a;
+    };
+    auto load1 = [&](int u) {
+      Slot q{};
+      const int s = threadIdx.x + u * kTileThreads;
+      if (s < M) {
+        q.j = tile_slot_index(T, s);
+        q.x = pv.s.x[q.j];
+        q.bits = static_cast<int>(pv.s.phase[q.j]) | (static_cast<int>(pv.s.flags[q.j]) << 8);
       }
-      const IfaceNbr d = load_iface_nbr<SPS>(pv, P, T.start[lo] + (s - T.base[lo]));
-      s_x[s] = d.x;
-      s_r[s] = d.r;
-      if (SPS) s_v[s] = d.v;
-    }
+      return q;
+    };
+    auto load2 = [&](int u, Slot q) {
+      const int s = threadIdx.x + u * kTileThreads;
+      if (s < M) {
+        const bool wall = (q.bits & 0xff) == kWall;
+        // (fluid normals and colours are being written by this pass)
+        const float4* src = wall ? pv.s.nrm : pv.t.nrec;  // two global arrays: a pointer select is fine
+        q.r = src[q.j];
+        q.a = (wall && P.wall_colour) ? pv.t.alpha[q.j] : 0.0f;
+        q.v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (SPS && (!wall || P.no_slip)) q.v = (wall ? pv.t.vw : pv.s.v)[q.j];
+      }
+      return q;
+    };
+    auto store = [&](int u, const Slot q) {
+      const int s = threadIdx.x + u * kTileThreads;
+      if (s < M) {
+        const bool wall = (q.bits & 0xff) == kWall;
+        s_x[s] = make_float4(q.x.x, q.x.y, q.x.z, __int_as_float(q.bits));
+        s_r[s] = wall ? make_float4(q.r.x, q.r.y, q.r.z, q.a) : make_float4(q.r.y / q.r.z, 0.0f, 0.0f, 0.0f);
+        if (SPS) s_v[s] = q.v;
+      }
+    };
+    static_assert(kStageSlots == 5, "staging is written out for five slots per thread");
+    Slot q0 = load1(0), q1 = load1(1), q2 = load1(2), q3 = load1(3), q4 = load1(4);
+    q0 = load2(0, q0);
+    q1 = load2(1, q1);
+    q2 = load2(2, q2);
+    q3 = load2(3, q3);
+    q4 = load2(4, q4);
+    store(0, q0);
+    store(1, q1);
+    store(2, q2);
+    store(3, q3);
+    store(4, q4);
   }
   __syncthreads();
-  auto none = [](auto&&) {};
   for (int round = 0; round * kTileThreads < ntarget; ++round) {
-    const int nfl = tile_round_fluid(pv, L, T, ntarget, round, s_work, s_wave, [&](int i) {
-      interface_particle<SPS, KS>(pv, P, i, csf_mode, none);
-    });
+    const int nfl = round == 0 ? nfl0 : tile_round_fluid(pv, L, T, ntarget, round, s_work, s_wave, other);
     if (static_cast<int>(threadIdx.x) >= nfl) continue;
     const int i = s_work[threadIdx.x];
     const int cnt = L.count[i];

@@ -428,7 +428,8 @@ struct cryo_solver {
     split_off.resize(cap);
     scan_sums.resize(static_cast<std::size_t>((cap + 1023) / 1024) + 1);
     scan_total.resize(1);
-    nbr_idx.resize(static_cast<std::size_t>(kListK) * cap);
+    // list chunks interleave over groups of 64 particles: whole groups
+    nbr_idx.resize(static_cast<std::size_t>(kListK) * ((static_cast<std::size_t>(cap) + 63) / 64 * 64));
     nbr_count.resize(cap);
     lists_valid = false;
     stride = (cap + 255) / 256 + 1;
