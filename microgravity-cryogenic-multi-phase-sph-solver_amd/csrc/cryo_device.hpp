@@ -489,6 +489,33 @@ __device__ inline void for_each_pair(const NbrList& L, const GridDev& g, int i, 
   }
 }
 
+// for_each_pair with the neighbours' data gathered four at a time:
+// load(j) returns a struct of what the pair needs (all loads issued before
+// any is used), f(j, data) consumes it, in the same order.  The sweep
+// fallback (count < 0 or invalid lists) loads one at a time.
+template <class Load, class F>
+__device__ inline void for_each_pair4(const NbrList& L, const GridDev& g, int i, V3 xi, Load&& load, F&& f) {
+  const int c = L.valid ? L.count[i] : -1;
+  if (c < 0) {
+    for_each_nbr(g, xi, [&](int j) { f(j, load(j)); });
+    return;
+  }
+  const int4* p = list_chunks(L, i);
+  const int nq = (c + 3) >> 2;
+  for (int q = 0; q < nq; ++q) {
+    const int4 t = p[q * kListGroup];
+    const int m = c - (q << 2);
+    const auto da = load(t.x);
+    const auto db = load(m > 1 ? t.y : t.x);
+    const auto dc = load(m > 2 ? t.z : t.x);
+    const auto dd = load(m > 3 ? t.w : t.x);
+    f(t.x, da);
+    if (m > 1) f(t.y, db);
+    if (m > 2) f(t.z, dc);
+    if (m > 3) f(t.w, dd);
+  }
+}
+
 // Holds the listed values in registers here: every load issued before this
 // point completes here, together, instead of each at its first use.
 __device__ inline void settle1(float v) { asm volatile("" ::"v"(v)); }

@@ -128,23 +128,29 @@ __global__ __launch_bounds__(kBlock) void k_wall_bc(PV pv, GridDev g, SimParams 
   const V3 xi = ld3(pv.s.x[i]);
   Acc<KS> asw[2], asp[2], asgr[2], srho;
   AccV3<KS> sv;
-  for_each_pair(L, g, i, xi, [&](int j) {
-    const uint8_t phj = pv.s.phase[j];
-    if (!is_fluid(phj)) return;
-    const V3 rwf = xi - ld3(pv.s.x[j]);
+  struct Nb {
+    float4 x, v;
+    float rho, p;
+    int ph;
+  };
+  auto load = [&](int j) { return Nb{pv.s.x[j], pv.s.v[j], pv.s.rho[j], pv.t.p[j], pv.s.phase[j]}; };
+  for_each_pair4(L, g, i, xi, load, [&](int, const Nb d) {
+    const int phj = d.ph;
+    if (!is_fluid(static_cast<uint8_t>(phj))) return;
+    const V3 rwf = xi - ld3(d.x);
     const float r2 = norm2(rwf);
     if (r2 >= P.support2) return;
     const float w = kW(P.kernel, sqrtf(r2));
-    const float rhoj = pv.s.rho[j];
-    const int k = pidx(phj);
+    const float rhoj = d.rho;
+    const int k = pidx(static_cast<uint8_t>(phj));
     for (int q = 0; q < 2; ++q) {  // constant indices: the sums stay in registers
       if (q != k) continue;
       asw[q] += w;
-      asp[q] += pv.t.p[j] * w;
+      asp[q] += d.p * w;
       asgr[q] += dot(grav, rwf) * rhoj * w;
     }
     srho += rhoj * w;
-    sv += ld3(pv.s.v[j]) * w;
+    sv += ld3(d.v) * w;
   });
   const float sw[2] = {float(asw[0]), float(asw[1])};
   const float sp[2] = {float(asp[0]), float(asp[1])}, sgr[2] = {float(asgr[0]), float(asgr[1])};

@@ -198,13 +198,21 @@ __global__ __launch_bounds__(kBlock) void k_pc_normalize(PV pv, GridDev g, SimPa
       const V3 xi = ld3(pv.s.x[i]);
       Acc<KS> awsum;
       float wr_max = 0.0f;
-      for_each_pair(L, g, i, xi, [&](int j) {
-        if (j == i || pv.s.phase[j] != target || (pv.s.flags[j] & kDying)) return;
-        const float r2 = norm2(xi - ld3(pv.s.x[j]));
+      struct Nb {
+        float4 x;
+        float m, rho;
+        int ph, fl;
+      };
+      auto load = [&](int j) {
+        return Nb{pv.s.x[j], pv.s.m[j], pv.s.rho[j], pv.s.phase[j], pv.s.flags[j]};
+      };
+      for_each_pair4(L, g, i, xi, load, [&](int j, const Nb d) {
+        if (j == i || d.ph != target || (d.fl & kDying)) return;
+        const float r2 = norm2(xi - ld3(d.x));
         if (r2 >= P.support2) return;
         const float w = kW(P.kernel, sqrtf(r2));
-        const float rhoj = pv.s.rho[j];
-        awsum += (pv.s.m[j] / rhoj) * w;
+        const float rhoj = d.rho;
+        awsum += (d.m / rhoj) * w;
         const float wr = w / rhoj;
         wr_max = wr > wr_max ? wr : wr_max;
       });
@@ -244,17 +252,25 @@ __global__ __launch_bounds__(kBlock) void k_pc_apply(PV pv, GridDev g, SimParams
   const V3 xi = ld3(pv.s.x[i]);
   Acc<KS> ain_m, ain_H;
   if (!(fl & kDying)) {
-    for_each_pair(L, g, i, xi, [&](int j) {
+    struct Nb {
+      float4 x;
+      float om, oh, wn;
+      int fl;
+    };
+    auto load = [&](int j) {
+      return Nb{pv.s.x[j], pv.t.out_m[j], pv.t.out_h[j], pv.t.w_norm[j], pv.s.flags[j]};
+    };
+    for_each_pair4(L, g, i, xi, load, [&](int j, const Nb d) {
       if (j == i) return;
-      const float om = pv.t.out_m[j];
+      const float om = d.om;
       if (!(om > 0.0f)) return;
-      const uint8_t target = (pv.s.flags[j] & kTargetVapor) ? kVapor : kLiquid;
+      const uint8_t target = (d.fl & kTargetVapor) ? kVapor : kLiquid;
       if (target != ph) return;
-      const float r2 = norm2(xi - ld3(pv.s.x[j]));
+      const float r2 = norm2(xi - ld3(d.x));
       if (r2 >= P.support2) return;
-      const float share = om * Vi * kW(P.kernel, sqrtf(r2)) / pv.t.w_norm[j];
+      const float share = om * Vi * kW(P.kernel, sqrtf(r2)) / d.wn;
       ain_m += share;
-      ain_H += share * pv.t.out_h[j];
+      ain_H += share * d.oh;
     });
   }
   const float in_m = ain_m, in_H = ain_H;
@@ -287,15 +303,21 @@ __global__ __launch_bounds__(kBlock) void k_rebalance(PV pv, GridDev g, SimParam
   const V3 vi = ld3(pv.s.v[i]);
   Acc<KS> dm, dH;
   AccV3<KS> dP;
-  for_each_pair(L, g, i, xi, [&](int j) {
-    if (j == i || pv.s.phase[j] != kVapor) return;
-    const float r2 = norm2(xi - ld3(pv.s.x[j]));
+  struct Nb {
+    float4 x, v;
+    float m, T;
+    int ph;
+  };
+  auto load = [&](int j) { return Nb{pv.s.x[j], pv.s.v[j], pv.s.m[j], pv.s.T[j], pv.s.phase[j]}; };
+  for_each_pair4(L, g, i, xi, load, [&](int j, const Nb d) {
+    if (j == i || d.ph != kVapor) return;
+    const float r2 = norm2(xi - ld3(d.x));
     if (r2 >= P.support2) return;
-    const float flow = beta * P.vol0 * kW(P.kernel, sqrtf(r2)) * (pv.s.m[j] - mi);
+    const float flow = beta * P.vol0 * kW(P.kernel, sqrtf(r2)) * (d.m - mi);
     const bool from_j = flow > 0.0f;
     dm += flow;
-    dH += flow * (from_j ? enthalpy(P, kVapor, pv.s.T[j]) : hi);
-    dP += (from_j ? ld3(pv.s.v[j]) : vi) * flow;
+    dH += flow * (from_j ? enthalpy(P, kVapor, d.T) : hi);
+    dP += (from_j ? ld3(d.v) : vi) * flow;
   });
   pv.t.out_m[i] = float(dm);
   pv.t.out_h[i] = float(dH);
