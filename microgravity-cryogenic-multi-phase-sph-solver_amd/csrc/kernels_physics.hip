@@ -725,6 +725,8 @@ constexpr int kTileTargetRows = kTileCells * kTileCells;
 struct TileRows {
   int start[kTileRows];     // first sorted index of each staged row
   int base[kTileRows + 1];  // its first LDS slot (exclusive prefix of row lengths)
+  int rend[kTileRows];      // start + length: first sorted index after the row
+  int roff[kTileRows];      // base - start: slot = j + roff for j in the row
   int nrows;
   int tstart[kTileTargetRows];  // the tile's own rows (targets)
   int tbase[kTileTargetRows + 1];
@@ -763,7 +765,12 @@ __device__ inline void tile_rows(const GridDev& g, int b, TileRows& T) {
   __syncthreads();
   if (t == 0) {
     T.base[0] = 0;
-    for (int r = 0; r < nrows; ++r) T.base[r + 1] += T.base[r];
+    for (int r = 0; r < nrows; ++r) {
+      const int len = T.base[r + 1];
+      T.base[r + 1] += T.base[r];
+      T.rend[r] = T.start[r] + len;
+      T.roff[r] = T.base[r] - T.start[r];
+    }
     T.nrows = nrows;
   } else if (t == 64) {
     T.tbase[0] = 0;
@@ -772,6 +779,23 @@ __device__ inline void tile_rows(const GridDev& g, int b, TileRows& T) {
   }
   __syncthreads();
 }
+
+// Maps a particle's ascending list entries to LDS slots: the row of the
+// current entry is kept with its end and offset, and only an entry past the
+// row's end moves it forward.
+struct RowCursor {
+  const TileRows& T;
+  int r, rend, roff;
+  __device__ explicit RowCursor(const TileRows& t) : T(t), r(0), rend(t.rend[0]), roff(t.roff[0]) {}
+  __device__ inline int slot(int j) {
+    while (j >= rend && r < T.nrows - 1) {
+      ++r;
+      rend = T.rend[r];
+      roff = T.roff[r];
+    }
+    return j + roff;
+  }
+};
 
 // Sorted index of LDS slot s of a tile (s < the tile's staged count).
 __device__ inline int tile_slot_index(const TileRows& T, int s) {
@@ -895,7 +919,7 @@ __global__ __launch_bounds__(kTileThreads) void k_forces_tiled(PV pv, GridDev g,
       // (occupancy is set by LDS here, so registers are free) before the four
       // pair bodies run, and the next four list entries are in flight.
       auto from_lds = [&](auto&& f) {
-        int r = 0;
+        RowCursor rc(T);
         const int4* lp = list_chunks(L, i);
         const int nq = (cnt + 3) >> 2;
         int4 cur = nq > 0 ? lp[0] : make_int4(0, 0, 0, 0);
@@ -908,9 +932,7 @@ __global__ __launch_bounds__(kTileThreads) void k_forces_tiled(PV pv, GridDev g,
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             if (u < m) {
-              const int j = js[u];
-              while (r < T.nrows - 1 && j >= T.start[r] + (T.base[r + 1] - T.base[r])) ++r;
-              const int s = T.base[r] + (j - T.start[r]);
+              const int s = rc.slot(js[u]);
               const float4 xs = s_x[s], rs = s_rec[s], vs = s_v[s];
               float4 vts = vs;
               if (need_vt) vts = s_vt[s];
@@ -1022,10 +1044,9 @@ __global__ __launch_bounds__(kTileThreads) void k_interface_tiled(PV pv, GridDev
     const int cnt = L.count[i];
     if (staged) {
       auto from_lds = [&](auto&& f) {
-        int r = 0;
+        RowCursor rc(T);
         for_each_listed(L, i, cnt, [&](int j) {
-          while (r < T.nrows - 1 && j >= T.start[r] + (T.base[r + 1] - T.base[r])) ++r;
-          const int s = T.base[r] + (j - T.start[r]);
+          const int s = rc.slot(j);
           const float4 xs = s_x[s], rs = s_r[s];
           float4 vs = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
           if (SPS) vs = s_v[s];
