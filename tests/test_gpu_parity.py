@@ -150,6 +150,20 @@ SCENARIOS = {
     "shifting_tvf_stress": dict(cfg=dict(dx=0.005, particle_shifting=1, transport_stress=1, thermal=0,
                                          phase_change=0, surface_tension=1, velocity_scale=1.0),
                                 geo=lambda: cabi.box(0.05, 0.05, 0.05), fill=dict(), steps=25),
+    # h = 1.4 dx: the lists still fit (<= 96 neighbours) while more staged
+    # 5x5x5-cell blocks exceed an LDS tile (the tiles' global-memory path)
+    "wide_kernel_h14": dict(cfg=dict(dx=0.005, h_factor=1.4, thermo_mode=1, lee_evaporation=5.0,
+                                     lee_condensation=5.0, gravity=(0, 0, -0.3), velocity_scale=0.5),
+                            geo=lambda: cabi.box(0.05, 0.05, 0.05), fill=dict(T_liquid=79.0), steps=20),
+    # h = 1.5 dx: every fluid list and some wall lists overflow (> 96
+    # neighbours: the cell-sweep instances)
+    "wide_kernel_h15": dict(cfg=dict(dx=0.005, h_factor=1.5, thermo_mode=1, lee_evaporation=5.0,
+                                     lee_condensation=5.0, gravity=(0, 0, -0.3), velocity_scale=0.5),
+                            geo=lambda: cabi.box(0.05, 0.05, 0.05), fill=dict(T_liquid=79.0), steps=20),
+    # h = 1.7 dx: every fluid list overflows
+    "wide_kernel_h17": dict(cfg=dict(dx=0.005, h_factor=1.7, density_mode=1, surface_tension=0, thermal=0,
+                                     phase_change=0, gravity=(0, 0, -0.3), velocity_scale=0.5),
+                            geo=lambda: cabi.box(0.05, 0.05, 0.05), fill=dict(include_vapor=0), steps=20),
 }
 
 
