@@ -178,6 +178,24 @@ int cryo_solver_damp_velocities(cryo_solver* s, float factor);
  * may use a member during the call. */
 int cryo_ensemble_run_steps(cryo_solver* const* solvers, int32_t count, int64_t steps);
 
+/* Extension (no counterpart in the reference, which has no adjoint or autodiff):
+ * runs `steps` steps, step k with frame acceleration frame_acc[3k..3k+2]
+ * (g_eff = gravity - acc, as cryo_solver_set_frame_motion sets it), and returns
+ *   loss = sum_k dt (S_k.x^2 + S_k.y^2),  S_k = sum of fw over the fluid
+ *   particles after step k's rates (the negated slosh force),
+ * S_k in wall_load[3k..3k+2] (NULL: not returned) and, when grad_frame_acc is
+ * not NULL, d loss / d frame_acc by a hand-written reverse sweep that
+ * checkpoints every `checkpoint_every` steps and re-runs each segment.  The
+ * solver ends in the forward end state either way.  Differentiated: density
+ * summation, EOS, Shepard wall pressure and ghost velocity, momentum
+ * (pressure, viscosity, artificial viscosity, wall terms), kick-drift-kick.
+ * Requires fixed_dt > 0, the Wendland C2 kernel, density summation, no
+ * surface tension, thermal, phase change, transport velocity, SPS or
+ * shifting, and no frame rotation or manoeuvre table
+ * (CRYO_ERR_INVALID_ARGUMENT otherwise). */
+int cryo_solver_adjoint_slosh(cryo_solver* s, int64_t steps, const float* frame_acc, int32_t checkpoint_every,
+                              double* loss, double* grad_frame_acc, double* wall_load);
+
 /* Solver::time / step_count / last_dt / num_particles (sph_solver.hpp:153-163). */
 int cryo_solver_time(cryo_solver* s, double* t);
 int cryo_solver_step_count(cryo_solver* s, int64_t* step);

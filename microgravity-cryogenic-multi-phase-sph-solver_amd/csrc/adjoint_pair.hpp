@@ -1,0 +1,300 @@
+// Per-pair vector-Jacobian products of the hydrodynamic step: the pieces of the
+// reverse-mode adjoint of one step, written out by hand.
+//
+// The forward pair terms they differentiate are those of
+//   wall_boundary  (navier_stokes.cu:16-69)   Shepard wall pressure + ghost velocity
+//   momentum       (navier_stokes.cu:121-245) pressure, Morris viscosity, Monaghan
+//                                             artificial viscosity, wall pressure,
+//                                             near-wall guard, no-slip shear, wall AV
+//   density        (SPH_kernels.cu:6-34)      sigma = sum_j W(r_ij)
+// with the Wendland C2 kernel (SPH_kernels.cuh:14-64).  Each function returns
+// d(c . term)/d(inputs) for the cotangent c of the quantity the term adds to.
+//
+// Templated on the scalar type: float on the GPU, double in the FP64 CPU
+// adjoint that checks it.  Plain expressions only, so the same source builds
+// with hipcc (device and host) and with g++.
+#pragma once
+
+#include <cmath>
+
+#if defined(__HIPCC__)
+#define CRYO_ADJ_HD __host__ __device__
+#else
+#define CRYO_ADJ_HD
+#endif
+
+namespace cryo_adj {
+
+template <class T>
+struct Vec {
+  T x, y, z;
+};
+template <class T>
+CRYO_ADJ_HD inline Vec<T> operator+(Vec<T> a, Vec<T> b) {
+  return {a.x + b.x, a.y + b.y, a.z + b.z};
+}
+template <class T>
+CRYO_ADJ_HD inline Vec<T> operator-(Vec<T> a, Vec<T> b) {
+  return {a.x - b.x, a.y - b.y, a.z - b.z};
+}
+template <class T>
+CRYO_ADJ_HD inline Vec<T> operator*(Vec<T> a, T s) {
+  return {a.x * s, a.y * s, a.z * s};
+}
+template <class T>
+CRYO_ADJ_HD inline Vec<T>& operator+=(Vec<T>& a, Vec<T> b) {
+  a.x += b.x;
+  a.y += b.y;
+  a.z += b.z;
+  return a;
+}
+template <class T>
+CRYO_ADJ_HD inline Vec<T>& operator-=(Vec<T>& a, Vec<T> b) {
+  a.x -= b.x;
+  a.y -= b.y;
+  a.z -= b.z;
+  return a;
+}
+template <class T>
+CRYO_ADJ_HD inline T dot(Vec<T> a, Vec<T> b) {
+  return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+
+// Constants of one rates evaluation.
+template <class T>
+struct Ctx {
+  T norm, inv_h, h;  // Wendland C2: W = norm (1 - q/2)^4 (2q + 1), q = r / h
+  T support2;        // (2h)^2
+  T eta2;            // 0.01 h^2 (the AV / viscosity regulariser)
+  T r_guard;         // 0.6 dx
+  T dt;              // the step's dt (the guard's damping term)
+  T vol0;            // wall particle volume dx^3
+  int no_slip;
+};
+
+// W(r), s = W'(r) / r and s' / r = (ds/dr) / r for Wendland C2, r < 2h.
+template <class T>
+CRYO_ADJ_HD inline T wendland_w(const Ctx<T>& c, T r) {
+  const T q = r * c.inv_h;
+  const T t = T(1) - T(0.5) * q;
+  return c.norm * t * t * t * t * (T(2) * q + T(1));
+}
+template <class T>
+CRYO_ADJ_HD inline T wendland_s(const Ctx<T>& c, T r) {
+  const T t = T(1) - T(0.5) * r * c.inv_h;
+  return T(-5) * c.norm * c.inv_h * c.inv_h * t * t * t;
+}
+template <class T>
+CRYO_ADJ_HD inline T wendland_spr(const Ctx<T>& c, T r) {
+  const T t = T(1) - T(0.5) * r * c.inv_h;
+  return T(7.5) * c.norm * c.inv_h * c.inv_h * c.inv_h * t * t / r;
+}
+
+// One fluid particle as a member of a pair.
+template <class T>
+struct Side {
+  Vec<T> x, v;
+  T m, rho, p;
+  T c;      // sound speed
+  T eta;    // dynamic viscosity mu[phase]
+  T alpha;  // AV coefficient of its phase
+  int ph;
+};
+
+// Fluid-fluid term of momentum with owner I and neighbour J (j != i):
+//   a_I += (T1 + T2) / m_I + T3,
+//   T1 = -gw vsum pt,  T2 = v_IJ vsum eta_IJ s,  T3 = -gw m_J Pi (same phase, v_IJ.r_IJ < 0)
+// with gw = r_IJ s, vsum = V_I^2 + V_J^2, pt = (rho_J p_I + rho_I p_J) / (rho_I + rho_J).
+// cF is the cotangent of the force sum (a_I's cotangent / m_I), cA that of the
+// artificial-viscosity sum.  r is the cotangent of r_IJ = x_I - x_J, vij that
+// of v_IJ = v_I - v_J.
+template <class T>
+struct FFGrad {
+  Vec<T> r, vij;
+  T rho_i, rho_j, p_i, p_j;
+};
+template <class T>
+CRYO_ADJ_HD inline FFGrad<T> ff_vjp(const Ctx<T>& c, const Side<T>& I, const Side<T>& J, Vec<T> cF, Vec<T> cA) {
+  FFGrad<T> o{{T(0), T(0), T(0)}, {T(0), T(0), T(0)}, T(0), T(0), T(0), T(0)};
+  const Vec<T> rij = I.x - J.x;
+  const T r2 = dot(rij, rij);
+  if (!(r2 < c.support2)) return o;
+  const T r = std::sqrt(r2);
+  if (!(r > T(1e-12) * c.h)) return o;  // gw = 0 and the viscous term is skipped
+  const T s = wendland_s(c, r), spr = wendland_spr(c, r);
+  const T Vi = I.m / I.rho, Vj = J.m / J.rho;
+  const T vsum = Vi * Vi + Vj * Vj;
+  const T rs = I.rho + J.rho;
+  const T pt = (J.rho * I.p + I.rho * J.p) / rs;
+  const T cFr = dot(cF, rij);
+  const T G = s * cFr;  // cF . gw
+  // T1
+  o.r = (cF * s + rij * (cFr * spr)) * (-vsum * pt);
+  T vsum_b = -pt * G;
+  const T pt_b = -vsum * G;
+  // T2
+  const Vec<T> vij = I.v - J.v;
+  const T es = I.eta + J.eta;
+  if (es > T(0)) {
+    const T eta = T(2) * I.eta * J.eta / es;
+    const T cFv = dot(cF, vij);
+    o.r += rij * (vsum * eta * cFv * spr);
+    o.vij += cF * (vsum * eta * s);
+    vsum_b += eta * s * cFv;
+  }
+  T rhoi_b = vsum_b * (T(-2) * Vi * Vi / I.rho) + pt_b * (J.p - pt) / rs;
+  T rhoj_b = vsum_b * (T(-2) * Vj * Vj / J.rho) + pt_b * (I.p - pt) / rs;
+  o.p_i = pt_b * J.rho / rs;
+  o.p_j = pt_b * I.rho / rs;
+  // T3: Pi = -alpha (c_I + c_J)/2 mu / ((rho_I + rho_J)/2), mu = h vr / (r^2 + eta2)
+  if (I.alpha > T(0) && I.ph == J.ph) {
+    const T vr = dot(vij, rij);
+    if (vr < T(0)) {
+      const T den = r2 + c.eta2;
+      const T mu = c.h * vr / den;
+      const T cbar = T(0.5) * (I.c + J.c), rbar = T(0.5) * rs;
+      const T Pi = -I.alpha * cbar * mu / rbar;
+      const T cAr = dot(cA, rij);
+      const T H = s * cAr;  // cA . gw
+      o.r += (cA * s + rij * (cAr * spr)) * (-J.m * Pi);
+      const T Pi_b = -J.m * H;
+      const T mu_b = Pi_b * (-I.alpha * cbar / rbar);
+      const T rbar_b = Pi_b * (-Pi / rbar);
+      rhoi_b += T(0.5) * rbar_b;
+      rhoj_b += T(0.5) * rbar_b;
+      const T vr_b = mu_b * c.h / den;
+      const T r2_b = -mu_b * mu / den;
+      o.vij += rij * vr_b;
+      o.r += vij * vr_b + rij * (T(2) * r2_b);
+    }
+  }
+  o.rho_i = rhoi_b;
+  o.rho_j = rhoj_b;
+  return o;
+}
+
+// Fluid-wall term of momentum with fluid owner I and a wall particle at xw
+// with ghost velocity vw and wall pressure pw (of I's phase):
+//   force += -gw vsum (p_I + pw)/2                       (W1)
+//   acc   += n 4 c_I^2 q^2 / r_g, q = 1 - r/r_g          (W2, r < r_g)
+//   acc   -= n q (v_I . n) / dt                          (W3, r < r_g, v_I . n < 0)
+//   force += v_Iw vsum eta_I s                           (W4, no-slip)
+//   acc   += gw vol0 alpha c_I mu                        (W5, no-slip, v_Iw . r < 0)
+// with vsum = V_I^2 + vol0^2 and n = r_Iw / r.  cF / cA: cotangents of the
+// force and acceleration sums (both carry a_I's and the wall-load cotangent).
+// r: cotangent of r_Iw = x_I - xw; vi: of v_I alone (W3); vij: of
+// v_Iw = v_I - vw; p_w: of pw.
+template <class T>
+struct FWGrad {
+  Vec<T> r, vi, vij;
+  T rho_i, p_i, p_w;
+};
+template <class T>
+CRYO_ADJ_HD inline FWGrad<T> fw_vjp(const Ctx<T>& c, const Side<T>& I, Vec<T> xw, Vec<T> vw, T pw, Vec<T> cF,
+                                    Vec<T> cA) {
+  FWGrad<T> o{{T(0), T(0), T(0)}, {T(0), T(0), T(0)}, {T(0), T(0), T(0)}, T(0), T(0), T(0)};
+  const Vec<T> rij = I.x - xw;
+  const T r2 = dot(rij, rij);
+  if (!(r2 < c.support2)) return o;
+  const T r = std::sqrt(r2);
+  if (!(r > T(1e-12) * c.h)) return o;  // every term vanishes or is skipped
+  const T s = wendland_s(c, r), spr = wendland_spr(c, r);
+  const T Vi = I.m / I.rho;
+  const T vsum = Vi * Vi + c.vol0 * c.vol0;
+  const T cFr = dot(cF, rij);
+  const T G = s * cFr;
+  // W1
+  const T pm = T(0.5) * (I.p + pw);
+  o.r = (cF * s + rij * (cFr * spr)) * (-vsum * pm);
+  o.p_i = T(-0.5) * vsum * G;
+  o.p_w = o.p_i;
+  T vsum_b = -pm * G;
+  // W2, W3
+  if (r < c.r_guard) {
+    const T q = T(1) - r / c.r_guard;
+    const Vec<T> n = rij * (T(1) / r);
+    const T a = dot(cA, n);
+    const T kap = T(4) * I.c * I.c / c.r_guard;
+    o.r += (cA - n * a) * (kap * q * q / r);
+    o.r -= n * (kap * T(2) * q * a / c.r_guard);
+    const T b = dot(I.v, n);
+    if (b < T(0) && c.dt > T(0)) {
+      const T k = T(-1) / c.dt;
+      o.vi += n * (k * q * a);
+      o.r += (n * (-(a * b) / c.r_guard) + (cA - n * a) * (q * b / r) + (I.v - n * b) * (q * a / r)) * k;
+    }
+  }
+  if (c.no_slip) {
+    const Vec<T> vij = I.v - vw;
+    const T cFv = dot(cF, vij);
+    // W4
+    o.r += rij * (vsum * I.eta * cFv * spr);
+    o.vij += cF * (vsum * I.eta * s);
+    vsum_b += I.eta * s * cFv;
+    // W5
+    const T vr = dot(vij, rij);
+    if (I.alpha > T(0) && vr < T(0)) {
+      const T den = r2 + c.eta2;
+      const T mu = c.h * vr / den;
+      const T cAr = dot(cA, rij);
+      const T H = s * cAr;
+      const T k5 = c.vol0 * I.alpha * I.c;
+      o.r += (cA * s + rij * (cAr * spr)) * (k5 * mu);
+      const T mu_b = k5 * H;
+      const T vr_b = mu_b * c.h / den;
+      const T r2_b = -mu_b * mu / den;
+      o.vij += rij * vr_b;
+      o.r += vij * vr_b + rij * (T(2) * r2_b);
+    }
+  }
+  o.rho_i = vsum_b * (T(-2) * Vi * Vi / I.rho);
+  return o;
+}
+
+// Shepard wall values seen from one fluid neighbour F of a wall particle at xw
+// (navier_stokes.cu:16-69), phase k = phase of F:
+//   pw_k = sum_F (p_F + rho_F g . r_wF) W / sum_F W,   vw = -sum_F v_F W / sum_F W
+// cp = (cotangent of pw_k) / sum_{F in k} W and cvw = (cotangent of vw) / sum_F W
+// are the wall's; pw = pw_k, vw its values.  Returns F's share: cotangents of
+// x_F, v_F, p_F, rho_F.  The cotangent of g is accumulated by the wall itself
+// (cp * sum rho_F r_wF W).
+template <class T>
+struct WBGrad {
+  Vec<T> x, v;
+  T p, rho;
+};
+template <class T>
+CRYO_ADJ_HD inline WBGrad<T> wb_vjp(const Ctx<T>& c, const Side<T>& F, Vec<T> xw, T pw, Vec<T> vw, T cp,
+                                    Vec<T> cvw, Vec<T> g) {
+  WBGrad<T> o{{T(0), T(0), T(0)}, {T(0), T(0), T(0)}, T(0), T(0)};
+  const Vec<T> rwf = xw - F.x;
+  const T r2 = dot(rwf, rwf);
+  if (!(r2 < c.support2)) return o;
+  const T r = std::sqrt(r2);
+  const T W = wendland_w(c, r), s = wendland_s(c, r);
+  const T gr = dot(g, rwf);
+  o.p = cp * W;
+  o.rho = cp * gr * W;
+  // dW/dx_F = -s r_wF
+  o.x = g * (-cp * F.rho * W) + rwf * (-cp * (F.p + F.rho * gr - pw) * s);
+  o.x += rwf * (dot(cvw, F.v + vw) * s);
+  o.v = cvw * (-W);
+  return o;
+}
+
+// dp/drho of the equation of state (sim_params.cuh:100-124); zero where the
+// pressure was clamped at 0.
+template <class T>
+CRYO_ADJ_HD inline T eos_dp_drho(int vapor, T rho, T temp, T p, T rho0, T tait_B, T tait_gamma, T gas_R,
+                                 int clamp_neg) {
+  if (clamp_neg && !(p > T(0))) return T(0);
+  if (vapor) return gas_R * temp;
+  const T ratio = rho / rho0;
+  if (tait_gamma == T(7)) {
+    const T r2 = ratio * ratio;
+    return T(7) * tait_B * r2 * r2 * r2 / rho0;
+  }
+  return tait_gamma * tait_B * std::pow(ratio, tait_gamma - T(1)) / rho0;
+}
+
+}  // namespace cryo_adj

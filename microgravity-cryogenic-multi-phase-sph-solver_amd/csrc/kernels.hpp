@@ -81,4 +81,23 @@ void launch_diagnostics(const PV& pv, const SimParams& P, int n, double* partial
                         double* out, hipStream_t s);
 void launch_telemetry_pack(const PV& pv, int n, float* rec, hipStream_t s);
 
+// Reverse-mode adjoint of the hydrodynamic step (kernels_adjoint.hip).
+struct AdjArgs {
+  const float4 *cx, *cv, *ca;  // cotangents of x, v, a at the step's end (this step's sorted order)
+  float4 *nx, *nv, *na;        // cotangents at the step's start (previous order), written through perm
+  float4* at;                  // a_bar + (dt/2) v_bar of each fluid particle
+  float4 *wcp, *wcv;           // walls: {cp0, cp1} and cvw, Shepard-weighted cotangents of pw0, pw1, vw
+  float4 *xr, *vr;             // the rates' share of the x and v cotangents
+  float* sb;                   // cotangent of sigma
+  const uint32_t* perm;        // the step's sort permutation (new i <- old perm[i])
+  const double* S;             // the step's sum of fw over fluid particles
+  double* partials;            // 6 rows of `stride` doubles
+  int stride;
+};
+void launch_adj_fill(float4* a, float4* b, float4* c, int n, hipStream_t s);
+void launch_adj_set_frame(DevState* st, const float* acc, hipStream_t s);
+void launch_adj_load_sum(const PV& pv, int n, double* partials, int stride, double* out, hipStream_t s);
+void launch_adj_step(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, const DevState* st,
+                     const AdjArgs& A, int n, double* grad_out, hipStream_t s);
+
 }  // namespace cryo

@@ -1,0 +1,310 @@
+// Reverse-mode adjoint of the hydrodynamic step on gfx950 (no counterpart in
+// the reference; the forward passes it differentiates are wall_boundary,
+// momentum and density summation, navier_stokes.cu:16-245, SPH_kernels.cu:6-34,
+// with the kick-drift-kick of sph_solver.cu:20-46).
+//
+// Loss: sum over steps k of dt (S_x^2 + S_y^2), S_k = sum of fw over the fluid
+// particles after step k's rates; parameter: the frame acceleration of each
+// step (g_eff = g - acc).  One reverse step runs after the rates of step k
+// have been re-evaluated on the taped post-sort state, in that step's sorted
+// order:
+//   k_adj_prep     a's total cotangent  at = a_bar + (dt/2) v_bar  (+ its g share)
+//   k_adj_wall     per wall: cotangents of its Shepard pressures and ghost
+//                  velocity, divided by the Shepard weights (+ its g share)
+//   k_adj_fluid    per fluid particle, gathered over its neighbours: both
+//                  members' share of every fluid pair, its own share of every
+//                  wall pair, the wall values' share, the EOS -> sigma_bar
+//   k_adj_density  per fluid particle: density summation, drift and kick,
+//                  then scattered back through the step's sort permutation
+// Every pair is evaluated by each member from its own side (a gather, no
+// atomics), so the result is bitwise reproducible.  The per-pair derivatives
+// are in adjoint_pair.hpp.
+#include "adjoint_pair.hpp"
+#include "kernels.hpp"
+
+namespace cryo {
+
+namespace {
+
+constexpr int kBlock = 256;
+__device__ inline int gtid() { return static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); }
+inline int blocks_for(long long n) { return static_cast<int>((n + kBlock - 1) / kBlock); }
+
+template <int NQ>
+__device__ inline void block_sums_store(double (&v)[NQ], double* partials, int stride) {
+  __shared__ double wsum[NQ][kBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    double x = v[q];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if (lane == 0) wsum[q][wave] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < NQ) {
+    double s = 0;
+    for (int w = 0; w < kBlock / 64; ++w) s += wsum[threadIdx.x][w];
+    partials[threadIdx.x * stride + blockIdx.x] = s;
+  }
+}
+
+__device__ inline double ordered_sum(const double* p, int nb, double* sh) {
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < nb; k += 1024) acc += p[k];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = 512; off > 0; off >>= 1) {
+    if (static_cast<int>(threadIdx.x) < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+using AV = cryo_adj::Vec<float>;
+__device__ inline AV av(float4 q) { return AV{q.x, q.y, q.z}; }
+__device__ inline AV av(V3 q) { return AV{q.x, q.y, q.z}; }
+__device__ inline float4 f4(AV a) { return make_float4(a.x, a.y, a.z, 0.0f); }
+
+__device__ inline cryo_adj::Ctx<float> adj_ctx(const SimParams& P, float dt) {
+  return cryo_adj::Ctx<float>{P.kernel.norm, P.kernel.inv_h, P.kernel.h, P.support2, 0.01f * P.kernel.h * P.kernel.h,
+                              0.6f * P.dx, dt, P.vol0, P.no_slip};
+}
+__device__ inline cryo_adj::Side<float> side(const PV& pv, const SimParams& P, int i) {
+  const uint8_t ph = pv.s.phase[i];
+  return cryo_adj::Side<float>{av(pv.s.x[i]),     av(pv.s.v[i]),
+                               pv.s.m[i],         pv.s.rho[i],
+                               pv.t.p[i],         sound_speed(P, ph, pv.s.T[i]),
+                               sel2(P.mu, pidx(ph)), ph == kVapor ? P.alpha_av_vap : P.alpha_av,
+                               static_cast<int>(ph)};
+}
+// Cotangent of the wall loads: d loss / d fw_i = 2 dt (S_x, S_y, 0).
+__device__ inline AV load_cot(const double* S, float dt) {
+  return AV{2.0f * dt * static_cast<float>(S[0]), 2.0f * dt * static_cast<float>(S[1]), 0.0f};
+}
+
+__global__ __launch_bounds__(kBlock) void k_adj_fill(float4* a, float4* b, float4* c, int n) {
+  const int i = gtid();
+  if (i >= n) return;
+  const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  a[i] = z;
+  b[i] = z;
+  c[i] = z;
+}
+
+__global__ __launch_bounds__(kBlock) void k_adj_set_frame(DevState* st, const float* acc) {
+  const V3 a{acc[0], acc[1], acc[2]};
+  st->frame_acc = a;
+  st->gravity = st->cfg_gravity - a;
+}
+
+// Sum of fw over fluid particles: block partials in rows [row, row + 3).
+__global__ __launch_bounds__(kBlock) void k_adj_load_sum(PV pv, int n, double* partials, int stride) {
+  const int i = gtid();
+  double v[3] = {0.0, 0.0, 0.0};
+  if (i < n && is_fluid(pv.s.phase[i])) {
+    const float4 f = pv.t.fw[i];
+    v[0] = f.x;
+    v[1] = f.y;
+    v[2] = f.z;
+  }
+  block_sums_store<3>(v, partials, stride);
+}
+
+// out[q] = scale * (sum rows r0+q [+ sum rows r1+q]), q = 0..2.
+__global__ __launch_bounds__(1024) void k_adj_finish(const double* partials, int stride, int nb, int r0, int r1,
+                                                    double scale, double* out) {
+  __shared__ double sh[1024];
+  for (int q = 0; q < 3; ++q) {
+    double s = ordered_sum(partials + (r0 + q) * stride, nb, sh);
+    if (r1 >= 0) s += ordered_sum(partials + (r1 + q) * stride, nb, sh);
+    if (threadIdx.x == 0) out[q] = scale * s;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_adj_prep(PV pv, AdjArgs A, const DevState* st, int n) {
+  const int i = gtid();
+  double g[3] = {0.0, 0.0, 0.0};
+  if (i < n) {
+    float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (is_fluid(pv.s.phase[i])) {
+      const float half = 0.5f * st->dt;
+      const float4 a = A.ca[i], v = A.cv[i];
+      r = make_float4(a.x + v.x * half, a.y + v.y * half, a.z + v.z * half, 0.0f);
+      g[0] = r.x;  // a = g_eff + ...
+      g[1] = r.y;
+      g[2] = r.z;
+    }
+    A.at[i] = r;
+  }
+  block_sums_store<3>(g, A.partials + 3 * A.stride, A.stride);
+}
+
+__global__ __launch_bounds__(kBlock) void k_adj_wall(PV pv, GridDev g, SimParams P, NbrList L, const DevState* st,
+                                                    AdjArgs A, int n) {
+  const int w = gtid();
+  double gs[3] = {0.0, 0.0, 0.0};
+  if (w < n && pv.s.phase[w] == kWall) {
+    const float dt = st->dt;
+    const cryo_adj::Ctx<float> c = adj_ctx(P, dt);
+    const AV Fb = load_cot(A.S, dt);
+    const AV xw = av(pv.s.x[w]), vw = av(pv.t.vw[w]);
+    const float pw0 = pv.t.p[w], pw1 = pv.t.p_wv[w];
+    float sw0 = 0.0f, sw1 = 0.0f, pb0 = 0.0f, pb1 = 0.0f;
+    AV G0{0.0f, 0.0f, 0.0f}, G1{0.0f, 0.0f, 0.0f}, vwb{0.0f, 0.0f, 0.0f};
+    for_each_pair(L, g, w, ld3(pv.s.x[w]), [&](int j) {
+      const uint8_t ph = pv.s.phase[j];
+      if (!is_fluid(ph)) return;  // the lists hold fluid only, the sweep fallback does not
+      const AV rwf = xw - av(pv.s.x[j]);
+      const float r2 = cryo_adj::dot(rwf, rwf);
+      if (!(r2 < P.support2)) return;
+      const float wk = kW(P.kernel, sqrtf(r2));
+      const float rho = pv.s.rho[j];
+      const cryo_adj::Side<float> I = side(pv, P, j);
+      const float4 at = A.at[j];
+      const AV cA{at.x, at.y, at.z};
+      const AV cF = cA * (1.0f / I.m);
+      const bool vap = ph == kVapor;
+      const cryo_adj::FWGrad<float> o = cryo_adj::fw_vjp(c, I, xw, vw, vap ? pw1 : pw0, cF + Fb, cA + Fb * I.m);
+      if (vap) {
+        sw1 += wk;
+        G1 += rwf * (rho * wk);
+        pb1 += o.p_w;
+      } else {
+        sw0 += wk;
+        G0 += rwf * (rho * wk);
+        pb0 += o.p_w;
+      }
+      vwb -= o.vij;
+    });
+    const bool cl0 = P.clamp_neg && !(pw0 > 0.0f), cl1 = P.clamp_neg && !(pw1 > 0.0f);
+    const float cp0 = (sw0 > 0.0f && !cl0) ? pb0 / sw0 : 0.0f;
+    const float cp1 = (sw1 > 0.0f && !cl1) ? pb1 / sw1 : 0.0f;
+    const float swt = sw0 + sw1;
+    const AV cvw = swt > 0.0f ? vwb * (1.0f / swt) : AV{0.0f, 0.0f, 0.0f};
+    A.wcp[w] = make_float4(cp0, cp1, 0.0f, 0.0f);
+    A.wcv[w] = f4(cvw);
+    const AV gw = G0 * cp0 + G1 * cp1;  // d pw / d g = sum rho r_wf W / sum W
+    gs[0] = gw.x;
+    gs[1] = gw.y;
+    gs[2] = gw.z;
+  }
+  block_sums_store<3>(gs, A.partials, A.stride);
+}
+
+__global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParams P, NbrList L, const DevState* st,
+                                                     AdjArgs A, int n) {
+  const int i = gtid();
+  if (i >= n) return;
+  const uint8_t phi = pv.s.phase[i];
+  if (!is_fluid(phi)) return;
+  const float dt = st->dt;
+  const cryo_adj::Ctx<float> c = adj_ctx(P, dt);
+  const AV Fb = load_cot(A.S, dt);
+  const AV grav = av(st->gravity);
+  const cryo_adj::Side<float> I = side(pv, P, i);
+  const bool vap = phi == kVapor;
+  const float4 ati = A.at[i];
+  const AV cA{ati.x, ati.y, ati.z};
+  const AV cF = cA * (1.0f / I.m);
+  const AV cFw = cF + Fb, cAw = cA + Fb * I.m;
+  AV xr{0.0f, 0.0f, 0.0f}, vr{0.0f, 0.0f, 0.0f};
+  float rb = 0.0f, pb = 0.0f;
+  for_each_pair(L, g, i, ld3(pv.s.x[i]), [&](int j) {
+    if (j == i) return;
+    const uint8_t phj = pv.s.phase[j];
+    if (phj == kWall) {
+      const AV xw = av(pv.s.x[j]), vw = av(pv.t.vw[j]);
+      const float pw = vap ? pv.t.p_wv[j] : pv.t.p[j];
+      const cryo_adj::FWGrad<float> o = cryo_adj::fw_vjp(c, I, xw, vw, pw, cFw, cAw);
+      xr += o.r;
+      vr += o.vi + o.vij;
+      rb += o.rho_i;
+      pb += o.p_i;
+      const float4 cp = A.wcp[j], cv = A.wcv[j];
+      const cryo_adj::WBGrad<float> b =
+          cryo_adj::wb_vjp(c, I, xw, pw, vw, vap ? cp.y : cp.x, AV{cv.x, cv.y, cv.z}, grav);
+      xr += b.x;
+      vr += b.v;
+      pb += b.p;
+      rb += b.rho;
+      return;
+    }
+    if (!is_fluid(phj)) return;
+    const cryo_adj::Side<float> J = side(pv, P, j);
+    const float4 atj = A.at[j];
+    const AV cAj{atj.x, atj.y, atj.z};
+    const cryo_adj::FFGrad<float> o1 = cryo_adj::ff_vjp(c, I, J, cF, cA);         // i's pair term
+    const cryo_adj::FFGrad<float> o2 = cryo_adj::ff_vjp(c, J, I, cAj * (1.0f / J.m), cAj);  // j's
+    xr += o1.r;
+    xr -= o2.r;
+    vr += o1.vij;
+    vr -= o2.vij;
+    rb += o1.rho_i + o2.rho_j;
+    pb += o1.p_i + o2.p_j;
+  });
+  const float dpdr = cryo_adj::eos_dp_drho(vap ? 1 : 0, I.rho, pv.s.T[i], I.p, P.rho0[0], P.tait_B, P.tait_gamma,
+                                           P.gas_R, P.clamp_neg);
+  A.sb[i] = (rb + pb * dpdr) * (vap ? I.m : pv.s.m0[i]);  // rho = m0 sigma (liquid), m sigma (vapour)
+  A.xr[i] = f4(xr);
+  A.vr[i] = f4(vr);
+}
+
+__global__ __launch_bounds__(kBlock) void k_adj_density(PV pv, GridDev g, SimParams P, NbrList L, const DevState* st,
+                                                       AdjArgs A, int n) {
+  const int i = gtid();
+  if (i >= n) return;
+  const uint32_t dst = A.perm[i];
+  if (!is_fluid(pv.s.phase[i])) {
+    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    A.nx[dst] = z;
+    A.nv[dst] = z;
+    A.na[dst] = z;
+    return;
+  }
+  const float dt = st->dt, half = 0.5f * dt;
+  const cryo_adj::Ctx<float> c = adj_ctx(P, dt);
+  const float sbi = A.sb[i];
+  const AV xi = av(pv.s.x[i]);
+  AV xr = av(A.xr[i]);
+  for_each_pair(L, g, i, ld3(pv.s.x[i]), [&](int j) {
+    if (j == i) return;
+    const AV rij = xi - av(pv.s.x[j]);
+    const float r2 = cryo_adj::dot(rij, rij);
+    if (!(r2 < P.support2)) return;
+    const float sj = is_fluid(pv.s.phase[j]) ? A.sb[j] : 0.0f;  // wall sigma is a constant
+    xr += rij * ((sbi + sj) * cryo_adj::wendland_s(c, sqrtf(r2)));
+  });
+  // x' = x + v_half dt, v' = v_half + a' dt/2, v_half = v + a dt/2
+  const AV X = av(A.cx[i]) + xr;
+  const AV Vh = av(A.cv[i]) + av(A.vr[i]) + X * dt;
+  A.nx[dst] = f4(X);
+  A.nv[dst] = f4(Vh);
+  A.na[dst] = f4(Vh * half);
+}
+
+}  // namespace
+
+void launch_adj_fill(float4* a, float4* b, float4* c, int n, hipStream_t s) {
+  if (n <= 0) return;
+  k_adj_fill<<<blocks_for(n), kBlock, 0, s>>>(a, b, c, n);
+}
+void launch_adj_set_frame(DevState* st, const float* acc, hipStream_t s) { k_adj_set_frame<<<1, 1, 0, s>>>(st, acc); }
+void launch_adj_load_sum(const PV& pv, int n, double* partials, int stride, double* out, hipStream_t s) {
+  const int nb = blocks_for(n > 0 ? n : 1);
+  k_adj_load_sum<<<nb, kBlock, 0, s>>>(pv, n, partials, stride);
+  k_adj_finish<<<1, 1024, 0, s>>>(partials, stride, nb, 0, -1, 1.0, out);
+}
+void launch_adj_step(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, const DevState* st,
+                     const AdjArgs& A, int n, double* grad_out, hipStream_t s) {
+  const int nb = blocks_for(n > 0 ? n : 1);
+  k_adj_prep<<<nb, kBlock, 0, s>>>(pv, A, st, n);
+  k_adj_wall<<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
+  k_adj_fluid<<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
+  k_adj_density<<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
+  // d loss / d acc = -(d loss / d g_eff): wall rows 0..2 + fluid rows 3..5
+  k_adj_finish<<<1, 1024, 0, s>>>(A.partials, A.stride, nb, 0, 3, -1.0, grad_out);
+}
+
+}  // namespace cryo

@@ -804,6 +804,7 @@ struct cryo_solver {
     ensure_initialized();
     launch_begin_step(state.get(), P, L, man, stream);
     grid_build(true);
+    if (tape_slot) snap_save(*tape_slot, false, last_perm);
     if (marks.size() > 4096) prof_collect();
     evaluate_rates();
     const bool shifting = P.shift_strength > 0.0f;
@@ -872,6 +873,182 @@ struct cryo_solver {
     push_state();
     evaluate_rates();
     end_step(0, 0, false);
+    sync_state();
+  }
+
+  // ------------------------------------------------------------ adjoint
+  // A device copy of the persistent particle state, the device state block,
+  // optionally `a` (at step boundaries) and the step's sort permutation (taped
+  // post-sort states).
+  struct Snap {
+    DevBuf<float4> x, v, vt, nrm, a;
+    DevBuf<float> m, m0, rho, T;
+    DevBuf<uint8_t> phase, flags;
+    DevBuf<uint32_t> id, perm;
+    DevBuf<DevState> st;
+    DevBuf<float4> sv[6];  // every scratch field (the end state: what the user reads after the call)
+    DevBuf<float> sf[14];
+    int n = 0;
+  };
+  // The scratch fields in a fixed order (view().t).
+  void scratch_fields(float4* (&v)[6], float* (&f)[14]) {
+    const Scratch t = view().t;
+    float4* vv[6] = {t.a, t.ab, t.ns, t.vw, t.fw, t.mw};
+    float* ff[14] = {t.p,      t.drho,   t.dTdt,  t.sigma, t.kappa,  t.nu_t, t.alpha,
+                     t.wall_w, t.heat_w, t.out_m, t.out_h, t.w_norm, t.p_wv, t.rmin};
+    for (int k = 0; k < 6; ++k) v[k] = vv[k];
+    for (int k = 0; k < 14; ++k) f[k] = ff[k];
+  }
+  void snap_save_scratch(Snap& z) {
+    float4* v[6];
+    float* f[14];
+    scratch_fields(v, f);
+    for (int k = 0; k < 6; ++k) fit(z.sv[k], n), d2d(z.sv[k].get(), v[k], n);
+    for (int k = 0; k < 14; ++k) fit(z.sf[k], n), d2d(z.sf[k].get(), f[k], n);
+  }
+  void snap_load_scratch(const Snap& z) {
+    float4* v[6];
+    float* f[14];
+    scratch_fields(v, f);
+    for (int k = 0; k < 6; ++k) d2d(v[k], z.sv[k].get(), z.n);
+    for (int k = 0; k < 14; ++k) d2d(f[k], z.sf[k].get(), z.n);
+  }
+  std::vector<std::unique_ptr<Snap>> adj_ckpt, adj_tape;
+  std::unique_ptr<Snap> adj_end;
+  DevBuf<float4> adj_c[2][3];  // two cotangent sets of (x, v, a)
+  DevBuf<float4> adj_at, adj_wcp, adj_wcv, adj_xr, adj_vr;
+  DevBuf<float> adj_sb, adj_acc;
+  DevBuf<double> adj_S, adj_G, adj_part;
+  Snap* tape_slot = nullptr;  // set: step_issue() records its post-sort state here
+
+  template <class T>
+  void d2d(T* dst, const T* src, int count) {
+    if (count > 0) CRYO_HIP(hipMemcpyAsync(dst, src, sizeof(T) * count, hipMemcpyDeviceToDevice, stream));
+  }
+  template <class T>
+  static void fit(DevBuf<T>& b, std::size_t count) {
+    if (b.size() < count) b.resize(count);
+  }
+  void snap_save(Snap& z, bool with_a, const uint32_t* perm) {
+    const int m = n;
+    const Persist q = persist(set);
+    fit(z.x, m), fit(z.v, m), fit(z.vt, m), fit(z.nrm, m), fit(z.m, m), fit(z.m0, m), fit(z.rho, m), fit(z.T, m);
+    fit(z.phase, m), fit(z.flags, m), fit(z.id, m), fit(z.st, 1);
+    d2d(z.x.get(), q.x, m), d2d(z.v.get(), q.v, m), d2d(z.vt.get(), q.vt, m), d2d(z.nrm.get(), q.nrm, m);
+    d2d(z.m.get(), q.m, m), d2d(z.m0.get(), q.m0, m), d2d(z.rho.get(), q.rho, m), d2d(z.T.get(), q.T, m);
+    d2d(z.phase.get(), q.phase, m), d2d(z.flags.get(), q.flags, m), d2d(z.id.get(), q.id, m);
+    d2d(z.st.get(), state.get(), 1);
+    if (with_a) fit(z.a, m), d2d(z.a.get(), a.get(), m);
+    if (perm) fit(z.perm, m), d2d(z.perm.get(), perm, m);
+    z.n = m;
+  }
+  void snap_load(const Snap& z, bool with_a) {
+    const int m = z.n;
+    const Persist q = persist(set);
+    d2d(q.x, z.x.get(), m), d2d(q.v, z.v.get(), m), d2d(q.vt, z.vt.get(), m), d2d(q.nrm, z.nrm.get(), m);
+    d2d(q.m, z.m.get(), m), d2d(q.m0, z.m0.get(), m), d2d(q.rho, z.rho.get(), m), d2d(q.T, z.T.get(), m);
+    d2d(q.phase, z.phase.get(), m), d2d(q.flags, z.flags.get(), m), d2d(q.id, z.id.get(), m);
+    d2d(state.get(), z.st.get(), 1);
+    if (with_a) d2d(a.get(), z.a.get(), m);
+    n = m;
+    lists_valid = false;
+    tiles_ok = false;
+  }
+
+  void check_adjoint_scope() const {
+    const cryo_config& c = cfg;
+    if (!(c.fixed_dt > 0.0f)) throw std::invalid_argument("adjoint: needs fixed_dt > 0");
+    if (c.kernel != CRYO_KERNEL_WENDLAND_C2) throw std::invalid_argument("adjoint: Wendland C2 kernel only");
+    if (c.density_mode != CRYO_DENSITY_SUMMATION) throw std::invalid_argument("adjoint: density summation only");
+    if (c.surface_tension != CRYO_ST_NONE) throw std::invalid_argument("adjoint: surface tension not differentiated");
+    if (c.thermal || c.phase_change) throw std::invalid_argument("adjoint: thermal / phase change not differentiated");
+    if (c.transport_velocity || c.sps_turbulence || c.particle_shifting)
+      throw std::invalid_argument("adjoint: transport velocity, SPS and shifting not differentiated");
+    if (has_maneuver || frame_rot) throw std::invalid_argument("adjoint: frame rotation / manoeuvre table not allowed");
+  }
+
+  // loss = sum_k dt (S_x^2 + S_y^2), S_k = sum of fw over fluid particles
+  // after step k's rates; step k runs with frame acceleration acc[3k..3k+2].
+  // grad (optional) = d loss / d acc by the reverse sweep: checkpoints every
+  // `every` steps, each segment re-run forward with its post-sort states
+  // taped, then walked back one step at a time (rates re-evaluated on the
+  // taped state).  The solver ends in the forward end state.
+  void adjoint_slosh(long long steps, const float* acc, int every, double* loss, double* grad, double* force) {
+    check_adjoint_scope();
+    if (steps < 0) throw std::invalid_argument("adjoint: steps must be >= 0");
+    if (every <= 0) throw std::invalid_argument("adjoint: checkpoint_every must be > 0");
+    if (!loss) throw std::invalid_argument("adjoint: loss output is required");
+    ensure_initialized();
+    *loss = 0.0;
+    if (steps == 0) return;
+    const std::size_t ns = static_cast<std::size_t>(steps);
+    fit(adj_acc, 3 * ns), fit(adj_S, 3 * ns), fit(adj_G, 3 * ns), fit(adj_part, 6 * static_cast<std::size_t>(stride));
+    CRYO_HIP(hipMemcpyAsync(adj_acc.get(), acc, sizeof(float) * 3 * ns, hipMemcpyHostToDevice, stream));
+    const long long K = every;
+    const long long nseg = (steps + K - 1) / K;
+    if (grad) {
+      adj_ckpt.resize(static_cast<std::size_t>(nseg));
+      for (auto& c : adj_ckpt)
+        if (!c) c = std::make_unique<Snap>();
+    }
+    for (long long k = 0; k < steps; ++k) {
+      if (grad && k % K == 0) snap_save(*adj_ckpt[static_cast<std::size_t>(k / K)], true, nullptr);
+      launch_adj_set_frame(state.get(), adj_acc.get() + 3 * k, stream);
+      step_issue();
+      step_finish();
+      launch_adj_load_sum(view(), n, adj_part.get(), stride, adj_S.get() + 3 * k, stream);
+    }
+    check_launch();
+    if (grad) {
+      if (!adj_end) adj_end = std::make_unique<Snap>();
+      snap_save(*adj_end, true, nullptr);
+      snap_save_scratch(*adj_end);
+      const std::size_t m = static_cast<std::size_t>(n);
+      for (auto& set3 : adj_c)
+        for (auto& b : set3) fit(b, m);
+      fit(adj_at, m), fit(adj_wcp, m), fit(adj_wcv, m), fit(adj_xr, m), fit(adj_vr, m), fit(adj_sb, m);
+      launch_adj_fill(adj_c[0][0].get(), adj_c[0][1].get(), adj_c[0][2].get(), n, stream);
+      adj_tape.resize(static_cast<std::size_t>(std::min(K, steps)));
+      for (auto& t : adj_tape)
+        if (!t) t = std::make_unique<Snap>();
+      int cur = 0;
+      for (long long seg = nseg - 1; seg >= 0; --seg) {
+        const long long k0 = seg * K, k1 = std::min(steps, k0 + K);
+        snap_load(*adj_ckpt[static_cast<std::size_t>(seg)], true);
+        for (long long k = k0; k < k1; ++k) {
+          launch_adj_set_frame(state.get(), adj_acc.get() + 3 * k, stream);
+          tape_slot = adj_tape[static_cast<std::size_t>(k - k0)].get();
+          step_issue();
+          tape_slot = nullptr;
+          step_finish();
+        }
+        for (long long k = k1 - 1; k >= k0; --k) {
+          const Snap& t = *adj_tape[static_cast<std::size_t>(k - k0)];
+          snap_load(t, false);
+          grid_build(false);  // already in cell order: the identity permutation, the same cell table
+          evaluate_rates();
+          AdjArgs A{adj_c[cur][0].get(), adj_c[cur][1].get(), adj_c[cur][2].get(),
+                    adj_c[1 - cur][0].get(), adj_c[1 - cur][1].get(), adj_c[1 - cur][2].get(),
+                    adj_at.get(), adj_wcp.get(), adj_wcv.get(), adj_xr.get(), adj_vr.get(), adj_sb.get(),
+                    t.perm.get(), adj_S.get() + 3 * k, adj_part.get(), stride};
+          launch_adj_step(view(), grid(), P, lists(), state.get(), A, n, adj_G.get() + 3 * k, stream);
+          check_launch();
+          cur = 1 - cur;
+        }
+      }
+      snap_load(*adj_end, true);
+      grid_build(false);  // the end state's own cell table (identity permutation)
+      snap_load_scratch(*adj_end);
+    }
+    std::vector<double> S(3 * ns);
+    CRYO_HIP(hipMemcpyAsync(S.data(), adj_S.get(), sizeof(double) * 3 * ns, hipMemcpyDeviceToHost, stream));
+    if (grad) CRYO_HIP(hipMemcpyAsync(grad, adj_G.get(), sizeof(double) * 3 * ns, hipMemcpyDeviceToHost, stream));
+    CRYO_HIP(hipStreamSynchronize(stream));
+    const double dt = static_cast<double>(cfg.fixed_dt);
+    double Lsum = 0.0;
+    for (std::size_t k = 0; k < ns; ++k) Lsum += dt * (S[3 * k] * S[3 * k] + S[3 * k + 1] * S[3 * k + 1]);
+    *loss = Lsum;
+    if (force) std::memcpy(force, S.data(), sizeof(double) * 3 * ns);
     sync_state();
   }
 
@@ -1273,6 +1450,11 @@ int cryo_ensemble_run_steps(cryo_solver* const* solvers, int32_t count, int64_t 
     }
     for (int32_t t = 0; t < count; ++t) S(solvers[t]).sync_state();
   })
+}
+int cryo_solver_adjoint_slosh(cryo_solver* s, int64_t steps, const float* frame_acc, int32_t checkpoint_every,
+                              double* loss, double* grad_frame_acc, double* wall_load) {
+  if (steps > 0 && !frame_acc) return fail("frame_acc is required", CRYO_ERR_INVALID_ARGUMENT);
+  CRYO_GUARD(S(s).adjoint_slosh(steps, frame_acc, checkpoint_every, loss, grad_frame_acc, wall_load))
 }
 int cryo_solver_run_until(cryo_solver* s, double t_end) {
   CRYO_GUARD({

@@ -25,6 +25,7 @@
 #include <string>
 #include <vector>
 
+#include "adjoint_pair.hpp"
 #include "cryosph.h"
 
 #ifndef ORC_REAL
@@ -1282,6 +1283,11 @@ struct Solver {
   double heat_input = 0, evaporated = 0, condensed = 0, vented = 0, vented_h = 0, p_ullage = 0;
   Maneuver maneuver;
   V3 frame_acc{0, 0, 0};
+  struct TapeEntry {
+    Particles ps;
+    std::vector<u32> perm;
+  };
+  TapeEntry* record = nullptr;
   double fluid_volume0 = 0;
   int splits = 0, merges = 0;
 
@@ -1542,6 +1548,10 @@ struct Solver {
       });
     }
     grid_build(grid, ps);
+    if (record) {  // the adjoint's tape: the sorted, drifted state and the sort permutation
+      record->ps = ps;
+      record->perm.assign(grid.perm.begin(), grid.perm.begin() + ps.n);
+    }
     if (has_maneuver) {
       V3 a, w, wd;
       maneuver.evaluate(time + dt, a, w, wd);
@@ -1779,6 +1789,207 @@ struct Solver {
       out[k] = den > R(0) ? num / den : std::numeric_limits<R>::quiet_NaN();
     }
   }
+
+  // ------------------------------------------------------------ adjoint
+  // Reverse-mode adjoint of the hydrodynamic step (no counterpart in the
+  // reference): loss = sum_k dt (S_x^2 + S_y^2), S_k = sum of fw over fluid
+  // particles after step k's rates, with the frame acceleration of step k
+  // acc[3k..3k+2] as the parameters (g_eff = g - acc).  The pair derivatives
+  // come from csrc/adjoint_pair.hpp; this restatement drives them in the
+  // scatter form (each pair once, both members updated), serially, and is
+  // itself checked against central differences of this file's forward step.
+  void check_adjoint_scope() const {
+    const cryo_config& c = cfg;
+    if (!(c.fixed_dt > 0)) throw std::invalid_argument("adjoint: needs fixed_dt > 0");
+    if (c.kernel != CRYO_KERNEL_WENDLAND_C2) throw std::invalid_argument("adjoint: Wendland C2 kernel only");
+    if (c.density_mode != CRYO_DENSITY_SUMMATION) throw std::invalid_argument("adjoint: density summation only");
+    if (c.surface_tension != CRYO_ST_NONE) throw std::invalid_argument("adjoint: surface tension not differentiated");
+    if (c.thermal || c.phase_change) throw std::invalid_argument("adjoint: thermal / phase change not differentiated");
+    if (c.transport_velocity || c.sps_turbulence || c.particle_shifting)
+      throw std::invalid_argument("adjoint: transport velocity, SPS and shifting not differentiated");
+    if (has_maneuver || P.frame_rot) throw std::invalid_argument("adjoint: frame rotation / manoeuvre table not allowed");
+  }
+
+  V3 adjoint_rates(const std::vector<u32>& perm, V3 Ssum, std::vector<V3>& xb, std::vector<V3>& vb,
+                   std::vector<V3>& ab) {
+    using namespace cryo_adj;
+    using VR = Vec<R>;
+    const int n = ps.n;
+    const R dt = P.dt, half = R(0.5) * dt;
+    Ctx<R> c{P.kernel.norm, P.kernel.inv_h, P.kernel.h, P.support2, R(0.01) * P.kernel.h * P.kernel.h,
+             R(0.6) * P.dx, dt, P.vol0, P.no_slip};
+    auto vr = [](V3 a) { return VR{a.x, a.y, a.z}; };
+    auto v3 = [](VR a) { return V3{a.x, a.y, a.z}; };
+    const VR Fb{R(2) * dt * Ssum.x, R(2) * dt * Ssum.y, R(0)};
+    const VR g = vr(P.gravity);
+    auto side = [&](int i) {
+      const u8 ph = ps.phase[i];
+      return Side<R>{vr(ps.x[i]), vr(ps.v[i]), ps.m[i], ps.rho[i], ps.p[i], sound_speed(P, ph, ps.T[i]),
+                     P.mu[pidx(ph)], ph == V ? P.alpha_av_vap : P.alpha_av, int(ph)};
+    };
+    std::vector<VR> At(n, VR{0, 0, 0}), xr(n, VR{0, 0, 0}), vrb(n, VR{0, 0, 0}), vwb(n, VR{0, 0, 0});
+    std::vector<R> rb(n, R(0)), pb(n, R(0)), pwb0(n, R(0)), pwb1(n, R(0));
+    VR gb{0, 0, 0};
+    for (int i = 0; i < n; ++i) {
+      if (!is_fluid(ps.phase[i])) continue;
+      At[i] = vr(ab[i]) + vr(vb[i]) * half;  // v' = v_half + a' dt/2
+      gb += At[i];                           // a = g_eff + ...
+    }
+    // momentum
+    for (int i = 0; i < n; ++i) {
+      if (!is_fluid(ps.phase[i])) continue;
+      const Side<R> I = side(i);
+      const VR cF = At[i] * (R(1) / I.m), cA = At[i];
+      const VR cFw = cF + Fb, cAw = cA + Fb * I.m;
+      grid.each(ps.x[i], [&](int j) {
+        if (j == i) return;
+        if (ps.phase[j] == W) {
+          const FWGrad<R> o = fw_vjp(c, I, vr(ps.x[j]), vr(ps.vw[j]), I.ph == V ? ps.p_wv[j] : ps.p[j], cFw, cAw);
+          xr[i] += o.r;
+          vrb[i] += o.vi + o.vij;
+          vwb[j] -= o.vij;
+          rb[i] += o.rho_i;
+          pb[i] += o.p_i;
+          (I.ph == V ? pwb1 : pwb0)[j] += o.p_w;
+          return;
+        }
+        if (!is_fluid(ps.phase[j])) return;
+        const FFGrad<R> o = ff_vjp(c, I, side(j), cF, cA);
+        xr[i] += o.r;
+        xr[j] -= o.r;
+        vrb[i] += o.vij;
+        vrb[j] -= o.vij;
+        rb[i] += o.rho_i;
+        rb[j] += o.rho_j;
+        pb[i] += o.p_i;
+        pb[j] += o.p_j;
+      });
+    }
+    // wall boundary values
+    for (int w = 0; w < n; ++w) {
+      if (ps.phase[w] != W) continue;
+      R sw0 = 0, sw1 = 0;
+      VR G0{0, 0, 0}, G1{0, 0, 0};
+      grid.each(ps.x[w], [&](int j) {
+        if (!is_fluid(ps.phase[j])) return;
+        const V3 rwf = ps.x[w] - ps.x[j];
+        const R r2 = norm2(rwf);
+        if (r2 >= P.support2) return;
+        const R wk = kW(P.kernel, std::sqrt(r2));
+        if (ps.phase[j] == V) {
+          sw1 += wk;
+          G1 += vr(rwf) * (ps.rho[j] * wk);
+        } else {
+          sw0 += wk;
+          G0 += vr(rwf) * (ps.rho[j] * wk);
+        }
+      });
+      const bool cl0 = P.clamp_neg && !(ps.p[w] > R(0)), cl1 = P.clamp_neg && !(ps.p_wv[w] > R(0));
+      const R cp0 = (sw0 > R(0) && !cl0) ? pwb0[w] / sw0 : R(0);
+      const R cp1 = (sw1 > R(0) && !cl1) ? pwb1[w] / sw1 : R(0);
+      const R swt = sw0 + sw1;
+      const VR cvw = swt > R(0) ? vwb[w] * (R(1) / swt) : VR{0, 0, 0};
+      gb += G0 * cp0 + G1 * cp1;
+      grid.each(ps.x[w], [&](int f) {
+        if (!is_fluid(ps.phase[f])) return;
+        const Side<R> F = side(f);
+        const bool vap = F.ph == V;
+        const WBGrad<R> o =
+            wb_vjp(c, F, vr(ps.x[w]), vap ? ps.p_wv[w] : ps.p[w], vr(ps.vw[w]), vap ? cp1 : cp0, cvw, g);
+        xr[f] += o.x;
+        vrb[f] += o.v;
+        pb[f] += o.p;
+        rb[f] += o.rho;
+      });
+    }
+    // equation of state, then density summation
+    std::vector<R> sb(n, R(0));
+    for (int i = 0; i < n; ++i) {
+      const u8 ph = ps.phase[i];
+      if (!is_fluid(ph)) continue;
+      const R dpdr = eos_dp_drho(ph == V ? 1 : 0, ps.rho[i], ps.T[i], ps.p[i], P.rho0[0], P.tait_B, P.tait_gamma,
+                                 P.gas_R, P.clamp_neg);
+      sb[i] = (rb[i] + pb[i] * dpdr) * (ph == L ? ps.m0[i] : ps.m[i]);
+    }
+    for (int i = 0; i < n; ++i) {
+      if (!is_fluid(ps.phase[i])) continue;
+      grid.each(ps.x[i], [&](int j) {
+        if (j == i) return;
+        const V3 rij = ps.x[i] - ps.x[j];
+        const R r2 = norm2(rij);
+        if (r2 >= P.support2) return;
+        const VR d = vr(rij) * (sb[i] * wendland_s(c, std::sqrt(r2)));
+        xr[i] += d;
+        xr[j] -= d;
+      });
+    }
+    // drift and kick, then back through the cell sort: old[perm[i]] <- new[i]
+    std::vector<V3> nx(n, V3{0, 0, 0}), nv(n, V3{0, 0, 0}), na(n, V3{0, 0, 0});
+    for (int i = 0; i < n; ++i) {
+      if (!is_fluid(ps.phase[i])) continue;
+      const VR X = vr(xb[i]) + xr[i];
+      const VR Vh = vr(vb[i]) + vrb[i] + X * dt;
+      nx[perm[i]] = v3(X);
+      nv[perm[i]] = v3(Vh);
+      na[perm[i]] = v3(Vh * half);
+    }
+    xb.swap(nx);
+    vb.swap(nv);
+    ab.swap(na);
+    return v3(gb);
+  }
+
+  void adjoint_slosh(long long steps, const float* acc, double* loss, double* grad, double* force) {
+    check_adjoint_scope();
+    if (steps < 0) throw std::invalid_argument("adjoint: steps must be >= 0");
+    ensure_initialized();
+    const V3 origin = P.frame_origin;
+    const V3 zero{0, 0, 0};
+    auto acc_of = [&](long long k) { return V3{R(acc[3 * k]), R(acc[3 * k + 1]), R(acc[3 * k + 2])}; };
+    std::vector<TapeEntry> tape(static_cast<std::size_t>(steps));
+    std::vector<V3> S(static_cast<std::size_t>(steps));
+    double L = 0;
+    for (long long k = 0; k < steps; ++k) {
+      apply_frame(acc_of(k), zero, zero, origin);
+      record = grad ? &tape[static_cast<std::size_t>(k)] : nullptr;
+      step();
+      record = nullptr;
+      double sx = 0, sy = 0, sz = 0;
+      for (int i = 0; i < ps.n; ++i) {
+        if (!is_fluid(ps.phase[i])) continue;
+        sx += double(ps.fw[i].x);
+        sy += double(ps.fw[i].y);
+        sz += double(ps.fw[i].z);
+      }
+      S[static_cast<std::size_t>(k)] = V3{R(sx), R(sy), R(sz)};
+      if (force) {
+        force[3 * k] = sx;
+        force[3 * k + 1] = sy;
+        force[3 * k + 2] = sz;
+      }
+      L += double(P.dt) * (sx * sx + sy * sy);
+    }
+    *loss = L;
+    if (!grad) return;
+    // reverse sweep over the taped states; the solver ends in the forward end state
+    Particles end_state = ps;
+    std::vector<V3> xb(ps.n, zero), vb(ps.n, zero), ab(ps.n, zero);
+    for (long long k = steps - 1; k >= 0; --k) {
+      TapeEntry& t = tape[static_cast<std::size_t>(k)];
+      ps = t.ps;
+      grid_build(grid, ps);  // already sorted: the identity permutation, the same cell table
+      apply_frame(acc_of(k), zero, zero, origin);
+      P.dt = R(cfg.fixed_dt);
+      evaluate_rates();
+      const V3 gb = adjoint_rates(t.perm, S[static_cast<std::size_t>(k)], xb, vb, ab);
+      grad[3 * k] = -double(gb.x);
+      grad[3 * k + 1] = -double(gb.y);
+      grad[3 * k + 2] = -double(gb.z);
+    }
+    ps = end_state;        // scratch included: a is the end state's
+    grid_build(grid, ps);  // sorted already: the identity permutation
+    if (steps > 0) apply_frame(acc_of(steps - 1), zero, zero, origin);
+  }
 };
 
 thread_local std::string g_err;
@@ -1854,6 +2065,11 @@ int orc_build_tank_particles(int, const cryo_geo_node*, int, const cryo_mesh*, c
 int orc_step(void* s) { GUARD(static_cast<Solver*>(s)->step()) }
 int orc_run_steps(void* s, long long n) {
   GUARD(for (long long k = 0; k < n; ++k) static_cast<Solver*>(s)->step())
+}
+int orc_adjoint_slosh(void* s, long long steps, const float* acc, int checkpoint_every, double* loss, double* grad,
+                      double* force) {
+  (void)checkpoint_every;  // the oracle tapes every step
+  GUARD(static_cast<Solver*>(s)->adjoint_slosh(steps, acc, loss, grad, force))
 }
 int orc_run_until(void* s, double t) {
   GUARD(Solver& S = *static_cast<Solver*>(s); while (S.time < t) S.step())

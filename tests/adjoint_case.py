@@ -1,0 +1,47 @@
+"""Inputs of the adjoint tests: a small closed box tank, half liquid / half
+vapour LN2, hydrodynamics only (no surface tension, thermal, phase change,
+transport velocity), fixed dt, jittered positions, random velocities, and a
+share of the fluid particles next to the walls moved inside the near-wall guard
+distance (0.6 dx) and sent wallward, so that every branch of the wall terms
+(guard repulsion, guard damping, no-slip shear, wall artificial viscosity) is
+on the differentiated path.  The parameters are per-step frame accelerations.
+"""
+import numpy as np
+
+import cabi
+from cabi import WALL
+
+
+def adjoint_case(steps=12, seed=1, dx=0.004, L=0.04, dt=2.5e-4, acc_scale=0.3):
+    fluid = cabi.hip_fluid_preset("LN2")
+    # strong artificial viscosity and gravity: their terms weigh in the gradient
+    cfg = cabi.default_config(dx=dx, gravity=(0.0, 0.0, -5.0), thermal=0, phase_change=0, surface_tension=0,
+                              transport_velocity=0, velocity_scale=0.3, fixed_dt=dt, artificial_viscosity=0.5,
+                              artificial_viscosity_vapor=1.0)
+    ps = cabi.hip_build_tank_particles(cabi.box(L, L, L), fluid, dx, cabi.default_fill())
+    rng = np.random.default_rng(seed)
+    fl = np.flatnonzero(ps.phase != WALL)
+    wl = np.flatnonzero(ps.phase == WALL)
+    ps.x[fl] += rng.uniform(-0.05 * dx, 0.05 * dx, size=(len(fl), 3)).astype(np.float32)
+    ps.v[fl] = rng.normal(0.0, 0.03, size=(len(fl), 3)).astype(np.float32)
+    # nearest wall particle of each fluid particle (brute force over a small tank)
+    xw = ps.x[wl].astype(np.float64)
+    near = []
+    for i in fl:
+        d = np.linalg.norm(xw - ps.x[i].astype(np.float64), axis=1)
+        k = int(np.argmin(d))
+        if d[k] < 1.1 * dx:
+            near.append((i, k, d[k]))
+    for i, k, d in near[::5]:
+        u = (xw[k] - ps.x[i]) / d
+        ps.x[i] += (u * 0.45 * dx).astype(np.float32)
+        ps.v[i] = (u * 0.02).astype(np.float32)
+    lo = ps.x.min(0) - np.float32(4 * dx)
+    hi = ps.x.max(0) + np.float32(4 * dx)
+    acc = rng.normal(0.0, acc_scale, size=(steps, 3)).astype(np.float32)
+    return fluid, cfg, ps, lo, hi, acc
+
+
+def loss_of(make_solver, acc):
+    """Forward-only loss of a fresh solver on the given accelerations."""
+    return make_solver().adjoint_slosh(acc, grad=False)[0]

@@ -229,6 +229,18 @@ def _check(lib, prefix, rc):
         raise exc(f"{prefix} rc={rc}: {msg}")
 
 
+def _adjoint_slosh(call, handle, acc, checkpoint_every, grad):
+    acc = np.ascontiguousarray(np.asarray(acc, np.float32).reshape(-1, 3))
+    steps = len(acc)
+    loss = C.c_double()
+    g = np.zeros((steps, 3), np.float64) if grad else None
+    f = np.zeros((steps, 3), np.float64)
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+    call("adjoint_slosh", handle, C.c_int64(steps), _f32p(acc), C.c_int32(checkpoint_every), C.byref(loss), dp(g),
+         dp(f))
+    return loss.value, g, f
+
+
 class CpuSolver:
     """Oracle restatement (prefix 'orc_') or reference build (prefix 'ref_')."""
 
@@ -291,6 +303,10 @@ class CpuSolver:
 
     def relax(self, duration, damping=0.98):
         self._call("relax", self.h, C.c_double(duration), C.c_double(damping))
+
+    def adjoint_slosh(self, acc, checkpoint_every=20, grad=True):
+        """(loss, dloss/dacc or None, per-step sum of fw) of the oracle adjoint."""
+        return _adjoint_slosh(self._call, self.h, acc, checkpoint_every, grad)
 
     def next_dt(self):
         d = C.c_double()
@@ -505,6 +521,10 @@ class HipSolver:
 
     def damp_velocities(self, f):
         self._call("solver_damp_velocities", self.h, C.c_float(f))
+
+    def adjoint_slosh(self, acc, checkpoint_every=20, grad=True):
+        """(loss, dloss/dacc or None, per-step sum of fw): cryo_solver_adjoint_slosh."""
+        return _adjoint_slosh(lambda name, *a: self._call("solver_" + name, *a), self.h, acc, checkpoint_every, grad)
 
     def initialize(self):
         self._call("solver_initialize", self.h)

@@ -1,0 +1,136 @@
+"""The hand-written adjoint of the hydrodynamic step on the GPU
+(cryo_solver_adjoint_slosh) against the FP64 CPU adjoint and central
+differences of the FP64 forward step, plus its own invariants.
+
+Loss: sum_k dt |S_k,xy|^2 with S_k the wall load after step k; parameters: the
+frame acceleration of each step.  Bar: the FP32 GPU gradient within 1e-3 (of
+its largest component) of the FP64 references.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from adjoint_case import adjoint_case
+from cabi import CpuSolver, HipSolver
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def hip(cfg, fluid, ps, lo, hi):
+    s = HipSolver(cfg, fluid)
+    s.set_particles(ps, lo, hi)
+    return s
+
+
+def orc(cfg, fluid, ps, lo, hi):
+    s = CpuSolver("orc", cfg, fluid, 64)
+    s.set_particles(ps, lo, hi)
+    return s
+
+
+def test_forward_pass_is_the_plain_step_sequence():
+    say("test_forward_pass_is_the_plain_step_sequence")
+    fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=8)
+    a = hip(cfg, fluid, ps, lo, hi)
+    loss, _, load = a.adjoint_slosh(acc, grad=False)
+    b = hip(cfg, fluid, ps, lo, hi)
+    b.initialize()
+    for k in range(len(acc)):
+        b.set_frame_motion(acc=acc[k])
+        b.step()
+    c = hip(cfg, fluid, ps, lo, hi)
+    loss_c, grad_c, load_c = c.adjoint_slosh(acc, checkpoint_every=3)
+    assert loss_c == loss and np.array_equal(load_c, load) and np.isfinite(grad_c).all()
+    for s in (a, c):  # with or without the reverse sweep: the forward end state
+        assert s.step_count == b.step_count == len(acc)
+        for f in ("x", "v", "rho", "p", "fw"):
+            assert np.array_equal(s.field(f).view(np.uint32), b.field(f).view(np.uint32)), f
+    assert np.allclose(-np.array(b.diagnostics().slosh_force.tup()), load[-1], rtol=1e-6, atol=0)
+    assert loss == pytest.approx(cfg.fixed_dt * float((load[:, :2] ** 2).sum()), rel=1e-15)
+
+
+def test_gradient_matches_fp64_adjoint_and_central_differences():
+    say("test_gradient_matches_fp64_adjoint_and_central_differences")
+    fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=12)
+    loss, g, load = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=5)
+    loss64, g64, load64 = orc(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
+    say("loss fp32", loss, "fp64", loss64)
+    err = np.abs(g - g64).max() / np.abs(g64).max()
+    say("gradient vs fp64 adjoint (max-normalised):", err)
+    assert abs(loss - loss64) <= 1e-4 * loss64
+    assert err <= 1e-3
+    for k, c in ((0, 0), (4, 1), (7, 0), (11, 1), (3, 2)):
+        a1, a2 = acc.copy(), acc.copy()
+        a1[k, c] += 1e-3
+        a2[k, c] -= 1e-3
+        h = float(a1[k, c]) - float(a2[k, c])
+        fd = (orc(cfg, fluid, ps, lo, hi).adjoint_slosh(a1, grad=False)[0] -
+              orc(cfg, fluid, ps, lo, hi).adjoint_slosh(a2, grad=False)[0]) / h
+        say(k, c, "gpu", g[k, c], "fd64", fd)
+        assert abs(g[k, c] - fd) <= 1e-3 * np.abs(g64).max()
+
+
+def test_checkpoint_interval_does_not_change_the_gradient():
+    say("test_checkpoint_interval_does_not_change_the_gradient")
+    fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=10)
+    ref = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=10)
+    for every in (1, 3, 4, 25):
+        out = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=every)
+        assert out[0] == ref[0]
+        assert np.array_equal(out[1], ref[1]), every  # the recompute is bitwise the forward pass
+
+
+def test_adjoint_rejects_undifferentiated_physics_and_bad_arguments():
+    say("test_adjoint_rejects_undifferentiated_physics_and_bad_arguments")
+    fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=2)
+    for field, value in (("surface_tension", 1), ("phase_change", 1), ("fixed_dt", 0.0), ("particle_shifting", 1)):
+        saved = getattr(cfg, field)
+        setattr(cfg, field, value)
+        with pytest.raises(ValueError):
+            hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
+        setattr(cfg, field, saved)
+    with pytest.raises(ValueError):
+        hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=0)
+
+
+def test_autograd_function_over_the_module():
+    say("test_autograd_function_over_the_module")
+    import torch
+
+    import CryoSPH as cs
+    from slosh_adjoint import SloshLoss, step_segments
+
+    fluid, cfg, ps, lo, hi, _ = adjoint_case(steps=2)
+    steps, nseg = 12, 4
+    seg = torch.tensor(np.random.default_rng(5).normal(0, 0.3, size=(nseg, 3)), dtype=torch.float64,
+                       requires_grad=True)
+
+    def make_solver():
+        c = cs.SolverConfig()
+        c.dx = cfg.dx
+        c.gravity = cfg.gravity.tup()
+        for f in ("thermal", "phase_change", "transport_velocity"):
+            setattr(c, f, False)
+        c.surface_tension = cs.SurfaceTensionModel.NoTension
+        for f in ("velocity_scale", "fixed_dt", "artificial_viscosity", "artificial_viscosity_vapor"):
+            setattr(c, f, getattr(cfg, f))
+        s = cs.Solver(c, cs.FluidProperties.liquid_nitrogen())
+        s.set_particles(ps.x.astype(np.float64), ps.m, ps.rho, ps.T, ps.phase, domain_lo=tuple(map(float, lo)),
+                        domain_hi=tuple(map(float, hi)), velocity=ps.v.astype(np.float64), flags=ps.flags)
+        return s
+
+    loss = SloshLoss.apply(seg, make_solver, steps, 5)
+    loss.backward()
+    # the same run through the C ABI, its per-step gradient summed per segment
+    acc = seg.detach().numpy()[step_segments(steps, nseg)].astype(np.float32)
+    l2, g2, _ = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=5)
+    gseg = np.zeros((nseg, 3))
+    np.add.at(gseg, step_segments(steps, nseg), g2)
+    assert loss.item() == l2
+    assert np.array_equal(seg.grad.numpy(), gseg)
