@@ -156,6 +156,55 @@ def ensemble_measure(cabi, tanks, steps):
             "ensemble_particle_steps_per_s": n * steps / ens}
 
 
+def c5_case(cabi, steps, segments=16, seed=0):
+    """The differentiable-slosh workload: LN2 cylinder tank r = 0.1 m, h = 0.25 m,
+    fill 0.5, dx = 3.7 mm, microgravity 1e-4 g, a `segments`-piece constant
+    frame-acceleration profile drawn N(0, 0.03) m/s^2 per component (seeded),
+    hydrodynamics only (the differentiated scope: no surface tension, thermal,
+    phase change or transport velocity), fixed dt = the start-up dt."""
+    fluid = cabi.hip_fluid_preset("LN2")
+    dx = 0.0037
+    cfg = cabi.default_config(dx=dx, gravity=(0.0, 0.0, -9.81e-4), velocity_scale=0.1, thermal=0, phase_change=0,
+                              surface_tension=0, transport_velocity=0)
+    ps = cabi.hip_build_tank_particles(cabi.cylinder(0.1, 0.25), fluid, dx, cabi.default_fill(fill_fraction=0.5))
+    pad = np.float32(4 * dx)
+    lo, hi = ps.x.min(0) - pad, ps.x.max(0) + pad
+    probe = cabi.HipSolver(cfg, fluid)
+    probe.set_particles(ps, lo, hi)
+    cfg.fixed_dt = float(np.float32(probe.next_dt()))
+    del probe
+    seg = np.random.default_rng(seed).normal(0.0, 0.03, size=(segments, 3))
+    acc = seg[(np.arange(steps) * segments) // steps].astype(np.float32)
+    return fluid, cfg, ps, lo, hi, acc
+
+
+def adjoint_measure(cabi, steps, every):
+    """Wall-clock of the forward pass alone and of forward + reverse sweep
+    (cryo_solver_adjoint_slosh with and without the gradient) on the
+    differentiable-slosh workload, each on a fresh, initialised solver."""
+    fluid, cfg, ps, lo, hi, acc = c5_case(cabi, steps)
+    times = {}
+    for name, grad in (("forward", False), ("adjoint", True), ("forward2", False), ("adjoint2", True)):
+        s = cabi.HipSolver(cfg, fluid)
+        s.set_particles(ps, lo, hi)
+        s.initialize()
+        s.time  # synchronises
+        t0 = time.perf_counter()
+        loss, g, _ = s.adjoint_slosh(acc, checkpoint_every=every, grad=grad)
+        times[name] = time.perf_counter() - t0
+        if grad:
+            gnorm = float(np.abs(g).max())
+        del s
+    fwd = min(times["forward"], times["forward2"])
+    adj = min(times["adjoint"], times["adjoint2"])
+    return {"workload": f"LN2 cylinder r=0.1 h=0.25 fill 0.5 dx=0.0037, {len(ps)} particles, {steps} steps of "
+                        f"dt={cfg.fixed_dt:.3e} s, 16-segment frame-acceleration profile; loss = sum dt |load_xy|^2",
+            "scope": "hydrodynamic step (density summation, EOS, Shepard wall BC, momentum, kick-drift-kick)",
+            "particles": len(ps), "steps": steps, "checkpoint_every": every, "loss": loss, "grad_max": gnorm,
+            "forward_ms_per_step": 1e3 * fwd / steps, "forward_and_adjoint_ms_per_step": 1e3 * adj / steps,
+            "cost_ratio": adj / fwd}
+
+
 def profile_section(lib, handle, index):
     name = C.create_string_buffer(64)
     ms = C.c_double()
@@ -209,6 +258,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=10, help="0 skips the 64-tank ensemble leg")
+    ap.add_argument("--adjoint-steps", type=int, default=200, help="0 skips the adjoint leg")
+    ap.add_argument("--checkpoint-every", type=int, default=20)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (scales the HBM byte counters) and exit")
     a = ap.parse_args()
@@ -270,6 +321,8 @@ def main():
     del s
     if a.ensemble_steps > 0:
         out["ensemble"] = ensemble_measure(cabi, c4_tanks(cabi), a.ensemble_steps)
+    if a.adjoint_steps > 0:
+        out["adjoint"] = adjoint_measure(cabi, a.adjoint_steps, a.checkpoint_every)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cabi, water, cfg, ps, lo, hi, a.cpu_steps)
     print(json.dumps(out))
