@@ -14,6 +14,11 @@
 //
 // Entry points mirror include/cryosph.h with the prefix orc_ and fields in the
 // oracle's native real type.
+//
+// orc_adjoint_slosh (the reverse-mode adjoint, which the reference does not
+// have) takes its per-pair derivative formulas from the library's
+// csrc/adjoint_pair.hpp and drives them in its own serial scatter form; the
+// tests pin it by central differences of this file's FP64 forward step.
 #include <omp.h>
 
 #include <algorithm>
