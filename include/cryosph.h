@@ -185,16 +185,20 @@ int cryo_ensemble_run_steps(cryo_solver* const* solvers, int32_t count, int64_t 
  *   particles after step k's rates (the negated slosh force),
  * S_k in wall_load[3k..3k+2] (NULL: not returned) and, when grad_frame_acc is
  * not NULL, d loss / d frame_acc by a hand-written reverse sweep that
- * checkpoints every `checkpoint_every` steps and re-runs each segment.  The
- * solver ends in the forward end state either way.  Differentiated: density
- * summation, EOS, Shepard wall pressure and ghost velocity, momentum
- * (pressure, viscosity, artificial viscosity, wall terms), kick-drift-kick.
- * Requires fixed_dt > 0, the Wendland C2 kernel, density summation, no
- * surface tension, thermal, phase change, transport velocity, SPS or
- * shifting, and no frame rotation or manoeuvre table
- * (CRYO_ERR_INVALID_ARGUMENT otherwise). */
+ * checkpoints every `checkpoint_every` steps and re-runs each segment, and
+ * (grad_contact_angle not NULL, which needs grad_frame_acc) d loss / d the
+ * contact angle in degrees (0 without CSF; on a solver not yet initialised
+ * the start-up rates' share is included).  The solver ends in the forward end
+ * state either way.  Differentiated: density summation, EOS, Shepard wall
+ * pressure, ghost velocity and wall colour, momentum (pressure, viscosity,
+ * artificial viscosity, wall terms), CSF surface tension (colour gradient,
+ * contact-angle rotation, curvature, force), kick-drift-kick.  Requires
+ * fixed_dt > 0, the Wendland C2 kernel, density summation, CSF only with
+ * unsmoothed normals, the classic force and normal rotation, no thermal,
+ * phase change, transport velocity, SPS or shifting, and no frame rotation
+ * or manoeuvre table (CRYO_ERR_INVALID_ARGUMENT otherwise). */
 int cryo_solver_adjoint_slosh(cryo_solver* s, int64_t steps, const float* frame_acc, int32_t checkpoint_every,
-                              double* loss, double* grad_frame_acc, double* wall_load);
+                              double* loss, double* grad_frame_acc, double* wall_load, double* grad_contact_angle);
 
 /* Solver::time / step_count / last_dt / num_particles (sph_solver.hpp:153-163). */
 int cryo_solver_time(cryo_solver* s, double* t);

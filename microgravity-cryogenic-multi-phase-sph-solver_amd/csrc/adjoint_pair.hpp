@@ -297,4 +297,130 @@ CRYO_ADJ_HD inline T eos_dp_drho(int vapor, T rho, T temp, T p, T rho0, T tait_B
   return tait_gamma * tait_B * std::pow(ratio, tait_gamma - T(1)) / rho0;
 }
 
+// ---------------------------------------------------------------- CSF
+// Curvature pair of the CSF pass (surface_tension.cu:120-177), owner I and
+// fluid J (j != i), both with |ns|^2 > 0.25:
+//   num_I += V_J (ns_I - ns_J) . gw,   den_I += V_J r (-W'(r)) = -V_J r^2 s
+// num_b / den_b: cotangents of num_I and den_I.  Returns the cotangents of
+// r_IJ, ns_I, ns_J and V_J.
+template <class T>
+struct CsfGrad {
+  Vec<T> r, ns_i, ns_j;
+  T V_j;
+};
+template <class T>
+CRYO_ADJ_HD inline CsfGrad<T> csf_vjp(const Ctx<T>& c, Vec<T> rij, Vec<T> ns_i, Vec<T> ns_j, T Vj, T num_b,
+                                      T den_b) {
+  CsfGrad<T> o{{T(0), T(0), T(0)}, {T(0), T(0), T(0)}, {T(0), T(0), T(0)}, T(0)};
+  const T r2 = dot(rij, rij);
+  if (!(r2 < c.support2)) return o;
+  const T r = std::sqrt(r2);
+  if (!(r > T(1e-12) * c.h)) return o;
+  const T s = wendland_s(c, r), spr = wendland_spr(c, r);
+  const Vec<T> u = ns_i - ns_j;
+  const Vec<T> gw = rij * s;
+  o.ns_i = gw * (num_b * Vj);
+  o.ns_j = gw * (-num_b * Vj);
+  o.V_j = num_b * dot(u, gw) - den_b * r2 * s;
+  o.r = (u * s + rij * (dot(u, rij) * spr)) * (num_b * Vj) - rij * (den_b * Vj * (T(2) * s + r2 * spr));
+  return o;
+}
+
+// Colour gradient of the interface pass (surface_tension.cu:5-85) for fluid I:
+//   gradc_I += gw V_J (c_J - c_I)                 fluid J != I
+//   gradc_I += gw vol0 (alpha_w - c_I)            wall w (wall colour on)
+//   wall_n_I += nrm_w vol0 W,  ww_I += vol0 W     wall w
+// gb, wnb, wwb: cotangents of gradc_I, wall_n_I, ww_I.
+template <class T>
+struct GcGrad {
+  Vec<T> r;
+  T V_j;  // fluid pair: cotangent of V_J; wall pair: of alpha_w
+};
+template <class T>
+CRYO_ADJ_HD inline GcGrad<T> gc_ff_vjp(const Ctx<T>& c, Vec<T> rij, T Vj, T dcol, Vec<T> gb) {
+  GcGrad<T> o{{T(0), T(0), T(0)}, T(0)};
+  if (dcol == T(0)) return o;
+  const T r2 = dot(rij, rij);
+  if (!(r2 < c.support2)) return o;
+  const T r = std::sqrt(r2);
+  if (!(r > T(1e-12) * c.h)) return o;
+  const T s = wendland_s(c, r), spr = wendland_spr(c, r);
+  const T gr = dot(gb, rij);
+  o.r = (gb * s + rij * (gr * spr)) * (Vj * dcol);
+  o.V_j = dcol * s * gr;
+  return o;
+}
+template <class T>
+CRYO_ADJ_HD inline GcGrad<T> gc_fw_vjp(const Ctx<T>& c, Vec<T> rij, T alpha_w, T c_i, Vec<T> nrm_w, int wall_colour,
+                                       Vec<T> gb, Vec<T> wnb, T wwb) {
+  GcGrad<T> o{{T(0), T(0), T(0)}, T(0)};
+  const T r2 = dot(rij, rij);
+  if (!(r2 < c.support2)) return o;
+  const T r = std::sqrt(r2);
+  const T s = wendland_s(c, r);
+  o.r = rij * (c.vol0 * (dot(wnb, nrm_w) + wwb) * s);  // dW/dr_IJ = s r_IJ
+  if (wall_colour && r > T(1e-12) * c.h) {
+    const T spr = wendland_spr(c, r);
+    const T gr = dot(gb, rij);
+    o.r += (gb * s + rij * (gr * spr)) * (c.vol0 * (alpha_w - c_i));
+    o.V_j = c.vol0 * s * gr;
+  }
+  return o;
+}
+
+// Contact-angle rotation of the colour gradient (surface_tension.cu:60-80):
+//   G = |g|, nh = g/G, nw = wall_n/|wall_n|, t = nh - nw (nh.nw),
+//   target = -cos(theta) nw + sin(theta) t/|t|, beta = clamp(ww / adh_w, 0, 1),
+//   nrm = G normalize(nh (1 - beta) + target beta)
+// applied when rotate && ww > 0 && G > normal_eps && |wall_n| > 1e-20 && |t| > 1e-6,
+// else nrm = g.  nb: cotangent of nrm.
+template <class T>
+struct RotGrad {
+  Vec<T> g, wall_n;
+  T ww, cos_t, sin_t;
+};
+template <class T>
+CRYO_ADJ_HD inline RotGrad<T> rot_vjp(Vec<T> g, Vec<T> wall_n, T ww, T cos_t, T sin_t, T adh_w, T normal_eps,
+                                      int rotate, Vec<T> nb) {
+  RotGrad<T> o{nb, {T(0), T(0), T(0)}, T(0), T(0), T(0)};
+  const T G = std::sqrt(dot(g, g));
+  if (!(rotate && ww > T(0) && G > normal_eps)) return o;
+  const T Wn = std::sqrt(dot(wall_n, wall_n));
+  if (!(Wn > T(1e-20))) return o;
+  const Vec<T> nw = wall_n * (T(1) / Wn);
+  const Vec<T> nh = g * (T(1) / G);
+  const T d = dot(nh, nw);
+  const Vec<T> t = nh - nw * d;
+  const T tm = std::sqrt(dot(t, t));
+  if (!(tm > T(1e-6))) return o;
+  const Vec<T> th = t * (T(1) / tm);
+  const Vec<T> target = nw * (-cos_t) + th * sin_t;
+  const T q = ww / adh_w;
+  const T beta = q < T(0) ? T(0) : (q > T(1) ? T(1) : q);
+  const Vec<T> b = nh * (T(1) - beta) + target * beta;
+  const T B = std::sqrt(dot(b, b));
+  if (!(B > T(1e-20))) {  // blended = 0: nrm = 0 whatever g is
+    o.g = Vec<T>{T(0), T(0), T(0)};
+    return o;
+  }
+  const Vec<T> bl = b * (T(1) / B);
+  const T Gb = dot(nb, bl);
+  const Vec<T> blb = nb * G;
+  const Vec<T> bb = (blb - bl * dot(bl, blb)) * (T(1) / B);
+  Vec<T> nhb = bb * (T(1) - beta);
+  const Vec<T> tgb = bb * beta;
+  const T betab = dot(bb, target - nh);
+  if (q > T(0) && q < T(1)) o.ww = betab / adh_w;
+  Vec<T> nwb = tgb * (-cos_t);
+  const Vec<T> thb = tgb * sin_t;
+  o.cos_t = -dot(tgb, nw);
+  o.sin_t = dot(tgb, th);
+  const Vec<T> tb = (thb - th * dot(th, thb)) * (T(1) / tm);
+  nhb += tb - nw * dot(nw, tb);
+  nwb += tb * (-d) - nh * dot(nw, tb);
+  o.wall_n = (nwb - nw * dot(nw, nwb)) * (T(1) / Wn);
+  o.g = (nhb - nh * dot(nh, nhb)) * (T(1) / G) + nh * Gb;
+  return o;
+}
+
 }  // namespace cryo_adj

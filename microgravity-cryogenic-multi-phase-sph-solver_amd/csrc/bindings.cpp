@@ -775,24 +775,27 @@ PYBIND11_MODULE(CryoSPH, m) {
         return vec_array(out);
       }, py::arg("points"))
       // Extension (cryo_solver_adjoint_slosh): no counterpart in the reference.
-      .def("adjoint_slosh", [](Solver& S, DArr acc, int checkpoint_every, bool gradient) {
+      .def("adjoint_slosh", [](Solver& S, DArr acc, int checkpoint_every, bool gradient) -> py::tuple {
         if (acc.ndim() != 2 || acc.shape(1) != 3) throw py::value_error("frame_acc must have shape (steps, 3)");
         const py::ssize_t k = acc.shape(0);
         std::vector<float> a(3 * static_cast<std::size_t>(k));
         for (py::ssize_t i = 0; i < 3 * k; ++i) a[i] = static_cast<float>(acc.data()[i]);
         py::array_t<double> load({k, py::ssize_t{3}}), grad({k, py::ssize_t{3}});
-        double loss = 0.0;
+        double loss = 0.0, dtheta = 0.0;
         int r;
         {
           py::gil_scoped_release rel;
           r = cryo_solver_adjoint_slosh(S.s, k, a.data(), checkpoint_every, &loss,
-                                        gradient ? grad.mutable_data() : nullptr, load.mutable_data());
+                                        gradient ? grad.mutable_data() : nullptr, load.mutable_data(),
+                                        gradient ? &dtheta : nullptr);
         }
         check(r);
-        return py::make_tuple(loss, gradient ? py::object(grad) : py::object(py::none()), load);
+        if (!gradient) return py::make_tuple(loss, py::none(), load, py::none());
+        return py::make_tuple(loss, grad, load, dtheta);
       }, py::arg("frame_acc"), py::arg("checkpoint_every") = 20, py::arg("gradient") = true,
          "Runs len(frame_acc) steps (step k with frame acceleration frame_acc[k]); returns (loss, d loss / "
-         "d frame_acc or None, wall load per step), loss = sum_k dt |load_k,xy|^2.")
+         "d frame_acc, wall load per step, d loss / d contact angle in degrees), loss = sum_k dt "
+         "|load_k,xy|^2; without gradient the two derivatives are None.")
       .def("request_telemetry", [](Solver& S) {
         const int r = cryo_solver_request_telemetry(S.s);
         check(r);

@@ -91,13 +91,23 @@ struct AdjArgs {
   float* sb;                   // cotangent of sigma
   const uint32_t* perm;        // the step's sort permutation (new i <- old perm[i])
   const double* S;             // the step's sum of fw over fluid particles
-  double* partials;            // 6 rows of `stride` doubles
+  double* partials;            // 8 rows of `stride` doubles
   int stride;
+  // surface tension (CSF with the contact-angle rotation); csf = 0: not used
+  int csf;
+  float4* cs;   // {num_bar, den_bar} of each fluid particle's curvature sums
+  float* csv;   // cotangent of V = m / rho from the curvature and colour sums
+  float4* gcb;  // cotangents of the colour gradient (xyz) and the wall weight ww (w)
+  float4* wnb;  // cotangent of the wall-normal sum
+  float4* xc;   // the curvature sums' share of the x cotangent
 };
 void launch_adj_fill(float4* a, float4* b, float4* c, int n, hipStream_t s);
 void launch_adj_set_frame(DevState* st, const float* acc, hipStream_t s);
 void launch_adj_load_sum(const PV& pv, int n, double* partials, int stride, double* out, hipStream_t s);
 void launch_adj_step(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, const DevState* st,
-                     const AdjArgs& A, int n, double* grad_out, hipStream_t s);
+                     const AdjArgs& A, int n, double* grad_out, double* theta_out, hipStream_t s);
+// The contact-angle share of the start-up rates: A.at holds a's cotangent.
+void launch_adj_csf_only(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, const DevState* st,
+                         const AdjArgs& A, int n, double* theta_out, hipStream_t s);
 
 }  // namespace cryo

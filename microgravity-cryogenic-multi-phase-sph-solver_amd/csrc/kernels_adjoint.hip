@@ -3,17 +3,27 @@
 // momentum and density summation, navier_stokes.cu:16-245, SPH_kernels.cu:6-34,
 // with the kick-drift-kick of sph_solver.cu:20-46).
 //
+// With CSF surface tension on, also the interface and CSF passes
+// (surface_tension.cu:5-177: colour gradient with the wall colour and the
+// contact-angle rotation, unsmoothed normals, curvature, classic CSF force).
+//
 // Loss: sum over steps k of dt (S_x^2 + S_y^2), S_k = sum of fw over the fluid
-// particles after step k's rates; parameter: the frame acceleration of each
-// step (g_eff = g - acc).  One reverse step runs after the rates of step k
-// have been re-evaluated on the taped post-sort state, in that step's sorted
-// order:
+// particles after step k's rates; parameters: the frame acceleration of each
+// step (g_eff = g - acc) and the contact angle.  One reverse step runs after
+// the rates of step k have been re-evaluated on the taped post-sort state, in
+// that step's sorted order:
 //   k_adj_prep     a's total cotangent  at = a_bar + (dt/2) v_bar  (+ its g share)
-//   k_adj_wall     per wall: cotangents of its Shepard pressures and ghost
-//                  velocity, divided by the Shepard weights (+ its g share)
+//   k_adj_csf1     (CSF) cotangents of each particle's curvature sums
+//   k_adj_csf2     (CSF) both sides of the curvature pairs, the normalisation
+//                  and the rotation: colour-gradient, wall-normal and wall-weight
+//                  cotangents (+ the contact angle's share)
+//   k_adj_wall     per wall: cotangents of its Shepard pressures, ghost
+//                  velocity and wall colour, divided by the Shepard weights
+//                  (+ its g share)
 //   k_adj_fluid    per fluid particle, gathered over its neighbours: both
 //                  members' share of every fluid pair, its own share of every
-//                  wall pair, the wall values' share, the EOS -> sigma_bar
+//                  wall pair, the wall values' share, the colour-gradient
+//                  pairs (CSF), the EOS and V = m / rho -> sigma_bar
 //   k_adj_density  per fluid particle: density summation, drift and kick,
 //                  then scattered back through the step's sort permutation
 // Every pair is evaluated by each member from its own side (a gather, no
@@ -112,11 +122,11 @@ __global__ __launch_bounds__(kBlock) void k_adj_load_sum(PV pv, int n, double* p
   block_sums_store<3>(v, partials, stride);
 }
 
-// out[q] = scale * (sum rows r0+q [+ sum rows r1+q]), q = 0..2.
+// out[q] = scale * (sum rows r0+q [+ sum rows r1+q]), q = 0..nq-1.
 __global__ __launch_bounds__(1024) void k_adj_finish(const double* partials, int stride, int nb, int r0, int r1,
-                                                    double scale, double* out) {
+                                                    double scale, double* out, int nq) {
   __shared__ double sh[1024];
-  for (int q = 0; q < 3; ++q) {
+  for (int q = 0; q < nq; ++q) {
     double s = ordered_sum(partials + (r0 + q) * stride, nb, sh);
     if (r1 >= 0) s += ordered_sum(partials + (r1 + q) * stride, nb, sh);
     if (threadIdx.x == 0) out[q] = scale * s;
@@ -141,6 +151,104 @@ __global__ __launch_bounds__(kBlock) void k_adj_prep(PV pv, AdjArgs A, const Dev
   block_sums_store<3>(g, A.partials + 3 * A.stride, A.stride);
 }
 
+__device__ inline float colour(uint8_t ph) { return ph == kLiquid ? 1.0f : 0.0f; }
+__device__ inline bool live_normal(const PV& pv, int i) { return norm2(ld3(pv.t.ns[i])) > 0.25f; }
+
+// Curvature sums' cotangents: kappa = 3 num / den, a += nrm coeff kappa.
+__global__ __launch_bounds__(kBlock) void k_adj_csf1(PV pv, GridDev g, SimParams P, NbrList L, AdjArgs A, int n) {
+  const int i = gtid();
+  if (i >= n) return;
+  float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (is_fluid(pv.s.phase[i]) && live_normal(pv, i)) {
+    const float coeff = 2.0f * P.sigma / (P.rho0[0] + P.rho0[1]);
+    const float kappa = pv.t.kappa[i];
+    const float4 at = A.at[i], nr = pv.s.nrm[i];
+    const float kb = coeff * (at.x * nr.x + at.y * nr.y + at.z * nr.z);
+    const V3 xi = ld3(pv.s.x[i]);
+    float den = 0.0f;
+    for_each_pair(L, g, i, xi, [&](int j) {
+      if (j == i || !is_fluid(pv.s.phase[j]) || !live_normal(pv, j)) return;
+      const float r2 = norm2(xi - ld3(pv.s.x[j]));
+      if (!(r2 < P.support2)) return;
+      const float r = sqrtf(r2);
+      den += pv.s.m[j] / pv.s.rho[j] * r * (-kdW(P.kernel, r));
+    });
+    if (den > 0.0f) out = make_float4(3.0f * kb / den, -kappa * kb / den, 0.0f, 0.0f);
+  }
+  A.cs[i] = out;
+}
+
+// Per fluid particle: both sides of the curvature pairs, the normalisation,
+// then the contact-angle rotation of its colour gradient (recomputed).
+__global__ __launch_bounds__(kBlock) void k_adj_csf2(PV pv, GridDev g, SimParams P, NbrList L, const DevState* st,
+                                                    AdjArgs A, int n, int row) {
+  const int k = gtid();
+  double th[2] = {0.0, 0.0};
+  if (k < n && is_fluid(pv.s.phase[k])) {
+    const cryo_adj::Ctx<float> c = adj_ctx(P, st->dt);
+    const float coeff = 2.0f * P.sigma / (P.rho0[0] + P.rho0[1]);
+    const V3 xk = ld3(pv.s.x[k]);
+    const AV nsk = av(pv.t.ns[k]);
+    const bool livek = cryo_adj::dot(nsk, nsk) > 0.25f;
+    const float4 csk = A.cs[k];
+    const float Vk = pv.s.m[k] / pv.s.rho[k];
+    const float ck = colour(pv.s.phase[k]);
+    AV xr{0.0f, 0.0f, 0.0f}, nsb{0.0f, 0.0f, 0.0f};
+    float Vb = 0.0f, ww = 0.0f;
+    V3 gradc{0.0f, 0.0f, 0.0f}, wall_n{0.0f, 0.0f, 0.0f};
+    for_each_pair(L, g, k, xk, [&](int j) {
+      const V3 xj = ld3(pv.s.x[j]);
+      const V3 rkj = xk - xj;
+      const float r2 = norm2(rkj);
+      if (!(r2 < P.support2)) return;
+      const float r = sqrtf(r2);
+      const uint8_t phj = pv.s.phase[j];
+      if (phj == kWall) {  // the interface pass's wall sums (surface_tension.cu:5-85)
+        const float w = kW(P.kernel, r);
+        ww += P.vol0 * w;
+        wall_n += ld3(pv.s.nrm[j]) * (P.vol0 * w);
+        if (P.wall_colour) gradc += kgradW(P.kernel, rkj, r) * (P.vol0 * (pv.t.alpha[j] - ck));
+        return;
+      }
+      if (j == k || !is_fluid(phj)) return;
+      const float Vj = pv.s.m[j] / pv.s.rho[j];
+      gradc += kgradW(P.kernel, rkj, r) * (Vj * (colour(phj) - ck));
+      const AV nsj = av(pv.t.ns[j]);
+      if (!livek || !(cryo_adj::dot(nsj, nsj) > 0.25f)) return;
+      const AV rv = av(rkj);
+      const cryo_adj::CsfGrad<float> o = cryo_adj::csf_vjp(c, rv, nsk, nsj, Vj, csk.x, csk.y);  // k's sums
+      xr += o.r;
+      nsb += o.ns_i;
+      const float4 csj = A.cs[j];
+      const cryo_adj::CsfGrad<float> o2 =
+          cryo_adj::csf_vjp(c, AV{-rv.x, -rv.y, -rv.z}, nsj, nsk, Vk, csj.x, csj.y);  // j's sums
+      xr -= o2.r;
+      nsb += o2.ns_j;
+      Vb += o2.V_j;
+    });
+    const float4 at = A.at[k];
+    AV nrmb = livek ? AV{at.x, at.y, at.z} * (coeff * pv.t.kappa[k]) : AV{0.0f, 0.0f, 0.0f};
+    const AV nrk = av(pv.s.nrm[k]);
+    const float G = sqrtf(cryo_adj::dot(nrk, nrk));
+    if (G > P.normal_eps) nrmb += (nsb - nsk * cryo_adj::dot(nsk, nsb)) * (1.0f / G);  // ns = nrm / |nrm|
+    const cryo_adj::RotGrad<float> ro = cryo_adj::rot_vjp(av(gradc), av(wall_n), ww, P.cos_theta, P.sin_theta,
+                                                          P.wall_adh_w, P.normal_eps, P.wetting != 1 ? 1 : 0, nrmb);
+    A.gcb[k] = make_float4(ro.g.x, ro.g.y, ro.g.z, ro.ww);
+    A.wnb[k] = f4(ro.wall_n);
+    A.csv[k] = Vb;
+    A.xc[k] = f4(xr);
+    th[0] = ro.cos_t;
+    th[1] = ro.sin_t;
+  } else if (k < n) {
+    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    A.gcb[k] = z;
+    A.wnb[k] = z;
+    A.csv[k] = 0.0f;
+    A.xc[k] = z;
+  }
+  block_sums_store<2>(th, A.partials + row * A.stride, A.stride);
+}
+
 __global__ __launch_bounds__(kBlock) void k_adj_wall(PV pv, GridDev g, SimParams P, NbrList L, const DevState* st,
                                                     AdjArgs A, int n) {
   const int w = gtid();
@@ -151,8 +259,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_wall(PV pv, GridDev g, SimParams
     const AV Fb = load_cot(A.S, dt);
     const AV xw = av(pv.s.x[w]), vw = av(pv.t.vw[w]);
     const float pw0 = pv.t.p[w], pw1 = pv.t.p_wv[w];
-    float sw0 = 0.0f, sw1 = 0.0f, pb0 = 0.0f, pb1 = 0.0f;
+    float sw0 = 0.0f, sw1 = 0.0f, pb0 = 0.0f, pb1 = 0.0f, alb = 0.0f;
     AV G0{0.0f, 0.0f, 0.0f}, G1{0.0f, 0.0f, 0.0f}, vwb{0.0f, 0.0f, 0.0f};
+    const float alpha_w = pv.t.alpha[w];
+    const AV nrm_w = av(pv.s.nrm[w]);
     for_each_pair(L, g, w, ld3(pv.s.x[w]), [&](int j) {
       const uint8_t ph = pv.s.phase[j];
       if (!is_fluid(ph)) return;  // the lists hold fluid only, the sweep fallback does not
@@ -177,6 +287,13 @@ __global__ __launch_bounds__(kBlock) void k_adj_wall(PV pv, GridDev g, SimParams
         pb0 += o.p_w;
       }
       vwb -= o.vij;
+      if (A.csf) {  // the fluid's colour gradient reads the wall colour alpha_w
+        const float4 gb = A.gcb[j], wb = A.wnb[j];
+        const cryo_adj::GcGrad<float> og = cryo_adj::gc_fw_vjp(c, AV{-rwf.x, -rwf.y, -rwf.z}, alpha_w,
+                                                               colour(ph), nrm_w, P.wall_colour,
+                                                               AV{gb.x, gb.y, gb.z}, AV{wb.x, wb.y, wb.z}, gb.w);
+        alb += og.V_j;
+      }
     });
     const bool cl0 = P.clamp_neg && !(pw0 > 0.0f), cl1 = P.clamp_neg && !(pw1 > 0.0f);
     const float cp0 = (sw0 > 0.0f && !cl0) ? pb0 / sw0 : 0.0f;
@@ -184,7 +301,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_wall(PV pv, GridDev g, SimParams
     const float swt = sw0 + sw1;
     const AV cvw = swt > 0.0f ? vwb * (1.0f / swt) : AV{0.0f, 0.0f, 0.0f};
     A.wcp[w] = make_float4(cp0, cp1, 0.0f, 0.0f);
-    A.wcv[w] = f4(cvw);
+    A.wcv[w] = make_float4(cvw.x, cvw.y, cvw.z, swt > 0.0f ? alb / swt : 0.0f);  // w: alpha_w = sw0 / swt
     const AV gw = G0 * cp0 + G1 * cp1;  // d pw / d g = sum rho r_wf W / sum W
     gs[0] = gw.x;
     gs[1] = gw.y;
@@ -211,6 +328,17 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
   const AV cFw = cF + Fb, cAw = cA + Fb * I.m;
   AV xr{0.0f, 0.0f, 0.0f}, vr{0.0f, 0.0f, 0.0f};
   float rb = 0.0f, pb = 0.0f;
+  const float ci = colour(phi), Vk = I.m / I.rho;
+  AV gbk{0.0f, 0.0f, 0.0f}, wbk{0.0f, 0.0f, 0.0f};
+  float wwbk = 0.0f, Vbk = 0.0f;
+  if (A.csf) {
+    const float4 gb = A.gcb[i], wb = A.wnb[i];
+    gbk = AV{gb.x, gb.y, gb.z};
+    wwbk = gb.w;
+    wbk = AV{wb.x, wb.y, wb.z};
+    Vbk = A.csv[i];
+    xr = av(A.xc[i]);
+  }
   for_each_pair(L, g, i, ld3(pv.s.x[i]), [&](int j) {
     if (j == i) return;
     const uint8_t phj = pv.s.phase[j];
@@ -229,9 +357,30 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
       vr += b.v;
       pb += b.p;
       rb += b.rho;
+      if (A.csf) {
+        const float alpha_w = pv.t.alpha[j];
+        const cryo_adj::GcGrad<float> og = cryo_adj::gc_fw_vjp(c, I.x - xw, alpha_w, ci, av(pv.s.nrm[j]),
+                                                               P.wall_colour, gbk, wbk, wwbk);
+        xr += og.r;
+        const AV rwk = xw - I.x;
+        const float r2 = cryo_adj::dot(rwk, rwk);
+        if (cv.w != 0.0f && r2 < P.support2)  // alpha_w = sw0 / swt: d/dx_k = (c_k - alpha_w) / swt (-s r_wk)
+          xr += rwk * (-cv.w * (ci - alpha_w) * cryo_adj::wendland_s(c, sqrtf(r2)));
+      }
       return;
     }
     if (!is_fluid(phj)) return;
+    if (A.csf) {  // colour-gradient pairs, both sides
+      const float cj = colour(phj);
+      const float4 gbj4 = A.gcb[j];
+      const AV rkj = I.x - av(pv.s.x[j]);
+      const cryo_adj::GcGrad<float> og = cryo_adj::gc_ff_vjp(c, rkj, pv.s.m[j] / pv.s.rho[j], cj - ci, gbk);
+      const cryo_adj::GcGrad<float> og2 =
+          cryo_adj::gc_ff_vjp(c, AV{-rkj.x, -rkj.y, -rkj.z}, Vk, ci - cj, AV{gbj4.x, gbj4.y, gbj4.z});
+      xr += og.r;
+      xr -= og2.r;
+      Vbk += og2.V_j;
+    }
     const cryo_adj::Side<float> J = side(pv, P, j);
     const float4 atj = A.at[j];
     const AV cAj{atj.x, atj.y, atj.z};
@@ -246,6 +395,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
   });
   const float dpdr = cryo_adj::eos_dp_drho(vap ? 1 : 0, I.rho, pv.s.T[i], I.p, P.rho0[0], P.tait_B, P.tait_gamma,
                                            P.gas_R, P.clamp_neg);
+  rb -= Vbk * Vk / I.rho;  // V = m / rho in the curvature and colour sums
   A.sb[i] = (rb + pb * dpdr) * (vap ? I.m : pv.s.m0[i]);  // rho = m0 sigma (liquid), m sigma (vapour)
   A.xr[i] = f4(xr);
   A.vr[i] = f4(vr);
@@ -294,17 +444,29 @@ void launch_adj_set_frame(DevState* st, const float* acc, hipStream_t s) { k_adj
 void launch_adj_load_sum(const PV& pv, int n, double* partials, int stride, double* out, hipStream_t s) {
   const int nb = blocks_for(n > 0 ? n : 1);
   k_adj_load_sum<<<nb, kBlock, 0, s>>>(pv, n, partials, stride);
-  k_adj_finish<<<1, 1024, 0, s>>>(partials, stride, nb, 0, -1, 1.0, out);
+  k_adj_finish<<<1, 1024, 0, s>>>(partials, stride, nb, 0, -1, 1.0, out, 3);
+}
+void launch_adj_csf_only(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, const DevState* st,
+                         const AdjArgs& A, int n, double* theta_out, hipStream_t s) {
+  const int nb = blocks_for(n > 0 ? n : 1);
+  k_adj_csf1<<<nb, kBlock, 0, s>>>(pv, g, P, L, A, n);
+  k_adj_csf2<<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n, 6);
+  k_adj_finish<<<1, 1024, 0, s>>>(A.partials, A.stride, nb, 6, -1, 1.0, theta_out, 2);
 }
 void launch_adj_step(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, const DevState* st,
-                     const AdjArgs& A, int n, double* grad_out, hipStream_t s) {
+                     const AdjArgs& A, int n, double* grad_out, double* theta_out, hipStream_t s) {
   const int nb = blocks_for(n > 0 ? n : 1);
   k_adj_prep<<<nb, kBlock, 0, s>>>(pv, A, st, n);
+  if (A.csf) {
+    k_adj_csf1<<<nb, kBlock, 0, s>>>(pv, g, P, L, A, n);
+    k_adj_csf2<<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n, 6);
+    k_adj_finish<<<1, 1024, 0, s>>>(A.partials, A.stride, nb, 6, -1, 1.0, theta_out, 2);
+  }
   k_adj_wall<<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
   k_adj_fluid<<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
   k_adj_density<<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
   // d loss / d acc = -(d loss / d g_eff): wall rows 0..2 + fluid rows 3..5
-  k_adj_finish<<<1, 1024, 0, s>>>(A.partials, A.stride, nb, 0, 3, -1.0, grad_out);
+  k_adj_finish<<<1, 1024, 0, s>>>(A.partials, A.stride, nb, 0, 3, -1.0, grad_out, 3);
 }
 
 }  // namespace cryo

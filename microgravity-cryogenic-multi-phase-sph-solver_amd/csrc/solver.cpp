@@ -916,9 +916,9 @@ struct cryo_solver {
   std::vector<std::unique_ptr<Snap>> adj_ckpt, adj_tape;
   std::unique_ptr<Snap> adj_end;
   DevBuf<float4> adj_c[2][3];  // two cotangent sets of (x, v, a)
-  DevBuf<float4> adj_at, adj_wcp, adj_wcv, adj_xr, adj_vr;
-  DevBuf<float> adj_sb, adj_acc;
-  DevBuf<double> adj_S, adj_G, adj_part;
+  DevBuf<float4> adj_at, adj_wcp, adj_wcv, adj_xr, adj_vr, adj_cs, adj_gcb, adj_wnb, adj_xc;
+  DevBuf<float> adj_sb, adj_acc, adj_csv;
+  DevBuf<double> adj_S, adj_G, adj_T, adj_part;
   Snap* tape_slot = nullptr;  // set: step_issue() records its post-sort state here
 
   template <class T>
@@ -955,12 +955,23 @@ struct cryo_solver {
     tiles_ok = false;
   }
 
+  AdjArgs adj_args(int cur, const uint32_t* perm, const double* S, bool csf) {
+    return AdjArgs{adj_c[cur][0].get(), adj_c[cur][1].get(), adj_c[cur][2].get(),
+                   adj_c[1 - cur][0].get(), adj_c[1 - cur][1].get(), adj_c[1 - cur][2].get(),
+                   adj_at.get(), adj_wcp.get(), adj_wcv.get(), adj_xr.get(), adj_vr.get(), adj_sb.get(),
+                   perm, S, adj_part.get(), stride, csf ? 1 : 0, adj_cs.get(), adj_csv.get(), adj_gcb.get(),
+                   adj_wnb.get(), adj_xc.get()};
+  }
+
   void check_adjoint_scope() const {
     const cryo_config& c = cfg;
     if (!(c.fixed_dt > 0.0f)) throw std::invalid_argument("adjoint: needs fixed_dt > 0");
     if (c.kernel != CRYO_KERNEL_WENDLAND_C2) throw std::invalid_argument("adjoint: Wendland C2 kernel only");
     if (c.density_mode != CRYO_DENSITY_SUMMATION) throw std::invalid_argument("adjoint: density summation only");
-    if (c.surface_tension != CRYO_ST_NONE) throw std::invalid_argument("adjoint: surface tension not differentiated");
+    if (c.surface_tension == CRYO_ST_AKINCI) throw std::invalid_argument("adjoint: Akinci surface tension not differentiated");
+    if (c.surface_tension == CRYO_ST_CSF &&
+        (c.csf_smooth_normals || c.csf_balanced || c.wetting_model != CRYO_WET_NORMAL_ROTATION))
+      throw std::invalid_argument("adjoint: CSF only with unsmoothed normals, the classic force and normal rotation");
     if (c.thermal || c.phase_change) throw std::invalid_argument("adjoint: thermal / phase change not differentiated");
     if (c.transport_velocity || c.sps_turbulence || c.particle_shifting)
       throw std::invalid_argument("adjoint: transport velocity, SPS and shifting not differentiated");
@@ -973,16 +984,22 @@ struct cryo_solver {
   // `every` steps, each segment re-run forward with its post-sort states
   // taped, then walked back one step at a time (rates re-evaluated on the
   // taped state).  The solver ends in the forward end state.
-  void adjoint_slosh(long long steps, const float* acc, int every, double* loss, double* grad, double* force) {
+  void adjoint_slosh(long long steps, const float* acc, int every, double* loss, double* grad, double* force,
+                     double* grad_theta) {
     check_adjoint_scope();
     if (steps < 0) throw std::invalid_argument("adjoint: steps must be >= 0");
     if (every <= 0) throw std::invalid_argument("adjoint: checkpoint_every must be > 0");
     if (!loss) throw std::invalid_argument("adjoint: loss output is required");
+    if (grad_theta && !grad) throw std::invalid_argument("adjoint: the contact-angle gradient needs grad_frame_acc");
+    const bool fresh = !initialized;  // the start-up rates (and their a) belong to this run
     ensure_initialized();
     *loss = 0.0;
+    if (grad_theta) *grad_theta = 0.0;
     if (steps == 0) return;
+    const bool csf = cfg.surface_tension == CRYO_ST_CSF && has_vapor;
     const std::size_t ns = static_cast<std::size_t>(steps);
-    fit(adj_acc, 3 * ns), fit(adj_S, 3 * ns), fit(adj_G, 3 * ns), fit(adj_part, 6 * static_cast<std::size_t>(stride));
+    fit(adj_acc, 3 * ns), fit(adj_S, 3 * ns), fit(adj_G, 3 * ns), fit(adj_T, 2 * ns + 2);
+    fit(adj_part, 8 * static_cast<std::size_t>(stride));
     CRYO_HIP(hipMemcpyAsync(adj_acc.get(), acc, sizeof(float) * 3 * ns, hipMemcpyHostToDevice, stream));
     const long long K = every;
     const long long nseg = (steps + K - 1) / K;
@@ -1007,6 +1024,7 @@ struct cryo_solver {
       for (auto& set3 : adj_c)
         for (auto& b : set3) fit(b, m);
       fit(adj_at, m), fit(adj_wcp, m), fit(adj_wcv, m), fit(adj_xr, m), fit(adj_vr, m), fit(adj_sb, m);
+      if (csf) fit(adj_cs, m), fit(adj_csv, m), fit(adj_gcb, m), fit(adj_wnb, m), fit(adj_xc, m);
       launch_adj_fill(adj_c[0][0].get(), adj_c[0][1].get(), adj_c[0][2].get(), n, stream);
       adj_tape.resize(static_cast<std::size_t>(std::min(K, steps)));
       for (auto& t : adj_tape)
@@ -1027,23 +1045,41 @@ struct cryo_solver {
           snap_load(t, false);
           grid_build(false);  // already in cell order: the identity permutation, the same cell table
           evaluate_rates();
-          AdjArgs A{adj_c[cur][0].get(), adj_c[cur][1].get(), adj_c[cur][2].get(),
-                    adj_c[1 - cur][0].get(), adj_c[1 - cur][1].get(), adj_c[1 - cur][2].get(),
-                    adj_at.get(), adj_wcp.get(), adj_wcv.get(), adj_xr.get(), adj_vr.get(), adj_sb.get(),
-                    t.perm.get(), adj_S.get() + 3 * k, adj_part.get(), stride};
-          launch_adj_step(view(), grid(), P, lists(), state.get(), A, n, adj_G.get() + 3 * k, stream);
+          const AdjArgs A = adj_args(cur, t.perm.get(), adj_S.get() + 3 * k, csf);
+          launch_adj_step(view(), grid(), P, lists(), state.get(), A, n, adj_G.get() + 3 * k, adj_T.get() + 2 * k,
+                          stream);
           check_launch();
           cur = 1 - cur;
         }
+      }
+      if (fresh && csf && grad_theta) {
+        // the start-up a (checkpoint 0) depends on the contact angle too:
+        // adj_c[cur][2] holds its cotangent
+        snap_load(*adj_ckpt[0], true);
+        grid_build(false);
+        evaluate_rates();
+        d2d(adj_at.get(), adj_c[cur][2].get(), n);
+        const AdjArgs A = adj_args(cur, nullptr, nullptr, true);
+        launch_adj_csf_only(view(), grid(), P, lists(), state.get(), A, n, adj_T.get() + 2 * ns, stream);
+        check_launch();
       }
       snap_load(*adj_end, true);
       grid_build(false);  // the end state's own cell table (identity permutation)
       snap_load_scratch(*adj_end);
     }
-    std::vector<double> S(3 * ns);
+    std::vector<double> S(3 * ns), T(2 * ns + 2, 0.0);
     CRYO_HIP(hipMemcpyAsync(S.data(), adj_S.get(), sizeof(double) * 3 * ns, hipMemcpyDeviceToHost, stream));
     if (grad) CRYO_HIP(hipMemcpyAsync(grad, adj_G.get(), sizeof(double) * 3 * ns, hipMemcpyDeviceToHost, stream));
+    const bool start_term = fresh && csf && grad_theta;
+    if (grad && csf) CRYO_HIP(hipMemcpyAsync(T.data(), adj_T.get(), sizeof(double) * (2 * ns + (start_term ? 2 : 0)),
+                                             hipMemcpyDeviceToHost, stream));
     CRYO_HIP(hipStreamSynchronize(stream));
+    if (grad_theta) {  // d loss / d theta = (cos_bar (-sin theta) + sin_bar cos theta) pi / 180
+      double cb = 0.0, sb = 0.0;
+      for (std::size_t q = 0; q < ns + (start_term ? 1 : 0); ++q) cb += T[2 * q], sb += T[2 * q + 1];
+      *grad_theta = (cb * -static_cast<double>(P.sin_theta) + sb * static_cast<double>(P.cos_theta)) *
+                    (3.14159265358979323846 / 180.0);
+    }
     const double dt = static_cast<double>(cfg.fixed_dt);
     double Lsum = 0.0;
     for (std::size_t k = 0; k < ns; ++k) Lsum += dt * (S[3 * k] * S[3 * k] + S[3 * k + 1] * S[3 * k + 1]);
@@ -1452,9 +1488,10 @@ int cryo_ensemble_run_steps(cryo_solver* const* solvers, int32_t count, int64_t 
   })
 }
 int cryo_solver_adjoint_slosh(cryo_solver* s, int64_t steps, const float* frame_acc, int32_t checkpoint_every,
-                              double* loss, double* grad_frame_acc, double* wall_load) {
+                              double* loss, double* grad_frame_acc, double* wall_load, double* grad_contact_angle) {
   if (steps > 0 && !frame_acc) return fail("frame_acc is required", CRYO_ERR_INVALID_ARGUMENT);
-  CRYO_GUARD(S(s).adjoint_slosh(steps, frame_acc, checkpoint_every, loss, grad_frame_acc, wall_load))
+  CRYO_GUARD(S(s).adjoint_slosh(steps, frame_acc, checkpoint_every, loss, grad_frame_acc, wall_load,
+                                grad_contact_angle))
 }
 int cryo_solver_run_until(cryo_solver* s, double t_end) {
   CRYO_GUARD({

@@ -30,7 +30,7 @@ class SloshLoss(torch.autograd.Function):
         idx = step_segments(int(steps), seg.shape[0])
         acc = np.ascontiguousarray(seg[idx])
         want = bool(ctx.needs_input_grad[0])
-        loss, grad, _ = make_solver().adjoint_slosh(acc, int(checkpoint_every), want)
+        loss, grad, _, _ = make_solver().adjoint_slosh(acc, int(checkpoint_every), want)
         if want:
             gseg = np.zeros_like(seg)
             np.add.at(gseg, idx, grad)

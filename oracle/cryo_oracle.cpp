@@ -1808,15 +1808,129 @@ struct Solver {
     if (!(c.fixed_dt > 0)) throw std::invalid_argument("adjoint: needs fixed_dt > 0");
     if (c.kernel != CRYO_KERNEL_WENDLAND_C2) throw std::invalid_argument("adjoint: Wendland C2 kernel only");
     if (c.density_mode != CRYO_DENSITY_SUMMATION) throw std::invalid_argument("adjoint: density summation only");
-    if (c.surface_tension != CRYO_ST_NONE) throw std::invalid_argument("adjoint: surface tension not differentiated");
+    if (c.surface_tension == CRYO_ST_AKINCI) throw std::invalid_argument("adjoint: Akinci surface tension not differentiated");
+    if (c.surface_tension == CRYO_ST_CSF &&
+        (c.csf_smooth_normals || c.csf_balanced || c.wetting_model != CRYO_WET_NORMAL_ROTATION))
+      throw std::invalid_argument("adjoint: CSF only with unsmoothed normals, the classic force and normal rotation");
     if (c.thermal || c.phase_change) throw std::invalid_argument("adjoint: thermal / phase change not differentiated");
     if (c.transport_velocity || c.sps_turbulence || c.particle_shifting)
       throw std::invalid_argument("adjoint: transport velocity, SPS and shifting not differentiated");
     if (has_maneuver || P.frame_rot) throw std::invalid_argument("adjoint: frame rotation / manoeuvre table not allowed");
   }
 
+  // Surface tension's share of the rates' adjoint: a += nrm coeff kappa
+  // (surface_tension.cu:120-177) with nrm the contact-angle-rotated colour
+  // gradient (surface_tension.cu:5-85).  At: cotangent of a.  Adds to xr
+  // (positions), Vb (V = m / rho) and alphab (wall colour); returns
+  // d loss / d contact angle (degrees) of this evaluation.
+  double csf_adjoint(const cryo_adj::Ctx<R>& c, const std::vector<cryo_adj::Vec<R>>& At,
+                     std::vector<cryo_adj::Vec<R>>& xr, std::vector<R>& Vb, std::vector<R>& alphab) {
+    using namespace cryo_adj;
+    using VR = Vec<R>;
+    const int n = ps.n;
+    auto vr = [](V3 a) { return VR{a.x, a.y, a.z}; };
+    double theta_b = 0;
+    const R coeff = R(2) * P.sigma / (P.rho0[0] + P.rho0[1]);
+    auto live = [&](int i) { return norm2(ps.ns[i]) > R(0.25); };
+    auto Vof = [&](int j) { return ps.m[j] / ps.rho[j]; };
+    std::vector<R> numb(n, R(0)), denb(n, R(0));
+    std::vector<VR> nsb(n, VR{0, 0, 0}), nrmb(n, VR{0, 0, 0});
+    for (int i = 0; i < n; ++i) {
+      if (!is_fluid(ps.phase[i]) || !live(i)) continue;
+      const R kappa = ps.kappa[i];
+      const R kb = coeff * dot(At[i], vr(ps.nrm[i]));
+      nrmb[i] += At[i] * (coeff * kappa);
+      R den = 0;
+      grid.each(ps.x[i], [&](int j) {
+        if (j == i || !is_fluid(ps.phase[j]) || !live(j)) return;
+        const R r2 = norm2(ps.x[i] - ps.x[j]);
+        if (r2 >= P.support2) return;
+        const R r = std::sqrt(r2);
+        den += Vof(j) * r * (-kdW(P.kernel, r));
+      });
+      if (den > R(0)) {
+        numb[i] = R(3) * kb / den;
+        denb[i] = -kappa * kb / den;
+      }
+    }
+    for (int i = 0; i < n; ++i) {
+      if (!is_fluid(ps.phase[i]) || !live(i)) continue;
+      grid.each(ps.x[i], [&](int j) {
+        if (j == i || !is_fluid(ps.phase[j]) || !live(j)) return;
+        const CsfGrad<R> o = csf_vjp(c, vr(ps.x[i] - ps.x[j]), vr(ps.ns[i]), vr(ps.ns[j]), Vof(j), numb[i], denb[i]);
+        xr[i] += o.r;
+        xr[j] -= o.r;
+        nsb[i] += o.ns_i;
+        nsb[j] += o.ns_j;
+        Vb[j] += o.V_j;
+      });
+    }
+    std::vector<VR> gcb(n, VR{0, 0, 0}), wnb(n, VR{0, 0, 0});
+    std::vector<R> wwb(n, R(0));
+    double cos_b = 0, sin_b = 0;
+    for (int i = 0; i < n; ++i) {
+      const u8 ph = ps.phase[i];
+      if (!is_fluid(ph)) continue;
+      const V3 g0 = ps.nrm[i];
+      const R G = norm(g0);
+      if (G > P.normal_eps) {  // ns = nrm / |nrm| (no smoothing)
+        const VR ns = vr(ps.ns[i]);
+        nrmb[i] += (nsb[i] - ns * dot(ns, nsb[i])) * (R(1) / G);
+      }
+      // the colour gradient before the rotation, the wall normal sum and weight
+      const R ci = ph == L ? R(1) : R(0);
+      V3 gradc{0, 0, 0}, wall_n{0, 0, 0};
+      R ww = 0;
+      grid.each(ps.x[i], [&](int j) {
+        const V3 rij = ps.x[i] - ps.x[j];
+        const R r2 = norm2(rij);
+        if (r2 >= P.support2) return;
+        const R r = std::sqrt(r2);
+        const R w = kW(P.kernel, r);
+        if (ps.phase[j] == W) {
+          ww += P.vol0 * w;
+          wall_n += ps.nrm[j] * (P.vol0 * w);
+          if (P.wall_colour) gradc += kgradW(P.kernel, rij, r) * (P.vol0 * (ps.alpha[j] - ci));
+          return;
+        }
+        if (j == i || !is_fluid(ps.phase[j])) return;
+        const R cj = ps.phase[j] == L ? R(1) : R(0);
+        gradc += kgradW(P.kernel, rij, r) * (Vof(j) * (cj - ci));
+      });
+      const RotGrad<R> ro = rot_vjp(vr(gradc), vr(wall_n), ww, P.cos_theta, P.sin_theta, P.wall_adh_w, P.normal_eps,
+                                    P.wetting != 1 ? 1 : 0, nrmb[i]);
+      gcb[i] = ro.g;
+      wnb[i] = ro.wall_n;
+      wwb[i] = ro.ww;
+      cos_b += double(ro.cos_t);
+      sin_b += double(ro.sin_t);
+    }
+    theta_b = (cos_b * -double(P.sin_theta) + sin_b * double(P.cos_theta)) * (3.14159265358979323846 / 180.0);
+    for (int i = 0; i < n; ++i) {
+      const u8 ph = ps.phase[i];
+      if (!is_fluid(ph)) continue;
+      const R ci = ph == L ? R(1) : R(0);
+      grid.each(ps.x[i], [&](int j) {
+        if (ps.phase[j] == W) {
+          const GcGrad<R> o = gc_fw_vjp(c, vr(ps.x[i] - ps.x[j]), ps.alpha[j], ci, vr(ps.nrm[j]), P.wall_colour,
+                                        gcb[i], wnb[i], wwb[i]);
+          xr[i] += o.r;
+          alphab[j] += o.V_j;
+          return;
+        }
+        if (j == i || !is_fluid(ps.phase[j])) return;
+        const R cj = ps.phase[j] == L ? R(1) : R(0);
+        const GcGrad<R> o = gc_ff_vjp(c, vr(ps.x[i] - ps.x[j]), Vof(j), cj - ci, gcb[i]);
+        xr[i] += o.r;
+        xr[j] -= o.r;
+        Vb[j] += o.V_j;
+      });
+    }
+    return theta_b;
+  }
+
   V3 adjoint_rates(const std::vector<u32>& perm, V3 Ssum, std::vector<V3>& xb, std::vector<V3>& vb,
-                   std::vector<V3>& ab) {
+                   std::vector<V3>& ab, double& theta_b) {
     using namespace cryo_adj;
     using VR = Vec<R>;
     const int n = ps.n;
@@ -1840,6 +1954,9 @@ struct Solver {
       At[i] = vr(ab[i]) + vr(vb[i]) * half;  // v' = v_half + a' dt/2
       gb += At[i];                           // a = g_eff + ...
     }
+    // surface tension
+    std::vector<R> Vb(n, R(0)), alphab(n, R(0));
+    if (cfg.surface_tension == CRYO_ST_CSF && has_vapor) theta_b += csf_adjoint(c, At, xr, Vb, alphab);
     // momentum
     for (int i = 0; i < n; ++i) {
       if (!is_fluid(ps.phase[i])) continue;
@@ -1894,6 +2011,7 @@ struct Solver {
       const R cp1 = (sw1 > R(0) && !cl1) ? pwb1[w] / sw1 : R(0);
       const R swt = sw0 + sw1;
       const VR cvw = swt > R(0) ? vwb[w] * (R(1) / swt) : VR{0, 0, 0};
+      const R calpha = swt > R(0) ? alphab[w] / swt : R(0);  // alpha_w = sw0 / swt
       gb += G0 * cp0 + G1 * cp1;
       grid.each(ps.x[w], [&](int f) {
         if (!is_fluid(ps.phase[f])) return;
@@ -1905,13 +2023,20 @@ struct Solver {
         vrb[f] += o.v;
         pb[f] += o.p;
         rb[f] += o.rho;
+        if (calpha != R(0)) {
+          const V3 rwf = ps.x[w] - ps.x[f];
+          const R r2 = norm2(rwf);
+          if (r2 < P.support2)  // d alpha_w / d x_f = (c_f - alpha_w) / swt * dW/dx_f, dW/dx_f = -s r_wf
+            xr[f] += vr(rwf) * (-calpha * ((vap ? R(0) : R(1)) - ps.alpha[w]) * wendland_s(c, std::sqrt(r2)));
+        }
       });
     }
-    // equation of state, then density summation
+    // equation of state (and V = m / rho of the surface-tension sums), then density summation
     std::vector<R> sb(n, R(0));
     for (int i = 0; i < n; ++i) {
       const u8 ph = ps.phase[i];
       if (!is_fluid(ph)) continue;
+      rb[i] -= Vb[i] * (ps.m[i] / ps.rho[i]) / ps.rho[i];
       const R dpdr = eos_dp_drho(ph == V ? 1 : 0, ps.rho[i], ps.T[i], ps.p[i], P.rho0[0], P.tait_B, P.tait_gamma,
                                  P.gas_R, P.clamp_neg);
       sb[i] = (rb[i] + pb[i] * dpdr) * (ph == L ? ps.m0[i] : ps.m[i]);
@@ -1944,10 +2069,13 @@ struct Solver {
     return v3(gb);
   }
 
-  void adjoint_slosh(long long steps, const float* acc, double* loss, double* grad, double* force) {
+  void adjoint_slosh(long long steps, const float* acc, double* loss, double* grad, double* force, double* grad_theta) {
     check_adjoint_scope();
     if (steps < 0) throw std::invalid_argument("adjoint: steps must be >= 0");
+    const bool fresh = !initialized;  // the start-up rates (and their a) belong to this run
     ensure_initialized();
+    const Particles start_state = ps;
+    const V3 start_acc = frame_acc;
     const V3 origin = P.frame_origin;
     const V3 zero{0, 0, 0};
     auto acc_of = [&](long long k) { return V3{R(acc[3 * k]), R(acc[3 * k + 1]), R(acc[3 * k + 2])}; };
@@ -1979,6 +2107,7 @@ struct Solver {
     // reverse sweep over the taped states; the solver ends in the forward end state
     Particles end_state = ps;
     std::vector<V3> xb(ps.n, zero), vb(ps.n, zero), ab(ps.n, zero);
+    double theta_b = 0;
     for (long long k = steps - 1; k >= 0; --k) {
       TapeEntry& t = tape[static_cast<std::size_t>(k)];
       ps = t.ps;
@@ -1986,11 +2115,27 @@ struct Solver {
       apply_frame(acc_of(k), zero, zero, origin);
       P.dt = R(cfg.fixed_dt);
       evaluate_rates();
-      const V3 gb = adjoint_rates(t.perm, S[static_cast<std::size_t>(k)], xb, vb, ab);
+      const V3 gb = adjoint_rates(t.perm, S[static_cast<std::size_t>(k)], xb, vb, ab, theta_b);
       grad[3 * k] = -double(gb.x);
       grad[3 * k + 1] = -double(gb.y);
       grad[3 * k + 2] = -double(gb.z);
     }
+    if (fresh && steps > 0 && cfg.surface_tension == CRYO_ST_CSF && has_vapor) {
+      // the start-up a depends on the contact angle too: ab now holds its cotangent
+      ps = start_state;
+      grid_build(grid, ps);
+      apply_frame(start_acc, zero, zero, origin);
+      evaluate_rates();
+      using VR = cryo_adj::Vec<R>;
+      const int n = ps.n;
+      cryo_adj::Ctx<R> c{P.kernel.norm, P.kernel.inv_h, P.kernel.h, P.support2, R(0.01) * P.kernel.h * P.kernel.h,
+                         R(0.6) * P.dx, P.dt, P.vol0, P.no_slip};
+      std::vector<VR> At(n), xr(n, VR{0, 0, 0});
+      for (int i = 0; i < n; ++i) At[i] = is_fluid(ps.phase[i]) ? VR{ab[i].x, ab[i].y, ab[i].z} : VR{0, 0, 0};
+      std::vector<R> Vb(n, R(0)), alphab(n, R(0));
+      theta_b += csf_adjoint(c, At, xr, Vb, alphab);
+    }
+    if (grad_theta) *grad_theta = theta_b;
     ps = end_state;        // scratch included: a is the end state's
     grid_build(grid, ps);  // sorted already: the identity permutation
     if (steps > 0) apply_frame(acc_of(steps - 1), zero, zero, origin);
@@ -2072,9 +2217,9 @@ int orc_run_steps(void* s, long long n) {
   GUARD(for (long long k = 0; k < n; ++k) static_cast<Solver*>(s)->step())
 }
 int orc_adjoint_slosh(void* s, long long steps, const float* acc, int checkpoint_every, double* loss, double* grad,
-                      double* force) {
+                      double* force, double* grad_contact_angle) {
   (void)checkpoint_every;  // the oracle tapes every step
-  GUARD(static_cast<Solver*>(s)->adjoint_slosh(steps, acc, loss, grad, force))
+  GUARD(static_cast<Solver*>(s)->adjoint_slosh(steps, acc, loss, grad, force, grad_contact_angle))
 }
 int orc_run_until(void* s, double t) {
   GUARD(Solver& S = *static_cast<Solver*>(s); while (S.time < t) S.step())

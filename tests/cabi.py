@@ -235,10 +235,11 @@ def _adjoint_slosh(call, handle, acc, checkpoint_every, grad):
     loss = C.c_double()
     g = np.zeros((steps, 3), np.float64) if grad else None
     f = np.zeros((steps, 3), np.float64)
+    th = C.c_double(0.0)
     dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
     call("adjoint_slosh", handle, C.c_int64(steps), _f32p(acc), C.c_int32(checkpoint_every), C.byref(loss), dp(g),
-         dp(f))
-    return loss.value, g, f
+         dp(f), C.byref(th) if grad else None)
+    return loss.value, g, f, (th.value if grad else None)
 
 
 class CpuSolver:
@@ -305,7 +306,7 @@ class CpuSolver:
         self._call("relax", self.h, C.c_double(duration), C.c_double(damping))
 
     def adjoint_slosh(self, acc, checkpoint_every=20, grad=True):
-        """(loss, dloss/dacc or None, per-step sum of fw) of the oracle adjoint."""
+        """(loss, dloss/dacc or None, per-step sum of fw, dloss/dcontact_angle_deg or None), oracle."""
         return _adjoint_slosh(self._call, self.h, acc, checkpoint_every, grad)
 
     def next_dt(self):
@@ -523,7 +524,7 @@ class HipSolver:
         self._call("solver_damp_velocities", self.h, C.c_float(f))
 
     def adjoint_slosh(self, acc, checkpoint_every=20, grad=True):
-        """(loss, dloss/dacc or None, per-step sum of fw): cryo_solver_adjoint_slosh."""
+        """(loss, dloss/dacc or None, per-step sum of fw, dloss/dcontact_angle_deg or None)."""
         return _adjoint_slosh(lambda name, *a: self._call("solver_" + name, *a), self.h, acc, checkpoint_every, grad)
 
     def initialize(self):

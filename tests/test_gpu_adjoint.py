@@ -1,9 +1,9 @@
-"""The hand-written adjoint of the hydrodynamic step on the GPU
-(cryo_solver_adjoint_slosh) against the FP64 CPU adjoint and central
-differences of the FP64 forward step, plus its own invariants.
+"""The hand-written adjoint of the step on the GPU (cryo_solver_adjoint_slosh)
+against the FP64 CPU adjoint and central differences of the FP64 forward step,
+plus its own invariants.
 
 Loss: sum_k dt |S_k,xy|^2 with S_k the wall load after step k; parameters: the
-frame acceleration of each step.  Bar: the FP32 GPU gradient within 1e-3 (of
+frame acceleration of each step and (CSF surface tension on) the contact angle.  Bar: the FP32 GPU gradient within 1e-3 (of
 its largest component) of the FP64 references.
 """
 import sys
@@ -38,14 +38,14 @@ def test_forward_pass_is_the_plain_step_sequence():
     say("test_forward_pass_is_the_plain_step_sequence")
     fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=8)
     a = hip(cfg, fluid, ps, lo, hi)
-    loss, _, load = a.adjoint_slosh(acc, grad=False)
+    loss, _, load, _ = a.adjoint_slosh(acc, grad=False)
     b = hip(cfg, fluid, ps, lo, hi)
     b.initialize()
     for k in range(len(acc)):
         b.set_frame_motion(acc=acc[k])
         b.step()
     c = hip(cfg, fluid, ps, lo, hi)
-    loss_c, grad_c, load_c = c.adjoint_slosh(acc, checkpoint_every=3)
+    loss_c, grad_c, load_c, _ = c.adjoint_slosh(acc, checkpoint_every=3)
     assert loss_c == loss and np.array_equal(load_c, load) and np.isfinite(grad_c).all()
     for s in (a, c):  # with or without the reverse sweep: the forward end state
         assert s.step_count == b.step_count == len(acc)
@@ -58,8 +58,8 @@ def test_forward_pass_is_the_plain_step_sequence():
 def test_gradient_matches_fp64_adjoint_and_central_differences():
     say("test_gradient_matches_fp64_adjoint_and_central_differences")
     fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=12)
-    loss, g, load = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=5)
-    loss64, g64, load64 = orc(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
+    loss, g, load, _ = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=5)
+    loss64, g64, load64, _ = orc(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
     say("loss fp32", loss, "fp64", loss64)
     err = np.abs(g - g64).max() / np.abs(g64).max()
     say("gradient vs fp64 adjoint (max-normalised):", err)
@@ -76,6 +76,30 @@ def test_gradient_matches_fp64_adjoint_and_central_differences():
         assert abs(g[k, c] - fd) <= 1e-3 * np.abs(g64).max()
 
 
+def test_csf_gradients_match_fp64_adjoint_and_central_differences():
+    say("test_csf_gradients_match_fp64_adjoint_and_central_differences")
+    fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=12, csf=True, contact_angle=30.0)
+    loss, g, _, dth = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=5)
+    loss64, g64, _, dth64 = orc(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
+    err = np.abs(g - g64).max() / np.abs(g64).max()
+    say("loss", loss, "fp64", loss64, "acc gradient error", err, "dL/dtheta", dth, "fp64", dth64)
+    assert abs(loss - loss64) <= 1e-4 * loss64
+    assert err <= 1e-3
+    assert abs(dth - dth64) <= 1e-3 * abs(dth64)
+    d = 0.01
+    out = []
+    for th in (30.0 + d, 30.0 - d):
+        cfg.contact_angle_deg = th
+        out.append(orc(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, grad=False)[0])
+    fd = (out[0] - out[1]) / (np.float32(30.0 + d) - np.float32(30.0 - d))
+    say("dL/dtheta central difference (fp64)", fd)
+    assert abs(dth - fd) <= 1e-3 * abs(fd)
+    # a checkpoint interval of one step gives the same bits
+    cfg.contact_angle_deg = 30.0
+    out1 = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=1)
+    assert np.array_equal(out1[1], g) and out1[3] == dth
+
+
 def test_checkpoint_interval_does_not_change_the_gradient():
     say("test_checkpoint_interval_does_not_change_the_gradient")
     fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=10)
@@ -88,13 +112,14 @@ def test_checkpoint_interval_does_not_change_the_gradient():
 
 def test_adjoint_rejects_undifferentiated_physics_and_bad_arguments():
     say("test_adjoint_rejects_undifferentiated_physics_and_bad_arguments")
-    fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=2)
-    for field, value in (("surface_tension", 1), ("phase_change", 1), ("fixed_dt", 0.0), ("particle_shifting", 1)):
-        saved = getattr(cfg, field)
+    for csf, field, value in ((False, "surface_tension", 2), (False, "phase_change", 1), (False, "fixed_dt", 0.0),
+                              (False, "particle_shifting", 1), (True, "csf_smooth_normals", 1),
+                              (True, "csf_balanced", 1)):
+        fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=2, csf=csf)
         setattr(cfg, field, value)
         with pytest.raises(ValueError):
             hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
-        setattr(cfg, field, saved)
+    fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=2)
     with pytest.raises(ValueError):
         hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=0)
 
@@ -129,7 +154,7 @@ def test_autograd_function_over_the_module():
     loss.backward()
     # the same run through the C ABI, its per-step gradient summed per segment
     acc = seg.detach().numpy()[step_segments(steps, nseg)].astype(np.float32)
-    l2, g2, _ = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=5)
+    l2, g2, _, _ = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=5)
     gseg = np.zeros((nseg, 3))
     np.add.at(gseg, step_segments(steps, nseg), g2)
     assert loss.item() == l2
@@ -152,15 +177,15 @@ def test_c5_tank_gradient_matches_cpu_adjoints():
 
     fluid, cfg, ps, lo, hi, acc = bench.c5_case(cabi, steps=6, segments=3)
     say("N =", len(ps), "dt =", cfg.fixed_dt)
-    loss, g, load = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=2)
-    loss_b, g_b, _ = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=6)
+    loss, g, load, _ = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=2)
+    loss_b, g_b, _, _ = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=6)
     assert loss_b == loss and np.array_equal(g_b, g)
     o32 = CpuSolver("orc", cfg, fluid, 32)
     o32.set_particles(ps, lo, hi)
-    loss32, g32, load32 = o32.adjoint_slosh(acc)
+    loss32, g32, load32, _ = o32.adjoint_slosh(acc)
     assert np.allclose(load, load32, rtol=1e-12, atol=0)
     err32 = np.abs(g - g32).max() / np.abs(g32).max()
-    loss64, g64, _ = orc(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
+    loss64, g64, _, _ = orc(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
     err64 = np.abs(g - g64).max() / np.abs(g64).max()
     say("loss", loss, "fp64", loss64, "rel", abs(loss - loss64) / loss64)
     say("gradient vs fp32 adjoint", err32, "vs fp64 adjoint", err64)

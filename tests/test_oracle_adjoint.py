@@ -1,12 +1,16 @@
 """The FP64 CPU adjoint (oracle/cryo_oracle.cpp, Solver::adjoint_slosh) against
-central differences of the FP64 forward step, every parameter of the case.
+central differences of the FP64 forward step, every parameter of the case, with
+and without CSF surface tension (then the contact angle is a parameter too).
 
 It pins the hand-written per-pair derivatives (csrc/adjoint_pair.hpp) that the
 GPU adjoint uses: dropping any one term of them (the guard repulsion or
 damping, no-slip shear, wall or fluid artificial viscosity, the Shepard wall
-pressure's gravity term) moves the gradient by 3e-6 to 5e-3 of its largest
-component, while the central differences of this case agree with the correct
-adjoint to better than 2e-7 of it.
+pressure's gravity term; with CSF the curvature denominator, the neighbour
+normals, the wall-normal sum, the tangent projection of the rotation, the
+blend weight, the wall colour) moves the acceleration gradient by 3e-6 to 5e-3
+of its largest component or the contact-angle gradient by 6e-5 or more, while
+the central differences of this case agree with the correct adjoint to 2e-7
+and 1e-8.
 """
 import numpy as np
 import pytest
@@ -25,10 +29,10 @@ def oracle(cfg, fluid, ps, lo, hi):
 
 def test_fp64_adjoint_matches_central_differences():
     fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=12)
-    loss, grad, load = oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
+    loss, grad, load, _ = oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
     assert loss > 0 and grad.shape == acc.shape
     # the forward pass alone gives the same loss and loads
-    loss_f, none, load_f = oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, grad=False)
+    loss_f, none, load_f, _ = oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, grad=False)
     assert none is None and loss_f == loss and np.array_equal(load_f, load)
     fd = np.zeros_like(grad)
     for k in range(acc.shape[0]):
@@ -46,7 +50,7 @@ def test_fp64_adjoint_matches_central_differences():
 def test_loss_is_the_integrated_wall_load():
     fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=5)
     s = oracle(cfg, fluid, ps, lo, hi)
-    loss, _, load = s.adjoint_slosh(acc, grad=False)
+    loss, _, load, _ = s.adjoint_slosh(acc, grad=False)
     assert loss == pytest.approx(cfg.fixed_dt * float((load[:, :2] ** 2).sum()), rel=1e-15)
     # the negated slosh force of the diagnostics (reported as float) is the last step's load
     d = s.diagnostics()
@@ -55,10 +59,37 @@ def test_loss_is_the_integrated_wall_load():
 
 
 def test_adjoint_rejects_undifferentiated_physics():
-    fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=2)
-    for field, value in (("surface_tension", 1), ("thermal", 1), ("fixed_dt", 0.0), ("transport_velocity", 1)):
-        saved = getattr(cfg, field)
+    for csf, field, value in ((False, "surface_tension", 2), (False, "thermal", 1), (False, "fixed_dt", 0.0),
+                              (False, "transport_velocity", 1), (True, "csf_balanced", 1), (True, "wetting_model", 1),
+                              (True, "csf_smooth_normals", 1)):
+        fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=2, csf=csf)
         setattr(cfg, field, value)
         with pytest.raises(ValueError):
             oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
-        setattr(cfg, field, saved)
+
+
+def test_fp64_adjoint_with_csf_matches_central_differences():
+    """CSF surface tension with the contact-angle rotation: every acceleration
+    component and the contact angle (set before start-up, so the start-up
+    rates' share counts)."""
+    fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=12, csf=True, contact_angle=30.0)
+    loss, grad, _, dtheta = oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
+    fd = np.zeros_like(grad)
+    for k in range(acc.shape[0]):
+        for c in range(3):
+            a1, a2 = acc.copy(), acc.copy()
+            a1[k, c] += STEP
+            a2[k, c] -= STEP
+            h = float(a1[k, c]) - float(a2[k, c])
+            fd[k, c] = (oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(a1, grad=False)[0] -
+                        oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(a2, grad=False)[0]) / h
+    assert np.abs(grad - fd).max() / np.abs(fd).max() <= 1e-6
+    d = 0.01  # degrees
+    out = []
+    for th in (30.0 + d, 30.0 - d):
+        cfg.contact_angle_deg = th
+        out.append(oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, grad=False)[0])
+    cfg.contact_angle_deg = 30.0
+    fd_theta = (out[0] - out[1]) / (np.float32(30.0 + d) - np.float32(30.0 - d))
+    assert dtheta != 0.0
+    assert abs(dtheta - fd_theta) <= 1e-5 * abs(fd_theta), (dtheta, fd_theta)
