@@ -95,7 +95,7 @@ struct AdjArgs {
   int stride;
   // surface tension (CSF with the contact-angle rotation); csf = 0: not used
   int csf;
-  float4* cs;   // {num_bar, den_bar} of each fluid particle's curvature sums
+  float4* cs;   // {num_bar, den_bar, kappa} of each fluid particle's curvature sums
   float* csv;   // cotangent of V = m / rho from the curvature and colour sums
   float4* gcb;  // cotangents of the colour gradient (xyz) and the wall weight ww (w)
   float4* wnb;  // cotangent of the wall-normal sum

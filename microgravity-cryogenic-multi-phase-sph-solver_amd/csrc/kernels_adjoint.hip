@@ -10,8 +10,9 @@
 // Loss: sum over steps k of dt (S_x^2 + S_y^2), S_k = sum of fw over the fluid
 // particles after step k's rates; parameters: the frame acceleration of each
 // step (g_eff = g - acc) and the contact angle.  One reverse step runs after
-// the rates of step k have been re-evaluated on the taped post-sort state, in
-// that step's sorted order:
+// the rates of step k (all passes but the forces pass, whose outputs the
+// reverse step does not read) have been re-evaluated on the taped post-sort
+// state, in that step's sorted order:
 //   k_adj_prep     a's total cotangent  at = a_bar + (dt/2) v_bar  (+ its g share)
 //   k_adj_csf1     (CSF) cotangents of each particle's curvature sums
 //   k_adj_csf2     (CSF) both sides of the curvature pairs, the normalisation
@@ -161,19 +162,25 @@ __global__ __launch_bounds__(kBlock) void k_adj_csf1(PV pv, GridDev g, SimParams
   float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (is_fluid(pv.s.phase[i]) && live_normal(pv, i)) {
     const float coeff = 2.0f * P.sigma / (P.rho0[0] + P.rho0[1]);
-    const float kappa = pv.t.kappa[i];
     const float4 at = A.at[i], nr = pv.s.nrm[i];
-    const float kb = coeff * (at.x * nr.x + at.y * nr.y + at.z * nr.z);
-    const V3 xi = ld3(pv.s.x[i]);
-    float den = 0.0f;
+    const V3 xi = ld3(pv.s.x[i]), ni = ld3(pv.t.ns[i]);
+    float num = 0.0f, den = 0.0f;  // the curvature sums (the forward pass's, recomputed)
     for_each_pair(L, g, i, xi, [&](int j) {
       if (j == i || !is_fluid(pv.s.phase[j]) || !live_normal(pv, j)) return;
-      const float r2 = norm2(xi - ld3(pv.s.x[j]));
+      const V3 rij = xi - ld3(pv.s.x[j]);
+      const float r2 = norm2(rij);
       if (!(r2 < P.support2)) return;
       const float r = sqrtf(r2);
-      den += pv.s.m[j] / pv.s.rho[j] * r * (-kdW(P.kernel, r));
+      const float dW = kdW(P.kernel, r);
+      const float Vj = pv.s.m[j] / pv.s.rho[j];
+      num += Vj * dot(ni - ld3(pv.t.ns[j]), rij * (dW / r));
+      den += Vj * r * (-dW);
     });
-    if (den > 0.0f) out = make_float4(3.0f * kb / den, -kappa * kb / den, 0.0f, 0.0f);
+    if (den > 0.0f) {
+      const float kappa = 3.0f * num / den;
+      const float kb = coeff * (at.x * nr.x + at.y * nr.y + at.z * nr.z);
+      out = make_float4(3.0f * kb / den, -kappa * kb / den, kappa, 0.0f);
+    }
   }
   A.cs[i] = out;
 }
@@ -227,7 +234,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_csf2(PV pv, GridDev g, SimParams
       Vb += o2.V_j;
     });
     const float4 at = A.at[k];
-    AV nrmb = livek ? AV{at.x, at.y, at.z} * (coeff * pv.t.kappa[k]) : AV{0.0f, 0.0f, 0.0f};
+    AV nrmb = livek ? AV{at.x, at.y, at.z} * (coeff * csk.z) : AV{0.0f, 0.0f, 0.0f};
     const AV nrk = av(pv.s.nrm[k]);
     const float G = sqrtf(cryo_adj::dot(nrk, nrk));
     if (G > P.normal_eps) nrmb += (nsb - nsk * cryo_adj::dot(nsk, nsb)) * (1.0f / G);  // ns = nrm / |nrm|

@@ -719,7 +719,9 @@ struct cryo_solver {
     pending_dead = 0;
   }
 
-  void evaluate_rates() {  // sph_solver.cu:449-480
+  // forces = false: everything but the forces pass (what the adjoint's
+  // reverse sweep reads: densities, pressures, wall values, normals, lists).
+  void evaluate_rates(bool forces = true) {  // sph_solver.cu:449-480
     PV pv = view();
     const GridDev gd = grid();
     const bool csf_mode = cfg.surface_tension == CRYO_ST_CSF && has_vapor;
@@ -740,6 +742,10 @@ struct cryo_solver {
     if (csf_mode && P.csf_smooth) launch_smooth_normals(pv, gd, P, L, n, stream);
     if (P.thermal) launch_heat_norm(state.get(), partials.get(), nb, heat_sum, stream);
     prof_end(5);
+    if (!forces) {
+      check_launch();
+      return;
+    }
     prof_begin(6);
     launch_forces(pv, gd, P, L, state.get(), n, csf_mode, cfg.surface_tension == CRYO_ST_AKINCI,
                   cfg.density_mode == CRYO_DENSITY_CONTINUITY, P.thermal != 0, tiles_ok && use_tiles, stream);
@@ -980,10 +986,11 @@ struct cryo_solver {
 
   // loss = sum_k dt (S_x^2 + S_y^2), S_k = sum of fw over fluid particles
   // after step k's rates; step k runs with frame acceleration acc[3k..3k+2].
-  // grad (optional) = d loss / d acc by the reverse sweep: checkpoints every
-  // `every` steps, each segment re-run forward with its post-sort states
-  // taped, then walked back one step at a time (rates re-evaluated on the
-  // taped state).  The solver ends in the forward end state.
+  // grad (optional) = d loss / d acc (and grad_theta = d loss / d contact
+  // angle) by the reverse sweep: checkpoints every `every` steps, each segment
+  // re-run forward with its post-sort states taped, then walked back one step
+  // at a time (the rates but the forces pass re-evaluated on the taped
+  // state).  The solver ends in the forward end state.
   void adjoint_slosh(long long steps, const float* acc, int every, double* loss, double* grad, double* force,
                      double* grad_theta) {
     check_adjoint_scope();
@@ -1044,7 +1051,7 @@ struct cryo_solver {
           const Snap& t = *adj_tape[static_cast<std::size_t>(k - k0)];
           snap_load(t, false);
           grid_build(false);  // already in cell order: the identity permutation, the same cell table
-          evaluate_rates();
+          evaluate_rates(false);
           const AdjArgs A = adj_args(cur, t.perm.get(), adj_S.get() + 3 * k, csf);
           launch_adj_step(view(), grid(), P, lists(), state.get(), A, n, adj_G.get() + 3 * k, adj_T.get() + 2 * k,
                           stream);
@@ -1057,7 +1064,7 @@ struct cryo_solver {
         // adj_c[cur][2] holds its cotangent
         snap_load(*adj_ckpt[0], true);
         grid_build(false);
-        evaluate_rates();
+        evaluate_rates(false);
         d2d(adj_at.get(), adj_c[cur][2].get(), n);
         const AdjArgs A = adj_args(cur, nullptr, nullptr, true);
         launch_adj_csf_only(view(), grid(), P, lists(), state.get(), A, n, adj_T.get() + 2 * ns, stream);
