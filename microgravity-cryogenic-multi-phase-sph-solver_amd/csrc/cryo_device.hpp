@@ -303,7 +303,11 @@ struct GridDev {
   int nx, ny, nz, num_cells;
   const int* off;
   float cull_pad;  // cell units: rounding allowance of the cell-culled sweep
+  // per cell: kCellWall | kCellFluid of the particles it holds (built with the
+  // cell table each sort)
+  const uint8_t* flags;
 };
+constexpr int kCellWall = 1, kCellFluid = 2;
 __host__ __device__ inline void cell_coords(const GridDev& g, V3 p, int& cx, int& cy, int& cz) {
   cx = clampv(static_cast<int>(floorf((p.x - g.ox) * g.inv_cell)), 0, g.nx - 1);
   cy = clampv(static_cast<int>(floorf((p.y - g.oy) * g.inv_cell)), 0, g.ny - 1);
@@ -330,6 +334,23 @@ __host__ __device__ inline void for_each_nbr(const GridDev& g, V3 p, F&& f) {
       for (int j = s; j < e; ++j) f(j);
     }
   }
+}
+
+// Whether any of the 27 cells around p holds a particle of the given kind
+// (a neighbour of that kind is possible only then).
+__device__ inline bool cells_any(const GridDev& g, V3 p, int mask) {
+  int cx, cy, cz;
+  cell_coords(g, p, cx, cy, cz);
+  const int z0 = cz > 0 ? cz - 1 : 0, z1 = cz < g.nz - 1 ? cz + 1 : cz;
+  const int y0 = cy > 0 ? cy - 1 : 0, y1 = cy < g.ny - 1 ? cy + 1 : cy;
+  const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx < g.nx - 1 ? cx + 1 : cx;
+  int any = 0;
+  for (int z = z0; z <= z1; ++z)
+    for (int y = y0; y <= y1; ++y) {
+      const int row = (z * g.ny + y) * g.nx;
+      for (int x = x0; x <= x1; ++x) any |= g.flags[row + x];
+    }
+  return (any & mask) != 0;
 }
 
 // The same sweep without the cells that provably hold nothing within `radius`

@@ -30,6 +30,7 @@ int radix_tiles(int n);
 void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, int n,
                        int key_bits, uint32_t* tile_hist, uint32_t* row_total, hipStream_t s, bool* result_in_alt);
 void launch_cell_offsets(const uint32_t* keys, int n, int num_cells, int* off, DevState* st, hipStream_t s);
+void launch_cell_flags(const int* off, const uint8_t* phase, int num_cells, uint8_t* flags, hipStream_t s);
 void launch_gather(const Persist& dst, const Persist& src, const uint32_t* perm, const DevState* st, int n_ub,
                    hipStream_t s);
 void launch_neighbor_stats(const float4* x, const uint32_t* id, const GridDev& g, int n, float r2max, int* counts,

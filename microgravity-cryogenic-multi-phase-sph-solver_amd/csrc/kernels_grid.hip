@@ -282,6 +282,24 @@ void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint3
   }
 }
 
+// Per-cell kinds of the particles each cell holds (kCellWall | kCellFluid).
+__global__ __launch_bounds__(kBlock) void k_cell_flags(const int* off, const uint8_t* phase, int num_cells,
+                                                      uint8_t* flags) {
+  const int c = gtid();
+  if (c >= num_cells) return;
+  int f = 0;
+  for (int j = off[c], e = off[c + 1]; j < e; ++j) {
+    const uint8_t ph = phase[j];
+    f |= ph == kWall ? kCellWall : (is_fluid(ph) ? kCellFluid : 0);
+  }
+  flags[c] = static_cast<uint8_t>(f);
+}
+
+void launch_cell_flags(const int* off, const uint8_t* phase, int num_cells, uint8_t* flags, hipStream_t s) {
+  if (num_cells <= 0) return;
+  k_cell_flags<<<blocks_for(num_cells), kBlock, 0, s>>>(off, phase, num_cells, flags);
+}
+
 void launch_cell_offsets(const uint32_t* keys, int n, int num_cells, int* off, DevState* st, hipStream_t s) {
   k_cell_offsets<<<blocks_for(static_cast<long long>(num_cells) + 1), kBlock, 0, s>>>(keys, n, num_cells, off, st);
 }

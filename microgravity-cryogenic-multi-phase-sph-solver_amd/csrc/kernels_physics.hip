@@ -65,7 +65,9 @@ __global__ __launch_bounds__(kBlock) void k_density(PV pv, GridDev g, SimParams 
   if (!is_fluid(ph)) {
     pv.t.sigma[i] = 1.0f / P.vol0;
     pv.t.rmin[i] = P.kernel.support;
-    if (ph == kWall) {  // wall targets only ever need their fluid neighbours
+    // wall targets only ever need their fluid neighbours: none without a fluid
+    // particle in the 27 cells
+    if (ph == kWall && cells_any(g, xi, kCellFluid)) {
       for_each_cand_x(g, xi, P.kernel.support, pv.s.x, [&](int j, float4 xj) {
         if (norm2(xi - ld3(xj)) >= P.support2) return;
         if (!is_fluid(pv.s.phase[j])) return;
@@ -322,6 +324,43 @@ __device__ inline float interface_particle(const PV& pv, const SimParams& P, int
     }
   }
   return hw;
+}
+
+// Runs without vapour, SPS or Akinci (every fluid particle is liquid): a
+// fluid neighbour then adds exact zeros (c_J - c_I = 0 to the colour
+// gradient, nothing to alpha's numerator), and a wall within 2h of a liquid
+// particle has colour alpha_w = 1, so only wall neighbours can add anything.
+// Particles with no wall in their 27 cells get the outputs of an empty sum
+// directly; the others sum over their wall neighbours only.  Bitwise the
+// full pass's results (adding a zero to a running sum that starts at +0
+// never changes it).  Resets the dt maxima and writes the heat partials as
+// k_interface does.
+template <bool KS>
+__global__ __launch_bounds__(kBlock) void k_interface_liquid(PV pv, GridDev g, SimParams P, NbrList L, DevState* st,
+                                                            int n, double* heat_partials) {
+  const int i = gtid();
+  if (i == 0) {
+    for (int k = 0; k < 5; ++k) st->max_bits[k] = 0u;
+  }
+  double heat_contrib = 0.0;
+  if (i < n && is_fluid(pv.s.phase[i])) {
+    const V3 xi = ld3(pv.s.x[i]);
+    if (cells_any(g, xi, kCellWall)) {
+      auto walls = [&](auto&& f) {
+        for_each_pair(L, g, i, xi, [&](int j) {
+          if (pv.s.phase[j] == kWall) f(j, load_iface_nbr<false>(pv, P, j));
+        });
+      };
+      heat_contrib = static_cast<double>(interface_particle<false, KS>(pv, P, i, 0, walls));
+    } else {
+      pv.t.alpha[i] = 0.0f;
+      pv.t.wall_w[i] = 0.0f;
+      pv.t.heat_w[i] = 0.0f;
+      pv.t.vw[i] = st3(V3{0, 0, 0});
+      pv.s.nrm[i] = st3(V3{0, 0, 0});
+    }
+  }
+  if (heat_partials) block_sum_store(heat_contrib, heat_partials + blockIdx.x);
 }
 
 // Global-memory instance.  only_overflow = 0: every particle, resets the dt
@@ -1202,6 +1241,11 @@ int launch_interface(const PV& pv, const GridDev& g, const SimParams& P, const N
                      bool csf_mode, double* heat_partials, bool tiled, hipStream_t s) {
   const int nb = blocks_for(n > 0 ? n : 1);
   const int cm = csf_mode ? 1 : 0;
+  if (!P.has_vapor && P.st_model != 2 && !P.sps && L.valid && g.flags) {  // liquid only
+    if (P.compensated) k_interface_liquid<true><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, n, heat_partials);
+    else k_interface_liquid<false><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, n, heat_partials);
+    return nb;
+  }
   tiled = tiled && L.valid;
   if (P.compensated) {
     if (P.sps) interface_variant<true, true>(pv, g, P, L, st, n, cm, heat_partials, tiled, s);

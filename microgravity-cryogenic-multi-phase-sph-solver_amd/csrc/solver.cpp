@@ -226,6 +226,7 @@ struct cryo_solver {
   // grid and scans
   DevBuf<uint32_t> keys_[2], vals_[2], tile_hist, row_total, split_flag, split_off, scan_sums, scan_total;
   DevBuf<int> cell_off;
+  DevBuf<uint8_t> cell_flags;
   // neighbour lists of the current step (NbrList in cryo_device.hpp)
   static constexpr int kListK = 96;
   DevBuf<int> nbr_idx, nbr_count;
@@ -391,6 +392,7 @@ struct cryo_solver {
   GridDev grid() {
     GridDev gd = g;
     gd.off = cell_off.get();
+    gd.flags = cell_flags.get();
     return gd;
   }
   NbrList lists() {
@@ -682,6 +684,7 @@ struct cryo_solver {
     // |error| of (p - origin) * inv_cell is a few ulp of the largest index
     g.cull_pad = 1e-3f + 5e-7f * static_cast<float>(std::max(nx, std::max(ny, nz)));
     cell_off.resize(static_cast<std::size_t>(cells) + 2);
+    cell_flags.resize(static_cast<std::size_t>(cells));
     key_bits = 1;
     while ((1ll << key_bits) <= cells) ++key_bits;
     last_perm = nullptr;
@@ -708,6 +711,7 @@ struct cryo_solver {
     prof_end(1);
     prof_begin(2);
     launch_gather(persist(1 - set), persist(set), perm, state.get(), n, stream);
+    launch_cell_flags(cell_off.get(), persist(1 - set).phase, g.num_cells, cell_flags.get(), stream);
     prof_end(2);
     check_launch();
     set = 1 - set;
