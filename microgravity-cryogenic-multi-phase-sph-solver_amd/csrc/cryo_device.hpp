@@ -303,11 +303,13 @@ struct GridDev {
   int nx, ny, nz, num_cells;
   const int* off;
   float cull_pad;  // cell units: rounding allowance of the cell-culled sweep
-  // per cell: kCellWall | kCellFluid of the particles it holds (built with the
-  // cell table each sort)
-  const uint8_t* flags;
+  // per cell: which kinds of particle it holds (kCellWall, kCellLiquid,
+  // kCellVapor; built with the cell table each sort) and, during the phase
+  // change pass, whether one of them gives mass to liquid / vapour receivers
+  // (kCellSrcL, kCellSrcV)
+  uint8_t* flags;
 };
-constexpr int kCellWall = 1, kCellFluid = 2;
+constexpr int kCellWall = 1, kCellLiquid = 2, kCellVapor = 4, kCellFluid = 6, kCellSrcL = 8, kCellSrcV = 16;
 __host__ __device__ inline void cell_coords(const GridDev& g, V3 p, int& cx, int& cy, int& cz) {
   cx = clampv(static_cast<int>(floorf((p.x - g.ox) * g.inv_cell)), 0, g.nx - 1);
   cy = clampv(static_cast<int>(floorf((p.y - g.oy) * g.inv_cell)), 0, g.ny - 1);

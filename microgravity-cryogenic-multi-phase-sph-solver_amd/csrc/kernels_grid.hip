@@ -290,7 +290,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_flags(const int* off, const uin
   int f = 0;
   for (int j = off[c], e = off[c + 1]; j < e; ++j) {
     const uint8_t ph = phase[j];
-    f |= ph == kWall ? kCellWall : (is_fluid(ph) ? kCellFluid : 0);
+    f |= ph == kWall ? kCellWall : (ph == kLiquid ? kCellLiquid : (ph == kVapor ? kCellVapor : 0));
   }
   flags[c] = static_cast<uint8_t>(f);
 }

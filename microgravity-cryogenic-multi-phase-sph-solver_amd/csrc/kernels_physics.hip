@@ -326,38 +326,47 @@ __device__ inline float interface_particle(const PV& pv, const SimParams& P, int
   return hw;
 }
 
-// Runs without vapour, SPS or Akinci (every fluid particle is liquid): a
-// fluid neighbour then adds exact zeros (c_J - c_I = 0 to the colour
-// gradient, nothing to alpha's numerator), and a wall within 2h of a liquid
-// particle has colour alpha_w = 1, so only wall neighbours can add anything.
-// Particles with no wall in their 27 cells get the outputs of an empty sum
-// directly; the others sum over their wall neighbours only.  Bitwise the
-// full pass's results (adding a zero to a running sum that starts at +0
-// never changes it).  Resets the dt maxima and writes the heat partials as
-// k_interface does.
+// The interface pass for runs without SPS or Akinci, skipping what is
+// provably zero.  A neighbour of the particle's own phase adds exact zeros
+// (c_J - c_I = 0 to the colour gradient, nothing to alpha's numerator), so a
+// particle with neither a wall nor the other phase in its 27 cells gets the
+// outputs of an empty sum directly.  Without vapour at all, a wall within 2h
+// of a liquid particle has colour alpha_w = 1 and so adds zeros too: only wall
+// neighbours are summed.  Bitwise the full pass's results (adding a zero to a
+// running sum that starts at +0 never changes it).  Resets the dt maxima and
+// writes the heat partials as k_interface does.
 template <bool KS>
-__global__ __launch_bounds__(kBlock) void k_interface_liquid(PV pv, GridDev g, SimParams P, NbrList L, DevState* st,
-                                                            int n, double* heat_partials) {
+__global__ __launch_bounds__(kBlock) void k_interface_sparse(PV pv, GridDev g, SimParams P, NbrList L, DevState* st,
+                                                            int n, int csf_mode, double* heat_partials) {
   const int i = gtid();
   if (i == 0) {
     for (int k = 0; k < 5; ++k) st->max_bits[k] = 0u;
   }
   double heat_contrib = 0.0;
-  if (i < n && is_fluid(pv.s.phase[i])) {
+  const uint8_t ph = i < n ? pv.s.phase[i] : kDead;
+  if (is_fluid(ph)) {
     const V3 xi = ld3(pv.s.x[i]);
-    if (cells_any(g, xi, kCellWall)) {
-      auto walls = [&](auto&& f) {
-        for_each_pair(L, g, i, xi, [&](int j) {
-          if (pv.s.phase[j] == kWall) f(j, load_iface_nbr<false>(pv, P, j));
-        });
-      };
-      heat_contrib = static_cast<double>(interface_particle<false, KS>(pv, P, i, 0, walls));
+    if (cells_any(g, xi, kCellWall | (ph == kLiquid ? kCellVapor : kCellLiquid))) {
+      if (P.has_vapor) {
+        auto all = [&](auto&& f) {
+          for_each_pair(L, g, i, xi, [&](int j) { f(j, load_iface_nbr<false>(pv, P, j)); });
+        };
+        heat_contrib = static_cast<double>(interface_particle<false, KS>(pv, P, i, csf_mode, all));
+      } else {
+        auto walls = [&](auto&& f) {
+          for_each_pair(L, g, i, xi, [&](int j) {
+            if (pv.s.phase[j] == kWall) f(j, load_iface_nbr<false>(pv, P, j));
+          });
+        };
+        heat_contrib = static_cast<double>(interface_particle<false, KS>(pv, P, i, csf_mode, walls));
+      }
     } else {
       pv.t.alpha[i] = 0.0f;
       pv.t.wall_w[i] = 0.0f;
       pv.t.heat_w[i] = 0.0f;
       pv.t.vw[i] = st3(V3{0, 0, 0});
       pv.s.nrm[i] = st3(V3{0, 0, 0});
+      if (csf_mode && !P.csf_smooth) pv.t.ns[i] = st3(V3{0, 0, 0});
     }
   }
   if (heat_partials) block_sum_store(heat_contrib, heat_partials + blockIdx.x);
@@ -1241,9 +1250,9 @@ int launch_interface(const PV& pv, const GridDev& g, const SimParams& P, const N
                      bool csf_mode, double* heat_partials, bool tiled, hipStream_t s) {
   const int nb = blocks_for(n > 0 ? n : 1);
   const int cm = csf_mode ? 1 : 0;
-  if (!P.has_vapor && P.st_model != 2 && !P.sps && L.valid && g.flags) {  // liquid only
-    if (P.compensated) k_interface_liquid<true><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, n, heat_partials);
-    else k_interface_liquid<false><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, n, heat_partials);
+  if (P.st_model != 2 && !P.sps && L.valid && g.flags) {
+    if (P.compensated) k_interface_sparse<true><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, n, cm, heat_partials);
+    else k_interface_sparse<false><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, n, cm, heat_partials);
     return nb;
   }
   tiled = tiled && L.valid;

@@ -239,6 +239,17 @@ __global__ __launch_bounds__(kBlock) void k_pc_normalize(PV pv, GridDev g, SimPa
   block_sums_store<3>(acc, partials, stride);
 }
 
+// Marks the cells holding a mass source (out_m > 0 after the normalisation)
+// by the phase its receivers have; the kind bits stay.
+__global__ __launch_bounds__(kBlock) void k_cell_sources(PV pv, GridDev g) {
+  const int c = gtid();
+  if (c >= g.num_cells) return;
+  int f = g.flags[c] & (kCellWall | kCellFluid);
+  for (int j = g.off[c], e = g.off[c + 1]; j < e; ++j)
+    if (pv.t.out_m[j] > 0.0f) f |= (pv.s.flags[j] & kTargetVapor) ? kCellSrcV : kCellSrcL;
+  g.flags[c] = static_cast<uint8_t>(f);
+}
+
 // phase_change.cu:174-215
 template <bool KS>
 __global__ __launch_bounds__(kBlock) void k_pc_apply(PV pv, GridDev g, SimParams P, NbrList L, int n) {
@@ -251,7 +262,8 @@ __global__ __launch_bounds__(kBlock) void k_pc_apply(PV pv, GridDev g, SimParams
   const float Vi = mi / pv.s.rho[i];
   const V3 xi = ld3(pv.s.x[i]);
   Acc<KS> ain_m, ain_H;
-  if (!(fl & kDying)) {
+  // receivers only near a source giving mass to this phase
+  if (!(fl & kDying) && cells_any(g, xi, ph == kVapor ? kCellSrcV : kCellSrcL)) {
     struct Nb {
       float4 x;
       float om, oh, wn;
@@ -654,6 +666,7 @@ int launch_pc_normalize(const PV& pv, const GridDev& g, const SimParams& P, cons
 }
 void launch_pc_apply(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, int n, hipStream_t s) {
   if (n <= 0) return;
+  if (g.num_cells > 0) k_cell_sources<<<blocks_for(g.num_cells), kBlock, 0, s>>>(pv, g);
   if (P.compensated) k_pc_apply<true><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, n);
   else k_pc_apply<false><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, n);
 }
