@@ -1490,9 +1490,16 @@ int cryo_ensemble_run_steps(cryo_solver* const* solvers, int32_t count, int64_t 
     }
     // Every member's step is queued on its own stream before any member
     // waits, so the device runs the tanks' kernels side by side and a
-    // member's host round trip overlaps the others' device work.
-    for (int64_t q = 0; q < steps; ++q) {
+    // member's host round trip overlaps the others' device work.  A member
+    // queues its next step as soon as its own bookkeeping is back, so the
+    // device never runs dry between ensemble steps.
+    if (steps > 0) {
       for (int32_t t = 0; t < count; ++t) S(solvers[t]).step_issue();
+      for (int64_t q = 1; q < steps; ++q)
+        for (int32_t t = 0; t < count; ++t) {
+          S(solvers[t]).step_finish();
+          S(solvers[t]).step_issue();
+        }
       for (int32_t t = 0; t < count; ++t) S(solvers[t]).step_finish();
     }
     for (int32_t t = 0; t < count; ++t) S(solvers[t]).sync_state();
