@@ -181,6 +181,9 @@ struct SimParams {
   float c0, tait_B, tait_gamma, p_bg, gas_R, vapor_K;
   int clamp_neg;
   float alpha_av, alpha_av_vap, delta_sph;
+  // harmonic viscosity 2 mu_a mu_b / (mu_a + mu_b) of the phase pairs LL, LV,
+  // VV (0 when mu_a + mu_b = 0), evaluated once in the pair loops' order
+  float eta_ll, eta_lv, eta_vv;
   int density_mode, no_slip, sps, tvf, tvf_stress;
   float tvf_factor, tvf_sigma_min, shift_strength;
   int tvf_interface_only;
@@ -564,6 +567,9 @@ struct Scratch {
   // pass; rho_tvf is the wall density liquid targets use for the transport
   // velocity).
   float4* nrec;
+  // V = m / rho of each fluid particle for one rates evaluation (density
+  // pass): the pair loops read it instead of dividing per pair.
+  float* vol;
 };
 struct PV {  // kernel view
   Persist s;

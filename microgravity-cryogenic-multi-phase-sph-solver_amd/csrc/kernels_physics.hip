@@ -99,6 +99,7 @@ __global__ __launch_bounds__(kBlock) void k_density(PV pv, GridDev g, SimParams 
   const float p = eos_pressure(P, ph, rho, Ti);
   pv.t.p[i] = p;
   pv.t.nrec[i] = make_float4(p, mi, rho, Ti);
+  pv.t.vol[i] = mi / rho;
 }
 
 // SPH_kernels.cu:36-54 (start-up only).
@@ -211,7 +212,7 @@ __device__ inline IfaceNbr load_iface_nbr(const PV& pv, const SimParams& P, int 
     if (SPS && P.no_slip) d.v = pv.t.vw[j];
   } else {
     const float4 rec = pv.t.nrec[j];
-    d.r = make_float4(rec.y / rec.z, 0.0f, 0.0f, 0.0f);
+    d.r = make_float4(pv.t.vol[j], 0.0f, 0.0f, 0.0f);  // m / rho (density pass)
     if (SPS) d.v = pv.s.v[j];
   }
   return d;
@@ -431,6 +432,7 @@ struct NbrData {
   float4 rec;  // {p, m, rho, T} fluid; {p, p_wv, rho_tvf, T} wall
   float4 v;
   float4 vt;
+  float V;  // m / rho (fluid)
 };
 
 __device__ inline bool forces_need_vt(const SimParams& P, bool continuity) {
@@ -443,13 +445,14 @@ __device__ inline NbrData load_nbr(const PV& pv, const SimParams& P, int j, bool
   d.x = pv.s.x[j];
   d.ph = pv.s.phase[j];
   d.rec = pv.t.nrec[j];
+  d.V = pv.t.vol[j];
   const float4 v = pv.s.v[j];
   float4 vw = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (P.no_slip) vw = pv.t.vw[j];
   d.vt = v;
   if (need_vt) d.vt = pv.s.vt[j];
   // every gather of the pair in flight together, then one wait
-  settle(d.x.x, d.x.y, d.x.z, d.ph, d.rec.x, d.rec.y, d.rec.z, d.rec.w, v.x, v.y, v.z);
+  settle(d.x.x, d.x.y, d.x.z, d.ph, d.rec.x, d.rec.y, d.rec.z, d.rec.w, v.x, v.y, v.z, d.V);
   if (P.no_slip) settle(vw.x, vw.y, vw.z);
   if (need_vt) settle(d.vt.x, d.vt.y, d.vt.z);
   d.v = d.ph == kWall ? vw : v;
@@ -617,7 +620,7 @@ __device__ inline void forces_particle(const PV& pv, const SimParams& P, DevStat
           const float Vj = 1.0f / pv.t.sigma[j];
           sum_t += gw * (Vt2 + Vj * Vj);
         }
-        const float Vj = mj / rhoj;
+        const float Vj = d.V;  // mj / rhoj, divided once per particle
         const float vsum = Vi2 + Vj * Vj;
         const float pt = (rhoj * pi + rhoi * pj) / (rhoi + rhoj);
         force -= gw * (vsum * pt);
@@ -630,7 +633,8 @@ __device__ inline void forces_particle(const PV& pv, const SimParams& P, DevStat
         const float etaj = sel2(P.mu, pidx(phj)) + (P.sps ? rhoj * pv.t.nu_t[j] : 0.0f);
         const float eta_sum = etai + etaj;
         if (eta_sum > 0.0f && separated) {
-          const float eta_ij = 2.0f * etai * etaj / eta_sum;
+          const float eta_ij = P.sps ? 2.0f * etai * etaj / eta_sum
+                                     : (ph == phj ? (ph == kLiquid ? P.eta_ll : P.eta_vv) : P.eta_lv);
           force += vij * (vsum * eta_ij * dW / r);
         }
         if (alpha_i > 0.0f && phj == ph) {
@@ -900,6 +904,7 @@ __global__ __launch_bounds__(kTileThreads) void k_forces_tiled(PV pv, GridDev g,
                                                              DevState* st) {
   __shared__ TileRows T;
   __shared__ float4 s_x[kTileCap], s_rec[kTileCap], s_v[kTileCap], s_vt[kTileCap];
+  __shared__ float s_V[kTileCap];
   __shared__ int s_work[kTileThreads], s_wave[kTileThreads / 64];
   tile_rows(g, blockIdx.x, T);
   const int ntarget = T.tbase[T.ntrows];
@@ -920,6 +925,7 @@ __global__ __launch_bounds__(kTileThreads) void k_forces_tiled(PV pv, GridDev g,
     struct Slot {
       float4 x, r, v, t;
       int ph;
+      float V;
     };
     auto load = [&](int u) {
       Slot q{};
@@ -929,6 +935,7 @@ __global__ __launch_bounds__(kTileThreads) void k_forces_tiled(PV pv, GridDev g,
         q.x = pv.s.x[j];
         q.ph = pv.s.phase[j];
         q.r = pv.t.nrec[j];
+        q.V = pv.t.vol[j];
         const float4 v = pv.s.v[j];
         float4 w = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (P.no_slip) w = pv.t.vw[j];
@@ -945,6 +952,7 @@ __global__ __launch_bounds__(kTileThreads) void k_forces_tiled(PV pv, GridDev g,
         s_x[s] = make_float4(q.x.x, q.x.y, q.x.z, __int_as_float(q.ph));
         s_rec[s] = q.r;
         s_v[s] = q.v;
+        s_V[s] = q.V;
         if (need_vt) s_vt[s] = q.t;
       }
     };
@@ -984,7 +992,7 @@ __global__ __launch_bounds__(kTileThreads) void k_forces_tiled(PV pv, GridDev g,
               const float4 xs = s_x[s], rs = s_rec[s], vs = s_v[s];
               float4 vts = vs;
               if (need_vt) vts = s_vt[s];
-              d[u] = NbrData{xs, __float_as_int(xs.w), rs, vs, vts};
+              d[u] = NbrData{xs, __float_as_int(xs.w), rs, vs, vts, s_V[s]};
             }
           }
 #pragma unroll

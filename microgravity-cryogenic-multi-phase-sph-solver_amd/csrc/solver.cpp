@@ -222,7 +222,7 @@ struct cryo_solver {
   DevBuf<uint32_t> id_[2];
   int set = 0;
   DevBuf<float4> a, ab, ns, vw, fw, mw, nrec;
-  DevBuf<float> p, drho, dTdt, sigma, kappa, nu_t, alpha, wall_w, heat_w, out_m, out_h, w_norm, p_wv, rmin;
+  DevBuf<float> p, drho, dTdt, sigma, kappa, nu_t, alpha, wall_w, heat_w, out_m, out_h, w_norm, p_wv, rmin, vol;
   // grid and scans
   DevBuf<uint32_t> keys_[2], vals_[2], tile_hist, row_total, split_flag, split_off, scan_sums, scan_total;
   DevBuf<int> cell_off;
@@ -381,7 +381,7 @@ struct cryo_solver {
                    rho_[k].get(), T_[k].get(), phase_[k].get(), flags_[k].get(), id_[k].get()};
     pv.t = Scratch{a.get(), ab.get(), ns.get(), vw.get(), fw.get(), mw.get(), p.get(), drho.get(), dTdt.get(),
                    sigma.get(), kappa.get(), nu_t.get(), alpha.get(), wall_w.get(), heat_w.get(), out_m.get(),
-                   out_h.get(), w_norm.get(), p_wv.get(), rmin.get(), nrec.get()};
+                   out_h.get(), w_norm.get(), p_wv.get(), rmin.get(), nrec.get(), vol.get()};
     pv.capacity = capacity;
     return pv;
   }
@@ -414,7 +414,7 @@ struct cryo_solver {
       f(flags_[k], k == set);
       f(id_[k], k == set);
     }
-    f(a, true); f(ab, true); f(ns, true); f(vw, true); f(fw, true); f(mw, true); f(nrec, false);
+    f(a, true); f(ab, true); f(ns, true); f(vw, true); f(fw, true); f(mw, true); f(nrec, false); f(vol, false);
     f(p, true); f(drho, true); f(dTdt, true); f(sigma, true); f(kappa, true); f(nu_t, true); f(alpha, true);
     f(wall_w, true); f(heat_w, true); f(out_m, true); f(out_h, true); f(w_norm, true); f(p_wv, true); f(rmin, true);
   }
@@ -510,6 +510,13 @@ struct cryo_solver {
     Q.clamp_neg = c.clamp_negative_pressure ? 1 : 0;
     Q.alpha_av = c.artificial_viscosity;
     Q.alpha_av_vap = c.artificial_viscosity_vapor;
+    auto harmonic = [](float a, float b) {
+      const float sum = a + b;
+      return sum > 0.0f ? 2.0f * a * b / sum : 0.0f;
+    };
+    Q.eta_ll = harmonic(Q.mu[0], Q.mu[0]);
+    Q.eta_lv = harmonic(Q.mu[0], Q.mu[1]);
+    Q.eta_vv = harmonic(Q.mu[1], Q.mu[1]);
     Q.delta_sph = c.delta_sph;
     Q.density_mode = c.density_mode;
     Q.no_slip = c.no_slip ? 1 : 0;
