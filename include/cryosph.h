@@ -192,11 +192,11 @@ int cryo_ensemble_run_steps(cryo_solver* const* solvers, int32_t count, int64_t 
  * state either way.  Differentiated: density summation, EOS, Shepard wall
  * pressure, ghost velocity and wall colour, momentum (pressure, viscosity,
  * artificial viscosity, wall terms), CSF surface tension (colour gradient,
- * contact-angle rotation, curvature, force), kick-drift-kick.  Requires
- * fixed_dt > 0, the Wendland C2 kernel, density summation, CSF only with
- * unsmoothed normals, the classic force and normal rotation, no thermal,
- * phase change, transport velocity, SPS or shifting, and no frame rotation
- * or manoeuvre table (CRYO_ERR_INVALID_ARGUMENT otherwise). */
+ * contact-angle rotation, curvature, force), the transport velocity,
+ * kick-drift-kick.  Requires fixed_dt > 0, the Wendland C2 kernel, density
+ * summation, CSF only with unsmoothed normals, the classic force and normal
+ * rotation, no thermal, phase change, transport stress, SPS or shifting, and
+ * no frame rotation or manoeuvre table (CRYO_ERR_INVALID_ARGUMENT otherwise). */
 int cryo_solver_adjoint_slosh(cryo_solver* s, int64_t steps, const float* frame_acc, int32_t checkpoint_every,
                               double* loss, double* grad_frame_acc, double* wall_load, double* grad_contact_angle);
 

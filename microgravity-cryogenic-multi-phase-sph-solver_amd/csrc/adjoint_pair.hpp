@@ -423,4 +423,53 @@ CRYO_ADJ_HD inline RotGrad<T> rot_vjp(Vec<T> g, Vec<T> wall_n, T ww, T cos_t, T 
   return o;
 }
 
+// ---------------------------------------------------- transport velocity
+// Transport acceleration of an active owner I (SPH_kernels.cu:56-104):
+//   ab_I = K_I sum_J gw (Vt_I^2 + V_J^2),  K_I = -pb_I / (rho_I Vt_I),  Vt_I = 1 / sigma_I,
+//   V_J = 1 / sigma_J (fluid) or the wall volume below.
+// Sb = K_I * (cotangent of ab_I) is the cotangent of the sum.  Returns the
+// cotangents of r_IJ, Vt_I^2 and V_J.
+template <class T>
+struct TvGrad {
+  Vec<T> r;
+  T vt2, vj;
+};
+template <class T>
+CRYO_ADJ_HD inline TvGrad<T> tvf_vjp(const Ctx<T>& c, Vec<T> rij, T Vt2, T Vj, Vec<T> Sb) {
+  TvGrad<T> o{{T(0), T(0), T(0)}, T(0), T(0)};
+  const T r2 = dot(rij, rij);
+  if (!(r2 < c.support2)) return o;
+  const T r = std::sqrt(r2);
+  if (!(r > T(1e-12) * c.h)) return o;
+  const T s = wendland_s(c, r), spr = wendland_spr(c, r);
+  const T sr = dot(Sb, rij);
+  const T G = s * sr;  // Sb . gw
+  o.r = (Sb * s + rij * (sr * spr)) * (Vt2 + Vj * Vj);
+  o.vt2 = G;
+  o.vj = T(2) * Vj * G;
+  return o;
+}
+// A wall neighbour's volume for the owner: V_J = rho_w > 0 ? Vt_I rho_I / rho_w : Vt_I.
+// Adds the owner's shares to vtb and rhob; returns the cotangent of rho_w.
+template <class T>
+CRYO_ADJ_HD inline T tvf_wall_vjp(T Vt, T rho_i, T rho_w, T vjb, T& vtb, T& rhob) {
+  if (!(rho_w > T(0))) {
+    vtb += vjb;
+    return T(0);
+  }
+  vtb += vjb * rho_i / rho_w;
+  rhob += vjb * Vt / rho_w;
+  return -vjb * (Vt * rho_i / rho_w) / rho_w;
+}
+// d rho_w / d p of the wall density a TVF owner uses: liquid owners
+// rho0 max(ratio, 0.5)^(1/gamma) with ratio = (p_w - p_bg) / B + 1 (the value
+// rho_w is passed in); vapour owners (p_wv - (p_bg - K)) / (R T_I).
+template <class T>
+CRYO_ADJ_HD inline T tvf_wall_drho_dp(int vapor, T p_w, T rho_w, T p_bg, T tait_B, T tait_gamma, T gas_R, T T_i) {
+  if (vapor) return T(1) / (gas_R * T_i);
+  const T ratio = (p_w - p_bg) / tait_B + T(1);
+  if (!(ratio > T(0.5))) return T(0);
+  return rho_w / (tait_gamma * ratio * tait_B);
+}
+
 }  // namespace cryo_adj

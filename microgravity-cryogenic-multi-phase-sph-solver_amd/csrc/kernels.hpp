@@ -101,6 +101,11 @@ struct AdjArgs {
   float4* gcb;  // cotangents of the colour gradient (xyz) and the wall weight ww (w)
   float4* wnb;  // cotangent of the wall-normal sum
   float4* xc;   // the curvature sums' share of the x cotangent
+  // transport velocity; tvf = 0: not used
+  int tvf;
+  const float4* ct;  // cotangent of the transport acceleration ab at the step's end
+  float4* nt;        // the same before the step (the drift reads ab), written through perm
+  float4* tvs;       // {K * ab_bar, 1} of each active transport owner, else 0
 };
 void launch_adj_fill(float4* a, float4* b, float4* c, int n, hipStream_t s);
 void launch_adj_set_frame(DevState* st, const float* acc, hipStream_t s);

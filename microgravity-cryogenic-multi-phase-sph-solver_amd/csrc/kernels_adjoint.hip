@@ -134,7 +134,25 @@ __global__ __launch_bounds__(1024) void k_adj_finish(const double* partials, int
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_adj_prep(PV pv, AdjArgs A, const DevState* st, int n) {
+__device__ inline float colour(uint8_t ph) { return ph == kLiquid ? 1.0f : 0.0f; }
+
+// Transport acceleration (SPH_kernels.cu:56-104): ab = K sum_J gw (Vt^2 + V_J^2),
+// Vt = 1 / sigma, K = -pb / (rho Vt); active where the forward pass computes it.
+__device__ inline bool tvf_active(const PV& pv, const SimParams& P, int i) {
+  return P.tvf && !(pv.t.sigma[i] < P.tvf_sigma_min) && !(P.tvf_interface_only && !(pv.t.alpha[i] > 0.0f));
+}
+__device__ inline float tvf_K(const PV& pv, const SimParams& P, int i) {
+  const uint8_t ph = pv.s.phase[i];
+  const float ci = sound_speed(P, ph, pv.s.T[i]);
+  const float Vt = 1.0f / pv.t.sigma[i];
+  return -(P.tvf_factor * sel2(P.rho0, pidx(ph)) * ci * ci) / (pv.s.rho[i] * Vt);
+}
+// The wall density a transport owner of phase ph uses for wall j.
+__device__ inline float tvf_rho_w(const PV& pv, const SimParams& P, uint8_t ph, float Ti, int j) {
+  if (ph == kVapor) return (pv.t.p_wv[j] - (P.p_bg - P.vapor_K)) / (P.gas_R * Ti);
+  return pv.t.nrec[j].z;  // rho0 max(ratio, 0.5)^(1/gamma), evaluated by the wall pass
+}
+__global__ __launch_bounds__(kBlock) void k_adj_prep(PV pv, SimParams P, AdjArgs A, const DevState* st, int n) {
   const int i = gtid();
   double g[3] = {0.0, 0.0, 0.0};
   if (i < n) {
@@ -148,11 +166,19 @@ __global__ __launch_bounds__(kBlock) void k_adj_prep(PV pv, AdjArgs A, const Dev
       g[2] = r.z;
     }
     A.at[i] = r;
+    if (A.tvf) {
+      float4 t = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (is_fluid(pv.s.phase[i]) && tvf_active(pv, P, i)) {
+        const float K = tvf_K(pv, P, i);
+        const float4 c = A.ct[i];
+        t = make_float4(c.x * K, c.y * K, c.z * K, 1.0f);
+      }
+      A.tvs[i] = t;
+    }
   }
   block_sums_store<3>(g, A.partials + 3 * A.stride, A.stride);
 }
 
-__device__ inline float colour(uint8_t ph) { return ph == kLiquid ? 1.0f : 0.0f; }
 __device__ inline bool live_normal(const PV& pv, int i) { return norm2(ld3(pv.t.ns[i])) > 0.25f; }
 
 // Curvature sums' cotangents: kappa = 3 num / den, a += nrm coeff kappa.
@@ -294,6 +320,23 @@ __global__ __launch_bounds__(kBlock) void k_adj_wall(PV pv, GridDev g, SimParams
         pb0 += o.p_w;
       }
       vwb -= o.vij;
+      if (A.tvf) {  // a transport owner's wall volume reads this wall's pressure
+        const float4 tv = A.tvs[j];
+        if (tv.w > 0.0f) {
+          const float Ti = pv.s.T[j];
+          const float Vt = 1.0f / pv.t.sigma[j];
+          const float rho_w = tvf_rho_w(pv, P, ph, Ti, w);
+          const float Vj = rho_w > 0.0f ? Vt * rho / rho_w : Vt;
+          const cryo_adj::TvGrad<float> ot =
+              cryo_adj::tvf_vjp(c, AV{-rwf.x, -rwf.y, -rwf.z}, Vt * Vt, Vj, AV{tv.x, tv.y, tv.z});
+          float dummy_vt = 0.0f, dummy_rho = 0.0f;
+          const float rwb = cryo_adj::tvf_wall_vjp(Vt, rho, rho_w, ot.vj, dummy_vt, dummy_rho);
+          const float dp = cryo_adj::tvf_wall_drho_dp(vap ? 1 : 0, vap ? pw1 : pw0, rho_w, P.p_bg, P.tait_B,
+                                                      P.tait_gamma, P.gas_R, Ti);
+          if (vap) pb1 += rwb * dp;
+          else pb0 += rwb * dp;
+        }
+      }
       if (A.csf) {  // the fluid's colour gradient reads the wall colour alpha_w
         const float4 gb = A.gcb[j], wb = A.wnb[j];
         const cryo_adj::GcGrad<float> og = cryo_adj::gc_fw_vjp(c, AV{-rwf.x, -rwf.y, -rwf.z}, alpha_w,
@@ -338,6 +381,13 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
   const float ci = colour(phi), Vk = I.m / I.rho;
   AV gbk{0.0f, 0.0f, 0.0f}, wbk{0.0f, 0.0f, 0.0f};
   float wwbk = 0.0f, Vbk = 0.0f;
+  // transport acceleration: own sums (owner active) and the neighbours' sums that read V_i = 1 / sigma_i
+  const float4 tvi = A.tvf ? A.tvs[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const bool tv_own = tvi.w > 0.0f;
+  const AV Sbi{tvi.x, tvi.y, tvi.z};
+  const float Vti = A.tvf ? 1.0f / pv.t.sigma[i] : 0.0f, Vti2 = Vti * Vti;
+  float vt2b = 0.0f, vtb = 0.0f, trhob = 0.0f, tsb = 0.0f;
+  AV tsum{0.0f, 0.0f, 0.0f};
   if (A.csf) {
     const float4 gb = A.gcb[i], wb = A.wnb[i];
     gbk = AV{gb.x, gb.y, gb.z};
@@ -364,6 +414,19 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
       vr += b.v;
       pb += b.p;
       rb += b.rho;
+      if (tv_own) {
+        const AV rkw = I.x - xw;
+        const float r2 = cryo_adj::dot(rkw, rkw);
+        if (r2 < P.support2) {
+          const float rho_w = tvf_rho_w(pv, P, phi, pv.s.T[i], j);
+          const float Vj = rho_w > 0.0f ? Vti * I.rho / rho_w : Vti;
+          tsum += av(kgradW(P.kernel, V3{rkw.x, rkw.y, rkw.z}, sqrtf(r2))) * (Vti2 + Vj * Vj);
+          const cryo_adj::TvGrad<float> ot = cryo_adj::tvf_vjp(c, rkw, Vti2, Vj, Sbi);
+          xr += ot.r;
+          vt2b += ot.vt2;
+          (void)cryo_adj::tvf_wall_vjp(Vti, I.rho, rho_w, ot.vj, vtb, trhob);  // the wall's share: k_adj_wall
+        }
+      }
       if (A.csf) {
         const float alpha_w = pv.t.alpha[j];
         const cryo_adj::GcGrad<float> og = cryo_adj::gc_fw_vjp(c, I.x - xw, alpha_w, ci, av(pv.s.nrm[j]),
@@ -388,6 +451,26 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
       xr -= og2.r;
       Vbk += og2.V_j;
     }
+    if (A.tvf) {
+      const AV rkj = I.x - av(pv.s.x[j]);
+      const float r2 = cryo_adj::dot(rkj, rkj);
+      if (r2 < P.support2) {
+        const float Vj = 1.0f / pv.t.sigma[j];
+        if (tv_own) {
+          tsum += av(kgradW(P.kernel, V3{rkj.x, rkj.y, rkj.z}, sqrtf(r2))) * (Vti2 + Vj * Vj);
+          const cryo_adj::TvGrad<float> ot = cryo_adj::tvf_vjp(c, rkj, Vti2, Vj, Sbi);
+          xr += ot.r;
+          vt2b += ot.vt2;
+        }
+        const float4 tvj = A.tvs[j];
+        if (tvj.w > 0.0f) {  // j's sum reads V_i
+          const cryo_adj::TvGrad<float> o2 =
+              cryo_adj::tvf_vjp(c, AV{-rkj.x, -rkj.y, -rkj.z}, Vj * Vj, Vti, AV{tvj.x, tvj.y, tvj.z});
+          xr -= o2.r;
+          tsb -= o2.vj * Vti * Vti;
+        }
+      }
+    }
     const cryo_adj::Side<float> J = side(pv, P, j);
     const float4 atj = A.at[j];
     const AV cAj{atj.x, atj.y, atj.z};
@@ -403,7 +486,17 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
   const float dpdr = cryo_adj::eos_dp_drho(vap ? 1 : 0, I.rho, pv.s.T[i], I.p, P.rho0[0], P.tait_B, P.tait_gamma,
                                            P.gas_R, P.clamp_neg);
   rb -= Vbk * Vk / I.rho;  // V = m / rho in the curvature and colour sums
-  A.sb[i] = (rb + pb * dpdr) * (vap ? I.m : pv.s.m0[i]);  // rho = m0 sigma (liquid), m sigma (vapour)
+  if (tv_own) {  // K = -pb / (rho Vt), Vt = 1 / sigma
+    const float K = tvf_K(pv, P, i);
+    const float4 ct = A.ct[i];
+    const float Kb = ct.x * tsum.x + ct.y * tsum.y + ct.z * tsum.z;
+    trhob -= K * Kb / I.rho;
+    vtb -= K * Kb / Vti;
+    vtb += 2.0f * Vti * vt2b;
+    tsb -= vtb * Vti * Vti;
+    rb += trhob;
+  }
+  A.sb[i] = (rb + pb * dpdr) * (vap ? I.m : pv.s.m0[i]) + tsb;  // rho = m0 sigma (liquid), m sigma (vapour)
   A.xr[i] = f4(xr);
   A.vr[i] = f4(vr);
 }
@@ -418,6 +511,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_density(PV pv, GridDev g, SimPar
     A.nx[dst] = z;
     A.nv[dst] = z;
     A.na[dst] = z;
+    if (A.tvf) A.nt[dst] = z;
     return;
   }
   const float dt = st->dt, half = 0.5f * dt;
@@ -439,6 +533,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_density(PV pv, GridDev g, SimPar
   A.nx[dst] = f4(X);
   A.nv[dst] = f4(Vh);
   A.na[dst] = f4(Vh * half);
+  if (A.tvf) A.nt[dst] = f4(X * (half * dt));  // x' = x + (v_half + ab dt/2) dt
 }
 
 }  // namespace
@@ -463,7 +558,7 @@ void launch_adj_csf_only(const PV& pv, const GridDev& g, const SimParams& P, con
 void launch_adj_step(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, const DevState* st,
                      const AdjArgs& A, int n, double* grad_out, double* theta_out, hipStream_t s) {
   const int nb = blocks_for(n > 0 ? n : 1);
-  k_adj_prep<<<nb, kBlock, 0, s>>>(pv, A, st, n);
+  k_adj_prep<<<nb, kBlock, 0, s>>>(pv, P, A, st, n);
   if (A.csf) {
     k_adj_csf1<<<nb, kBlock, 0, s>>>(pv, g, P, L, A, n);
     k_adj_csf2<<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n, 6);

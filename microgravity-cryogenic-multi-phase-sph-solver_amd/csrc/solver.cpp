@@ -898,7 +898,7 @@ struct cryo_solver {
   // optionally `a` (at step boundaries) and the step's sort permutation (taped
   // post-sort states).
   struct Snap {
-    DevBuf<float4> x, v, vt, nrm, a;
+    DevBuf<float4> x, v, vt, nrm, a, ab;
     DevBuf<float> m, m0, rho, T;
     DevBuf<uint8_t> phase, flags;
     DevBuf<uint32_t> id, perm;
@@ -932,8 +932,8 @@ struct cryo_solver {
   }
   std::vector<std::unique_ptr<Snap>> adj_ckpt, adj_tape;
   std::unique_ptr<Snap> adj_end;
-  DevBuf<float4> adj_c[2][3];  // two cotangent sets of (x, v, a)
-  DevBuf<float4> adj_at, adj_wcp, adj_wcv, adj_xr, adj_vr, adj_cs, adj_gcb, adj_wnb, adj_xc;
+  DevBuf<float4> adj_c[2][4];  // two cotangent sets of (x, v, a, ab)
+  DevBuf<float4> adj_at, adj_wcp, adj_wcv, adj_xr, adj_vr, adj_cs, adj_gcb, adj_wnb, adj_xc, adj_tvs;
   DevBuf<float> adj_sb, adj_acc, adj_csv;
   DevBuf<double> adj_S, adj_G, adj_T, adj_part;
   Snap* tape_slot = nullptr;  // set: step_issue() records its post-sort state here
@@ -955,7 +955,7 @@ struct cryo_solver {
     d2d(z.m.get(), q.m, m), d2d(z.m0.get(), q.m0, m), d2d(z.rho.get(), q.rho, m), d2d(z.T.get(), q.T, m);
     d2d(z.phase.get(), q.phase, m), d2d(z.flags.get(), q.flags, m), d2d(z.id.get(), q.id, m);
     d2d(z.st.get(), state.get(), 1);
-    if (with_a) fit(z.a, m), d2d(z.a.get(), a.get(), m);
+    if (with_a) fit(z.a, m), d2d(z.a.get(), a.get(), m), fit(z.ab, m), d2d(z.ab.get(), ab.get(), m);
     if (perm) fit(z.perm, m), d2d(z.perm.get(), perm, m);
     z.n = m;
   }
@@ -966,7 +966,7 @@ struct cryo_solver {
     d2d(q.m, z.m.get(), m), d2d(q.m0, z.m0.get(), m), d2d(q.rho, z.rho.get(), m), d2d(q.T, z.T.get(), m);
     d2d(q.phase, z.phase.get(), m), d2d(q.flags, z.flags.get(), m), d2d(q.id, z.id.get(), m);
     d2d(state.get(), z.st.get(), 1);
-    if (with_a) d2d(a.get(), z.a.get(), m);
+    if (with_a) d2d(a.get(), z.a.get(), m), d2d(ab.get(), z.ab.get(), m);
     n = m;
     lists_valid = false;
     tiles_ok = false;
@@ -977,7 +977,8 @@ struct cryo_solver {
                    adj_c[1 - cur][0].get(), adj_c[1 - cur][1].get(), adj_c[1 - cur][2].get(),
                    adj_at.get(), adj_wcp.get(), adj_wcv.get(), adj_xr.get(), adj_vr.get(), adj_sb.get(),
                    perm, S, adj_part.get(), stride, csf ? 1 : 0, adj_cs.get(), adj_csv.get(), adj_gcb.get(),
-                   adj_wnb.get(), adj_xc.get()};
+                   adj_wnb.get(), adj_xc.get(), P.tvf ? 1 : 0, adj_c[cur][3].get(), adj_c[1 - cur][3].get(),
+                   adj_tvs.get()};
   }
 
   void check_adjoint_scope() const {
@@ -990,8 +991,8 @@ struct cryo_solver {
         (c.csf_smooth_normals || c.csf_balanced || c.wetting_model != CRYO_WET_NORMAL_ROTATION))
       throw std::invalid_argument("adjoint: CSF only with unsmoothed normals, the classic force and normal rotation");
     if (c.thermal || c.phase_change) throw std::invalid_argument("adjoint: thermal / phase change not differentiated");
-    if (c.transport_velocity || c.sps_turbulence || c.particle_shifting)
-      throw std::invalid_argument("adjoint: transport velocity, SPS and shifting not differentiated");
+    if (c.transport_stress || c.sps_turbulence || c.particle_shifting)
+      throw std::invalid_argument("adjoint: transport stress, SPS and shifting not differentiated");
     if (has_maneuver || frame_rot) throw std::invalid_argument("adjoint: frame rotation / manoeuvre table not allowed");
   }
 
@@ -1043,7 +1044,9 @@ struct cryo_solver {
         for (auto& b : set3) fit(b, m);
       fit(adj_at, m), fit(adj_wcp, m), fit(adj_wcv, m), fit(adj_xr, m), fit(adj_vr, m), fit(adj_sb, m);
       if (csf) fit(adj_cs, m), fit(adj_csv, m), fit(adj_gcb, m), fit(adj_wnb, m), fit(adj_xc, m);
+      fit(adj_tvs, m);
       launch_adj_fill(adj_c[0][0].get(), adj_c[0][1].get(), adj_c[0][2].get(), n, stream);
+      launch_adj_fill(adj_c[0][3].get(), adj_c[1][3].get(), adj_tvs.get(), n, stream);
       adj_tape.resize(static_cast<std::size_t>(std::min(K, steps)));
       for (auto& t : adj_tape)
         if (!t) t = std::make_unique<Snap>();

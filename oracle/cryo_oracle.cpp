@@ -1813,8 +1813,8 @@ struct Solver {
         (c.csf_smooth_normals || c.csf_balanced || c.wetting_model != CRYO_WET_NORMAL_ROTATION))
       throw std::invalid_argument("adjoint: CSF only with unsmoothed normals, the classic force and normal rotation");
     if (c.thermal || c.phase_change) throw std::invalid_argument("adjoint: thermal / phase change not differentiated");
-    if (c.transport_velocity || c.sps_turbulence || c.particle_shifting)
-      throw std::invalid_argument("adjoint: transport velocity, SPS and shifting not differentiated");
+    if (c.transport_stress || c.sps_turbulence || c.particle_shifting)
+      throw std::invalid_argument("adjoint: transport stress, SPS and shifting not differentiated");
     if (has_maneuver || P.frame_rot) throw std::invalid_argument("adjoint: frame rotation / manoeuvre table not allowed");
   }
 
@@ -1930,7 +1930,7 @@ struct Solver {
   }
 
   V3 adjoint_rates(const std::vector<u32>& perm, V3 Ssum, std::vector<V3>& xb, std::vector<V3>& vb,
-                   std::vector<V3>& ab, double& theta_b) {
+                   std::vector<V3>& ab, std::vector<V3>& tb, double& theta_b) {
     using namespace cryo_adj;
     using VR = Vec<R>;
     const int n = ps.n;
@@ -1957,6 +1957,61 @@ struct Solver {
     // surface tension
     std::vector<R> Vb(n, R(0)), alphab(n, R(0));
     if (cfg.surface_tension == CRYO_ST_CSF && has_vapor) theta_b += csf_adjoint(c, At, xr, Vb, alphab);
+    // transport acceleration ab = K sum_J gw (Vt^2 + V_J^2), K = -pb / (rho Vt)
+    // (SPH_kernels.cu:56-104), read by the next step's drift: tb is its cotangent
+    std::vector<R> tsb(n, R(0));  // sigma cotangent of the transport sums
+    if (P.tvf) {
+      for (int i = 0; i < n; ++i) {
+        const u8 ph = ps.phase[i];
+        if (!is_fluid(ph) || ps.sigma[i] < P.tvf_sigma_min || (P.tvf_interface_only && !(ps.alpha[i] > R(0))))
+          continue;
+        const VR Ab = vr(tb[i]);
+        const R Vt = R(1) / ps.sigma[i], Vt2 = Vt * Vt, rhoi = ps.rho[i];
+        const R ci = sound_speed(P, ph, ps.T[i]);
+        const R K = -(P.tvf_factor * P.rho0[pidx(ph)] * ci * ci) / (rhoi * Vt);
+        const VR Sb = Ab * K;
+        R vt2b = 0, vtb = 0, rhob = 0;
+        VR sum{0, 0, 0};
+        grid.each(ps.x[i], [&](int j) {
+          if (j == i) return;
+          const V3 rij = ps.x[i] - ps.x[j];
+          const R r2 = norm2(rij);
+          if (r2 >= P.support2) return;
+          const bool wall = ps.phase[j] == W;
+          R Vj, rho_w = 0;
+          if (wall) {
+            if (ph == V) {
+              rho_w = (ps.p_wv[j] - (P.p_bg - P.vapor_K)) / (P.gas_R * ps.T[i]);
+            } else {
+              const R ratio = (ps.p[j] - P.p_bg) / P.tait_B + R(1);
+              rho_w = P.rho0[0] * std::pow(ratio > R(0.5) ? ratio : R(0.5), R(1) / P.tait_gamma);
+            }
+            Vj = rho_w > R(0) ? Vt * rhoi / rho_w : Vt;
+          } else {
+            Vj = R(1) / ps.sigma[j];
+          }
+          sum += vr(kgradW(P.kernel, rij, std::sqrt(r2))) * (Vt2 + Vj * Vj);
+          const TvGrad<R> o = tvf_vjp(c, vr(rij), Vt2, Vj, Sb);
+          xr[i] += o.r;
+          xr[j] -= o.r;
+          vt2b += o.vt2;
+          if (wall) {
+            const R rwb = tvf_wall_vjp(Vt, rhoi, rho_w, o.vj, vtb, rhob);
+            const R dp = tvf_wall_drho_dp(ph == V ? 1 : 0, ph == V ? ps.p_wv[j] : ps.p[j], rho_w, P.p_bg, P.tait_B,
+                                          P.tait_gamma, P.gas_R, ps.T[i]);
+            (ph == V ? pwb1 : pwb0)[j] += rwb * dp;
+          } else {
+            tsb[j] -= o.vj * Vj * Vj;  // V_J = 1 / sigma_J
+          }
+        });
+        const R Kb = dot(Ab, sum);
+        rhob -= K * Kb / rhoi;
+        vtb -= K * Kb / Vt;
+        vtb += R(2) * Vt * vt2b;
+        tsb[i] -= vtb * Vt * Vt;  // Vt = 1 / sigma
+        rb[i] += rhob;
+      }
+    }
     // momentum
     for (int i = 0; i < n; ++i) {
       if (!is_fluid(ps.phase[i])) continue;
@@ -2039,7 +2094,7 @@ struct Solver {
       rb[i] -= Vb[i] * (ps.m[i] / ps.rho[i]) / ps.rho[i];
       const R dpdr = eos_dp_drho(ph == V ? 1 : 0, ps.rho[i], ps.T[i], ps.p[i], P.rho0[0], P.tait_B, P.tait_gamma,
                                  P.gas_R, P.clamp_neg);
-      sb[i] = (rb[i] + pb[i] * dpdr) * (ph == L ? ps.m0[i] : ps.m[i]);
+      sb[i] = (rb[i] + pb[i] * dpdr) * (ph == L ? ps.m0[i] : ps.m[i]) + tsb[i];
     }
     for (int i = 0; i < n; ++i) {
       if (!is_fluid(ps.phase[i])) continue;
@@ -2054,7 +2109,7 @@ struct Solver {
       });
     }
     // drift and kick, then back through the cell sort: old[perm[i]] <- new[i]
-    std::vector<V3> nx(n, V3{0, 0, 0}), nv(n, V3{0, 0, 0}), na(n, V3{0, 0, 0});
+    std::vector<V3> nx(n, V3{0, 0, 0}), nv(n, V3{0, 0, 0}), na(n, V3{0, 0, 0}), nt(n, V3{0, 0, 0});
     for (int i = 0; i < n; ++i) {
       if (!is_fluid(ps.phase[i])) continue;
       const VR X = vr(xb[i]) + xr[i];
@@ -2062,10 +2117,12 @@ struct Solver {
       nx[perm[i]] = v3(X);
       nv[perm[i]] = v3(Vh);
       na[perm[i]] = v3(Vh * half);
+      if (P.tvf) nt[perm[i]] = v3(X * (half * dt));  // x' = x + (v_half + ab dt/2) dt
     }
     xb.swap(nx);
     vb.swap(nv);
     ab.swap(na);
+    tb.swap(nt);
     return v3(gb);
   }
 
@@ -2106,7 +2163,7 @@ struct Solver {
     if (!grad) return;
     // reverse sweep over the taped states; the solver ends in the forward end state
     Particles end_state = ps;
-    std::vector<V3> xb(ps.n, zero), vb(ps.n, zero), ab(ps.n, zero);
+    std::vector<V3> xb(ps.n, zero), vb(ps.n, zero), ab(ps.n, zero), tb(ps.n, zero);
     double theta_b = 0;
     for (long long k = steps - 1; k >= 0; --k) {
       TapeEntry& t = tape[static_cast<std::size_t>(k)];
@@ -2115,7 +2172,7 @@ struct Solver {
       apply_frame(acc_of(k), zero, zero, origin);
       P.dt = R(cfg.fixed_dt);
       evaluate_rates();
-      const V3 gb = adjoint_rates(t.perm, S[static_cast<std::size_t>(k)], xb, vb, ab, theta_b);
+      const V3 gb = adjoint_rates(t.perm, S[static_cast<std::size_t>(k)], xb, vb, ab, tb, theta_b);
       grad[3 * k] = -double(gb.x);
       grad[3 * k + 1] = -double(gb.y);
       grad[3 * k + 2] = -double(gb.z);

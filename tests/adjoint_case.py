@@ -1,6 +1,6 @@
 """Inputs of the adjoint tests: a small closed box tank, half liquid / half
 vapour LN2, hydrodynamics and optionally CSF surface tension with a contact
-angle (no thermal, phase change, transport velocity), fixed dt, jittered positions, random velocities, and a
+angle and the transport velocity (no thermal or phase change), fixed dt, jittered positions, random velocities, and a
 share of the fluid particles next to the walls moved inside the near-wall guard
 distance (0.6 dx) and sent wallward, so that every branch of the wall terms
 (guard repulsion, guard damping, no-slip shear, wall artificial viscosity) is
@@ -12,14 +12,16 @@ import cabi
 from cabi import WALL
 
 
-def adjoint_case(steps=12, seed=1, dx=0.004, L=0.04, dt=2.5e-4, acc_scale=0.3, csf=False, contact_angle=30.0):
+def adjoint_case(steps=12, seed=1, dx=0.004, L=0.04, dt=2.5e-4, acc_scale=0.3, csf=False, contact_angle=30.0,
+                 tvf=False):
     fluid = cabi.hip_fluid_preset("LN2")
     if csf:  # a surface tension large enough to weigh in the wall loads
         fluid.sigma = 0.2
     # strong artificial viscosity and gravity: their terms weigh in the gradient
     cfg = cabi.default_config(dx=dx, gravity=(0.0, 0.0, -5.0), thermal=0, phase_change=0,
                               surface_tension=1 if csf else 0, contact_angle_deg=contact_angle,
-                              transport_velocity=0, velocity_scale=0.3, fixed_dt=dt, artificial_viscosity=0.5,
+                              transport_velocity=1 if tvf else 0, velocity_scale=0.3, fixed_dt=dt,
+                              artificial_viscosity=0.5,
                               artificial_viscosity_vapor=1.0)
     ps = cabi.hip_build_tank_particles(cabi.box(L, L, L), fluid, dx, cabi.default_fill())
     rng = np.random.default_rng(seed)

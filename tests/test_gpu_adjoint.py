@@ -100,6 +100,20 @@ def test_csf_gradients_match_fp64_adjoint_and_central_differences():
     assert np.array_equal(out1[1], g) and out1[3] == dth
 
 
+def test_transport_velocity_gradients_match_fp64_adjoint():
+    say("test_transport_velocity_gradients_match_fp64_adjoint")
+    fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=12, csf=True, tvf=True)
+    loss, g, _, dth = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=5)
+    loss64, g64, _, dth64 = orc(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
+    err = np.abs(g - g64).max() / np.abs(g64).max()
+    say("loss", loss, "fp64", loss64, "acc gradient error", err, "dL/dtheta", dth, "fp64", dth64)
+    assert abs(loss - loss64) <= 1e-4 * loss64
+    assert err <= 1e-3
+    assert abs(dth - dth64) <= 1e-3 * abs(dth64)
+    out1 = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=12)
+    assert np.array_equal(out1[1], g) and out1[3] == dth
+
+
 def test_checkpoint_interval_does_not_change_the_gradient():
     say("test_checkpoint_interval_does_not_change_the_gradient")
     fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=10)

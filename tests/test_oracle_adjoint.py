@@ -60,7 +60,7 @@ def test_loss_is_the_integrated_wall_load():
 
 def test_adjoint_rejects_undifferentiated_physics():
     for csf, field, value in ((False, "surface_tension", 2), (False, "thermal", 1), (False, "fixed_dt", 0.0),
-                              (False, "transport_velocity", 1), (True, "csf_balanced", 1), (True, "wetting_model", 1),
+                              (False, "transport_stress", 1), (True, "csf_balanced", 1), (True, "wetting_model", 1),
                               (True, "csf_smooth_normals", 1)):
         fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=2, csf=csf)
         setattr(cfg, field, value)
@@ -93,3 +93,22 @@ def test_fp64_adjoint_with_csf_matches_central_differences():
     fd_theta = (out[0] - out[1]) / (np.float32(30.0 + d) - np.float32(30.0 - d))
     assert dtheta != 0.0
     assert abs(dtheta - fd_theta) <= 1e-5 * abs(fd_theta), (dtheta, fd_theta)
+
+
+def test_fp64_adjoint_with_transport_velocity_matches_central_differences():
+    """The transport velocity on (interface band, with CSF): its acceleration is
+    read by the next step's drift, so its cotangent is part of the state.
+    Dropping that cotangent, or only its wall-pressure share, moves the
+    gradient by 1e-2 of its largest component."""
+    fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=12, csf=True, tvf=True)
+    loss, grad, _, _ = oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
+    fd = np.zeros_like(grad)
+    for k in range(acc.shape[0]):
+        for c in range(3):
+            a1, a2 = acc.copy(), acc.copy()
+            a1[k, c] += STEP
+            a2[k, c] -= STEP
+            h = float(a1[k, c]) - float(a2[k, c])
+            fd[k, c] = (oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(a1, grad=False)[0] -
+                        oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(a2, grad=False)[0]) / h
+    assert np.abs(grad - fd).max() / np.abs(fd).max() <= 1e-6
