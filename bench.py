@@ -156,18 +156,22 @@ def ensemble_measure(cabi, tanks, steps):
             "ensemble_particle_steps_per_s": n * steps / ens}
 
 
-def c5_case(cabi, steps, segments=16, seed=0):
+def c5_case(cabi, steps, segments=16, seed=0, superheat=0.2):
     """The differentiable-slosh workload: LN2 cylinder tank r = 0.1 m, h = 0.25 m,
     fill 0.5, dx = 3.7 mm, microgravity 1e-4 g, CSF surface tension with a 30
-    degree contact angle, a `segments`-piece constant frame-acceleration profile
+    degree contact angle, conduction and Lee phase change with coefficient 5
+    (vented tank, vapour rebalance), the liquid `superheat` K above saturation
+    (so that Lee transfer runs; small enough that no particle splits within a
+    few hundred steps), a `segments`-piece constant frame-acceleration profile
     drawn N(0, 0.03) m/s^2 per component (seeded), fixed dt = the start-up dt;
-    thermal, phase change and transport velocity off (outside the
-    differentiated scope)."""
+    transport velocity off."""
     fluid = cabi.hip_fluid_preset("LN2")
     dx = 0.0037
-    cfg = cabi.default_config(dx=dx, gravity=(0.0, 0.0, -9.81e-4), velocity_scale=0.1, thermal=0, phase_change=0,
+    cfg = cabi.default_config(dx=dx, gravity=(0.0, 0.0, -9.81e-4), velocity_scale=0.1, thermal=1, phase_change=1,
+                              lee_evaporation=5.0, lee_condensation=5.0, thermo_mode=0,
                               surface_tension=1, contact_angle_deg=30.0, transport_velocity=0)
     ps = cabi.hip_build_tank_particles(cabi.cylinder(0.1, 0.25), fluid, dx, cabi.default_fill(fill_fraction=0.5))
+    ps.T[ps.phase == 0] += np.float32(superheat)
     pad = np.float32(4 * dx)
     lo, hi = ps.x.min(0) - pad, ps.x.max(0) + pad
     probe = cabi.HipSolver(cfg, fluid)
@@ -191,20 +195,22 @@ def adjoint_measure(cabi, steps, every):
         s.initialize()
         s.time  # synchronises
         t0 = time.perf_counter()
-        loss, g, _, dtheta = s.adjoint_slosh(acc, checkpoint_every=every, grad=grad)
+        out = s.adjoint_slosh(acc, checkpoint_every=every, grad=grad)
         times[name] = time.perf_counter() - t0
+        loss = out[0]
         if grad:
-            gnorm, gtheta = float(np.abs(g).max()), dtheta
+            gnorm, gtheta, glee = float(np.abs(out[1]).max()), out[3], out.lee
         del s
     fwd = min(times["forward"], times["forward2"])
     adj = min(times["adjoint"], times["adjoint2"])
-    return {"workload": f"LN2 cylinder r=0.1 h=0.25 fill 0.5 dx=0.0037 CSF contact angle 30 deg, {len(ps)} "
-                        f"particles, {steps} steps of dt={cfg.fixed_dt:.3e} s, 16-segment frame-acceleration "
-                        f"profile; loss = sum dt |load_xy|^2",
+    return {"workload": f"LN2 cylinder r=0.1 h=0.25 fill 0.5 dx=0.0037 CSF contact angle 30 deg, Lee 5, "
+                        f"liquid 0.2 K superheated, vented, {len(ps)} particles, {steps} steps of "
+                        f"dt={cfg.fixed_dt:.3e} s, 16-segment frame-acceleration profile; "
+                        f"loss = sum dt |load_xy|^2",
             "scope": "density summation, EOS, Shepard wall BC, momentum, CSF with contact-angle rotation, "
-                     "kick-drift-kick; not thermal / Lee phase change",
+                     "conduction, Lee sources / normalisation / apply, vapour rebalance, venting, kick-drift-kick",
             "particles": len(ps), "steps": steps, "checkpoint_every": every, "loss": loss, "grad_acc_max": gnorm,
-            "grad_contact_angle": gtheta,
+            "grad_contact_angle": gtheta, "grad_lee_evaporation": glee[0], "grad_lee_condensation": glee[1],
             "forward_ms_per_step": 1e3 * fwd / steps, "forward_and_adjoint_ms_per_step": 1e3 * adj / steps,
             "cost_ratio": adj / fwd}
 

@@ -198,7 +198,8 @@ int cryo_ensemble_run_steps(cryo_solver* const* solvers, int32_t count, int64_t 
  * rotation, no thermal, phase change, transport stress, SPS or shifting, and
  * no frame rotation or manoeuvre table (CRYO_ERR_INVALID_ARGUMENT otherwise). */
 int cryo_solver_adjoint_slosh(cryo_solver* s, int64_t steps, const float* frame_acc, int32_t checkpoint_every,
-                              double* loss, double* grad_frame_acc, double* wall_load, double* grad_contact_angle);
+                              double* loss, double* grad_frame_acc, double* wall_load, double* grad_contact_angle,
+                              double* grad_lee);
 
 /* Solver::time / step_count / last_dt / num_particles (sph_solver.hpp:153-163). */
 int cryo_solver_time(cryo_solver* s, double* t);

@@ -107,15 +107,19 @@ struct Side {
 // with gw = r_IJ s, vsum = V_I^2 + V_J^2, pt = (rho_J p_I + rho_I p_J) / (rho_I + rho_J).
 // cF is the cotangent of the force sum (a_I's cotangent / m_I), cA that of the
 // artificial-viscosity sum.  r is the cotangent of r_IJ = x_I - x_J, vij that
-// of v_IJ = v_I - v_J.
+// of v_IJ = v_I - v_J; m_* and c_* those of the masses (through V = m / rho
+// and m_J in T3) and sound speeds.  f is the value T1 + T2 of the pair (the
+// owner needs the force sum for d(F / m_I) / d m_I).
 template <class T>
 struct FFGrad {
   Vec<T> r, vij;
-  T rho_i, rho_j, p_i, p_j;
+  T rho_i, rho_j, p_i, p_j, m_i, m_j, c_i, c_j;
+  Vec<T> f;
 };
 template <class T>
 CRYO_ADJ_HD inline FFGrad<T> ff_vjp(const Ctx<T>& c, const Side<T>& I, const Side<T>& J, Vec<T> cF, Vec<T> cA) {
-  FFGrad<T> o{{T(0), T(0), T(0)}, {T(0), T(0), T(0)}, T(0), T(0), T(0), T(0)};
+  FFGrad<T> o{{T(0), T(0), T(0)}, {T(0), T(0), T(0)}, T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0),
+              {T(0), T(0), T(0)}};
   const Vec<T> rij = I.x - J.x;
   const T r2 = dot(rij, rij);
   if (!(r2 < c.support2)) return o;
@@ -130,6 +134,7 @@ CRYO_ADJ_HD inline FFGrad<T> ff_vjp(const Ctx<T>& c, const Side<T>& I, const Sid
   const T G = s * cFr;  // cF . gw
   // T1
   o.r = (cF * s + rij * (cFr * spr)) * (-vsum * pt);
+  o.f = rij * (-s * vsum * pt);
   T vsum_b = -pt * G;
   const T pt_b = -vsum * G;
   // T2
@@ -140,8 +145,11 @@ CRYO_ADJ_HD inline FFGrad<T> ff_vjp(const Ctx<T>& c, const Side<T>& I, const Sid
     const T cFv = dot(cF, vij);
     o.r += rij * (vsum * eta * cFv * spr);
     o.vij += cF * (vsum * eta * s);
+    o.f += vij * (vsum * eta * s);
     vsum_b += eta * s * cFv;
   }
+  o.m_i = vsum_b * T(2) * Vi / I.rho;
+  o.m_j = vsum_b * T(2) * Vj / J.rho;
   T rhoi_b = vsum_b * (T(-2) * Vi * Vi / I.rho) + pt_b * (J.p - pt) / rs;
   T rhoj_b = vsum_b * (T(-2) * Vj * Vj / J.rho) + pt_b * (I.p - pt) / rs;
   o.p_i = pt_b * J.rho / rs;
@@ -158,6 +166,8 @@ CRYO_ADJ_HD inline FFGrad<T> ff_vjp(const Ctx<T>& c, const Side<T>& I, const Sid
       const T H = s * cAr;  // cA . gw
       o.r += (cA * s + rij * (cAr * spr)) * (-J.m * Pi);
       const T Pi_b = -J.m * H;
+      o.m_j -= Pi * H;
+      o.c_i = o.c_j = T(0.5) * Pi_b * (-I.alpha * mu / rbar);
       const T mu_b = Pi_b * (-I.alpha * cbar / rbar);
       const T rbar_b = Pi_b * (-Pi / rbar);
       rhoi_b += T(0.5) * rbar_b;
@@ -183,16 +193,19 @@ CRYO_ADJ_HD inline FFGrad<T> ff_vjp(const Ctx<T>& c, const Side<T>& I, const Sid
 // with vsum = V_I^2 + vol0^2 and n = r_Iw / r.  cF / cA: cotangents of the
 // force and acceleration sums (both carry a_I's and the wall-load cotangent).
 // r: cotangent of r_Iw = x_I - xw; vi: of v_I alone (W3); vij: of
-// v_Iw = v_I - vw; p_w: of pw.
+// v_Iw = v_I - vw; p_w: of pw; m_i, c_i: of m_I (through V_I) and c_I.
+// f and acc are the pair's values W1 + W4 and W2 + W3 + W5.
 template <class T>
 struct FWGrad {
   Vec<T> r, vi, vij;
-  T rho_i, p_i, p_w;
+  T rho_i, p_i, p_w, m_i, c_i;
+  Vec<T> f, acc;
 };
 template <class T>
 CRYO_ADJ_HD inline FWGrad<T> fw_vjp(const Ctx<T>& c, const Side<T>& I, Vec<T> xw, Vec<T> vw, T pw, Vec<T> cF,
                                     Vec<T> cA) {
-  FWGrad<T> o{{T(0), T(0), T(0)}, {T(0), T(0), T(0)}, {T(0), T(0), T(0)}, T(0), T(0), T(0)};
+  FWGrad<T> o{{T(0), T(0), T(0)}, {T(0), T(0), T(0)}, {T(0), T(0), T(0)}, T(0), T(0), T(0), T(0), T(0),
+              {T(0), T(0), T(0)}, {T(0), T(0), T(0)}};
   const Vec<T> rij = I.x - xw;
   const T r2 = dot(rij, rij);
   if (!(r2 < c.support2)) return o;
@@ -206,6 +219,7 @@ CRYO_ADJ_HD inline FWGrad<T> fw_vjp(const Ctx<T>& c, const Side<T>& I, Vec<T> xw
   // W1
   const T pm = T(0.5) * (I.p + pw);
   o.r = (cF * s + rij * (cFr * spr)) * (-vsum * pm);
+  o.f = rij * (-s * vsum * pm);
   o.p_i = T(-0.5) * vsum * G;
   o.p_w = o.p_i;
   T vsum_b = -pm * G;
@@ -217,9 +231,12 @@ CRYO_ADJ_HD inline FWGrad<T> fw_vjp(const Ctx<T>& c, const Side<T>& I, Vec<T> xw
     const T kap = T(4) * I.c * I.c / c.r_guard;
     o.r += (cA - n * a) * (kap * q * q / r);
     o.r -= n * (kap * T(2) * q * a / c.r_guard);
+    o.c_i = a * T(8) * I.c * q * q / c.r_guard;
+    o.acc = n * (kap * q * q);
     const T b = dot(I.v, n);
     if (b < T(0) && c.dt > T(0)) {
       const T k = T(-1) / c.dt;
+      o.acc += n * (k * q * b);
       o.vi += n * (k * q * a);
       o.r += (n * (-(a * b) / c.r_guard) + (cA - n * a) * (q * b / r) + (I.v - n * b) * (q * a / r)) * k;
     }
@@ -230,6 +247,7 @@ CRYO_ADJ_HD inline FWGrad<T> fw_vjp(const Ctx<T>& c, const Side<T>& I, Vec<T> xw
     // W4
     o.r += rij * (vsum * I.eta * cFv * spr);
     o.vij += cF * (vsum * I.eta * s);
+    o.f += vij * (vsum * I.eta * s);
     vsum_b += I.eta * s * cFv;
     // W5
     const T vr = dot(vij, rij);
@@ -240,6 +258,8 @@ CRYO_ADJ_HD inline FWGrad<T> fw_vjp(const Ctx<T>& c, const Side<T>& I, Vec<T> xw
       const T H = s * cAr;
       const T k5 = c.vol0 * I.alpha * I.c;
       o.r += (cA * s + rij * (cAr * spr)) * (k5 * mu);
+      o.acc += rij * (s * k5 * mu);
+      o.c_i += H * c.vol0 * I.alpha * mu;
       const T mu_b = k5 * H;
       const T vr_b = mu_b * c.h / den;
       const T r2_b = -mu_b * mu / den;
@@ -248,6 +268,7 @@ CRYO_ADJ_HD inline FWGrad<T> fw_vjp(const Ctx<T>& c, const Side<T>& I, Vec<T> xw
     }
   }
   o.rho_i = vsum_b * (T(-2) * Vi * Vi / I.rho);
+  o.m_i = vsum_b * T(2) * Vi / I.rho;
   return o;
 }
 
@@ -295,6 +316,47 @@ CRYO_ADJ_HD inline T eos_dp_drho(int vapor, T rho, T temp, T p, T rho0, T tait_B
     return T(7) * tait_B * r2 * r2 * r2 / rho0;
   }
   return tait_gamma * tait_B * std::pow(ratio, tait_gamma - T(1)) / rho0;
+}
+
+// dp/dT of the equation of state: rho R for vapour, 0 for liquid (and where
+// the pressure was clamped at 0).
+template <class T>
+CRYO_ADJ_HD inline T eos_dp_dT(int vapor, T rho, T p, T gas_R, int clamp_neg) {
+  if (!vapor || (clamp_neg && !(p > T(0)))) return T(0);
+  return rho * gas_R;
+}
+
+// ------------------------------------------------------------ heat
+// Conduction pair of the thermal rates (phase_change.cu:31-75), owner I:
+//   q_I += V_I V_J k_IJ (T_I - T_J) f(r),  f = r^2 s / (r^2 + eta2)   (r W'(r) = r^2 s)
+// qb: cotangent of q_I.  Returns the cotangents of r_IJ, V_I, V_J, T_I, T_J.
+template <class T>
+struct ThGrad {
+  Vec<T> r;
+  T Vi, Vj, Ti, Tj;
+};
+template <class T>
+CRYO_ADJ_HD inline ThGrad<T> th_vjp(const Ctx<T>& c, Vec<T> rij, T Vi, T Vj, T kij, T Ti, T Tj, T qb) {
+  ThGrad<T> o{{T(0), T(0), T(0)}, T(0), T(0), T(0), T(0)};
+  const T r2 = dot(rij, rij);
+  if (!(r2 < c.support2)) return o;
+  const T r = std::sqrt(r2);
+  const T s = wendland_s(c, r);
+  const T den = r2 + c.eta2;
+  const T f = r2 * s / den;
+  const T dT = Ti - Tj;
+  const T k = kij * qb;
+  o.Vi = k * Vj * dT * f;
+  o.Vj = k * Vi * dT * f;
+  o.Ti = k * Vi * Vj * f;
+  o.Tj = -o.Ti;
+  if (r > T(1e-12) * c.h) {
+    const T spr = wendland_spr(c, r);
+    // (1/r) df/dr = ((2 s + r^2 spr) den - 2 r^2 s) / den^2
+    const T fr = ((T(2) * s + r2 * spr) * den - T(2) * r2 * s) / (den * den);
+    o.r = rij * (k * Vi * Vj * dT * fr);
+  }
+  return o;
 }
 
 // ---------------------------------------------------------------- CSF

@@ -781,21 +781,22 @@ PYBIND11_MODULE(CryoSPH, m) {
         std::vector<float> a(3 * static_cast<std::size_t>(k));
         for (py::ssize_t i = 0; i < 3 * k; ++i) a[i] = static_cast<float>(acc.data()[i]);
         py::array_t<double> load({k, py::ssize_t{3}}), grad({k, py::ssize_t{3}});
-        double loss = 0.0, dtheta = 0.0;
+        double loss = 0.0, dtheta = 0.0, dlee[2] = {0.0, 0.0};
         int r;
         {
           py::gil_scoped_release rel;
           r = cryo_solver_adjoint_slosh(S.s, k, a.data(), checkpoint_every, &loss,
                                         gradient ? grad.mutable_data() : nullptr, load.mutable_data(),
-                                        gradient ? &dtheta : nullptr);
+                                        gradient ? &dtheta : nullptr, gradient ? dlee : nullptr);
         }
         check(r);
-        if (!gradient) return py::make_tuple(loss, py::none(), load, py::none());
-        return py::make_tuple(loss, grad, load, dtheta);
+        if (!gradient) return py::make_tuple(loss, py::none(), load, py::none(), py::none());
+        return py::make_tuple(loss, grad, load, dtheta, py::make_tuple(dlee[0], dlee[1]));
       }, py::arg("frame_acc"), py::arg("checkpoint_every") = 20, py::arg("gradient") = true,
          "Runs len(frame_acc) steps (step k with frame acceleration frame_acc[k]); returns (loss, d loss / "
-         "d frame_acc, wall load per step, d loss / d contact angle in degrees), loss = sum_k dt "
-         "|load_k,xy|^2; without gradient the two derivatives are None.")
+         "d frame_acc, wall load per step, d loss / d contact angle in degrees, (d loss / d "
+         "lee_evaporation, d loss / d lee_condensation)), loss = sum_k dt |load_k,xy|^2; without gradient "
+         "the derivatives are None.")
       .def("request_telemetry", [](Solver& S) {
         const int r = cryo_solver_request_telemetry(S.s);
         check(r);

@@ -106,6 +106,30 @@ struct AdjArgs {
   const float4* ct;  // cotangent of the transport acceleration ab at the step's end
   float4* nt;        // the same before the step (the drift reads ab), written through perm
   float4* tvs;       // {K * ab_bar, 1} of each active transport owner, else 0
+  // heat and phase change; mT = 0: masses and temperatures are constants
+  int mT, heat, pc, reb, vent;
+  float relax;              // venting relaxation
+  const float *cm, *cT;     // cotangents of m and T at the step's end
+  float *nm, *nT;           // the same at the step's start, written through perm
+  float *mr, *Tr;           // the step's m and T cotangents (sorted order)
+  float* qb;                // cotangent of each fluid particle's conduction sum
+  double* hsum;             // sum of qb heat_w (the heater normalisation's cotangent)
+  // the phase-change re-run: kicked v and T, normalised out_m, out_h, applied m and T,
+  // the sources' flags
+  const float4* v1;
+  const float *T1, *om, *oh, *m2, *T2;
+  const uint8_t* fl;
+  float4* pa;    // {m_bar before phase change, kicked T_bar, in_m_bar, in_H_bar}
+  float4* po;    // {out_m_bar, out_h_bar, w_norm_bar (apply share)}
+  float4* pb4;   // rebalance {dm_bar, dH_bar, m2_bar (own), T2_bar (own)}
+  float4* pdp;   // rebalance {dP_bar, rho_bar (venting)}
+  float4* v1b;   // cotangent of the kicked velocity
+  float2* pm2;   // {m2_bar, T2_bar}
+  float4* pn;    // {w_norm_bar, wr_max_bar, s_all_bar, s_other_bar}
+  int* amax;     // the neighbour giving a capped source's wr_max, else -1
+  float *pV, *pr;  // phase change's share of the V = m / rho and rho cotangents
+  float4* xp;      // phase change's share of the x cotangent
+  double* lee_out; // this step's d loss / d (lee_evap, lee_cond)
 };
 void launch_adj_fill(float4* a, float4* b, float4* c, int n, hipStream_t s);
 void launch_adj_set_frame(DevState* st, const float* acc, hipStream_t s);

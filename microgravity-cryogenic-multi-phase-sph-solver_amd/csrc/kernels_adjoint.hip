@@ -27,6 +27,17 @@
 //                  pairs (CSF), the EOS and V = m / rho -> sigma_bar
 //   k_adj_density  per fluid particle: density summation, drift and kick,
 //                  then scattered back through the step's sort permutation
+// With conduction and Lee phase change on, masses and temperatures are state
+// too.  The step's kick and the phase-change passes up to apply are re-run
+// forward on the taped state first (their values are kept), then, ahead of
+// k_adj_prep:
+//   k_adj_reb1/2   venting and vapour rebalance (phase_change.cu:217-268,337-358)
+//   k_adj_app1/2   apply: both sides of each source-receiver pair (:174-215)
+//   k_adj_src      receiver normalisation with the vapour density cap and the
+//                  Lee sources (:77-172), d loss / d Lee coefficients
+//   k_adj_nrm      the normalisation and interface-fraction pairs, both sides
+// and k_adj_fluid adds conduction pairs (phase_change.cu:31-75), the heater
+// weights and the masses' and temperatures' shares of every rates term.
 // Every pair is evaluated by each member from its own side (a gather, no
 // atomics), so the result is bitwise reproducible.  The per-pair derivatives
 // are in adjoint_pair.hpp.
@@ -155,17 +166,28 @@ __device__ inline float tvf_rho_w(const PV& pv, const SimParams& P, uint8_t ph, 
 __global__ __launch_bounds__(kBlock) void k_adj_prep(PV pv, SimParams P, AdjArgs A, const DevState* st, int n) {
   const int i = gtid();
   double g[3] = {0.0, 0.0, 0.0};
+  double hw = 0.0;
   if (i < n) {
     float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (is_fluid(pv.s.phase[i])) {
       const float half = 0.5f * st->dt;
-      const float4 a = A.ca[i], v = A.cv[i];
+      const float4 a = A.ca[i], v = A.pc ? A.v1b[i] : A.cv[i];
       r = make_float4(a.x + v.x * half, a.y + v.y * half, a.z + v.z * half, 0.0f);
       g[0] = r.x;  // a = g_eff + ...
       g[1] = r.y;
       g[2] = r.z;
     }
     A.at[i] = r;
+    if (A.heat) {  // T1 = T + dTdt dt, dTdt = q / (m cp)
+      float q = 0.0f;
+      const uint8_t ph = pv.s.phase[i];
+      if (is_fluid(ph)) {
+        const float T1b = A.pc ? A.pa[i].y : A.cT[i];
+        q = T1b * st->dt / (pv.s.m[i] * sel2(P.cp, pidx(ph)));
+        hw = static_cast<double>(q) * static_cast<double>(pv.t.heat_w[i]);
+      }
+      A.qb[i] = q;
+    }
     if (A.tvf) {
       float4 t = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       if (is_fluid(pv.s.phase[i]) && tvf_active(pv, P, i)) {
@@ -177,6 +199,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_prep(PV pv, SimParams P, AdjArgs
     }
   }
   block_sums_store<3>(g, A.partials + 3 * A.stride, A.stride);
+  if (A.heat) {
+    double h1[1] = {hw};
+    block_sums_store<1>(h1, A.partials + 8 * A.stride, A.stride);
+  }
 }
 
 __device__ inline bool live_normal(const PV& pv, int i) { return norm2(ld3(pv.t.ns[i])) > 0.25f; }
@@ -386,8 +412,20 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
   const bool tv_own = tvi.w > 0.0f;
   const AV Sbi{tvi.x, tvi.y, tvi.z};
   const float Vti = A.tvf ? 1.0f / pv.t.sigma[i] : 0.0f, Vti2 = Vti * Vti;
-  float vt2b = 0.0f, vtb = 0.0f, trhob = 0.0f, tsb = 0.0f;
+  float vt2b = 0.0f, vtb = 0.0f, trhob = 0.0f, tsb = 0.0f, tTb = 0.0f;
   AV tsum{0.0f, 0.0f, 0.0f};
+  // masses, temperatures (sound speeds), the force sum for d(F / m) / dm, conduction
+  float mb = 0.0f, Tb = 0.0f, cb = 0.0f;
+  AV F{0.0f, 0.0f, 0.0f};
+  const float Ti = pv.s.T[i];
+  const float qi = A.heat ? A.qb[i] : 0.0f;
+  const float ki = sel2(P.k, pidx(phi));
+  const bool iso = P.wall_bc == 1;
+  float hwb = 0.0f;  // heater: hw_i's cotangent, q_i P hinv + (sum's share)
+  if (A.heat && P.heat_power != 0.0f && st->heat_inv_norm > 0.0f) {
+    const float hinv = st->heat_inv_norm;
+    hwb = qi * P.heat_power * hinv - static_cast<float>(A.hsum[0]) * P.heat_power * hinv * hinv;
+  }
   if (A.csf) {
     const float4 gb = A.gcb[i], wb = A.wnb[i];
     gbk = AV{gb.x, gb.y, gb.z};
@@ -407,6 +445,23 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
       vr += o.vi + o.vij;
       rb += o.rho_i;
       pb += o.p_i;
+      if (A.mT) {
+        F += o.f;
+        mb += o.m_i + cryo_adj::dot(Fb, o.acc);  // the wall load carries m_i acc
+        cb += o.c_i;
+        if (A.heat) {
+          const AV rkw = I.x - xw;
+          if (iso) {
+            const cryo_adj::ThGrad<float> oh = cryo_adj::th_vjp(c, rkw, Vk, P.vol0, 2.0f * ki, Ti, pv.s.T[j], qi);
+            xr += oh.r;
+            Vbk += oh.Vi;
+            Tb += oh.Ti;
+          }
+          const float r2 = cryo_adj::dot(rkw, rkw);
+          if (hwb != 0.0f && (pv.s.flags[j] & kHeated) && r2 < P.support2)
+            xr += rkw * (hwb * P.vol0 * cryo_adj::wendland_s(c, sqrtf(r2)));
+        }
+      }
       const float4 cp = A.wcp[j], cv = A.wcv[j];
       const cryo_adj::WBGrad<float> b =
           cryo_adj::wb_vjp(c, I, xw, pw, vw, vap ? cp.y : cp.x, AV{cv.x, cv.y, cv.z}, grav);
@@ -424,7 +479,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
           const cryo_adj::TvGrad<float> ot = cryo_adj::tvf_vjp(c, rkw, Vti2, Vj, Sbi);
           xr += ot.r;
           vt2b += ot.vt2;
-          (void)cryo_adj::tvf_wall_vjp(Vti, I.rho, rho_w, ot.vj, vtb, trhob);  // the wall's share: k_adj_wall
+          const float rwb = cryo_adj::tvf_wall_vjp(Vti, I.rho, rho_w, ot.vj, vtb, trhob);  // wall: k_adj_wall
+          if (vap) tTb -= rwb * rho_w / pv.s.T[i];  // vapour wall density (p_wv - p0) / (R T_i)
         }
       }
       if (A.csf) {
@@ -482,10 +538,33 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
     vr -= o2.vij;
     rb += o1.rho_i + o2.rho_j;
     pb += o1.p_i + o2.p_j;
+    if (A.mT) {
+      F += o1.f;
+      mb += o1.m_i + o2.m_j;
+      cb += o1.c_i + o2.c_j;
+      if (A.heat) {  // conduction pairs, both sides
+        const AV rkj = I.x - J.x;
+        const float kj = sel2(P.k, pidx(phj));
+        const float kij = 4.0f * ki * kj / (ki + kj);
+        const float Vj = J.m / J.rho;
+        const cryo_adj::ThGrad<float> h1 = cryo_adj::th_vjp(c, rkj, Vk, Vj, kij, Ti, pv.s.T[j], qi);
+        const cryo_adj::ThGrad<float> h2 =
+            cryo_adj::th_vjp(c, AV{-rkj.x, -rkj.y, -rkj.z}, Vj, Vk, kij, pv.s.T[j], Ti, A.qb[j]);
+        xr += h1.r;
+        xr -= h2.r;
+        Vbk += h1.Vi + h2.Vj;
+        Tb += h1.Ti + h2.Tj;
+      }
+    }
   });
   const float dpdr = cryo_adj::eos_dp_drho(vap ? 1 : 0, I.rho, pv.s.T[i], I.p, P.rho0[0], P.tait_B, P.tait_gamma,
                                            P.gas_R, P.clamp_neg);
-  rb -= Vbk * Vk / I.rho;  // V = m / rho in the curvature and colour sums
+  if (A.pc) {  // phase change's shares of V, rho and x
+    Vbk += A.pV[i];
+    rb += A.pr[i];
+    xr += av(A.xp[i]);
+  }
+  rb -= Vbk * Vk / I.rho;  // V = m / rho in the curvature, colour, conduction and phase-change sums
   if (tv_own) {  // K = -pb / (rho Vt), Vt = 1 / sigma
     const float K = tvf_K(pv, P, i);
     const float4 ct = A.ct[i];
@@ -495,8 +574,21 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
     vtb += 2.0f * Vti * vt2b;
     tsb -= vtb * Vti * Vti;
     rb += trhob;
+    if (vap) tTb += K * Kb / pv.s.T[i];  // c^2 = R T
   }
-  A.sb[i] = (rb + pb * dpdr) * (vap ? I.m : pv.s.m0[i]) + tsb;  // rho = m0 sigma (liquid), m sigma (vapour)
+  const float rhob = rb + pb * dpdr;
+  A.sb[i] = rhob * (vap ? I.m : pv.s.m0[i]) + tsb;  // rho = m0 sigma (liquid), m sigma (vapour)
+  if (A.mT) {
+    const float4 pa = A.pc ? A.pa[i] : make_float4(A.cm[i], A.cT[i], 0.0f, 0.0f);
+    mb += pa.x + Vbk / I.rho;
+    mb -= (cA.x * F.x + cA.y * F.y + cA.z * F.z) / (I.m * I.m);  // a = g + F / m + ...
+    if (A.heat) mb -= pa.y * dt * pv.t.dTdt[i] / I.m;      // dTdt = q / (m cp)
+    if (vap) mb += rhob * pv.t.sigma[i];
+    Tb += pa.y + tTb + pb * cryo_adj::eos_dp_dT(vap ? 1 : 0, I.rho, I.p, P.gas_R, P.clamp_neg);
+    if (vap && Ti > 0.0f) Tb += cb * P.gas_R / (2.0f * I.c);  // c = sqrt(R T)
+    A.mr[i] = mb;
+    A.Tr[i] = Tb;
+  }
   A.xr[i] = f4(xr);
   A.vr[i] = f4(vr);
 }
@@ -512,6 +604,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_density(PV pv, GridDev g, SimPar
     A.nv[dst] = z;
     A.na[dst] = z;
     if (A.tvf) A.nt[dst] = z;
+    if (A.mT) A.nm[dst] = A.nT[dst] = 0.0f;
     return;
   }
   const float dt = st->dt, half = 0.5f * dt;
@@ -529,11 +622,396 @@ __global__ __launch_bounds__(kBlock) void k_adj_density(PV pv, GridDev g, SimPar
   });
   // x' = x + v_half dt, v' = v_half + a' dt/2, v_half = v + a dt/2
   const AV X = av(A.cx[i]) + xr;
-  const AV Vh = av(A.cv[i]) + av(A.vr[i]) + X * dt;
+  const AV Vh = av(A.pc ? A.v1b[i] : A.cv[i]) + av(A.vr[i]) + X * dt;
+  if (A.mT) {
+    A.nm[dst] = A.mr[i];
+    A.nT[dst] = A.Tr[i];
+  }
   A.nx[dst] = f4(X);
   A.nv[dst] = f4(Vh);
   A.na[dst] = f4(Vh * half);
   if (A.tvf) A.nt[dst] = f4(X * (half * dt));  // x' = x + (v_half + ab dt/2) dt
+}
+
+// ------------------------------------------------------ phase change
+__device__ inline uint8_t pc_target(uint8_t fl) { return (fl & kTargetVapor) ? kVapor : kLiquid; }
+
+// Venting (m4 = m3 - relax (m3 - rho0_v m3 / rho)) and the owner's part of the
+// vapour rebalance (m3 = m2 + dm, T3 = T(H / m3), v3 = (v1 m2 + dP) / m3, with
+// dm, dH, dP recomputed from the applied masses, temperatures and kicked velocities).
+__global__ __launch_bounds__(kBlock) void k_adj_reb1(PV pv, GridDev g, SimParams P, NbrList L, AdjArgs A, int n) {
+  const int i = gtid();
+  if (i >= n) return;
+  const uint8_t ph = pv.s.phase[i];
+  float m2b = A.cm[i], T2b = A.cT[i], dmb = 0.0f, dHb = 0.0f, rhob = 0.0f;
+  AV v1b = av(A.cv[i]), dPb{0.0f, 0.0f, 0.0f};
+  if (ph == kVapor) {
+    const float beta = P.vapor_rebalance, cpv = P.cp[1];
+    const V3 xi = ld3(pv.s.x[i]);
+    const float mi = A.m2[i], hi = enthalpy(P, kVapor, A.T2[i]);
+    const AV vi = av(A.v1[i]);
+    float dm = 0.0f, dH = 0.0f;
+    AV dP{0.0f, 0.0f, 0.0f};
+    if (A.reb) {
+      for_each_pair(L, g, i, xi, [&](int j) {
+        if (j == i || pv.s.phase[j] != kVapor) return;
+        const float r2 = norm2(xi - ld3(pv.s.x[j]));
+        if (r2 >= P.support2) return;
+        const float flow = beta * P.vol0 * kW(P.kernel, sqrtf(r2)) * (A.m2[j] - mi);
+        const bool from_j = flow > 0.0f;
+        dm += flow;
+        dH += flow * (from_j ? enthalpy(P, kVapor, A.T2[j]) : hi);
+        dP += (from_j ? av(A.v1[j]) : vi) * flow;
+      });
+    }
+    const bool upd = A.reb && dm != 0.0f;
+    const float m3 = upd ? mi + dm : mi;
+    float m3b = A.cm[i];
+    if (A.vent) {
+      const float rho = pv.s.rho[i], r0 = P.rho0[1];
+      if (m3 > r0 * (m3 / rho)) {
+        m3b = A.cm[i] * (1.0f - A.relax + A.relax * r0 / rho);
+        rhob = A.cm[i] * (-A.relax * r0 * m3 / (rho * rho));
+      }
+    }
+    m2b = m3b;
+    if (upd) {
+      const float mn = m3, hh = (mi * hi + dH) / mn;
+      const AV v3 = (vi * mi + dP) * (1.0f / mn);
+      const float hhb = A.cT[i] / cpv, Hb = hhb / mn;
+      float mnb = m3b - hhb * hh / mn;
+      const AV Vb3 = av(A.cv[i]);
+      v1b = Vb3 * (mi / mn);
+      float m2bi = cryo_adj::dot(Vb3, vi) / mn;
+      dPb = Vb3 * (1.0f / mn);
+      mnb -= cryo_adj::dot(Vb3, v3) / mn;
+      m2b = m2bi + mnb + Hb * hi;
+      dmb = mnb;
+      dHb = Hb;
+      T2b = Hb * mi * cpv;
+    }
+  }
+  A.pb4[i] = make_float4(dmb, dHb, m2b, T2b);
+  A.pdp[i] = make_float4(dPb.x, dPb.y, dPb.z, rhob);
+  A.v1b[i] = f4(v1b);
+}
+
+// Both sides of each rebalance pair: flow_ij = beta vol0 W (m2_j - m2_i).
+__global__ __launch_bounds__(kBlock) void k_adj_reb2(PV pv, GridDev g, SimParams P, NbrList L, AdjArgs A, int n) {
+  const int i = gtid();
+  if (i >= n) return;
+  const float4 own = A.pb4[i];
+  float m2b = own.z, T2b = own.w;
+  AV xr{0.0f, 0.0f, 0.0f};
+  if (A.reb && pv.s.phase[i] == kVapor) {
+    const float beta = P.vapor_rebalance, cpv = P.cp[1];
+    const cryo_adj::Ctx<float> c = adj_ctx(P, 0.0f);
+    const V3 xi = ld3(pv.s.x[i]);
+    const float mi = A.m2[i], hi = enthalpy(P, kVapor, A.T2[i]);
+    const AV vi = av(A.v1[i]);
+    const float4 dpi = A.pdp[i];
+    const AV dPi{dpi.x, dpi.y, dpi.z};
+    AV v1b = av(A.v1b[i]);
+    for_each_pair(L, g, i, xi, [&](int j) {
+      if (j == i || pv.s.phase[j] != kVapor) return;
+      const V3 rv = xi - ld3(pv.s.x[j]);
+      const float r2 = norm2(rv);
+      if (r2 >= P.support2) return;
+      const float r = sqrtf(r2), W = kW(P.kernel, r), sW = cryo_adj::wendland_s(c, r);
+      const AV rij = av(rv);
+      const float mj = A.m2[j], hj = enthalpy(P, kVapor, A.T2[j]);
+      const AV vj = av(A.v1[j]);
+      // i's own flow from j
+      const float flow = beta * P.vol0 * W * (mj - mi);
+      const bool from_j = flow > 0.0f;
+      const float fb = own.x + own.y * (from_j ? hj : hi) + cryo_adj::dot(dPi, from_j ? vj : vi);
+      if (!from_j) {
+        T2b += own.y * flow * cpv;
+        v1b += dPi * flow;
+      }
+      const float k = fb * beta * P.vol0;
+      m2b -= k * W;
+      xr += rij * (k * (mj - mi) * sW);
+      // j's flow from i
+      const float4 oj = A.pb4[j], dpj = A.pdp[j];
+      const AV dPj{dpj.x, dpj.y, dpj.z};
+      const float flow2 = beta * P.vol0 * W * (mi - mj);
+      const bool from_i = flow2 > 0.0f;
+      const float fb2 = oj.x + oj.y * (from_i ? hi : hj) + cryo_adj::dot(dPj, from_i ? vi : vj);
+      if (from_i) {
+        T2b += oj.y * flow2 * cpv;
+        v1b += dPj * flow2;
+      }
+      const float k2 = fb2 * beta * P.vol0;
+      m2b += k2 * W;
+      xr += rij * (k2 * (mi - mj) * sW);
+    });
+    A.v1b[i] = f4(v1b);
+  }
+  A.pm2[i] = make_float2(m2b, T2b);
+  A.xp[i] = f4(xr);
+}
+
+// Apply, the receiver's own part: m2 = m - om + in_m, H2 = m h(T1) - om oh + in_H,
+// T2 = T(H2 / m2) (in_m, in_H recomputed).
+__global__ __launch_bounds__(kBlock) void k_adj_app1(PV pv, GridDev g, SimParams P, NbrList L, AdjArgs A, int n) {
+  const int i = gtid();
+  if (i >= n) return;
+  const uint8_t ph = pv.s.phase[i];
+  const float2 b2 = A.pm2[i];
+  float m0b = b2.x, T1b = b2.y, inmb = 0.0f, inHb = 0.0f, omb = 0.0f, ohb = 0.0f;
+  if (is_fluid(ph)) {
+    const V3 xi = ld3(pv.s.x[i]);
+    const float mi = pv.s.m[i], Vi = mi / pv.s.rho[i];
+    float in_m = 0.0f, in_H = 0.0f;
+    if (!(A.fl[i] & kDying)) {
+      for_each_pair(L, g, i, xi, [&](int j) {
+        if (j == i || !(A.om[j] > 0.0f) || pc_target(A.fl[j]) != ph) return;
+        const float r2 = norm2(xi - ld3(pv.s.x[j]));
+        if (r2 >= P.support2) return;
+        const float share = A.om[j] * Vi * kW(P.kernel, sqrtf(r2)) / pv.t.w_norm[j];
+        in_m += share;
+        in_H += share * A.oh[j];
+      });
+    }
+    const float om = A.om[i];
+    if (om > 0.0f || in_m > 0.0f) {
+      const float mn = mi - om + in_m, hT = enthalpy(P, ph, A.T1[i]);
+      const float hh = (mi * hT - om * A.oh[i] + in_H) / mn;
+      const float cpp = sel2(P.cp, pidx(ph));
+      const float hhb = b2.y / cpp, Hb = hhb / mn;
+      const float mnb = b2.x - hhb * hh / mn;
+      m0b = mnb + Hb * hT;
+      T1b = Hb * mi * cpp;
+      omb = -mnb - Hb * A.oh[i];
+      ohb = -Hb * om;
+      inmb = mnb;
+      inHb = Hb;
+    }
+  }
+  A.pa[i] = make_float4(m0b, T1b, inmb, inHb);
+  A.po[i] = make_float4(omb, ohb, 0.0f, 0.0f);
+}
+
+// Apply pairs, both sides: share = om_j V_i W / wn_j.
+__global__ __launch_bounds__(kBlock) void k_adj_app2(PV pv, GridDev g, SimParams P, NbrList L, AdjArgs A, int n) {
+  const int i = gtid();
+  if (i >= n) return;
+  const uint8_t ph = pv.s.phase[i];
+  if (!is_fluid(ph)) {
+    A.pV[i] = 0.0f;
+    return;
+  }
+  const cryo_adj::Ctx<float> c = adj_ctx(P, 0.0f);
+  const float4 pai = A.pa[i];
+  float4 po = A.po[i];
+  const bool recv = pai.z != 0.0f || pai.w != 0.0f;
+  const float om = A.om[i], oh = A.oh[i], wn = pv.t.w_norm[i];
+  const uint8_t tgt = pc_target(A.fl[i]);
+  const V3 xi = ld3(pv.s.x[i]);
+  const float Vi = pv.s.m[i] / pv.s.rho[i];
+  float Vb = 0.0f;
+  AV xr{0.0f, 0.0f, 0.0f};
+  for_each_pair(L, g, i, xi, [&](int j) {
+    if (j == i) return;
+    const uint8_t phj = pv.s.phase[j];
+    if (!is_fluid(phj)) return;
+    const V3 rv = xi - ld3(pv.s.x[j]);
+    const float r2 = norm2(rv);
+    if (r2 >= P.support2) return;
+    const float r = sqrtf(r2), W = kW(P.kernel, r), sW = cryo_adj::wendland_s(c, r);
+    const float omj = A.om[j];
+    if (recv && omj > 0.0f && pc_target(A.fl[j]) == ph) {  // i receives from j
+      const float wnj = pv.t.w_norm[j];
+      const float shb = pai.z + pai.w * A.oh[j];
+      Vb += shb * omj * W / wnj;
+      xr += av(rv) * (shb * omj * Vi / wnj * sW);
+    }
+    if (om > 0.0f && phj == tgt && !(A.fl[j] & kDying)) {  // j receives from i
+      const float4 paj = A.pa[j];
+      if (paj.z != 0.0f || paj.w != 0.0f) {
+        const float Vj = pv.s.m[j] / pv.s.rho[j];
+        const float share = om * Vj * W / wn;
+        const float shb = paj.z + paj.w * oh;
+        po.y += paj.w * share;
+        po.x += shb * Vj * W / wn;
+        po.z -= shb * share / wn;
+        xr += av(rv) * (shb * om * Vj / wn * sW);
+      }
+    }
+  });
+  A.po[i] = po;
+  A.pV[i] = Vb;
+  A.xp[i] = f4(av(A.xp[i]) + xr);
+}
+
+// Receiver normalisation (with the vapour density cap) and the Lee sources of
+// each source; the interface-fraction cotangent's sums for the next kernel.
+__global__ __launch_bounds__(kBlock) void k_adj_src(PV pv, GridDev g, SimParams P, NbrList L, const DevState* st,
+                                                   AdjArgs A, int n) {
+  const int i = gtid();
+  double lee[2] = {0.0, 0.0};
+  if (i < n) {
+    const uint8_t ph = pv.s.phase[i];
+    float4 pa = A.pa[i];
+    const float4 po = A.po[i];
+    float wnb = po.z, wrb = 0.0f, fab = 0.0f, sab = 0.0f, sob = 0.0f;
+    int amax = -1;
+    const uint8_t fl = A.fl[i];
+    const V3 xi = ld3(pv.s.x[i]);
+    if (is_fluid(ph) && (fl & kSource) && pv.t.w_norm[i] > 0.0f) {
+      const float dt = st->dt, Ts = st->T_sat;
+      const float m = pv.s.m[i], T = A.T1[i], alpha = pv.t.alpha[i];
+      // the source's out_m before the cap (k_kick_sources' arithmetic)
+      float om_pre = 0.0f, dm_r = 0.0f, dm_e = 0.0f;
+      int br = 0;  // 1 rate, 2 energy cap, 3 mass cap
+      if (ph == kLiquid) {
+        const float h_out = P.h_fg + P.cp[1] * (Ts - P.T_ref);
+        const float latent = h_out - P.cp[0] * (Ts - P.T_ref);
+        const float rate = P.lee_evap * m * alpha * (T - Ts) / Ts;
+        dm_r = rate * dt;
+        dm_e = m * P.cp[0] * (T - Ts) / latent;
+        float dm = dm_r < dm_e ? dm_r : dm_e;
+        br = dm_r < dm_e ? 1 : 2;
+        const float cap = P.max_transfer * m;
+        if (!(dm < cap)) br = 3, dm = cap;
+        om_pre = dm;
+      } else {
+        const float rate = P.lee_cond * m * alpha * (Ts - T) / Ts;
+        dm_r = rate * dt;
+        float cap = P.max_transfer * m;
+        if (P.pc_density_limit > 0.0f && P.pc_density_limit * m < cap) cap = P.pc_density_limit * m;
+        br = dm_r < cap ? 1 : 3;
+        om_pre = dm_r < cap ? dm_r : cap;
+      }
+      float ompb = po.x;
+      if (A.om[i] != om_pre) {  // the vapour density cap: om = limit wn / wr_max
+        const uint8_t tgt = pc_target(fl);
+        float wr_max = 0.0f;
+        for_each_pair(L, g, i, xi, [&](int j) {
+          if (j == i || pv.s.phase[j] != tgt || (A.fl[j] & kDying)) return;
+          const float r2 = norm2(xi - ld3(pv.s.x[j]));
+          if (r2 >= P.support2) return;
+          const float wr = kW(P.kernel, sqrtf(r2)) / pv.s.rho[j];
+          if (wr > wr_max) {
+            wr_max = wr;
+            amax = j;
+          }
+        });
+        if (wr_max > 0.0f) {
+          wnb += ompb * P.pc_density_limit / wr_max;
+          wrb = -ompb * A.om[i] / wr_max;
+        }
+        ompb = 0.0f;
+      }
+      const float capf = om_pre / m;
+      if (ph == kLiquid) {
+        const float h_out = P.h_fg + P.cp[1] * (Ts - P.T_ref);
+        const float latent = h_out - P.cp[0] * (Ts - P.T_ref);
+        if (br == 3) {
+          pa.x += ompb * capf;
+        } else if (br == 1) {
+          pa.x += ompb * dm_r / m;
+          fab = ompb * dm_r / alpha;
+          pa.y += ompb * P.lee_evap * m * alpha / Ts * dt;
+          lee[0] = static_cast<double>(ompb * m * alpha * (T - Ts) / Ts * dt);
+        } else {
+          pa.x += ompb * P.cp[0] * (T - Ts) / latent;
+          pa.y += ompb * m * P.cp[0] / latent;
+        }
+      } else {
+        pa.y += po.y * P.cp[1];  // out_h = h(T) of the condensing vapour
+        if (br == 3) {
+          pa.x += ompb * capf;
+        } else {
+          pa.x += ompb * dm_r / m;
+          fab = ompb * dm_r / alpha;
+          pa.y -= ompb * P.lee_cond * m * alpha / Ts * dt;
+          lee[1] = static_cast<double>(ompb * m * alpha * (Ts - T) / Ts * dt);
+        }
+      }
+      A.pa[i] = pa;
+    }
+    if (fab != 0.0f) {  // alpha = s_other / s_all over fluid neighbours (self included in s_all)
+      float sa = 0.0f, so = 0.0f;
+      for_each_pair(L, g, i, xi, [&](int j) {
+        if (!is_fluid(pv.s.phase[j])) return;
+        const float r2 = norm2(xi - ld3(pv.s.x[j]));
+        if (r2 >= P.support2) return;
+        const float Wv = pv.s.m[j] / pv.s.rho[j] * kW(P.kernel, sqrtf(r2));
+        sa += Wv;
+        if (pv.s.phase[j] != ph) so += Wv;
+      });
+      if (sa > 0.0f) {
+        sob = fab / sa;
+        sab = -fab * (so / sa) / sa;
+      }
+    }
+    if (i < n) {
+      A.pn[i] = make_float4(wnb, wrb, sab, sob);
+      A.amax[i] = amax;
+    }
+  }
+  block_sums_store<2>(lee, A.partials + 9 * A.stride, A.stride);
+}
+
+// Normalisation pairs (wn_i = sum V_j W over the target phase; the cap's wr_max
+// pair) and interface-fraction pairs, both sides.
+__global__ __launch_bounds__(kBlock) void k_adj_nrm(PV pv, GridDev g, SimParams P, NbrList L, AdjArgs A, int n) {
+  const int i = gtid();
+  if (i >= n) return;
+  const uint8_t ph = pv.s.phase[i];
+  if (!is_fluid(ph)) {
+    A.pr[i] = 0.0f;
+    return;
+  }
+  const cryo_adj::Ctx<float> c = adj_ctx(P, 0.0f);
+  const float4 pni = A.pn[i];
+  const uint8_t fli = A.fl[i];
+  const bool src = (fli & kSource) && pni.x != 0.0f;
+  const uint8_t tgt = pc_target(fli);
+  const int ami = A.amax[i];
+  const V3 xi = ld3(pv.s.x[i]);
+  const float Vi = pv.s.m[i] / pv.s.rho[i], rhoi = pv.s.rho[i];
+  float Vb = 0.0f, rhob = 0.0f;
+  AV xr{0.0f, 0.0f, 0.0f};
+  for_each_pair(L, g, i, xi, [&](int j) {
+    const uint8_t phj = pv.s.phase[j];
+    if (!is_fluid(phj)) return;
+    const V3 rv = xi - ld3(pv.s.x[j]);
+    const float r2 = norm2(rv);
+    if (r2 >= P.support2) return;
+    const float r = sqrtf(r2), W = kW(P.kernel, r);
+    if (j == i) {  // s_all holds the owner itself
+      Vb += pni.z * W;
+      return;
+    }
+    const float sW = cryo_adj::wendland_s(c, r);
+    const AV rij = av(rv);
+    const float Vj = pv.s.m[j] / pv.s.rho[j];
+    if (src && phj == tgt && !(A.fl[j] & kDying)) xr += rij * (pni.x * Vj * sW);
+    if (ami == j) xr += rij * (pni.y / pv.s.rho[j] * sW);
+    const float4 pnj = A.pn[j];
+    const uint8_t flj = A.fl[j];
+    if ((flj & kSource) && pnj.x != 0.0f && ph == pc_target(flj) && !(fli & kDying)) {
+      Vb += pnj.x * W;
+      xr += rij * (pnj.x * Vi * sW);
+    }
+    if (A.amax[j] == i) {
+      rhob -= pnj.y * (W / rhoi) / rhoi;
+      xr += rij * (pnj.y / rhoi * sW);
+    }
+    const float k1 = pni.z + (phj != ph ? pni.w : 0.0f);  // i's interface fraction
+    if (k1 != 0.0f) xr += rij * (k1 * Vj * sW);
+    const float k2 = pnj.z + (phj != ph ? pnj.w : 0.0f);  // j's
+    if (k2 != 0.0f) {
+      Vb += k2 * W;
+      xr += rij * (k2 * Vi * sW);
+    }
+  });
+  A.pV[i] += Vb;
+  A.pr[i] = rhob + A.pdp[i].w;
+  A.xp[i] = f4(av(A.xp[i]) + xr);
 }
 
 }  // namespace
@@ -558,7 +1036,17 @@ void launch_adj_csf_only(const PV& pv, const GridDev& g, const SimParams& P, con
 void launch_adj_step(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, const DevState* st,
                      const AdjArgs& A, int n, double* grad_out, double* theta_out, hipStream_t s) {
   const int nb = blocks_for(n > 0 ? n : 1);
+  if (A.pc) {
+    k_adj_reb1<<<nb, kBlock, 0, s>>>(pv, g, P, L, A, n);
+    k_adj_reb2<<<nb, kBlock, 0, s>>>(pv, g, P, L, A, n);
+    k_adj_app1<<<nb, kBlock, 0, s>>>(pv, g, P, L, A, n);
+    k_adj_app2<<<nb, kBlock, 0, s>>>(pv, g, P, L, A, n);
+    k_adj_src<<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
+    k_adj_finish<<<1, 1024, 0, s>>>(A.partials, A.stride, nb, 9, -1, 1.0, A.lee_out, 2);
+    k_adj_nrm<<<nb, kBlock, 0, s>>>(pv, g, P, L, A, n);
+  }
   k_adj_prep<<<nb, kBlock, 0, s>>>(pv, P, A, st, n);
+  if (A.heat) k_adj_finish<<<1, 1024, 0, s>>>(A.partials, A.stride, nb, 8, -1, 1.0, A.hsum, 1);
   if (A.csf) {
     k_adj_csf1<<<nb, kBlock, 0, s>>>(pv, g, P, L, A, n);
     k_adj_csf2<<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n, 6);

@@ -824,6 +824,10 @@ struct cryo_solver {
     if (tape_slot) snap_save(*tape_slot, false, last_perm);
     if (marks.size() > 4096) prof_collect();
     evaluate_rates();
+    if (tape_slot && tape_rates) {  // the kick reads a and dTdt: the reverse step re-runs it
+      fit(tape_slot->a, n), d2d(tape_slot->a.get(), a.get(), n);
+      fit(tape_slot->dTdt, n), d2d(tape_slot->dTdt.get(), view().t.dTdt, n);
+    }
     const bool shifting = P.shift_strength > 0.0f;
     const bool pc = cfg.phase_change != 0;
     PV pv = view();
@@ -899,7 +903,7 @@ struct cryo_solver {
   // post-sort states).
   struct Snap {
     DevBuf<float4> x, v, vt, nrm, a, ab;
-    DevBuf<float> m, m0, rho, T;
+    DevBuf<float> m, m0, rho, T, dTdt;
     DevBuf<uint8_t> phase, flags;
     DevBuf<uint32_t> id, perm;
     DevBuf<DevState> st;
@@ -936,7 +940,15 @@ struct cryo_solver {
   DevBuf<float4> adj_at, adj_wcp, adj_wcv, adj_xr, adj_vr, adj_cs, adj_gcb, adj_wnb, adj_xc, adj_tvs;
   DevBuf<float> adj_sb, adj_acc, adj_csv;
   DevBuf<double> adj_S, adj_G, adj_T, adj_part;
+  // heat and phase change: m and T cotangents (two sets), the phase-change re-run, its cotangents
+  DevBuf<float> adj_cm[2], adj_cT[2], adj_mr, adj_Tr, adj_qb, adj_T1, adj_om, adj_oh, adj_m2, adj_T2, adj_pV, adj_pr;
+  DevBuf<float4> adj_v1, adj_pa, adj_po, adj_pb4, adj_pdp, adj_v1b, adj_pn, adj_xp;
+  DevBuf<float2> adj_pm2;
+  DevBuf<uint8_t> adj_fl;
+  DevBuf<int> adj_amax;
+  DevBuf<double> adj_hsum, adj_lee;
   Snap* tape_slot = nullptr;  // set: step_issue() records its post-sort state here
+  bool tape_rates = false;    // and, after the rates, a and dTdt
 
   template <class T>
   void d2d(T* dst, const T* src, int count) {
@@ -972,13 +984,51 @@ struct cryo_solver {
     tiles_ok = false;
   }
 
-  AdjArgs adj_args(int cur, const uint32_t* perm, const double* S, bool csf) {
-    return AdjArgs{adj_c[cur][0].get(), adj_c[cur][1].get(), adj_c[cur][2].get(),
-                   adj_c[1 - cur][0].get(), adj_c[1 - cur][1].get(), adj_c[1 - cur][2].get(),
-                   adj_at.get(), adj_wcp.get(), adj_wcv.get(), adj_xr.get(), adj_vr.get(), adj_sb.get(),
-                   perm, S, adj_part.get(), stride, csf ? 1 : 0, adj_cs.get(), adj_csv.get(), adj_gcb.get(),
-                   adj_wnb.get(), adj_xc.get(), P.tvf ? 1 : 0, adj_c[cur][3].get(), adj_c[1 - cur][3].get(),
-                   adj_tvs.get()};
+  bool adj_heat() const { return P.thermal != 0; }
+  bool adj_pc() const { return cfg.phase_change != 0; }
+  AdjArgs adj_args(int cur, const uint32_t* perm, const double* S, bool csf, double* lee_out = nullptr) {
+    AdjArgs A{adj_c[cur][0].get(), adj_c[cur][1].get(), adj_c[cur][2].get(),
+              adj_c[1 - cur][0].get(), adj_c[1 - cur][1].get(), adj_c[1 - cur][2].get(),
+              adj_at.get(), adj_wcp.get(), adj_wcv.get(), adj_xr.get(), adj_vr.get(), adj_sb.get(),
+              perm, S, adj_part.get(), stride, csf ? 1 : 0, adj_cs.get(), adj_csv.get(), adj_gcb.get(),
+              adj_wnb.get(), adj_xc.get(), P.tvf ? 1 : 0, adj_c[cur][3].get(), adj_c[1 - cur][3].get(),
+              adj_tvs.get()};
+    A.heat = adj_heat() ? 1 : 0;
+    A.pc = adj_pc() ? 1 : 0;
+    A.mT = (A.heat || A.pc) ? 1 : 0;
+    A.reb = A.pc && P.vapor_rebalance > 0.0f && has_vapor;
+    A.vent = A.pc && cfg.thermo_mode == CRYO_VENTED && has_vapor;
+    A.relax = cfg.vent_relaxation;
+    if (A.mT) {
+      A.cm = adj_cm[cur].get(), A.cT = adj_cT[cur].get(), A.nm = adj_cm[1 - cur].get(), A.nT = adj_cT[1 - cur].get();
+      A.mr = adj_mr.get(), A.Tr = adj_Tr.get(), A.qb = adj_qb.get(), A.hsum = adj_hsum.get();
+    }
+    if (A.pc) {
+      A.v1 = adj_v1.get(), A.T1 = adj_T1.get(), A.om = adj_om.get(), A.oh = adj_oh.get(), A.m2 = adj_m2.get();
+      A.T2 = adj_T2.get(), A.fl = adj_fl.get(), A.pa = adj_pa.get(), A.po = adj_po.get(), A.pb4 = adj_pb4.get();
+      A.pdp = adj_pdp.get(), A.v1b = adj_v1b.get(), A.pm2 = adj_pm2.get(), A.pn = adj_pn.get();
+      A.amax = adj_amax.get(), A.pV = adj_pV.get(), A.pr = adj_pr.get(), A.xp = adj_xp.get();
+      A.lee_out = lee_out;
+    }
+    return A;
+  }
+  // The reverse step's forward re-run of the kick and of phase change up to
+  // apply on the taped state t (rates evaluated): keeps the kicked v and T,
+  // the sources' flags, the normalised out_m / out_h and the applied m and T,
+  // then puts the taped v, T, m, flags and phase back.
+  void adj_pc_rerun(const Snap& t) {
+    const Persist q = persist(set);
+    d2d(a.get(), t.a.get(), n);
+    d2d(view().t.dTdt, t.dTdt.get(), n);
+    if (!adj_pc()) return;
+    launch_kick_sources(view(), P, state.get(), n, true, P.thermal != 0, true, stream);
+    d2d(adj_v1.get(), q.v, n), d2d(adj_T1.get(), q.T, n), d2d(adj_fl.get(), q.flags, n);
+    launch_pc_normalize(view(), grid(), P, lists(), n, partials.get(), stride, stream);
+    d2d(adj_om.get(), view().t.out_m, n), d2d(adj_oh.get(), view().t.out_h, n);
+    launch_pc_apply(view(), grid(), P, lists(), n, stream);
+    d2d(adj_m2.get(), q.m, n), d2d(adj_T2.get(), q.T, n);
+    d2d(q.v, t.v.get(), n), d2d(q.T, t.T.get(), n), d2d(q.m, t.m.get(), n);
+    d2d(q.flags, t.flags.get(), n), d2d(q.phase, t.phase.get(), n);
   }
 
   void check_adjoint_scope() const {
@@ -990,7 +1040,8 @@ struct cryo_solver {
     if (c.surface_tension == CRYO_ST_CSF &&
         (c.csf_smooth_normals || c.csf_balanced || c.wetting_model != CRYO_WET_NORMAL_ROTATION))
       throw std::invalid_argument("adjoint: CSF only with unsmoothed normals, the classic force and normal rotation");
-    if (c.thermal || c.phase_change) throw std::invalid_argument("adjoint: thermal / phase change not differentiated");
+    if ((c.thermal || c.phase_change) && c.thermo_mode == CRYO_CLOSED && has_vapor)
+      throw std::invalid_argument("adjoint: the closed-tank saturation feedback is not differentiated");
     if (c.transport_stress || c.sps_turbulence || c.particle_shifting)
       throw std::invalid_argument("adjoint: transport stress, SPS and shifting not differentiated");
     if (has_maneuver || frame_rot) throw std::invalid_argument("adjoint: frame rotation / manoeuvre table not allowed");
@@ -1004,21 +1055,28 @@ struct cryo_solver {
   // at a time (the rates but the forces pass re-evaluated on the taped
   // state).  The solver ends in the forward end state.
   void adjoint_slosh(long long steps, const float* acc, int every, double* loss, double* grad, double* force,
-                     double* grad_theta) {
+                     double* grad_theta, double* grad_lee) {
     check_adjoint_scope();
     if (steps < 0) throw std::invalid_argument("adjoint: steps must be >= 0");
     if (every <= 0) throw std::invalid_argument("adjoint: checkpoint_every must be > 0");
     if (!loss) throw std::invalid_argument("adjoint: loss output is required");
     if (grad_theta && !grad) throw std::invalid_argument("adjoint: the contact-angle gradient needs grad_frame_acc");
+    if (grad_lee && !grad) throw std::invalid_argument("adjoint: the Lee-coefficient gradient needs grad_frame_acc");
     const bool fresh = !initialized;  // the start-up rates (and their a) belong to this run
     ensure_initialized();
     *loss = 0.0;
     if (grad_theta) *grad_theta = 0.0;
+    if (grad_lee) grad_lee[0] = grad_lee[1] = 0.0;
     if (steps == 0) return;
+    const bool mT = adj_heat() || adj_pc();
+    sync_state();
+    const long long events0 = static_cast<long long>(hs->merges) + hs->splits;
     const bool csf = cfg.surface_tension == CRYO_ST_CSF && has_vapor;
     const std::size_t ns = static_cast<std::size_t>(steps);
     fit(adj_acc, 3 * ns), fit(adj_S, 3 * ns), fit(adj_G, 3 * ns), fit(adj_T, 2 * ns + 2);
-    fit(adj_part, 8 * static_cast<std::size_t>(stride));
+    fit(adj_part, 11 * static_cast<std::size_t>(stride));
+    if (adj_pc()) fit(adj_lee, 2 * ns);
+    tape_rates = mT;
     CRYO_HIP(hipMemcpyAsync(adj_acc.get(), acc, sizeof(float) * 3 * ns, hipMemcpyHostToDevice, stream));
     const long long K = every;
     const long long nseg = (steps + K - 1) / K;
@@ -1035,6 +1093,11 @@ struct cryo_solver {
       launch_adj_load_sum(view(), n, adj_part.get(), stride, adj_S.get() + 3 * k, stream);
     }
     check_launch();
+    if (grad && mT) {
+      sync_state();
+      if (static_cast<long long>(hs->merges) + hs->splits != events0)
+        throw std::invalid_argument("adjoint: a phase-change merge or split inside the window is not differentiable");
+    }
     if (grad) {
       if (!adj_end) adj_end = std::make_unique<Snap>();
       snap_save(*adj_end, true, nullptr);
@@ -1047,6 +1110,19 @@ struct cryo_solver {
       fit(adj_tvs, m);
       launch_adj_fill(adj_c[0][0].get(), adj_c[0][1].get(), adj_c[0][2].get(), n, stream);
       launch_adj_fill(adj_c[0][3].get(), adj_c[1][3].get(), adj_tvs.get(), n, stream);
+      if (mT) {
+        for (auto& b : adj_cm) fit(b, m);
+        for (auto& b : adj_cT) fit(b, m);
+        fit(adj_mr, m), fit(adj_Tr, m), fit(adj_qb, m), fit(adj_hsum, 1);
+        CRYO_HIP(hipMemsetAsync(adj_cm[0].get(), 0, sizeof(float) * m, stream));
+        CRYO_HIP(hipMemsetAsync(adj_cT[0].get(), 0, sizeof(float) * m, stream));
+        CRYO_HIP(hipMemsetAsync(adj_hsum.get(), 0, sizeof(double), stream));
+      }
+      if (adj_pc()) {
+        fit(adj_v1, m), fit(adj_T1, m), fit(adj_om, m), fit(adj_oh, m), fit(adj_m2, m), fit(adj_T2, m);
+        fit(adj_fl, m), fit(adj_pa, m), fit(adj_po, m), fit(adj_pb4, m), fit(adj_pdp, m), fit(adj_v1b, m);
+        fit(adj_pm2, m), fit(adj_pn, m), fit(adj_amax, m), fit(adj_pV, m), fit(adj_pr, m), fit(adj_xp, m);
+      }
       adj_tape.resize(static_cast<std::size_t>(std::min(K, steps)));
       for (auto& t : adj_tape)
         if (!t) t = std::make_unique<Snap>();
@@ -1066,7 +1142,9 @@ struct cryo_solver {
           snap_load(t, false);
           grid_build(false);  // already in cell order: the identity permutation, the same cell table
           evaluate_rates(false);
-          const AdjArgs A = adj_args(cur, t.perm.get(), adj_S.get() + 3 * k, csf);
+          if (mT) adj_pc_rerun(t);
+          const AdjArgs A = adj_args(cur, t.perm.get(), adj_S.get() + 3 * k, csf,
+                                     adj_pc() ? adj_lee.get() + 2 * k : nullptr);
           launch_adj_step(view(), grid(), P, lists(), state.get(), A, n, adj_G.get() + 3 * k, adj_T.get() + 2 * k,
                           stream);
           check_launch();
@@ -1094,7 +1172,13 @@ struct cryo_solver {
     const bool start_term = fresh && csf && grad_theta;
     if (grad && csf) CRYO_HIP(hipMemcpyAsync(T.data(), adj_T.get(), sizeof(double) * (2 * ns + (start_term ? 2 : 0)),
                                              hipMemcpyDeviceToHost, stream));
+    std::vector<double> lee(2 * ns, 0.0);
+    if (grad && adj_pc())
+      CRYO_HIP(hipMemcpyAsync(lee.data(), adj_lee.get(), sizeof(double) * 2 * ns, hipMemcpyDeviceToHost, stream));
     CRYO_HIP(hipStreamSynchronize(stream));
+    tape_rates = false;
+    if (grad_lee)
+      for (std::size_t q = 0; q < ns; ++q) grad_lee[0] += lee[2 * q], grad_lee[1] += lee[2 * q + 1];
     if (grad_theta) {  // d loss / d theta = (cos_bar (-sin theta) + sin_bar cos theta) pi / 180
       double cb = 0.0, sb = 0.0;
       for (std::size_t q = 0; q < ns + (start_term ? 1 : 0); ++q) cb += T[2 * q], sb += T[2 * q + 1];
@@ -1516,10 +1600,11 @@ int cryo_ensemble_run_steps(cryo_solver* const* solvers, int32_t count, int64_t 
   })
 }
 int cryo_solver_adjoint_slosh(cryo_solver* s, int64_t steps, const float* frame_acc, int32_t checkpoint_every,
-                              double* loss, double* grad_frame_acc, double* wall_load, double* grad_contact_angle) {
+                              double* loss, double* grad_frame_acc, double* wall_load, double* grad_contact_angle,
+                              double* grad_lee) {
   if (steps > 0 && !frame_acc) return fail("frame_acc is required", CRYO_ERR_INVALID_ARGUMENT);
   CRYO_GUARD(S(s).adjoint_slosh(steps, frame_acc, checkpoint_every, loss, grad_frame_acc, wall_load,
-                                grad_contact_angle))
+                                grad_contact_angle, grad_lee))
 }
 int cryo_solver_run_until(cryo_solver* s, double t_end) {
   CRYO_GUARD({

@@ -1812,7 +1812,8 @@ struct Solver {
     if (c.surface_tension == CRYO_ST_CSF &&
         (c.csf_smooth_normals || c.csf_balanced || c.wetting_model != CRYO_WET_NORMAL_ROTATION))
       throw std::invalid_argument("adjoint: CSF only with unsmoothed normals, the classic force and normal rotation");
-    if (c.thermal || c.phase_change) throw std::invalid_argument("adjoint: thermal / phase change not differentiated");
+    if ((c.thermal || c.phase_change) && c.thermo_mode == CRYO_CLOSED && has_vapor)
+      throw std::invalid_argument("adjoint: the closed-tank saturation feedback is not differentiated");
     if (c.transport_stress || c.sps_turbulence || c.particle_shifting)
       throw std::invalid_argument("adjoint: transport stress, SPS and shifting not differentiated");
     if (has_maneuver || P.frame_rot) throw std::invalid_argument("adjoint: frame rotation / manoeuvre table not allowed");
@@ -1929,8 +1930,280 @@ struct Solver {
     return theta_b;
   }
 
+  // Phase change's share of one step's adjoint: kick -> Lee sources -> receiver
+  // normalisation -> apply -> vapour rebalance -> venting (phase_change.cu:77-268,
+  // 337-358, sph_solver.cu:36-46,545-584), re-run from the rates state ps.  In:
+  // the cotangents of the step's end state (vb velocities, mb masses, Tb
+  // temperatures).  Out: v1b, T1b (the kicked velocity and temperature), m0b (the
+  // mass before phase change) and shares for positions (xr), V = m / rho (Vb),
+  // rho (rb) and the interface fraction (fab); lee[0], lee[1] += d loss / d the
+  // evaporation and condensation coefficients.
+  void pc_adjoint(const cryo_adj::Ctx<R>& c, const std::vector<V3>& vb, const std::vector<R>& mb,
+                  const std::vector<R>& Tb, std::vector<cryo_adj::Vec<R>>& v1b, std::vector<R>& T1b,
+                  std::vector<R>& m0b, std::vector<cryo_adj::Vec<R>>& xr, std::vector<R>& Vb, std::vector<R>& rb,
+                  std::vector<R>& fab, double* lee) {
+    using namespace cryo_adj;
+    using VR = Vec<R>;
+    const int n = ps.n;
+    const R dt = P.dt, half = R(0.5) * dt;
+    auto vr = [](V3 a) { return VR{a.x, a.y, a.z}; };
+    auto first = [n](const auto& v) { return std::vector<typename std::decay_t<decltype(v)>::value_type>(v.begin(), v.begin() + n); };
+    // forward re-run on a copy
+    Particles w = ps;
+    for (int i = 0; i < n; ++i) {
+      if (!is_fluid(w.phase[i])) continue;
+      w.v[i] += w.a[i] * half;
+      if (P.thermal) w.T[i] += w.dTdt[i] * dt;
+    }
+    const std::vector<V3> v1 = first(w.v);
+    const std::vector<R> T1 = first(w.T);
+    const bool pc = cfg.phase_change != 0;
+    std::vector<R> om_pre(n, R(0)), om(n, R(0)), oh(n, R(0)), wn(n, R(0));
+    std::vector<u8> fl(n, 0);
+    if (pc) {
+      pc_sources(w, P);
+      om_pre = first(w.out_m);
+      pc_normalize(w, grid, P);
+      om = first(w.out_m);
+      oh = first(w.out_h);
+      wn = first(w.w_norm);
+      fl = first(w.flags);
+      for (int i = 0; i < n; ++i)
+        if ((fl[i] & F_DYING) && om[i] > R(0)) throw std::invalid_argument("adjoint: a merge inside the window");
+      pc_apply(w, grid, P);
+    }
+    const std::vector<R> m2 = first(w.m), T2 = first(w.T);
+    const bool reb = pc && P.vapor_rebalance > R(0) && has_vapor;
+    if (reb) rebalance_vapor(w, grid, P);
+    const std::vector<R> m3 = first(w.m), T3 = first(w.T);
+    const std::vector<V3> v3 = first(w.v);
+    const bool vent = pc && cfg.thermo_mode == CRYO_VENTED && has_vapor;
+    auto Vof = [&](int j) { return ps.m[j] / ps.rho[j]; };
+    // venting: m4 = m3 - relax (m3 - rho0_v m3 / rho)
+    std::vector<R> m3b(mb.begin(), mb.begin() + n), T3b(Tb.begin(), Tb.begin() + n);
+    std::vector<VR> v3b(n);
+    for (int i = 0; i < n; ++i) v3b[i] = vr(vb[i]);
+    if (vent) {
+      const R relax = R(cfg.vent_relaxation), r0 = P.rho0[1];
+      for (int i = 0; i < n; ++i) {
+        if (ps.phase[i] != V) continue;
+        const R m = m3[i], rho = ps.rho[i];
+        if (!(m > r0 * (m / rho))) continue;
+        m3b[i] = mb[i] * (R(1) - relax + relax * r0 / rho);
+        rb[i] += mb[i] * (-relax * r0 * m / (rho * rho));
+      }
+    }
+    // vapour rebalance: flow_ij = beta vol0 W (m_j - m_i)
+    std::vector<R> m2b = m3b, T2b = T3b;
+    std::vector<VR> v1b2 = v3b;
+    if (reb) {
+      const R beta = P.vapor_rebalance, cpv = P.cp[1];
+      std::vector<R> dmb(n, R(0)), dHb(n, R(0));
+      std::vector<VR> dPb(n, VR{0, 0, 0});
+      std::vector<char> upd(n, 0);
+      for (int i = 0; i < n; ++i) {
+        if (ps.phase[i] != V) continue;
+        const V3 xi = ps.x[i];
+        const R mi = m2[i], hi = enthalpy(P, V, T2[i]);
+        R dm = 0, dH = 0;
+        grid.each(xi, [&](int j) {
+          if (j == i || ps.phase[j] != V) return;
+          const R r2 = norm2(xi - ps.x[j]);
+          if (r2 >= P.support2) return;
+          const R flow = beta * P.vol0 * kW(P.kernel, std::sqrt(r2)) * (m2[j] - mi);
+          dm += flow;
+          dH += flow * (flow > R(0) ? enthalpy(P, V, T2[j]) : hi);
+        });
+        if (dm == R(0)) continue;
+        upd[i] = 1;
+        const R mn = mi + dm, hh = (mi * hi + dH) / mn;
+        const R hhb = T3b[i] / cpv, Hb = hhb / mn;
+        R mnb = m3b[i] - hhb * hh / mn;
+        const VR Vb3 = v3b[i];
+        v1b2[i] = Vb3 * (mi / mn);
+        R m2bi = dot(Vb3, vr(v1[i])) / mn;
+        dPb[i] = Vb3 * (R(1) / mn);
+        mnb -= dot(Vb3, vr(v3[i])) / mn;
+        m2bi += mnb + Hb * hi;
+        dmb[i] = mnb;
+        dHb[i] = Hb;
+        T2b[i] = Hb * mi * cpv;
+        m2b[i] = m2bi;
+      }
+      for (int i = 0; i < n; ++i) {
+        if (!upd[i]) continue;
+        const V3 xi = ps.x[i];
+        grid.each(xi, [&](int j) {
+          if (j == i || ps.phase[j] != V) return;
+          const V3 rij = xi - ps.x[j];
+          const R r2 = norm2(rij);
+          if (r2 >= P.support2) return;
+          const R r = std::sqrt(r2), W = kW(P.kernel, r);
+          const R flow = beta * P.vol0 * W * (m2[j] - m2[i]);
+          const bool from_j = flow > R(0);
+          const int src = from_j ? j : i;
+          const R fb = dmb[i] + dHb[i] * enthalpy(P, V, T2[src]) + dot(dPb[i], vr(v1[src]));
+          T2b[src] += dHb[i] * flow * cpv;
+          v1b2[src] += dPb[i] * flow;
+          const R k = fb * beta * P.vol0;
+          m2b[j] += k * W;
+          m2b[i] -= k * W;
+          const VR d = vr(rij) * (k * (m2[j] - m2[i]) * cryo_adj::wendland_s(c, r));
+          xr[i] += d;
+          xr[j] -= d;
+        });
+      }
+    }
+    for (int i = 0; i < n; ++i) v1b[i] = v1b2[i];
+    // apply: m2 = m - om_i + sum_j om_j V_i W / wn_j, H likewise, T2 = T(H / m2)
+    std::vector<R> omb(n, R(0)), ohb(n, R(0)), wnb(n, R(0)), inmb(n, R(0)), inHb(n, R(0));
+    for (int i = 0; i < n; ++i) {
+      const u8 ph = ps.phase[i];
+      if (!is_fluid(ph)) continue;
+      m0b[i] = m2b[i];
+      T1b[i] = T2b[i];
+      if (!pc) continue;
+      const V3 xi = ps.x[i];
+      const R mi = ps.m[i], Vi = Vof(i);
+      R in_m = 0, in_H = 0;
+      if (!(fl[i] & F_DYING)) grid.each(xi, [&](int j) {
+        if (j == i || !(om[j] > R(0))) return;
+        if (((fl[j] & F_TARGET_VAPOR) ? V : L) != ph) return;
+        const R r2 = norm2(xi - ps.x[j]);
+        if (r2 >= P.support2) return;
+        const R share = om[j] * Vi * kW(P.kernel, std::sqrt(r2)) / wn[j];
+        in_m += share;
+        in_H += share * oh[j];
+      });
+      if (!(om[i] > R(0)) && !(in_m > R(0))) continue;
+      const R mn = mi - om[i] + in_m, hT = enthalpy(P, ph, T1[i]);
+      const R hh = (mi * hT - om[i] * oh[i] + in_H) / mn;
+      const R cpp = P.cp[pidx(ph)];
+      const R hhb = T2b[i] / cpp, Hb = hhb / mn;
+      const R mnb = m2b[i] - hhb * hh / mn;
+      m0b[i] = mnb + Hb * hT;
+      T1b[i] = Hb * mi * cpp;
+      omb[i] += -mnb - Hb * oh[i];
+      ohb[i] += -Hb * om[i];
+      inmb[i] = mnb;
+      inHb[i] = Hb;
+    }
+    if (pc) {
+      for (int i = 0; i < n; ++i) {
+        if (inmb[i] == R(0) && inHb[i] == R(0)) continue;
+        const u8 ph = ps.phase[i];
+        const V3 xi = ps.x[i];
+        const R Vi = Vof(i);
+        grid.each(xi, [&](int j) {
+          if (j == i || !(om[j] > R(0))) return;
+          if (((fl[j] & F_TARGET_VAPOR) ? V : L) != ph) return;
+          const V3 rij = xi - ps.x[j];
+          const R r2 = norm2(rij);
+          if (r2 >= P.support2) return;
+          const R r = std::sqrt(r2), W = kW(P.kernel, r);
+          const R share = om[j] * Vi * W / wn[j];
+          const R shb = inmb[i] + inHb[i] * oh[j];
+          ohb[j] += inHb[i] * share;
+          omb[j] += shb * Vi * W / wn[j];
+          Vb[i] += shb * om[j] * W / wn[j];
+          wnb[j] -= shb * share / wn[j];
+          const VR d = vr(rij) * (shb * om[j] * Vi / wn[j] * cryo_adj::wendland_s(c, r));
+          xr[i] += d;
+          xr[j] -= d;
+        });
+      }
+      // normalisation (wn_i = sum V_j W over the target phase, vapour density cap) and the Lee sources
+      const R Ts = P.T_sat;
+      for (int i = 0; i < n; ++i) {
+        const u8 ph = ps.phase[i];
+        if (!is_fluid(ph) || !(fl[i] & F_SOURCE)) continue;
+        const u8 target = (fl[i] & F_TARGET_VAPOR) ? V : L;
+        const V3 xi = ps.x[i];
+        R wsum = 0, wr_max = 0;
+        int jmax = -1;
+        grid.each(xi, [&](int j) {
+          if (j == i || ps.phase[j] != target || (fl[j] & F_DYING)) return;
+          const R r2 = norm2(xi - ps.x[j]);
+          if (r2 >= P.support2) return;
+          const R W = kW(P.kernel, std::sqrt(r2));
+          wsum += Vof(j) * W;
+          const R wr = W / ps.rho[j];
+          if (wr > wr_max) {
+            wr_max = wr;
+            jmax = j;
+          }
+        });
+        if (!(wsum > R(0))) continue;  // om = 0
+        R ompb = omb[i];
+        if (target == V && P.pc_density_limit > R(0) && wr_max > R(0)) {
+          const R cap = P.pc_density_limit * wsum / wr_max;
+          if (om_pre[i] > cap) {
+            wnb[i] += omb[i] * P.pc_density_limit / wr_max;
+            const R wrb = -omb[i] * cap / wr_max;
+            ompb = 0;
+            const V3 rij = xi - ps.x[jmax];
+            const R r = std::sqrt(norm2(rij));
+            rb[jmax] -= wrb * wr_max / ps.rho[jmax];
+            const VR d = vr(rij) * (wrb / ps.rho[jmax] * cryo_adj::wendland_s(c, r));
+            xr[i] += d;
+            xr[jmax] -= d;
+          }
+        }
+        if (wnb[i] != R(0)) {
+          grid.each(xi, [&](int j) {
+            if (j == i || ps.phase[j] != target || (fl[j] & F_DYING)) return;
+            const V3 rij = xi - ps.x[j];
+            const R r2 = norm2(rij);
+            if (r2 >= P.support2) return;
+            const R r = std::sqrt(r2);
+            Vb[j] += wnb[i] * kW(P.kernel, r);
+            const VR d = vr(rij) * (wnb[i] * Vof(j) * cryo_adj::wendland_s(c, r));
+            xr[i] += d;
+            xr[j] -= d;
+          });
+        }
+        // Lee source: dm = min(rate dt, [energy cap], cap m), rate = lee m alpha |T - Ts| / Ts
+        const R m = ps.m[i], T = T1[i], alpha = ps.alpha[i];
+        if (!(alpha > R(0)) || !(dt > R(0))) continue;
+        if (ph == L && T > Ts) {
+          const R h_out = P.h_fg + P.cp[1] * (Ts - P.T_ref);
+          const R latent = h_out - P.cp[0] * (Ts - P.T_ref);
+          const R dm_r = P.lee_evap * m * alpha * (T - Ts) / Ts * dt;
+          const R dm_e = m * P.cp[0] * (T - Ts) / latent;
+          const R dm_c = P.max_transfer * m;
+          const R d1 = dm_r < dm_e ? dm_r : dm_e;
+          if (!(d1 < dm_c)) {
+            m0b[i] += ompb * P.max_transfer;
+          } else if (dm_r < dm_e) {
+            m0b[i] += ompb * dm_r / m;
+            fab[i] += ompb * dm_r / alpha;
+            T1b[i] += ompb * P.lee_evap * m * alpha / Ts * dt;
+            lee[0] += double(ompb * m * alpha * (T - Ts) / Ts * dt);
+          } else {
+            m0b[i] += ompb * P.cp[0] * (T - Ts) / latent;
+            T1b[i] += ompb * m * P.cp[0] / latent;
+          }
+        } else if (ph == V && T < Ts) {
+          const R dm_r = P.lee_cond * m * alpha * (Ts - T) / Ts * dt;
+          R capf = P.max_transfer;
+          if (P.pc_density_limit > R(0) && P.pc_density_limit * m < capf * m) capf = P.pc_density_limit;
+          T1b[i] += ohb[i] * P.cp[1];  // out_h = h(T) of the condensing vapour
+          if (!(dm_r < capf * m)) {
+            m0b[i] += ompb * capf;
+          } else {
+            m0b[i] += ompb * dm_r / m;
+            fab[i] += ompb * dm_r / alpha;
+            T1b[i] -= ompb * P.lee_cond * m * alpha / Ts * dt;
+            lee[1] += double(ompb * m * alpha * (Ts - T) / Ts * dt);
+          }
+        }
+      }
+    }
+  }
+
   V3 adjoint_rates(const std::vector<u32>& perm, V3 Ssum, std::vector<V3>& xb, std::vector<V3>& vb,
-                   std::vector<V3>& ab, std::vector<V3>& tb, double& theta_b) {
+                   std::vector<V3>& ab, std::vector<V3>& tb, std::vector<R>& mb, std::vector<R>& Tb, double& theta_b,
+                   double* lee) {
     using namespace cryo_adj;
     using VR = Vec<R>;
     const int n = ps.n;
@@ -1948,14 +2221,22 @@ struct Solver {
     };
     std::vector<VR> At(n, VR{0, 0, 0}), xr(n, VR{0, 0, 0}), vrb(n, VR{0, 0, 0}), vwb(n, VR{0, 0, 0});
     std::vector<R> rb(n, R(0)), pb(n, R(0)), pwb0(n, R(0)), pwb1(n, R(0));
+    std::vector<R> Vb(n, R(0)), alphab(n, R(0));
+    // phase change, then the kick: v1 = v_half + a dt/2, T1 = T + dTdt dt
+    std::vector<VR> v1b(n, VR{0, 0, 0});
+    std::vector<R> T1b(n, R(0)), mr(n, R(0)), Tr(n, R(0)), fab(n, R(0));
+    pc_adjoint(c, vb, mb, Tb, v1b, T1b, mr, xr, Vb, rb, fab, lee);
     VR gb{0, 0, 0};
     for (int i = 0; i < n; ++i) {
       if (!is_fluid(ps.phase[i])) continue;
-      At[i] = vr(ab[i]) + vr(vb[i]) * half;  // v' = v_half + a' dt/2
-      gb += At[i];                           // a = g_eff + ...
+      At[i] = vr(ab[i]) + v1b[i] * half;
+      gb += At[i];  // a = g_eff + ...
+      Tr[i] += T1b[i];
     }
+    auto c_to_T = [&](int i, R cb) {  // c = sqrt(R T) for vapour
+      if (ps.phase[i] == V && ps.T[i] > R(0) && cb != R(0)) Tr[i] += cb * P.gas_R / (R(2) * sound_speed(P, V, ps.T[i]));
+    };
     // surface tension
-    std::vector<R> Vb(n, R(0)), alphab(n, R(0));
     if (cfg.surface_tension == CRYO_ST_CSF && has_vapor) theta_b += csf_adjoint(c, At, xr, Vb, alphab);
     // transport acceleration ab = K sum_J gw (Vt^2 + V_J^2), K = -pb / (rho Vt)
     // (SPH_kernels.cu:56-104), read by the next step's drift: tb is its cotangent
@@ -1997,6 +2278,7 @@ struct Solver {
           vt2b += o.vt2;
           if (wall) {
             const R rwb = tvf_wall_vjp(Vt, rhoi, rho_w, o.vj, vtb, rhob);
+            if (ph == V) Tr[i] -= rwb * rho_w / ps.T[i];
             const R dp = tvf_wall_drho_dp(ph == V ? 1 : 0, ph == V ? ps.p_wv[j] : ps.p[j], rho_w, P.p_bg, P.tait_B,
                                           P.tait_gamma, P.gas_R, ps.T[i]);
             (ph == V ? pwb1 : pwb0)[j] += rwb * dp;
@@ -2006,6 +2288,7 @@ struct Solver {
         });
         const R Kb = dot(Ab, sum);
         rhob -= K * Kb / rhoi;
+        if (ph == V) Tr[i] += K * Kb / ps.T[i];  // c^2 = R T
         vtb -= K * Kb / Vt;
         vtb += R(2) * Vt * vt2b;
         tsb[i] -= vtb * Vt * Vt;  // Vt = 1 / sigma
@@ -2018,10 +2301,14 @@ struct Solver {
       const Side<R> I = side(i);
       const VR cF = At[i] * (R(1) / I.m), cA = At[i];
       const VR cFw = cF + Fb, cAw = cA + Fb * I.m;
+      VR F{0, 0, 0};
       grid.each(ps.x[i], [&](int j) {
         if (j == i) return;
         if (ps.phase[j] == W) {
           const FWGrad<R> o = fw_vjp(c, I, vr(ps.x[j]), vr(ps.vw[j]), I.ph == V ? ps.p_wv[j] : ps.p[j], cFw, cAw);
+          F += o.f;
+          mr[i] += o.m_i + dot(Fb, o.acc);  // the wall load carries m_I acc
+          c_to_T(i, o.c_i);
           xr[i] += o.r;
           vrb[i] += o.vi + o.vij;
           vwb[j] -= o.vij;
@@ -2032,6 +2319,11 @@ struct Solver {
         }
         if (!is_fluid(ps.phase[j])) return;
         const FFGrad<R> o = ff_vjp(c, I, side(j), cF, cA);
+        F += o.f;
+        mr[i] += o.m_i;
+        mr[j] += o.m_j;
+        c_to_T(i, o.c_i);
+        c_to_T(j, o.c_j);
         xr[i] += o.r;
         xr[j] -= o.r;
         vrb[i] += o.vij;
@@ -2040,6 +2332,92 @@ struct Solver {
         rb[j] += o.rho_j;
         pb[i] += o.p_i;
         pb[j] += o.p_j;
+      });
+      mr[i] -= dot(At[i], F) / (I.m * I.m);  // a = ... + F / m
+    }
+    // heat: dTdt = (sum_J q_IJ + P hw_I / sum_K hw_K) / (m cp)
+    if (P.thermal) {
+      const bool iso = P.wall_bc == 1, heat = P.heat_power != R(0) && P.heat_inv_norm > R(0);
+      std::vector<R> hwb(n, R(0));
+      R Sb = 0;
+      for (int i = 0; i < n; ++i) {
+        const u8 ph = ps.phase[i];
+        if (!is_fluid(ph)) continue;
+        const R D = T1b[i] * dt;
+        if (D == R(0)) continue;
+        const R mi = ps.m[i], cpi = P.cp[pidx(ph)], qb = D / (mi * cpi);
+        mr[i] -= D * ps.dTdt[i] / mi;
+        const R Vi = mi / ps.rho[i], ki = P.k[pidx(ph)];
+        grid.each(ps.x[i], [&](int j) {
+          if (j == i) return;
+          const u8 phj = ps.phase[j];
+          R Vj, kij;
+          if (phj == W) {
+            if (!iso) return;
+            Vj = P.vol0;
+            kij = R(2) * ki;
+          } else {
+            if (!is_fluid(phj)) return;
+            const R kj = P.k[pidx(phj)];
+            Vj = ps.m[j] / ps.rho[j];
+            kij = R(4) * ki * kj / (ki + kj);
+          }
+          const ThGrad<R> o = th_vjp(c, vr(ps.x[i] - ps.x[j]), Vi, Vj, kij, ps.T[i], ps.T[j], qb);
+          xr[i] += o.r;
+          Vb[i] += o.Vi;
+          Tr[i] += o.Ti;
+          if (phj != W) {
+            xr[j] -= o.r;
+            Vb[j] += o.Vj;
+            Tr[j] += o.Tj;
+          }
+        });
+        if (heat) {
+          hwb[i] += qb * P.heat_power * P.heat_inv_norm;
+          Sb -= qb * P.heat_power * ps.heat_w[i] * P.heat_inv_norm * P.heat_inv_norm;
+        }
+      }
+      if (heat) {  // hw_I = sum over heated walls of vol0 W
+        for (int i = 0; i < n; ++i) {
+          if (!is_fluid(ps.phase[i])) continue;
+          const R hb = hwb[i] + Sb;
+          grid.each(ps.x[i], [&](int j) {
+            if (ps.phase[j] != W || !(ps.flags[j] & F_HEATED)) return;
+            const V3 rij = ps.x[i] - ps.x[j];
+            const R r2 = norm2(rij);
+            if (r2 >= P.support2) return;
+            xr[i] += vr(rij) * (hb * P.vol0 * wendland_s(c, std::sqrt(r2)));
+          });
+        }
+      }
+    }
+    // interface fraction alpha = s_other / s_all (the Lee rate reads it)
+    for (int i = 0; i < n; ++i) {
+      const u8 ph = ps.phase[i];
+      if (!is_fluid(ph) || fab[i] == R(0)) continue;
+      R sa = 0, so = 0;
+      grid.each(ps.x[i], [&](int j) {
+        if (!is_fluid(ps.phase[j])) return;
+        const R r2 = norm2(ps.x[i] - ps.x[j]);
+        if (r2 >= P.support2) return;
+        const R Wv = ps.m[j] / ps.rho[j] * kW(P.kernel, std::sqrt(r2));
+        sa += Wv;
+        if (ps.phase[j] != ph) so += Wv;
+      });
+      if (!(sa > R(0))) continue;
+      const R sob = fab[i] / sa, sab = -fab[i] * (so / sa) / sa;
+      grid.each(ps.x[i], [&](int j) {
+        if (!is_fluid(ps.phase[j])) return;
+        const V3 rij = ps.x[i] - ps.x[j];
+        const R r2 = norm2(rij);
+        if (r2 >= P.support2) return;
+        const R r = std::sqrt(r2);
+        const R k = sab + (ps.phase[j] != ph ? sob : R(0));
+        Vb[j] += k * kW(P.kernel, r);
+        if (j == i) return;
+        const VR d = vr(rij) * (k * (ps.m[j] / ps.rho[j]) * wendland_s(c, r));
+        xr[i] += d;
+        xr[j] -= d;
       });
     }
     // wall boundary values
@@ -2092,9 +2470,13 @@ struct Solver {
       const u8 ph = ps.phase[i];
       if (!is_fluid(ph)) continue;
       rb[i] -= Vb[i] * (ps.m[i] / ps.rho[i]) / ps.rho[i];
+      mr[i] += Vb[i] / ps.rho[i];
       const R dpdr = eos_dp_drho(ph == V ? 1 : 0, ps.rho[i], ps.T[i], ps.p[i], P.rho0[0], P.tait_B, P.tait_gamma,
                                  P.gas_R, P.clamp_neg);
-      sb[i] = (rb[i] + pb[i] * dpdr) * (ph == L ? ps.m0[i] : ps.m[i]) + tsb[i];
+      Tr[i] += pb[i] * eos_dp_dT(ph == V ? 1 : 0, ps.rho[i], ps.p[i], P.gas_R, P.clamp_neg);
+      const R rhob = rb[i] + pb[i] * dpdr;
+      if (ph == V) mr[i] += rhob * ps.sigma[i];  // rho = m sigma
+      sb[i] = rhob * (ph == L ? ps.m0[i] : ps.m[i]) + tsb[i];
     }
     for (int i = 0; i < n; ++i) {
       if (!is_fluid(ps.phase[i])) continue;
@@ -2110,10 +2492,13 @@ struct Solver {
     }
     // drift and kick, then back through the cell sort: old[perm[i]] <- new[i]
     std::vector<V3> nx(n, V3{0, 0, 0}), nv(n, V3{0, 0, 0}), na(n, V3{0, 0, 0}), nt(n, V3{0, 0, 0});
+    std::vector<R> nm(n, R(0)), nT(n, R(0));
     for (int i = 0; i < n; ++i) {
       if (!is_fluid(ps.phase[i])) continue;
+      nm[perm[i]] = mr[i];
+      nT[perm[i]] = Tr[i];
       const VR X = vr(xb[i]) + xr[i];
-      const VR Vh = vr(vb[i]) + vrb[i] + X * dt;
+      const VR Vh = v1b[i] + vrb[i] + X * dt;
       nx[perm[i]] = v3(X);
       nv[perm[i]] = v3(Vh);
       na[perm[i]] = v3(Vh * half);
@@ -2123,10 +2508,13 @@ struct Solver {
     vb.swap(nv);
     ab.swap(na);
     tb.swap(nt);
+    mb.swap(nm);
+    Tb.swap(nT);
     return v3(gb);
   }
 
-  void adjoint_slosh(long long steps, const float* acc, double* loss, double* grad, double* force, double* grad_theta) {
+  void adjoint_slosh(long long steps, const float* acc, double* loss, double* grad, double* force, double* grad_theta,
+                     double* grad_lee) {
     check_adjoint_scope();
     if (steps < 0) throw std::invalid_argument("adjoint: steps must be >= 0");
     const bool fresh = !initialized;  // the start-up rates (and their a) belong to this run
@@ -2139,6 +2527,7 @@ struct Solver {
     std::vector<TapeEntry> tape(static_cast<std::size_t>(steps));
     std::vector<V3> S(static_cast<std::size_t>(steps));
     double L = 0;
+    const int events0 = merges + splits;
     for (long long k = 0; k < steps; ++k) {
       apply_frame(acc_of(k), zero, zero, origin);
       record = grad ? &tape[static_cast<std::size_t>(k)] : nullptr;
@@ -2161,10 +2550,13 @@ struct Solver {
     }
     *loss = L;
     if (!grad) return;
+    if (merges + splits != events0)
+      throw std::invalid_argument("adjoint: a phase-change merge or split inside the window is not differentiable");
     // reverse sweep over the taped states; the solver ends in the forward end state
     Particles end_state = ps;
     std::vector<V3> xb(ps.n, zero), vb(ps.n, zero), ab(ps.n, zero), tb(ps.n, zero);
-    double theta_b = 0;
+    std::vector<R> mb(ps.n, R(0)), Tb(ps.n, R(0));
+    double theta_b = 0, lee[2] = {0, 0};
     for (long long k = steps - 1; k >= 0; --k) {
       TapeEntry& t = tape[static_cast<std::size_t>(k)];
       ps = t.ps;
@@ -2172,7 +2564,7 @@ struct Solver {
       apply_frame(acc_of(k), zero, zero, origin);
       P.dt = R(cfg.fixed_dt);
       evaluate_rates();
-      const V3 gb = adjoint_rates(t.perm, S[static_cast<std::size_t>(k)], xb, vb, ab, tb, theta_b);
+      const V3 gb = adjoint_rates(t.perm, S[static_cast<std::size_t>(k)], xb, vb, ab, tb, mb, Tb, theta_b, lee);
       grad[3 * k] = -double(gb.x);
       grad[3 * k + 1] = -double(gb.y);
       grad[3 * k + 2] = -double(gb.z);
@@ -2193,6 +2585,10 @@ struct Solver {
       theta_b += csf_adjoint(c, At, xr, Vb, alphab);
     }
     if (grad_theta) *grad_theta = theta_b;
+    if (grad_lee) {
+      grad_lee[0] = lee[0];
+      grad_lee[1] = lee[1];
+    }
     ps = end_state;        // scratch included: a is the end state's
     grid_build(grid, ps);  // sorted already: the identity permutation
     if (steps > 0) apply_frame(acc_of(steps - 1), zero, zero, origin);
@@ -2274,9 +2670,9 @@ int orc_run_steps(void* s, long long n) {
   GUARD(for (long long k = 0; k < n; ++k) static_cast<Solver*>(s)->step())
 }
 int orc_adjoint_slosh(void* s, long long steps, const float* acc, int checkpoint_every, double* loss, double* grad,
-                      double* force, double* grad_contact_angle) {
+                      double* force, double* grad_contact_angle, double* grad_lee) {
   (void)checkpoint_every;  // the oracle tapes every step
-  GUARD(static_cast<Solver*>(s)->adjoint_slosh(steps, acc, loss, grad, force, grad_contact_angle))
+  GUARD(static_cast<Solver*>(s)->adjoint_slosh(steps, acc, loss, grad, force, grad_contact_angle, grad_lee))
 }
 int orc_run_until(void* s, double t) {
   GUARD(Solver& S = *static_cast<Solver*>(s); while (S.time < t) S.step())

@@ -229,6 +229,12 @@ def _check(lib, prefix, rc):
         raise exc(f"{prefix} rc={rc}: {msg}")
 
 
+class AdjointResult(tuple):
+    """(loss, d loss / d frame_acc, wall loads, d loss / d contact angle); .lee holds
+    (d loss / d lee_evaporation, d loss / d lee_condensation) when the gradient was asked for."""
+    lee = None
+
+
 def _adjoint_slosh(call, handle, acc, checkpoint_every, grad):
     acc = np.ascontiguousarray(np.asarray(acc, np.float32).reshape(-1, 3))
     steps = len(acc)
@@ -236,10 +242,13 @@ def _adjoint_slosh(call, handle, acc, checkpoint_every, grad):
     g = np.zeros((steps, 3), np.float64) if grad else None
     f = np.zeros((steps, 3), np.float64)
     th = C.c_double(0.0)
+    lee = np.zeros(2, np.float64)
     dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
     call("adjoint_slosh", handle, C.c_int64(steps), _f32p(acc), C.c_int32(checkpoint_every), C.byref(loss), dp(g),
-         dp(f), C.byref(th) if grad else None)
-    return loss.value, g, f, (th.value if grad else None)
+         dp(f), C.byref(th) if grad else None, dp(lee) if grad else None)
+    out = AdjointResult((loss.value, g, f, (th.value if grad else None)))
+    out.lee = (float(lee[0]), float(lee[1])) if grad else None
+    return out
 
 
 class CpuSolver:

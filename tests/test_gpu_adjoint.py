@@ -114,6 +114,39 @@ def test_transport_velocity_gradients_match_fp64_adjoint():
     assert np.array_equal(out1[1], g) and out1[3] == dth
 
 
+def test_heat_and_phase_change_gradients_match_cpu_adjoints():
+    """Conduction, heater, Lee evaporation and condensation, normalisation,
+    apply, vapour rebalance and venting on: the acceleration gradient and the
+    two Lee-coefficient derivatives, with and without CSF, against the FP32 CPU
+    adjoint (same arithmetic, serial sums) and the FP64 one (pinned by central
+    differences).  The evaporation derivative of this case is a small net of
+    larger opposite shares (rebalance against the vapour's own uptake), and the
+    FP32 forward's temperatures, under the scaled-up conduction and heater,
+    move it by 1e-3 (2.5e-3 with CSF) from FP64: the FP32 CPU adjoint shows
+    the same gap, and the GPU agrees with it to 2e-5.  The checkpoint interval
+    leaves the results bitwise unchanged."""
+    say("test_heat_and_phase_change_gradients_match_cpu_adjoints")
+    for csf in (False, True):
+        fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=12, lee=50.0, csf=csf)
+        out = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=5)
+        out64 = orc(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
+        o32 = CpuSolver("orc", cfg, fluid, 32)
+        o32.set_particles(ps, lo, hi)
+        out32 = o32.adjoint_slosh(acc)
+        err = np.abs(out[1] - out64[1]).max() / np.abs(out64[1]).max()
+        lee64 = [abs(out.lee[q] / out64.lee[q] - 1) for q in (0, 1)]
+        lee32 = [abs(out.lee[q] / out32.lee[q] - 1) for q in (0, 1)]
+        say("csf", csf, "loss", out[0], "fp64", out64[0], "acc gradient error", err, "lee", out.lee, "fp32",
+            out32.lee, "fp64", out64.lee, "rel fp32", lee32, "rel fp64", lee64)
+        assert abs(out[0] - out64[0]) <= 1e-4 * out64[0]
+        assert err <= 1e-3
+        assert max(lee32) <= 2e-4 and max(lee64) <= 5e-3
+        if csf:
+            assert abs(out[3] - out64[3]) <= 1e-3 * abs(out64[3])
+        again = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=12)
+        assert np.array_equal(again[1], out[1]) and again.lee == out.lee
+
+
 def test_checkpoint_interval_does_not_change_the_gradient():
     say("test_checkpoint_interval_does_not_change_the_gradient")
     fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=10)
@@ -126,7 +159,7 @@ def test_checkpoint_interval_does_not_change_the_gradient():
 
 def test_adjoint_rejects_undifferentiated_physics_and_bad_arguments():
     say("test_adjoint_rejects_undifferentiated_physics_and_bad_arguments")
-    for csf, field, value in ((False, "surface_tension", 2), (False, "phase_change", 1), (False, "fixed_dt", 0.0),
+    for csf, field, value in ((False, "surface_tension", 2), (False, "sps_turbulence", 1), (False, "fixed_dt", 0.0),
                               (False, "particle_shifting", 1), (True, "csf_smooth_normals", 1),
                               (True, "csf_balanced", 1)):
         fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=2, csf=csf)
@@ -136,6 +169,13 @@ def test_adjoint_rejects_undifferentiated_physics_and_bad_arguments():
     fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=2)
     with pytest.raises(ValueError):
         hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=0)
+    fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=2, lee=50.0, vented=False)
+    with pytest.raises(ValueError, match="closed"):
+        hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
+    fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=3, lee=50.0)
+    cfg.split_ratio = 0.5  # every particle splits in the first step
+    with pytest.raises(ValueError, match="merge or split"):
+        hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
 
 
 def test_autograd_function_over_the_module():
@@ -173,11 +213,44 @@ def test_autograd_function_over_the_module():
     np.add.at(gseg, step_segments(steps, nseg), g2)
     assert loss.item() == l2
     assert np.array_equal(seg.grad.numpy(), gseg)
+    # with heat and phase change, the Lee coefficient as a second input
+    fluid, cfg, ps, lo, hi, _ = adjoint_case(steps=2, lee=50.0)
+
+    base = {f: getattr(cfg, f) for f in ("velocity_scale", "fixed_dt", "artificial_viscosity",
+                                         "artificial_viscosity_vapor", "conductivity_scale", "heater_power",
+                                         "vapor_mass_rebalance")}
+
+    def make_lee_solver(lee):
+        c = cs.SolverConfig()
+        c.dx = cfg.dx
+        c.gravity = cfg.gravity.tup()
+        c.transport_velocity = False
+        c.surface_tension = cs.SurfaceTensionModel.NoTension
+        c.thermal = c.phase_change = True
+        c.thermo_mode = cs.TankThermoMode.Vented
+        for f, v in base.items():
+            setattr(c, f, v)
+        c.lee_evaporation = c.lee_condensation = lee
+        s = cs.Solver(c, cs.FluidProperties.liquid_nitrogen())
+        s.set_particles(ps.x.astype(np.float64), ps.m, ps.rho, ps.T, ps.phase, domain_lo=tuple(map(float, lo)),
+                        domain_hi=tuple(map(float, hi)), velocity=ps.v.astype(np.float64), flags=ps.flags)
+        return s
+
+    seg2 = seg.detach().clone().requires_grad_(True)
+    lee = torch.tensor(50.0, dtype=torch.float64, requires_grad=True)
+    loss = SloshLoss.apply(seg2, make_lee_solver, steps, 5, lee)
+    loss.backward()
+    out = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=5)
+    np.add.at(gseg := np.zeros((nseg, 3)), step_segments(steps, nseg), out[1])
+    say("autograd with lee: loss", loss.item(), "c abi", out[0], "dlee", lee.grad.item(), sum(out.lee))
+    assert loss.item() == out[0]
+    assert np.array_equal(seg2.grad.numpy(), gseg)
+    assert lee.grad.item() == out.lee[0] + out.lee[1]
 
 
 def test_c5_tank_gradient_matches_cpu_adjoints():
     """The differentiable-slosh workload at full size (LN2 cylinder, 209k
-    particles), a short horizon.  The FP32 GPU forward is bitwise the FP32
+    particles, CSF, conduction and Lee phase change), a short horizon.  The FP32 GPU forward is bitwise the FP32
     oracle's, so the GPU gradient is held to the FP32 CPU adjoint on the same
     trajectory; against FP64 the FP32 forward itself moves the wall loads by
     about 1e-3 here (tiny lateral loads out of large, nearly cancelling wall
@@ -191,12 +264,14 @@ def test_c5_tank_gradient_matches_cpu_adjoints():
 
     fluid, cfg, ps, lo, hi, acc = bench.c5_case(cabi, steps=6, segments=3)
     say("N =", len(ps), "dt =", cfg.fixed_dt)
-    loss, g, load, _ = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=2)
+    out = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=2)
+    loss, g, load, out_lee = out[0], out[1], out[2], out.lee
     loss_b, g_b, _, _ = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=6)
     assert loss_b == loss and np.array_equal(g_b, g)
     o32 = CpuSolver("orc", cfg, fluid, 32)
     o32.set_particles(ps, lo, hi)
-    loss32, g32, load32, _ = o32.adjoint_slosh(acc)
+    out32 = o32.adjoint_slosh(acc)
+    loss32, g32, load32, lee_32 = out32[0], out32[1], out32[2], out32.lee
     assert np.allclose(load, load32, rtol=1e-12, atol=0)
     err32 = np.abs(g - g32).max() / np.abs(g32).max()
     loss64, g64, _, _ = orc(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
@@ -206,5 +281,8 @@ def test_c5_tank_gradient_matches_cpu_adjoints():
     cfg.compensated_sums = 1
     loss_c = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, grad=False)[0]
     say("compensated forward: loss rel error vs fp64", abs(loss_c - loss64) / loss64)
+    lee32 = abs(out_lee[0] / lee_32[0] - 1)
+    say("d loss / d lee_evaporation", out_lee, "fp32", lee_32, "rel", lee32)
+    assert lee32 <= 1e-3
     assert err32 <= 1e-3  # the two FP32 adjoints differ only in their sums' rounding (measured 1.7e-4)
     assert abs(loss - loss64) <= 1e-2 * loss64 and err64 <= 1e-2  # measured 7e-4 and 3.4e-3

@@ -59,7 +59,7 @@ def test_loss_is_the_integrated_wall_load():
 
 
 def test_adjoint_rejects_undifferentiated_physics():
-    for csf, field, value in ((False, "surface_tension", 2), (False, "thermal", 1), (False, "fixed_dt", 0.0),
+    for csf, field, value in ((False, "surface_tension", 2), (False, "sps_turbulence", 1), (False, "fixed_dt", 0.0),
                               (False, "transport_stress", 1), (True, "csf_balanced", 1), (True, "wetting_model", 1),
                               (True, "csf_smooth_normals", 1)):
         fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=2, csf=csf)
@@ -112,3 +112,50 @@ def test_fp64_adjoint_with_transport_velocity_matches_central_differences():
             fd[k, c] = (oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(a1, grad=False)[0] -
                         oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(a2, grad=False)[0]) / h
     assert np.abs(grad - fd).max() / np.abs(fd).max() <= 1e-6
+
+
+def test_fp64_adjoint_with_heat_and_phase_change_matches_central_differences():
+    """Conduction (scaled up), the heater, Lee evaporation and condensation,
+    receiver normalisation with the vapour density cap, apply, vapour
+    rebalance and venting on: masses and temperatures are then part of the
+    state.  Both the acceleration gradient and the two Lee-coefficient
+    derivatives are checked.  Dropping one derivative term moves the
+    acceleration gradient by 3e-9 to 2e-5 of its largest component (the
+    force-over-mass term, the sound speed's temperature, conduction pairs,
+    the interface fraction, rebalance pairs, the vapour EOS temperature,
+    rho = m sigma of vapour, apply and normalisation volumes, the heater
+    weights and their normalisation) or a Lee derivative by 9e-3 or more
+    (venting)."""
+    fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=12, lee=50.0)
+    out = oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
+    grad = out[1]
+    fd = np.zeros_like(grad)
+    for k in range(acc.shape[0]):
+        for c in range(3):
+            a1, a2 = acc.copy(), acc.copy()
+            a1[k, c] += STEP
+            a2[k, c] -= STEP
+            h = float(a1[k, c]) - float(a2[k, c])
+            fd[k, c] = (oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(a1, grad=False)[0] -
+                        oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(a2, grad=False)[0]) / h
+    assert np.abs(grad - fd).max() / np.abs(fd).max() <= 2e-9
+    for which, name in enumerate(("lee_evaporation", "lee_condensation")):
+        vals = []
+        for sgn in (1, -1):
+            c2 = adjoint_case(steps=12, lee=50.0)[1]
+            setattr(c2, name, 50.0 * (1 + sgn * 1e-3))
+            vals.append(oracle(c2, fluid, ps, lo, hi).adjoint_slosh(acc, grad=False)[0])
+        fd_lee = (vals[0] - vals[1]) / (2 * 50.0 * 1e-3)
+        assert out.lee[which] != 0 and abs(out.lee[which] / fd_lee - 1) <= 2e-4
+
+
+def test_phase_change_adjoint_rejects_closed_tanks_and_splits():
+    fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=4, lee=50.0, vented=False)
+    with pytest.raises(ValueError, match="closed"):
+        oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
+    # every particle above the split ratio: the first step splits them
+    fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=4, lee=50.0)
+    cfg.split_ratio = 0.5
+    with pytest.raises(ValueError, match="merge or split"):
+        oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
+    oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, grad=False)  # the forward alone runs
