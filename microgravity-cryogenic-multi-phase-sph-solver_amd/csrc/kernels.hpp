@@ -131,6 +131,14 @@ struct AdjArgs {
   float4* xp;      // phase change's share of the x cotangent
   double* lee_out; // this step's d loss / d (lee_evap, lee_cond)
 };
+// Several independent device-to-device copies in one launch (16-byte aligned
+// pointers): the adjoint's snapshots.
+struct CopyItem {
+  void* dst;
+  const void* src;
+  unsigned long long bytes;
+};
+void launch_copy_batch(const CopyItem* items, int count, hipStream_t s);
 void launch_adj_fill(float4* a, float4* b, float4* c, int n, hipStream_t s);
 void launch_adj_set_frame(DevState* st, const float* acc, hipStream_t s);
 void launch_adj_load_sum(const PV& pv, int n, double* partials, int stride, double* out, hipStream_t s);

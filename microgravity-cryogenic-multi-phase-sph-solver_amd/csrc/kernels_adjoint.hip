@@ -764,7 +764,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_app1(PV pv, GridDev g, SimParams
     const V3 xi = ld3(pv.s.x[i]);
     const float mi = pv.s.m[i], Vi = mi / pv.s.rho[i];
     float in_m = 0.0f, in_H = 0.0f;
-    if (!(A.fl[i] & kDying)) {
+    // receivers only near a source giving mass to this phase (the cells' source bits of the re-run apply)
+    if (!(A.fl[i] & kDying) && cells_any(g, xi, ph == kVapor ? kCellSrcV : kCellSrcL)) {
       for_each_pair(L, g, i, xi, [&](int j) {
         if (j == i || !(A.om[j] > 0.0f) || pc_target(A.fl[j]) != ph) return;
         const float r2 = norm2(xi - ld3(pv.s.x[j]));
@@ -807,6 +808,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_app2(PV pv, GridDev g, SimParams
   float4 po = A.po[i];
   const bool recv = pai.z != 0.0f || pai.w != 0.0f;
   const float om = A.om[i], oh = A.oh[i], wn = pv.t.w_norm[i];
+  if (!recv && !(om > 0.0f)) {  // neither side of any apply pair
+    A.pV[i] = 0.0f;
+    return;
+  }
   const uint8_t tgt = pc_target(A.fl[i]);
   const V3 xi = ld3(pv.s.x[i]);
   const float Vi = pv.s.m[i] / pv.s.rho[i];
@@ -972,6 +977,11 @@ __global__ __launch_bounds__(kBlock) void k_adj_nrm(PV pv, GridDev g, SimParams 
   const uint8_t tgt = pc_target(fli);
   const int ami = A.amax[i];
   const V3 xi = ld3(pv.s.x[i]);
+  // every pair here has a source with out_m > 0 on one side (or is i's own)
+  if (!src && ami < 0 && pni.z == 0.0f && pni.w == 0.0f && !cells_any(g, xi, kCellSrcV | kCellSrcL)) {
+    A.pr[i] = A.pdp[i].w;
+    return;
+  }
   const float Vi = pv.s.m[i] / pv.s.rho[i], rhoi = pv.s.rho[i];
   float Vb = 0.0f, rhob = 0.0f;
   AV xr{0.0f, 0.0f, 0.0f};
@@ -1014,7 +1024,41 @@ __global__ __launch_bounds__(kBlock) void k_adj_nrm(PV pv, GridDev g, SimParams 
   A.xp[i] = f4(av(A.xp[i]) + xr);
 }
 
+constexpr int kCopyTable = 32;
+struct CopyTable {
+  CopyItem it[kCopyTable];
+};
+// blockIdx.y: the copy; 16-byte words over the x blocks, the byte tail by block 0.
+__global__ __launch_bounds__(kBlock) void k_copy_batch(CopyTable t) {
+  const CopyItem c = t.it[blockIdx.y];
+  const unsigned long long nv = c.bytes / 16;
+  const uint4* src = static_cast<const uint4*>(c.src);
+  uint4* dst = static_cast<uint4*>(c.dst);
+  for (unsigned long long k = blockIdx.x * static_cast<unsigned long long>(blockDim.x) + threadIdx.x; k < nv;
+       k += static_cast<unsigned long long>(gridDim.x) * blockDim.x)
+    dst[k] = src[k];
+  const unsigned long long done = nv * 16;
+  if (blockIdx.x == 0 && threadIdx.x < c.bytes - done)
+    static_cast<uint8_t*>(c.dst)[done + threadIdx.x] = static_cast<const uint8_t*>(c.src)[done + threadIdx.x];
+}
+
 }  // namespace
+
+void launch_copy_batch(const CopyItem* items, int count, hipStream_t s) {
+  for (int b = 0; b < count; b += kCopyTable) {
+    const int m = count - b < kCopyTable ? count - b : kCopyTable;
+    CopyTable t{};
+    unsigned long long most = 0;
+    for (int k = 0; k < m; ++k) {
+      t.it[k] = items[b + k];
+      if (t.it[k].bytes > most) most = t.it[k].bytes;
+    }
+    const unsigned long long words = most / 16 + 1;
+    const unsigned long long bx = (words + kBlock - 1) / kBlock;
+    const unsigned gx = static_cast<unsigned>(bx < 512 ? bx : 512);
+    k_copy_batch<<<dim3(gx, static_cast<unsigned>(m)), kBlock, 0, s>>>(t);
+  }
+}
 
 void launch_adj_fill(float4* a, float4* b, float4* c, int n, hipStream_t s) {
   if (n <= 0) return;

@@ -824,10 +824,7 @@ struct cryo_solver {
     if (tape_slot) snap_save(*tape_slot, false, last_perm);
     if (marks.size() > 4096) prof_collect();
     evaluate_rates();
-    if (tape_slot && tape_rates) {  // the kick reads a and dTdt: the reverse step re-runs it
-      fit(tape_slot->a, n), d2d(tape_slot->a.get(), a.get(), n);
-      fit(tape_slot->dTdt, n), d2d(tape_slot->dTdt.get(), view().t.dTdt, n);
-    }
+    if (tape_slot) snap_save_rates(*tape_slot);
     const bool shifting = P.shift_strength > 0.0f;
     const bool pc = cfg.phase_change != 0;
     PV pv = view();
@@ -902,8 +899,10 @@ struct cryo_solver {
   // optionally `a` (at step boundaries) and the step's sort permutation (taped
   // post-sort states).
   struct Snap {
-    DevBuf<float4> x, v, vt, nrm, a, ab;
-    DevBuf<float> m, m0, rho, T, dTdt;
+    DevBuf<float4> x, v, vt, nrm, a, ab, nrec;
+    DevBuf<float> m, m0, rho, T, vol;
+    DevBuf<int> lidx, lcnt, coff;  // neighbour lists and cell table of a taped rates evaluation
+    DevBuf<uint8_t> cflags;
     DevBuf<uint8_t> phase, flags;
     DevBuf<uint32_t> id, perm;
     DevBuf<DevState> st;
@@ -924,15 +923,51 @@ struct cryo_solver {
     float4* v[6];
     float* f[14];
     scratch_fields(v, f);
+    copy_begin();
     for (int k = 0; k < 6; ++k) fit(z.sv[k], n), d2d(z.sv[k].get(), v[k], n);
     for (int k = 0; k < 14; ++k) fit(z.sf[k], n), d2d(z.sf[k].get(), f[k], n);
+    copy_end();
+  }
+  // A taped step's rates evaluation, after evaluate_rates(): the densities and
+  // normals it wrote, every scratch field, the neighbour lists, the cell table
+  // and the device state.  The reverse step loads it instead of re-sorting and
+  // re-evaluating.
+  void snap_save_rates(Snap& z) {
+    const Persist q = persist(set);
+    const Scratch t = view().t;
+    const std::size_t nl = static_cast<std::size_t>(kListK) * ((static_cast<std::size_t>(n) + 63) / 64 * 64);
+    copy_begin();
+    d2d(z.rho.get(), q.rho, n), d2d(z.nrm.get(), q.nrm, n);
+    snap_save_scratch(z);
+    fit(z.nrec, n), d2d(z.nrec.get(), t.nrec, n), fit(z.vol, n), d2d(z.vol.get(), t.vol, n);
+    fit(z.lidx, nl), d2d(z.lidx.get(), nbr_idx.get(), static_cast<int>(nl));
+    fit(z.lcnt, n), d2d(z.lcnt.get(), nbr_count.get(), n);
+    fit(z.coff, g.num_cells + 2), d2d(z.coff.get(), cell_off.get(), g.num_cells + 2);
+    fit(z.cflags, g.num_cells), d2d(z.cflags.get(), cell_flags.get(), g.num_cells);
+    d2d(z.st.get(), state.get(), 1);
+    copy_end();
+  }
+  void snap_load_rates(const Snap& z) {
+    const Scratch t = view().t;
+    const std::size_t nl = static_cast<std::size_t>(kListK) * ((static_cast<std::size_t>(z.n) + 63) / 64 * 64);
+    copy_begin();
+    snap_load(z, false);
+    snap_load_scratch(z);
+    d2d(t.nrec, z.nrec.get(), z.n), d2d(t.vol, z.vol.get(), z.n);
+    d2d(nbr_idx.get(), z.lidx.get(), static_cast<int>(nl)), d2d(nbr_count.get(), z.lcnt.get(), z.n);
+    d2d(cell_off.get(), z.coff.get(), g.num_cells + 2), d2d(cell_flags.get(), z.cflags.get(), g.num_cells);
+    copy_end();
+    lists_valid = true;
+    tiles_ok = true;
   }
   void snap_load_scratch(const Snap& z) {
     float4* v[6];
     float* f[14];
     scratch_fields(v, f);
+    copy_begin();
     for (int k = 0; k < 6; ++k) d2d(v[k], z.sv[k].get(), z.n);
     for (int k = 0; k < 14; ++k) d2d(f[k], z.sf[k].get(), z.n);
+    copy_end();
   }
   std::vector<std::unique_ptr<Snap>> adj_ckpt, adj_tape;
   std::unique_ptr<Snap> adj_end;
@@ -947,12 +982,27 @@ struct cryo_solver {
   DevBuf<uint8_t> adj_fl;
   DevBuf<int> adj_amax;
   DevBuf<double> adj_hsum, adj_lee;
-  Snap* tape_slot = nullptr;  // set: step_issue() records its post-sort state here
-  bool tape_rates = false;    // and, after the rates, a and dTdt
+  Snap* tape_slot = nullptr;  // set: step_issue() records its post-sort state and its rates here
 
+  // d2d between copy_begin() and copy_end() queues the copy; copy_end() issues
+  // every queued copy in one launch (they must not depend on each other).
+  std::vector<CopyItem> copy_q;
+  int copy_depth = 0;
+  void copy_begin() { ++copy_depth; }
+  void copy_end() {
+    if (--copy_depth > 0) return;
+    if (!copy_q.empty()) launch_copy_batch(copy_q.data(), static_cast<int>(copy_q.size()), stream);
+    copy_q.clear();
+  }
   template <class T>
   void d2d(T* dst, const T* src, int count) {
-    if (count > 0) CRYO_HIP(hipMemcpyAsync(dst, src, sizeof(T) * count, hipMemcpyDeviceToDevice, stream));
+    if (count <= 0) return;
+    const std::size_t bytes = sizeof(T) * static_cast<std::size_t>(count);
+    if (copy_depth > 0 && ((reinterpret_cast<std::uintptr_t>(dst) | reinterpret_cast<std::uintptr_t>(src)) & 15) == 0) {
+      copy_q.push_back(CopyItem{dst, src, bytes});
+      return;
+    }
+    CRYO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream));
   }
   template <class T>
   static void fit(DevBuf<T>& b, std::size_t count) {
@@ -963,22 +1013,28 @@ struct cryo_solver {
     const Persist q = persist(set);
     fit(z.x, m), fit(z.v, m), fit(z.vt, m), fit(z.nrm, m), fit(z.m, m), fit(z.m0, m), fit(z.rho, m), fit(z.T, m);
     fit(z.phase, m), fit(z.flags, m), fit(z.id, m), fit(z.st, 1);
+    if (with_a) fit(z.a, m), fit(z.ab, m);
+    if (perm) fit(z.perm, m);
+    copy_begin();
     d2d(z.x.get(), q.x, m), d2d(z.v.get(), q.v, m), d2d(z.vt.get(), q.vt, m), d2d(z.nrm.get(), q.nrm, m);
     d2d(z.m.get(), q.m, m), d2d(z.m0.get(), q.m0, m), d2d(z.rho.get(), q.rho, m), d2d(z.T.get(), q.T, m);
     d2d(z.phase.get(), q.phase, m), d2d(z.flags.get(), q.flags, m), d2d(z.id.get(), q.id, m);
     d2d(z.st.get(), state.get(), 1);
-    if (with_a) fit(z.a, m), d2d(z.a.get(), a.get(), m), fit(z.ab, m), d2d(z.ab.get(), ab.get(), m);
-    if (perm) fit(z.perm, m), d2d(z.perm.get(), perm, m);
+    if (with_a) d2d(z.a.get(), a.get(), m), d2d(z.ab.get(), ab.get(), m);
+    if (perm) d2d(z.perm.get(), perm, m);
+    copy_end();
     z.n = m;
   }
   void snap_load(const Snap& z, bool with_a) {
     const int m = z.n;
     const Persist q = persist(set);
+    copy_begin();
     d2d(q.x, z.x.get(), m), d2d(q.v, z.v.get(), m), d2d(q.vt, z.vt.get(), m), d2d(q.nrm, z.nrm.get(), m);
     d2d(q.m, z.m.get(), m), d2d(q.m0, z.m0.get(), m), d2d(q.rho, z.rho.get(), m), d2d(q.T, z.T.get(), m);
     d2d(q.phase, z.phase.get(), m), d2d(q.flags, z.flags.get(), m), d2d(q.id, z.id.get(), m);
     d2d(state.get(), z.st.get(), 1);
     if (with_a) d2d(a.get(), z.a.get(), m), d2d(ab.get(), z.ab.get(), m);
+    copy_end();
     n = m;
     lists_valid = false;
     tiles_ok = false;
@@ -1018,17 +1074,23 @@ struct cryo_solver {
   // then puts the taped v, T, m, flags and phase back.
   void adj_pc_rerun(const Snap& t) {
     const Persist q = persist(set);
-    d2d(a.get(), t.a.get(), n);
-    d2d(view().t.dTdt, t.dTdt.get(), n);
     if (!adj_pc()) return;
     launch_kick_sources(view(), P, state.get(), n, true, P.thermal != 0, true, stream);
+    copy_begin();
     d2d(adj_v1.get(), q.v, n), d2d(adj_T1.get(), q.T, n), d2d(adj_fl.get(), q.flags, n);
+    copy_end();
     launch_pc_normalize(view(), grid(), P, lists(), n, partials.get(), stride, stream);
+    copy_begin();
     d2d(adj_om.get(), view().t.out_m, n), d2d(adj_oh.get(), view().t.out_h, n);
+    copy_end();
     launch_pc_apply(view(), grid(), P, lists(), n, stream);
+    copy_begin();
     d2d(adj_m2.get(), q.m, n), d2d(adj_T2.get(), q.T, n);
+    copy_end();
+    copy_begin();
     d2d(q.v, t.v.get(), n), d2d(q.T, t.T.get(), n), d2d(q.m, t.m.get(), n);
     d2d(q.flags, t.flags.get(), n), d2d(q.phase, t.phase.get(), n);
+    copy_end();
   }
 
   void check_adjoint_scope() const {
@@ -1062,6 +1124,8 @@ struct cryo_solver {
     if (!loss) throw std::invalid_argument("adjoint: loss output is required");
     if (grad_theta && !grad) throw std::invalid_argument("adjoint: the contact-angle gradient needs grad_frame_acc");
     if (grad_lee && !grad) throw std::invalid_argument("adjoint: the Lee-coefficient gradient needs grad_frame_acc");
+    copy_depth = 0;  // no batch left open by an earlier failure
+    copy_q.clear();
     const bool fresh = !initialized;  // the start-up rates (and their a) belong to this run
     ensure_initialized();
     *loss = 0.0;
@@ -1076,19 +1140,24 @@ struct cryo_solver {
     fit(adj_acc, 3 * ns), fit(adj_S, 3 * ns), fit(adj_G, 3 * ns), fit(adj_T, 2 * ns + 2);
     fit(adj_part, 11 * static_cast<std::size_t>(stride));
     if (adj_pc()) fit(adj_lee, 2 * ns);
-    tape_rates = mT;
     CRYO_HIP(hipMemcpyAsync(adj_acc.get(), acc, sizeof(float) * 3 * ns, hipMemcpyHostToDevice, stream));
     const long long K = every;
     const long long nseg = (steps + K - 1) / K;
+    const long long k_last = (nseg - 1) * K;  // the last segment is taped by the forward pass itself
     if (grad) {
       adj_ckpt.resize(static_cast<std::size_t>(nseg));
       for (auto& c : adj_ckpt)
         if (!c) c = std::make_unique<Snap>();
+      adj_tape.resize(static_cast<std::size_t>(std::min(K, steps)));
+      for (auto& t : adj_tape)
+        if (!t) t = std::make_unique<Snap>();
     }
     for (long long k = 0; k < steps; ++k) {
       if (grad && k % K == 0) snap_save(*adj_ckpt[static_cast<std::size_t>(k / K)], true, nullptr);
       launch_adj_set_frame(state.get(), adj_acc.get() + 3 * k, stream);
+      tape_slot = (grad && k >= k_last) ? adj_tape[static_cast<std::size_t>(k - k_last)].get() : nullptr;
       step_issue();
+      tape_slot = nullptr;
       step_finish();
       launch_adj_load_sum(view(), n, adj_part.get(), stride, adj_S.get() + 3 * k, stream);
     }
@@ -1123,14 +1192,11 @@ struct cryo_solver {
         fit(adj_fl, m), fit(adj_pa, m), fit(adj_po, m), fit(adj_pb4, m), fit(adj_pdp, m), fit(adj_v1b, m);
         fit(adj_pm2, m), fit(adj_pn, m), fit(adj_amax, m), fit(adj_pV, m), fit(adj_pr, m), fit(adj_xp, m);
       }
-      adj_tape.resize(static_cast<std::size_t>(std::min(K, steps)));
-      for (auto& t : adj_tape)
-        if (!t) t = std::make_unique<Snap>();
       int cur = 0;
       for (long long seg = nseg - 1; seg >= 0; --seg) {
         const long long k0 = seg * K, k1 = std::min(steps, k0 + K);
-        snap_load(*adj_ckpt[static_cast<std::size_t>(seg)], true);
-        for (long long k = k0; k < k1; ++k) {
+        if (seg < nseg - 1) snap_load(*adj_ckpt[static_cast<std::size_t>(seg)], true);
+        for (long long k = k0; k < k1 && seg < nseg - 1; ++k) {
           launch_adj_set_frame(state.get(), adj_acc.get() + 3 * k, stream);
           tape_slot = adj_tape[static_cast<std::size_t>(k - k0)].get();
           step_issue();
@@ -1139,9 +1205,7 @@ struct cryo_solver {
         }
         for (long long k = k1 - 1; k >= k0; --k) {
           const Snap& t = *adj_tape[static_cast<std::size_t>(k - k0)];
-          snap_load(t, false);
-          grid_build(false);  // already in cell order: the identity permutation, the same cell table
-          evaluate_rates(false);
+          snap_load_rates(t);  // the step's sorted state and its rates
           if (mT) adj_pc_rerun(t);
           const AdjArgs A = adj_args(cur, t.perm.get(), adj_S.get() + 3 * k, csf,
                                      adj_pc() ? adj_lee.get() + 2 * k : nullptr);
@@ -1176,7 +1240,6 @@ struct cryo_solver {
     if (grad && adj_pc())
       CRYO_HIP(hipMemcpyAsync(lee.data(), adj_lee.get(), sizeof(double) * 2 * ns, hipMemcpyDeviceToHost, stream));
     CRYO_HIP(hipStreamSynchronize(stream));
-    tape_rates = false;
     if (grad_lee)
       for (std::size_t q = 0; q < ns; ++q) grad_lee[0] += lee[2 * q], grad_lee[1] += lee[2 * q + 1];
     if (grad_theta) {  // d loss / d theta = (cos_bar (-sin theta) + sin_bar cos theta) pi / 180
