@@ -188,15 +188,20 @@ int cryo_ensemble_run_steps(cryo_solver* const* solvers, int32_t count, int64_t 
  * checkpoints every `checkpoint_every` steps and re-runs each segment, and
  * (grad_contact_angle not NULL, which needs grad_frame_acc) d loss / d the
  * contact angle in degrees (0 without CSF; on a solver not yet initialised
- * the start-up rates' share is included).  The solver ends in the forward end
- * state either way.  Differentiated: density summation, EOS, Shepard wall
- * pressure, ghost velocity and wall colour, momentum (pressure, viscosity,
- * artificial viscosity, wall terms), CSF surface tension (colour gradient,
- * contact-angle rotation, curvature, force), the transport velocity,
- * kick-drift-kick.  Requires fixed_dt > 0, the Wendland C2 kernel, density
- * summation, CSF only with unsmoothed normals, the classic force and normal
- * rotation, no thermal, phase change, transport stress, SPS or shifting, and
- * no frame rotation or manoeuvre table (CRYO_ERR_INVALID_ARGUMENT otherwise). */
+ * the start-up rates' share is included), and (grad_lee not NULL, two
+ * doubles, which needs grad_frame_acc) d loss / d lee_evaporation and
+ * d loss / d lee_condensation.  The solver ends in the forward end state
+ * either way.  Differentiated: density summation, EOS, Shepard wall pressure,
+ * ghost velocity and wall colour, momentum (pressure, viscosity, artificial
+ * viscosity, wall terms), CSF surface tension (colour gradient, contact-angle
+ * rotation, curvature, force), the transport velocity, conduction and the
+ * heater, Lee phase change (sources, receiver normalisation, apply, vapour
+ * rebalance, venting; masses and temperatures are state), kick-drift-kick.
+ * Requires fixed_dt > 0, the Wendland C2 kernel, density summation, CSF only
+ * with unsmoothed normals, the classic force and normal rotation, no
+ * closed-tank saturation feedback, transport stress, SPS or shifting, and no
+ * frame rotation or manoeuvre table; a window in which a particle splits or
+ * merges is not differentiable (CRYO_ERR_INVALID_ARGUMENT otherwise). */
 int cryo_solver_adjoint_slosh(cryo_solver* s, int64_t steps, const float* frame_acc, int32_t checkpoint_every,
                               double* loss, double* grad_frame_acc, double* wall_load, double* grad_contact_angle,
                               double* grad_lee);
