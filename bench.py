@@ -129,10 +129,13 @@ def c4_tanks(cabi, count=64, seed=0):
     return out
 
 
-def ensemble_measure(cabi, tanks, steps):
-    """The same tanks advanced `steps` steps by the per-tank loop of Solvers and
-    then by cryo_ensemble_run_steps (bitwise the same results; the GPU test
-    checks that), wall-clock per ensemble step each way."""
+def ensemble_measure(cabi, tanks, steps, tune_steps=3):
+    """The same tanks advanced `steps` steps by the per-tank loop of Solvers (the
+    default launch shape), then autotuned (cryo_solver_autotune: each member
+    keeps the fastest launch variant of its neighbour passes; the results are
+    the same bits), then by the loop again and by cryo_ensemble_run_steps
+    (bitwise the loop's results; the GPU test checks that), wall-clock per
+    ensemble step each way."""
     solvers = []
     for t in tanks:
         s = cabi.HipSolver(t["cfg"], t["fluid"])
@@ -146,13 +149,29 @@ def ensemble_measure(cabi, tanks, steps):
         s.run_steps(steps)
     loop = time.perf_counter() - t0
     t0 = time.perf_counter()
+    picks = [s.autotune(tune_steps)[0] for s in solvers]
+    tune = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for s in solvers:
+        s.run_steps(steps)
+    loop_tuned = time.perf_counter() - t0
+    t0 = time.perf_counter()
     cabi.hip_ensemble_run_steps(solvers, steps)
     ens = time.perf_counter() - t0
+    for s in solvers:  # the same with captured step graphs (captures included)
+        s.set_step_graphs(True)
+    t0 = time.perf_counter()
+    cabi.hip_ensemble_run_steps(solvers, steps)
+    ens_graphs = time.perf_counter() - t0
     n = sum(s.num_particles for s in solvers)
     del solvers
     return {"tanks": len(tanks), "particles": n, "steps": steps,
-            "per_tank_loop_ms_per_step": 1e3 * loop / steps, "ensemble_ms_per_step": 1e3 * ens / steps,
-            "speedup": loop / ens,
+            "per_tank_loop_ms_per_step": 1e3 * loop / steps,
+            "autotune_s": tune, "launch_variants_chosen": [picks.count(v) for v in range(4)],
+            "per_tank_loop_tuned_ms_per_step": 1e3 * loop_tuned / steps,
+            "ensemble_ms_per_step": 1e3 * ens / steps,
+            "ensemble_step_graphs_ms_per_step": 1e3 * ens_graphs / steps,
+            "speedup": loop / ens, "speedup_vs_tuned_loop": loop_tuned / ens,
             "ensemble_particle_steps_per_s": n * steps / ens}
 
 

@@ -178,6 +178,26 @@ int cryo_solver_damp_velocities(cryo_solver* s, float factor);
  * may use a member during the call. */
 int cryo_ensemble_run_steps(cryo_solver* const* solvers, int32_t count, int64_t steps);
 
+/* Extension (no counterpart in the reference, whose CUDA kernels have one launch
+ * shape): the launch variant of the interface and forces passes, bit 0 set =
+ * forces by per-particle neighbour-list gathers instead of LDS-staged cell
+ * tiles, bit 1 set = the same for the interface pass.  Every variant gives the
+ * same results bit for bit; only the speed differs with tank size and fill.
+ * cryo_solver_autotune times `trial_steps` steps of each variant from the
+ * solver's current state (after one untimed step each), keeps the fastest,
+ * returns it in *variant (may be NULL) and each variant's milliseconds per step
+ * in ms_per_step[0..3] (may be NULL), and puts the solver's state back as it was. */
+int cryo_solver_set_launch_variant(cryo_solver* s, int32_t variant);
+/* Extension (no counterpart in the reference): on != 0 makes cryo_solver_run_steps
+ * and cryo_ensemble_run_steps capture this solver's step as a HIP graph (one per
+ * persistent buffer set, captured again when the particle count or capacity
+ * changes) and replay it for the rest of the call.  Results are the same bits.
+ * Default off: on MI355X the replay has not been faster than plain launches for
+ * these steps (the device, not the launch path, bounds them). */
+int cryo_solver_set_step_graphs(cryo_solver* s, int32_t on);
+int cryo_solver_launch_variant(cryo_solver* s, int32_t* variant);
+int cryo_solver_autotune(cryo_solver* s, int32_t trial_steps, int32_t* variant, double* ms_per_step);
+
 /* Extension (no counterpart in the reference, which has no adjoint or autodiff):
  * runs `steps` steps, step k with frame acceleration frame_acc[3k..3k+2]
  * (g_eff = gravity - acc, as cryo_solver_set_frame_motion sets it), and returns

@@ -774,6 +774,26 @@ PYBIND11_MODULE(CryoSPH, m) {
         check(cryo_solver_probe_pressure(S.s, static_cast<int32_t>(np), p.data(), out.data()));
         return vec_array(out);
       }, py::arg("points"))
+      // Extension (cryo_solver_autotune / set_launch_variant): no counterpart in the reference.
+      .def("autotune", [](Solver& S, int trial_steps) {
+        int32_t v = 0;
+        double ms[4] = {0, 0, 0, 0};
+        int r;
+        {
+          py::gil_scoped_release rel;
+          r = cryo_solver_autotune(S.s, trial_steps, &v, ms);
+        }
+        check(r);
+        return py::make_tuple(v, py::make_tuple(ms[0], ms[1], ms[2], ms[3]));
+      }, py::arg("trial_steps") = 5,
+         "Times trial_steps steps of each launch variant of the neighbour passes (LDS cell tiles or list "
+         "gathers, for forces and interface), keeps the fastest, restores the state; returns (variant, "
+         "ms per step of variants 0..3).  Results do not depend on the variant.")
+      .def_property("launch_variant", [](Solver& S) {
+        int32_t v = 0;
+        check(cryo_solver_launch_variant(S.s, &v));
+        return v;
+      }, [](Solver& S, int v) { check(cryo_solver_set_launch_variant(S.s, v)); })
       // Extension (cryo_solver_adjoint_slosh): no counterpart in the reference.
       .def("adjoint_slosh", [](Solver& S, DArr acc, int checkpoint_every, bool gradient) -> py::tuple {
         if (acc.ndim() != 2 || acc.shape(1) != 3) throw py::value_error("frame_acc must have shape (steps, 3)");

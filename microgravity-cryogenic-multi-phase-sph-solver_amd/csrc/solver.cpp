@@ -234,7 +234,10 @@ struct cryo_solver {
   // Every particle [0, n) sits in the cell it was sorted into (no move or
   // append since the last sort): the cell-tile passes may be used.
   bool tiles_ok = false;
-  bool use_tiles = true;
+  // Launch variants of the neighbour passes (results are the same bits either
+  // way): LDS-staged cell tiles or per-particle list gathers, for the
+  // interface and the forces pass.  autotune() times them.
+  bool tile_iface = true, tile_forces = true;
   const uint32_t* last_perm = nullptr;
   int n_sorted = 0;
   // reductions
@@ -349,6 +352,7 @@ struct cryo_solver {
   ~cryo_solver() {
     if (stream) (void)hipStreamSynchronize(stream);
     if (copy_stream) (void)hipStreamSynchronize(copy_stream);
+    graph_reset();
     release_telemetry();
     for (const Mark& m : marks) {
       (void)hipEventDestroy(m.a);
@@ -749,7 +753,7 @@ struct cryo_solver {
     prof_begin(5);
     const int nb =
         launch_interface(pv, gd, P, L, state.get(), n, csf_mode, heat_sum ? partials.get() : nullptr,
-                         tiles_ok && use_tiles, stream);
+                         tiles_ok && tile_iface, stream);
     if (csf_mode && P.csf_smooth) launch_smooth_normals(pv, gd, P, L, n, stream);
     if (P.thermal) launch_heat_norm(state.get(), partials.get(), nb, heat_sum, stream);
     prof_end(5);
@@ -759,7 +763,7 @@ struct cryo_solver {
     }
     prof_begin(6);
     launch_forces(pv, gd, P, L, state.get(), n, csf_mode, cfg.surface_tension == CRYO_ST_AKINCI,
-                  cfg.density_mode == CRYO_DENSITY_CONTINUITY, P.thermal != 0, tiles_ok && use_tiles, stream);
+                  cfg.density_mode == CRYO_DENSITY_CONTINUITY, P.thermal != 0, tiles_ok && tile_forces, stream);
     prof_end(6);
     check_launch();
   }
@@ -817,6 +821,67 @@ struct cryo_solver {
   }
   // Everything of one step that the device can do without the host: queued
   // on the solver stream, returns at once.
+  // ------------------------------------------------------ captured steps
+  // One step's launches as a HIP graph, replayed while the host state it was
+  // captured from holds: the persistent buffer set, the particle count, the
+  // merged particles to drop and the capacity (a step's kernels, grids and
+  // pointers depend on nothing else).  Replay sets the host state the
+  // captured issue left.  With use_graphs set, cryo_solver_run_steps and the
+  // ensemble runner enable it for the duration of their call, during which
+  // SimParams cannot change.
+  struct StepGraph {
+    hipGraphExec_t exec = nullptr;
+    int n = -1, pending_dead = -1, capacity = -1;
+    int set_after = 0, n_after = 0, n_sorted = 0, pd_after = 0;
+    const uint32_t* last_perm = nullptr;
+    bool lists_valid = false, tiles_ok = false;
+  };
+  StepGraph sg[2];  // by persistent buffer set
+  bool use_graphs = false, graphs_on = false;
+  void graph_reset() {
+    for (StepGraph& x : sg) {
+      if (x.exec) (void)hipGraphExecDestroy(x.exec);
+      x = StepGraph{};
+    }
+  }
+  void step_issue_graph() {
+    ensure_initialized();
+    const bool grows = cfg.phase_change != 0 &&
+                       static_cast<std::size_t>(n) * 4 > static_cast<std::size_t>(capacity) * 3;  // allocates
+    if (!graphs_on || prof_mask != 0 || tape_slot || grows || n == 0) {
+      step_issue();
+      return;
+    }
+    StepGraph& G = sg[set];
+    if (!G.exec || G.n != n || G.pending_dead != pending_dead || G.capacity != capacity) {
+      if (G.exec) (void)hipGraphExecDestroy(G.exec);
+      G = StepGraph{};
+      const int n0 = n, pd0 = pending_dead, cap0 = capacity;
+      CRYO_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      try {
+        step_issue();
+      } catch (...) {
+        hipGraph_t partial = nullptr;
+        (void)hipStreamEndCapture(stream, &partial);
+        if (partial) (void)hipGraphDestroy(partial);
+        throw;
+      }
+      hipGraph_t graph = nullptr;
+      CRYO_HIP(hipStreamEndCapture(stream, &graph));
+      const hipError_t e = hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      CRYO_HIP(e);
+      G.n = n0, G.pending_dead = pd0, G.capacity = cap0;
+      G.set_after = set, G.n_after = n, G.n_sorted = n_sorted, G.pd_after = pending_dead;
+      G.last_perm = last_perm, G.lists_valid = lists_valid, G.tiles_ok = tiles_ok;
+      CRYO_HIP(hipGraphLaunch(G.exec, stream));
+      return;
+    }
+    CRYO_HIP(hipGraphLaunch(G.exec, stream));
+    set = G.set_after, n = G.n_after, n_sorted = G.n_sorted, pending_dead = G.pd_after;
+    last_perm = G.last_perm, lists_valid = G.lists_valid, tiles_ok = G.tiles_ok;
+  }
+
   void step_issue() {
     ensure_initialized();
     launch_begin_step(state.get(), P, L, man, stream);
@@ -1256,6 +1321,67 @@ struct cryo_solver {
     sync_state();
   }
 
+  // ----------------------------------------------------------- autotune
+  int launch_variant() const { return (tile_forces ? 0 : 1) | (tile_iface ? 0 : 2); }
+  void set_launch_variant(int v) {
+    if (v < 0 || v > 3) throw std::invalid_argument("launch variant must be 0..3");
+    tile_forces = (v & 1) == 0;
+    tile_iface = (v & 2) == 0;
+  }
+  // Times `trials` steps of every launch variant from the current state (after
+  // one untimed step each), keeps the fastest and puts the state back.  ms[v]
+  // (optional, 4 entries) receives each variant's milliseconds per step.
+  int autotune(int trials, double* ms) {
+    if (trials <= 0) throw std::invalid_argument("autotune: trial_steps must be > 0");
+    ensure_initialized();
+    sync_state();
+    Snap z;
+    snap_save(z, true, nullptr);
+    snap_save_scratch(z);
+    const int pd = pending_dead;
+    const int ms_seen = merges_seen;
+    const int v0 = launch_variant();
+    hipEvent_t e0, e1;
+    CRYO_HIP(hipEventCreate(&e0));
+    CRYO_HIP(hipEventCreate(&e1));
+    double best = 0.0;
+    int pick = v0;
+    auto restore = [&]() {
+      snap_load(z, true);
+      snap_load_scratch(z);
+      pending_dead = pd;
+      merges_seen = ms_seen;
+      sync_state();
+    };
+    try {
+      for (int v = 0; v < 4; ++v) {
+        restore();
+        set_launch_variant(v);
+        step();
+        CRYO_HIP(hipEventRecord(e0, stream));
+        for (int k = 0; k < trials; ++k) step();
+        CRYO_HIP(hipEventRecord(e1, stream));
+        CRYO_HIP(hipEventSynchronize(e1));
+        float t = 0.0f;
+        CRYO_HIP(hipEventElapsedTime(&t, e0, e1));
+        const double per = static_cast<double>(t) / trials;
+        if (ms) ms[v] = per;
+        if (v == 0 || per < best) best = per, pick = v;
+      }
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      set_launch_variant(v0);
+      restore();
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    set_launch_variant(pick);
+    restore();
+    return pick;
+  }
+
   // -------------------------------------------------------------- frame
   void apply_frame(V3 acc, V3 w, V3 wd, V3 origin) {  // sph_solver.cu:660-667
     sync_state();
@@ -1632,8 +1758,38 @@ int cryo_solver_set_particles(cryo_solver* s, int32_t n, const float* x, const f
 int cryo_solver_step(cryo_solver* s) { CRYO_GUARD(S(s).step()) }
 int cryo_solver_run_steps(cryo_solver* s, int64_t k) {
   CRYO_GUARD({
-    for (int64_t q = 0; q < k; ++q) S(s).step();
-    S(s).sync_state();
+    cryo_solver& z = S(s);
+    if (!z.use_graphs) {
+      for (int64_t q = 0; q < k; ++q) z.step();
+    } else {
+      struct Off {
+        cryo_solver& z;
+        ~Off() {
+          (void)hipStreamSynchronize(z.stream);
+          z.graph_reset();
+          z.graphs_on = false;
+        }
+      } off{z};
+      z.ensure_initialized();
+      z.graphs_on = true;
+      for (int64_t q = 0; q < k; ++q) {
+        z.step_issue_graph();
+        z.step_finish();
+      }
+    }
+    z.sync_state();
+  })
+}
+int cryo_solver_set_launch_variant(cryo_solver* s, int32_t variant) { CRYO_GUARD(S(s).set_launch_variant(variant)) }
+int cryo_solver_set_step_graphs(cryo_solver* s, int32_t on) { CRYO_GUARD(S(s).use_graphs = on != 0) }
+int cryo_solver_launch_variant(cryo_solver* s, int32_t* variant) {
+  if (!variant) return fail("variant is required", CRYO_ERR_INVALID_ARGUMENT);
+  CRYO_GUARD(*variant = S(s).launch_variant())
+}
+int cryo_solver_autotune(cryo_solver* s, int32_t trial_steps, int32_t* variant, double* ms_per_step) {
+  CRYO_GUARD({
+    const int v = S(s).autotune(trial_steps, ms_per_step);
+    if (variant) *variant = v;
   })
 }
 int cryo_ensemble_run_steps(cryo_solver* const* solvers, int32_t count, int64_t steps) {
@@ -1650,12 +1806,29 @@ int cryo_ensemble_run_steps(cryo_solver* const* solvers, int32_t count, int64_t 
     // member's host round trip overlaps the others' device work.  A member
     // queues its next step as soon as its own bookkeeping is back, so the
     // device never runs dry between ensemble steps.
+    // Each member's step is a captured graph, replayed step after step.
+    struct Graphs {
+      cryo_solver* const* v;
+      int32_t c;
+      ~Graphs() {
+        for (int32_t t = 0; t < c; ++t) {
+          (void)hipStreamSynchronize(v[t]->stream);
+          v[t]->graph_reset();
+          v[t]->graphs_on = false;
+        }
+      }
+    } graphs{solvers, count};
+    for (int32_t t = 0; t < count; ++t) {
+      cryo_solver& z = S(solvers[t]);
+      z.ensure_initialized();
+      z.graphs_on = z.use_graphs;
+    }
     if (steps > 0) {
-      for (int32_t t = 0; t < count; ++t) S(solvers[t]).step_issue();
+      for (int32_t t = 0; t < count; ++t) S(solvers[t]).step_issue_graph();
       for (int64_t q = 1; q < steps; ++q)
         for (int32_t t = 0; t < count; ++t) {
           S(solvers[t]).step_finish();
-          S(solvers[t]).step_issue();
+          S(solvers[t]).step_issue_graph();
         }
       for (int32_t t = 0; t < count; ++t) S(solvers[t]).step_finish();
     }

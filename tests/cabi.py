@@ -536,6 +536,24 @@ class HipSolver:
         """(loss, dloss/dacc or None, per-step sum of fw, dloss/dcontact_angle_deg or None)."""
         return _adjoint_slosh(lambda name, *a: self._call("solver_" + name, *a), self.h, acc, checkpoint_every, grad)
 
+    def set_launch_variant(self, v):
+        self._call("solver_set_launch_variant", self.h, C.c_int32(v))
+
+    def set_step_graphs(self, on):
+        self._call("solver_set_step_graphs", self.h, C.c_int32(1 if on else 0))
+
+    def launch_variant(self):
+        v = C.c_int32()
+        self._call("solver_launch_variant", self.h, C.byref(v))
+        return v.value
+
+    def autotune(self, trial_steps=5):
+        """See cryo_solver_autotune: (chosen variant, ms per step of variants 0..3)."""
+        v = C.c_int32()
+        ms = (C.c_double * 4)()
+        self._call("solver_autotune", self.h, C.c_int32(trial_steps), C.byref(v), ms)
+        return v.value, list(ms)
+
     def initialize(self):
         self._call("solver_initialize", self.h)
 
