@@ -392,10 +392,12 @@ __host__ __device__ inline void for_each_cand(const GridDev& g, V3 p, float radi
   }
 }
 
-// for_each_cand with the candidates' positions gathered eight at a time
-// (one wait per eight loads instead of one per candidate): f(j, x[j]).
-template <class F>
-__device__ inline void for_each_cand_x(const GridDev& g, V3 p, float radius, const float4* __restrict__ xs, F&& f) {
+// for_each_cand with the candidates' positions (and, WP, phases) gathered
+// eight at a time (one wait per eight loads instead of one per candidate):
+// f(j, x[j]) or f(j, x[j], phase[j]).
+template <bool WP, class F>
+__device__ inline void for_each_cand_impl(const GridDev& g, V3 p, float radius, const float4* __restrict__ xs,
+                                          const uint8_t* __restrict__ phs, F&& f) {
   const float ux = (p.x - g.ox) * g.inv_cell, uy = (p.y - g.oy) * g.inv_cell, uz = (p.z - g.oz) * g.inv_cell;
   const int cx = clampv(static_cast<int>(floorf(ux)), 0, g.nx - 1);
   const int cy = clampv(static_cast<int>(floorf(uy)), 0, g.ny - 1);
@@ -419,14 +421,33 @@ __device__ inline void for_each_cand_x(const GridDev& g, V3 p, float radius, con
       int j = s;
       for (; j + 8 <= e; j += 8) {
         float4 b[8];
+        uint8_t q[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) b[u] = xs[j + u];
+        for (int u = 0; u < 8; ++u) {
+          b[u] = xs[j + u];
+          if constexpr (WP) q[u] = phs[j + u];
+        }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) f(j + u, b[u]);
+        for (int u = 0; u < 8; ++u) {
+          if constexpr (WP) f(j + u, b[u], q[u]);
+          else f(j + u, b[u]);
+        }
       }
-      for (; j < e; ++j) f(j, xs[j]);
+      for (; j < e; ++j) {
+        if constexpr (WP) f(j, xs[j], phs[j]);
+        else f(j, xs[j]);
+      }
     }
   }
+}
+template <class F>
+__device__ inline void for_each_cand_x(const GridDev& g, V3 p, float radius, const float4* __restrict__ xs, F&& f) {
+  for_each_cand_impl<false>(g, p, radius, xs, nullptr, f);
+}
+template <class F>
+__device__ inline void for_each_cand_xph(const GridDev& g, V3 p, float radius, const float4* __restrict__ xs,
+                                         const uint8_t* __restrict__ phs, F&& f) {
+  for_each_cand_impl<true>(g, p, radius, xs, phs, f);
 }
 
 // ---------------------------------------------------------- neighbour lists

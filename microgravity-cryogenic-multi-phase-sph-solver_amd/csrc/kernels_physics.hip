@@ -68,9 +68,9 @@ __global__ __launch_bounds__(kBlock) void k_density(PV pv, GridDev g, SimParams 
     // wall targets only ever need their fluid neighbours: none without a fluid
     // particle in the 27 cells
     if (ph == kWall && cells_any(g, xi, kCellFluid)) {
-      for_each_cand_x(g, xi, P.kernel.support, pv.s.x, [&](int j, float4 xj) {
+      for_each_cand_xph(g, xi, P.kernel.support, pv.s.x, pv.s.phase, [&](int j, float4 xj, uint8_t phj) {
         if (norm2(xi - ld3(xj)) >= P.support2) return;
-        if (!is_fluid(pv.s.phase[j])) return;
+        if (!is_fluid(phj)) return;
         lw.push(j);
       });
     }

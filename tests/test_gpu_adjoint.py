@@ -123,11 +123,12 @@ def test_heat_and_phase_change_gradients_match_cpu_adjoints():
     larger opposite shares (rebalance against the vapour's own uptake), and the
     FP32 forward's temperatures, under the scaled-up conduction and heater,
     move it by 1e-3 (2.5e-3 with CSF) from FP64: the FP32 CPU adjoint shows
-    the same gap, and the GPU agrees with it to 2e-5.  The checkpoint interval
-    leaves the results bitwise unchanged."""
+    the same gap, and the GPU agrees with it to 2e-5.  Isothermal walls (wall
+    conduction pairs) are a third case.  The checkpoint interval leaves the
+    results bitwise unchanged."""
     say("test_heat_and_phase_change_gradients_match_cpu_adjoints")
-    for csf in (False, True):
-        fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=12, lee=50.0, csf=csf)
+    for csf, iso in ((False, False), (True, False), (False, True)):
+        fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=12, lee=50.0, csf=csf, iso_walls=iso)
         out = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=5)
         out64 = orc(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
         o32 = CpuSolver("orc", cfg, fluid, 32)
@@ -136,7 +137,7 @@ def test_heat_and_phase_change_gradients_match_cpu_adjoints():
         err = np.abs(out[1] - out64[1]).max() / np.abs(out64[1]).max()
         lee64 = [abs(out.lee[q] / out64.lee[q] - 1) for q in (0, 1)]
         lee32 = [abs(out.lee[q] / out32.lee[q] - 1) for q in (0, 1)]
-        say("csf", csf, "loss", out[0], "fp64", out64[0], "acc gradient error", err, "lee", out.lee, "fp32",
+        say("csf", csf, "iso walls", iso, "loss", out[0], "fp64", out64[0], "acc gradient error", err, "lee", out.lee, "fp32",
             out32.lee, "fp64", out64.lee, "rel fp32", lee32, "rel fp64", lee64)
         assert abs(out[0] - out64[0]) <= 1e-4 * out64[0]
         assert err <= 1e-3

@@ -114,7 +114,8 @@ def test_fp64_adjoint_with_transport_velocity_matches_central_differences():
     assert np.abs(grad - fd).max() / np.abs(fd).max() <= 1e-6
 
 
-def test_fp64_adjoint_with_heat_and_phase_change_matches_central_differences():
+@pytest.mark.parametrize("iso_walls", [False, True])
+def test_fp64_adjoint_with_heat_and_phase_change_matches_central_differences(iso_walls):
     """Conduction (scaled up), the heater, Lee evaporation and condensation,
     receiver normalisation with the vapour density cap, apply, vapour
     rebalance and venting on: masses and temperatures are then part of the
@@ -125,8 +126,9 @@ def test_fp64_adjoint_with_heat_and_phase_change_matches_central_differences():
     the interface fraction, rebalance pairs, the vapour EOS temperature,
     rho = m sigma of vapour, apply and normalisation volumes, the heater
     weights and their normalisation) or a Lee derivative by 9e-3 or more
-    (venting)."""
-    fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=12, lee=50.0)
+    (venting).  Run with adiabatic and with isothermal walls (wall conduction
+    pairs)."""
+    fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=12, lee=50.0, iso_walls=iso_walls)
     out = oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
     grad = out[1]
     fd = np.zeros_like(grad)
@@ -142,7 +144,7 @@ def test_fp64_adjoint_with_heat_and_phase_change_matches_central_differences():
     for which, name in enumerate(("lee_evaporation", "lee_condensation")):
         vals = []
         for sgn in (1, -1):
-            c2 = adjoint_case(steps=12, lee=50.0)[1]
+            c2 = adjoint_case(steps=12, lee=50.0, iso_walls=iso_walls)[1]
             setattr(c2, name, 50.0 * (1 + sgn * 1e-3))
             vals.append(oracle(c2, fluid, ps, lo, hi).adjoint_slosh(acc, grad=False)[0])
         fd_lee = (vals[0] - vals[1]) / (2 * 50.0 * 1e-3)
