@@ -216,12 +216,13 @@ int cryo_solver_autotune(cryo_solver* s, int32_t trial_steps, int32_t* variant, 
  * viscosity, wall terms), CSF surface tension (colour gradient, contact-angle
  * rotation, curvature, force), the transport velocity, conduction and the
  * heater, Lee phase change (sources, receiver normalisation, apply, vapour
- * rebalance, venting; masses and temperatures are state), kick-drift-kick.
- * Requires fixed_dt > 0, the Wendland C2 kernel, density summation, CSF only
- * with unsmoothed normals, the classic force and normal rotation, no
- * closed-tank saturation feedback, transport stress, SPS or shifting, and no
- * frame rotation or manoeuvre table; a window in which a particle splits or
- * merges is not differentiable (CRYO_ERR_INVALID_ARGUMENT otherwise). */
+ * rebalance, venting, a closed tank's saturation-temperature feedback;
+ * masses and temperatures are state), kick-drift-kick.  Requires
+ * fixed_dt > 0, the Wendland C2 kernel, density summation, CSF only with
+ * unsmoothed normals, the classic force and normal rotation, no transport
+ * stress, SPS or shifting, and no frame rotation or manoeuvre table; a window
+ * in which a particle splits or merges is not differentiable
+ * (CRYO_ERR_INVALID_ARGUMENT otherwise). */
 int cryo_solver_adjoint_slosh(cryo_solver* s, int64_t steps, const float* frame_acc, int32_t checkpoint_every,
                               double* loss, double* grad_frame_acc, double* wall_load, double* grad_contact_angle,
                               double* grad_lee);

@@ -130,6 +130,10 @@ struct AdjArgs {
   float *pV, *pr;  // phase change's share of the V = m / rho and rho cotangents
   float4* xp;      // phase change's share of the x cotangent
   double* lee_out; // this step's d loss / d (lee_evap, lee_cond)
+  // closed tank: this step's T_sat came from the previous step's end state
+  int sat;
+  FluidConst fluid;
+  double* satv;    // {sum_L m bar, sum_V m T bar} of that feedback
 };
 // Several independent device-to-device copies in one launch (16-byte aligned
 // pointers): the adjoint's snapshots.

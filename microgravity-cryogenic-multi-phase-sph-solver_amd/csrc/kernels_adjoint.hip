@@ -579,7 +579,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
   const float rhob = rb + pb * dpdr;
   A.sb[i] = rhob * (vap ? I.m : pv.s.m0[i]) + tsb;  // rho = m0 sigma (liquid), m sigma (vapour)
   if (A.mT) {
-    const float4 pa = A.pc ? A.pa[i] : make_float4(A.cm[i], A.cT[i], 0.0f, 0.0f);
+    const float4 pa = A.pc ? A.pa[i] : make_float4(A.cm[i], A.cT[i], 0.0f, 0.0f);  // (own shares)
     mb += pa.x + Vbk / I.rho;
     mb -= (cA.x * F.x + cA.y * F.y + cA.z * F.z) / (I.m * I.m);  // a = g + F / m + ...
     if (A.heat) mb -= pa.y * dt * pv.t.dTdt[i] / I.m;      // dTdt = q / (m cp)
@@ -624,8 +624,17 @@ __global__ __launch_bounds__(kBlock) void k_adj_density(PV pv, GridDev g, SimPar
   const AV X = av(A.cx[i]) + xr;
   const AV Vh = av(A.pc ? A.v1b[i] : A.cv[i]) + av(A.vr[i]) + X * dt;
   if (A.mT) {
-    A.nm[dst] = A.mr[i];
-    A.nT[dst] = A.Tr[i];
+    float nm = A.mr[i], nT = A.Tr[i];
+    if (A.sat) {  // the closed tank's T_sat feedback (k_adj_sat)
+      const uint8_t ph = pv.s.phase[i];
+      if (ph == kLiquid) nm += static_cast<float>(A.satv[0]);
+      if (ph == kVapor) {
+        nm += static_cast<float>(A.satv[1] * pv.s.T[i]);
+        nT += static_cast<float>(A.satv[1] * pv.s.m[i]);
+      }
+    }
+    A.nm[dst] = nm;
+    A.nT[dst] = nT;
   }
   A.nx[dst] = f4(X);
   A.nv[dst] = f4(Vh);
@@ -855,7 +864,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_app2(PV pv, GridDev g, SimParams
 __global__ __launch_bounds__(kBlock) void k_adj_src(PV pv, GridDev g, SimParams P, NbrList L, const DevState* st,
                                                    AdjArgs A, int n) {
   const int i = gtid();
-  double lee[2] = {0.0, 0.0};
+  double lee[3] = {0.0, 0.0, 0.0};  // d loss / d lee_evap, lee_cond, T_sat
   if (i < n) {
     const uint8_t ph = pv.s.phase[i];
     float4 pa = A.pa[i];
@@ -913,6 +922,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_src(PV pv, GridDev g, SimParams 
       if (ph == kLiquid) {
         const float h_out = P.h_fg + P.cp[1] * (Ts - P.T_ref);
         const float latent = h_out - P.cp[0] * (Ts - P.T_ref);
+        lee[2] = static_cast<double>(po.y * P.cp[1]);  // out_h = h_v(T_sat)
         if (br == 3) {
           pa.x += ompb * capf;
         } else if (br == 1) {
@@ -920,9 +930,11 @@ __global__ __launch_bounds__(kBlock) void k_adj_src(PV pv, GridDev g, SimParams 
           fab = ompb * dm_r / alpha;
           pa.y += ompb * P.lee_evap * m * alpha / Ts * dt;
           lee[0] = static_cast<double>(ompb * m * alpha * (T - Ts) / Ts * dt);
+          lee[2] -= static_cast<double>(ompb * P.lee_evap * m * alpha * T / (Ts * Ts) * dt);
         } else {
           pa.x += ompb * P.cp[0] * (T - Ts) / latent;
           pa.y += ompb * m * P.cp[0] / latent;
+          lee[2] -= static_cast<double>(ompb * (m * P.cp[0] + dm_e * (P.cp[1] - P.cp[0])) / latent);
         }
       } else {
         pa.y += po.y * P.cp[1];  // out_h = h(T) of the condensing vapour
@@ -933,6 +945,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_src(PV pv, GridDev g, SimParams 
           fab = ompb * dm_r / alpha;
           pa.y -= ompb * P.lee_cond * m * alpha / Ts * dt;
           lee[1] = static_cast<double>(ompb * m * alpha * (Ts - T) / Ts * dt);
+          lee[2] += static_cast<double>(ompb * P.lee_cond * m * alpha * T / (Ts * Ts) * dt);
         }
       }
       A.pa[i] = pa;
@@ -957,7 +970,34 @@ __global__ __launch_bounds__(kBlock) void k_adj_src(PV pv, GridDev g, SimParams 
       A.amax[i] = amax;
     }
   }
-  block_sums_store<2>(lee, A.partials + 9 * A.stride, A.stride);
+  block_sums_store<3>(lee, A.partials + 9 * A.stride, A.stride);
+}
+
+// Closed tank (sph_solver.cu:586-607): this step's T_sat came from the previous
+// step's end state, whose masses and temperatures are this taped state's:
+// p = p_sat (S_mT / v_ull) / (rho_v T_sat0), v_ull = V0 - S_l / rho_l,
+// T_sat = 1 / (1/T_sat0 - (R/h_fg) ln(p/p_sat)).  Rows 12..14: the inventory
+// partials (S_l, S_v, S_mT); the T_sat cotangent is row 11.  Writes the
+// cotangents of S_l and S_mT.
+__global__ __launch_bounds__(1024) void k_adj_sat(const DevState* st, AdjArgs A, int nb, int nb_inv) {
+  __shared__ double sh[1024];
+  const double tsb = ordered_sum(A.partials + 11 * A.stride, nb, sh);
+  const double m_liq = ordered_sum(A.partials + 12 * A.stride, nb_inv, sh);
+  const double m_vap = ordered_sum(A.partials + 13 * A.stride, nb_inv, sh);
+  const double mT = ordered_sum(A.partials + 14 * A.stride, nb_inv, sh);
+  if (threadIdx.x != 0) return;
+  double mlb = 0.0, mTb = 0.0;
+  const FluidConst fl = A.fluid;
+  const double v_u = st->fluid_volume0 - m_liq / fl.rho_liquid;
+  const double ratio = v_u > 0 && m_vap > 0 ? (mT / v_u) / (static_cast<double>(fl.rho_vapor) * fl.T_sat) : 0.0;
+  if (tsb != 0.0 && isfinite(ratio) && ratio > 0) {
+    const double pu = fl.p_sat * ratio, Ts = st->T_sat;
+    const double g = tsb * Ts * Ts * (static_cast<double>(fl.R_gas) / fl.h_fg) / pu;  // dT_sat / dp
+    mTb = g * pu / mT;
+    mlb = g * pu / (v_u * fl.rho_liquid);
+  }
+  A.satv[0] = mlb;
+  A.satv[1] = mTb;
 }
 
 // Normalisation pairs (wn_i = sum V_j W over the target phase; the cap's wr_max
@@ -1098,6 +1138,10 @@ void launch_adj_step(const PV& pv, const GridDev& g, const SimParams& P, const N
   }
   k_adj_wall<<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
   k_adj_fluid<<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
+  if (A.sat) {
+    const int nb_inv = launch_inventory(pv, st, n, A.partials + 12 * A.stride, A.stride, s);
+    k_adj_sat<<<1, 1024, 0, s>>>(st, A, nb, nb_inv);
+  }
   k_adj_density<<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
   // d loss / d acc = -(d loss / d g_eff): wall rows 0..2 + fluid rows 3..5
   k_adj_finish<<<1, 1024, 0, s>>>(A.partials, A.stride, nb, 0, 3, -1.0, grad_out, 3);

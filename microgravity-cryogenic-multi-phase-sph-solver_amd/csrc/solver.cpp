@@ -1046,7 +1046,7 @@ struct cryo_solver {
   DevBuf<float2> adj_pm2;
   DevBuf<uint8_t> adj_fl;
   DevBuf<int> adj_amax;
-  DevBuf<double> adj_hsum, adj_lee;
+  DevBuf<double> adj_hsum, adj_lee, adj_satv;
   Snap* tape_slot = nullptr;  // set: step_issue() records its post-sort state and its rates here
 
   // d2d between copy_begin() and copy_end() queues the copy; copy_end() issues
@@ -1107,7 +1107,8 @@ struct cryo_solver {
 
   bool adj_heat() const { return P.thermal != 0; }
   bool adj_pc() const { return cfg.phase_change != 0; }
-  AdjArgs adj_args(int cur, const uint32_t* perm, const double* S, bool csf, double* lee_out = nullptr) {
+  AdjArgs adj_args(int cur, const uint32_t* perm, const double* S, bool csf, double* lee_out = nullptr,
+                   bool sat_prev = false) {
     AdjArgs A{adj_c[cur][0].get(), adj_c[cur][1].get(), adj_c[cur][2].get(),
               adj_c[1 - cur][0].get(), adj_c[1 - cur][1].get(), adj_c[1 - cur][2].get(),
               adj_at.get(), adj_wcp.get(), adj_wcv.get(), adj_xr.get(), adj_vr.get(), adj_sb.get(),
@@ -1130,6 +1131,9 @@ struct cryo_solver {
       A.pdp = adj_pdp.get(), A.v1b = adj_v1b.get(), A.pm2 = adj_pm2.get(), A.pn = adj_pn.get();
       A.amax = adj_amax.get(), A.pV = adj_pV.get(), A.pr = adj_pr.get(), A.xp = adj_xp.get();
       A.lee_out = lee_out;
+      A.sat = sat_prev && cfg.thermo_mode == CRYO_CLOSED && has_vapor ? 1 : 0;
+      A.fluid = fluid_const();
+      A.satv = adj_satv.get();
     }
     return A;
   }
@@ -1167,8 +1171,6 @@ struct cryo_solver {
     if (c.surface_tension == CRYO_ST_CSF &&
         (c.csf_smooth_normals || c.csf_balanced || c.wetting_model != CRYO_WET_NORMAL_ROTATION))
       throw std::invalid_argument("adjoint: CSF only with unsmoothed normals, the classic force and normal rotation");
-    if ((c.thermal || c.phase_change) && c.thermo_mode == CRYO_CLOSED && has_vapor)
-      throw std::invalid_argument("adjoint: the closed-tank saturation feedback is not differentiated");
     if (c.transport_stress || c.sps_turbulence || c.particle_shifting)
       throw std::invalid_argument("adjoint: transport stress, SPS and shifting not differentiated");
     if (has_maneuver || frame_rot) throw std::invalid_argument("adjoint: frame rotation / manoeuvre table not allowed");
@@ -1203,7 +1205,8 @@ struct cryo_solver {
     const bool csf = cfg.surface_tension == CRYO_ST_CSF && has_vapor;
     const std::size_t ns = static_cast<std::size_t>(steps);
     fit(adj_acc, 3 * ns), fit(adj_S, 3 * ns), fit(adj_G, 3 * ns), fit(adj_T, 2 * ns + 2);
-    fit(adj_part, 11 * static_cast<std::size_t>(stride));
+    fit(adj_part, 15 * static_cast<std::size_t>(stride));
+    fit(adj_satv, 2);
     if (adj_pc()) fit(adj_lee, 2 * ns);
     CRYO_HIP(hipMemcpyAsync(adj_acc.get(), acc, sizeof(float) * 3 * ns, hipMemcpyHostToDevice, stream));
     const long long K = every;
@@ -1273,7 +1276,7 @@ struct cryo_solver {
           snap_load_rates(t);  // the step's sorted state and its rates
           if (mT) adj_pc_rerun(t);
           const AdjArgs A = adj_args(cur, t.perm.get(), adj_S.get() + 3 * k, csf,
-                                     adj_pc() ? adj_lee.get() + 2 * k : nullptr);
+                                     adj_pc() ? adj_lee.get() + 2 * k : nullptr, k > 0);
           launch_adj_step(view(), grid(), P, lists(), state.get(), A, n, adj_G.get() + 3 * k, adj_T.get() + 2 * k,
                           stream);
           check_launch();

@@ -1291,6 +1291,7 @@ struct Solver {
   struct TapeEntry {
     Particles ps;
     std::vector<u32> perm;
+    R T_sat = 0;  // the step's saturation temperature (a closed tank's changes every step)
   };
   TapeEntry* record = nullptr;
   double fluid_volume0 = 0;
@@ -1556,6 +1557,7 @@ struct Solver {
     if (record) {  // the adjoint's tape: the sorted, drifted state and the sort permutation
       record->ps = ps;
       record->perm.assign(grid.perm.begin(), grid.perm.begin() + ps.n);
+      record->T_sat = P.T_sat;
     }
     if (has_maneuver) {
       V3 a, w, wd;
@@ -1812,8 +1814,6 @@ struct Solver {
     if (c.surface_tension == CRYO_ST_CSF &&
         (c.csf_smooth_normals || c.csf_balanced || c.wetting_model != CRYO_WET_NORMAL_ROTATION))
       throw std::invalid_argument("adjoint: CSF only with unsmoothed normals, the classic force and normal rotation");
-    if ((c.thermal || c.phase_change) && c.thermo_mode == CRYO_CLOSED && has_vapor)
-      throw std::invalid_argument("adjoint: the closed-tank saturation feedback is not differentiated");
     if (c.transport_stress || c.sps_turbulence || c.particle_shifting)
       throw std::invalid_argument("adjoint: transport stress, SPS and shifting not differentiated");
     if (has_maneuver || P.frame_rot) throw std::invalid_argument("adjoint: frame rotation / manoeuvre table not allowed");
@@ -1941,7 +1941,7 @@ struct Solver {
   void pc_adjoint(const cryo_adj::Ctx<R>& c, const std::vector<V3>& vb, const std::vector<R>& mb,
                   const std::vector<R>& Tb, std::vector<cryo_adj::Vec<R>>& v1b, std::vector<R>& T1b,
                   std::vector<R>& m0b, std::vector<cryo_adj::Vec<R>>& xr, std::vector<R>& Vb, std::vector<R>& rb,
-                  std::vector<R>& fab, double* lee) {
+                  std::vector<R>& fab, double* lee, double& tsatb) {
     using namespace cryo_adj;
     using VR = Vec<R>;
     const int n = ps.n;
@@ -2172,6 +2172,7 @@ struct Solver {
           const R dm_e = m * P.cp[0] * (T - Ts) / latent;
           const R dm_c = P.max_transfer * m;
           const R d1 = dm_r < dm_e ? dm_r : dm_e;
+          tsatb += double(ohb[i] * P.cp[1]);  // out_h = h_v(T_sat)
           if (!(d1 < dm_c)) {
             m0b[i] += ompb * P.max_transfer;
           } else if (dm_r < dm_e) {
@@ -2179,9 +2180,11 @@ struct Solver {
             fab[i] += ompb * dm_r / alpha;
             T1b[i] += ompb * P.lee_evap * m * alpha / Ts * dt;
             lee[0] += double(ompb * m * alpha * (T - Ts) / Ts * dt);
+            tsatb -= double(ompb * P.lee_evap * m * alpha * T / (Ts * Ts) * dt);
           } else {
             m0b[i] += ompb * P.cp[0] * (T - Ts) / latent;
             T1b[i] += ompb * m * P.cp[0] / latent;
+            tsatb -= double(ompb * (m * P.cp[0] + dm_e * (P.cp[1] - P.cp[0])) / latent);
           }
         } else if (ph == V && T < Ts) {
           const R dm_r = P.lee_cond * m * alpha * (Ts - T) / Ts * dt;
@@ -2195,6 +2198,7 @@ struct Solver {
             fab[i] += ompb * dm_r / alpha;
             T1b[i] -= ompb * P.lee_cond * m * alpha / Ts * dt;
             lee[1] += double(ompb * m * alpha * (Ts - T) / Ts * dt);
+            tsatb += double(ompb * P.lee_cond * m * alpha * T / (Ts * Ts) * dt);
           }
         }
       }
@@ -2203,7 +2207,7 @@ struct Solver {
 
   V3 adjoint_rates(const std::vector<u32>& perm, V3 Ssum, std::vector<V3>& xb, std::vector<V3>& vb,
                    std::vector<V3>& ab, std::vector<V3>& tb, std::vector<R>& mb, std::vector<R>& Tb, double& theta_b,
-                   double* lee) {
+                   double* lee, bool sat_prev) {
     using namespace cryo_adj;
     using VR = Vec<R>;
     const int n = ps.n;
@@ -2225,7 +2229,8 @@ struct Solver {
     // phase change, then the kick: v1 = v_half + a dt/2, T1 = T + dTdt dt
     std::vector<VR> v1b(n, VR{0, 0, 0});
     std::vector<R> T1b(n, R(0)), mr(n, R(0)), Tr(n, R(0)), fab(n, R(0));
-    pc_adjoint(c, vb, mb, Tb, v1b, T1b, mr, xr, Vb, rb, fab, lee);
+    double tsatb = 0;  // cotangent of this step's saturation temperature
+    pc_adjoint(c, vb, mb, Tb, v1b, T1b, mr, xr, Vb, rb, fab, lee, tsatb);
     VR gb{0, 0, 0};
     for (int i = 0; i < n; ++i) {
       if (!is_fluid(ps.phase[i])) continue;
@@ -2493,10 +2498,32 @@ struct Solver {
     // drift and kick, then back through the cell sort: old[perm[i]] <- new[i]
     std::vector<V3> nx(n, V3{0, 0, 0}), nv(n, V3{0, 0, 0}), na(n, V3{0, 0, 0}), nt(n, V3{0, 0, 0});
     std::vector<R> nm(n, R(0)), nT(n, R(0));
+    // Closed tank: this step's T_sat came from the previous step's end state
+    // (sph_solver.cu:586-607), p = p_sat (sum_V m T / v_ull) / (rho_v T_sat0),
+    // v_ull = V0 - sum_L m / rho_l, T_sat = 1 / (1/T_sat0 - (R/h_fg) ln(p/p_sat)).
+    // That state's masses and temperatures are this taped state's.
+    double mlb = 0, mTb = 0;
+    if (sat_prev && tsatb != 0) {
+      double m_liq = 0, m_vap = 0, mT = 0;
+      for (int i = 0; i < n; ++i) {
+        if (ps.phase[i] == L) m_liq += double(ps.m[i]);
+        if (ps.phase[i] == V) m_vap += double(ps.m[i]), mT += double(ps.m[i]) * ps.T[i];
+      }
+      const double v_u = fluid_volume0 - m_liq / fluid.rho_liquid;
+      const double ratio = v_u > 0 && m_vap > 0 ? (mT / v_u) / (double(fluid.rho_vapor) * fluid.T_sat) : 0.0;
+      if (std::isfinite(ratio) && ratio > 0) {
+        const double pu = fluid.p_sat * ratio, Ts = P.T_sat;
+        const double g = tsatb * Ts * Ts * (double(fluid.R_gas) / fluid.h_fg) / pu;  // dT_sat/dp
+        mTb = g * pu / mT;
+        mlb = g * pu / (v_u * fluid.rho_liquid);
+      }
+    }
     for (int i = 0; i < n; ++i) {
       if (!is_fluid(ps.phase[i])) continue;
       nm[perm[i]] = mr[i];
       nT[perm[i]] = Tr[i];
+      if (ps.phase[i] == L) nm[perm[i]] += R(mlb);
+      if (ps.phase[i] == V) nm[perm[i]] += R(mTb * ps.T[i]), nT[perm[i]] += R(mTb * ps.m[i]);
       const VR X = vr(xb[i]) + xr[i];
       const VR Vh = v1b[i] + vrb[i] + X * dt;
       nx[perm[i]] = v3(X);
@@ -2554,6 +2581,7 @@ struct Solver {
       throw std::invalid_argument("adjoint: a phase-change merge or split inside the window is not differentiable");
     // reverse sweep over the taped states; the solver ends in the forward end state
     Particles end_state = ps;
+    const R T_sat_end = P.T_sat;
     std::vector<V3> xb(ps.n, zero), vb(ps.n, zero), ab(ps.n, zero), tb(ps.n, zero);
     std::vector<R> mb(ps.n, R(0)), Tb(ps.n, R(0));
     double theta_b = 0, lee[2] = {0, 0};
@@ -2563,8 +2591,10 @@ struct Solver {
       grid_build(grid, ps);  // already sorted: the identity permutation, the same cell table
       apply_frame(acc_of(k), zero, zero, origin);
       P.dt = R(cfg.fixed_dt);
+      P.T_sat = t.T_sat;
       evaluate_rates();
-      const V3 gb = adjoint_rates(t.perm, S[static_cast<std::size_t>(k)], xb, vb, ab, tb, mb, Tb, theta_b, lee);
+      const bool sat_prev = k > 0 && cfg.thermo_mode == CRYO_CLOSED && has_vapor && cfg.phase_change;
+      const V3 gb = adjoint_rates(t.perm, S[static_cast<std::size_t>(k)], xb, vb, ab, tb, mb, Tb, theta_b, lee, sat_prev);
       grad[3 * k] = -double(gb.x);
       grad[3 * k + 1] = -double(gb.y);
       grad[3 * k + 2] = -double(gb.z);
@@ -2590,6 +2620,7 @@ struct Solver {
       grad_lee[1] = lee[1];
     }
     ps = end_state;        // scratch included: a is the end state's
+    P.T_sat = T_sat_end;
     grid_build(grid, ps);  // sorted already: the identity permutation
     if (steps > 0) apply_frame(acc_of(steps - 1), zero, zero, origin);
   }
