@@ -965,10 +965,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_src(PV pv, GridDev g, SimParams 
         sab = -fab * (so / sa) / sa;
       }
     }
-    if (i < n) {
-      A.pn[i] = make_float4(wnb, wrb, sab, sob);
-      A.amax[i] = amax;
-    }
+    A.pn[i] = make_float4(wnb, wrb, sab, sob);
+    A.amax[i] = amax;
   }
   block_sums_store<3>(lee, A.partials + 9 * A.stride, A.stride);
 }

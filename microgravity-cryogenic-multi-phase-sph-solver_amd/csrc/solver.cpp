@@ -988,10 +988,11 @@ struct cryo_solver {
     float4* v[6];
     float* f[14];
     scratch_fields(v, f);
-    copy_begin();
-    for (int k = 0; k < 6; ++k) fit(z.sv[k], n), d2d(z.sv[k].get(), v[k], n);
-    for (int k = 0; k < 14; ++k) fit(z.sf[k], n), d2d(z.sf[k].get(), f[k], n);
-    copy_end();
+    {
+      CopyBatch cb(*this);
+      for (int k = 0; k < 6; ++k) fit(z.sv[k], n), d2d(z.sv[k].get(), v[k], n);
+      for (int k = 0; k < 14; ++k) fit(z.sf[k], n), d2d(z.sf[k].get(), f[k], n);
+    }
   }
   // A taped step's rates evaluation, after evaluate_rates(): the densities and
   // normals it wrote, every scratch field, the neighbour lists, the cell table
@@ -1001,27 +1002,29 @@ struct cryo_solver {
     const Persist q = persist(set);
     const Scratch t = view().t;
     const std::size_t nl = static_cast<std::size_t>(kListK) * ((static_cast<std::size_t>(n) + 63) / 64 * 64);
-    copy_begin();
-    d2d(z.rho.get(), q.rho, n), d2d(z.nrm.get(), q.nrm, n);
-    snap_save_scratch(z);
-    fit(z.nrec, n), d2d(z.nrec.get(), t.nrec, n), fit(z.vol, n), d2d(z.vol.get(), t.vol, n);
-    fit(z.lidx, nl), d2d(z.lidx.get(), nbr_idx.get(), static_cast<int>(nl));
-    fit(z.lcnt, n), d2d(z.lcnt.get(), nbr_count.get(), n);
-    fit(z.coff, g.num_cells + 2), d2d(z.coff.get(), cell_off.get(), g.num_cells + 2);
-    fit(z.cflags, g.num_cells), d2d(z.cflags.get(), cell_flags.get(), g.num_cells);
-    d2d(z.st.get(), state.get(), 1);
-    copy_end();
+    {
+      CopyBatch cb(*this);
+      d2d(z.rho.get(), q.rho, n), d2d(z.nrm.get(), q.nrm, n);
+      snap_save_scratch(z);
+      fit(z.nrec, n), d2d(z.nrec.get(), t.nrec, n), fit(z.vol, n), d2d(z.vol.get(), t.vol, n);
+      fit(z.lidx, nl), d2d(z.lidx.get(), nbr_idx.get(), static_cast<int>(nl));
+      fit(z.lcnt, n), d2d(z.lcnt.get(), nbr_count.get(), n);
+      fit(z.coff, g.num_cells + 2), d2d(z.coff.get(), cell_off.get(), g.num_cells + 2);
+      fit(z.cflags, g.num_cells), d2d(z.cflags.get(), cell_flags.get(), g.num_cells);
+      d2d(z.st.get(), state.get(), 1);
+    }
   }
   void snap_load_rates(const Snap& z) {
     const Scratch t = view().t;
     const std::size_t nl = static_cast<std::size_t>(kListK) * ((static_cast<std::size_t>(z.n) + 63) / 64 * 64);
-    copy_begin();
-    snap_load(z, false);
-    snap_load_scratch(z);
-    d2d(t.nrec, z.nrec.get(), z.n), d2d(t.vol, z.vol.get(), z.n);
-    d2d(nbr_idx.get(), z.lidx.get(), static_cast<int>(nl)), d2d(nbr_count.get(), z.lcnt.get(), z.n);
-    d2d(cell_off.get(), z.coff.get(), g.num_cells + 2), d2d(cell_flags.get(), z.cflags.get(), g.num_cells);
-    copy_end();
+    {
+      CopyBatch cb(*this);
+      snap_load(z, false);
+      snap_load_scratch(z);
+      d2d(t.nrec, z.nrec.get(), z.n), d2d(t.vol, z.vol.get(), z.n);
+      d2d(nbr_idx.get(), z.lidx.get(), static_cast<int>(nl)), d2d(nbr_count.get(), z.lcnt.get(), z.n);
+      d2d(cell_off.get(), z.coff.get(), g.num_cells + 2), d2d(cell_flags.get(), z.cflags.get(), g.num_cells);
+    }
     lists_valid = true;
     tiles_ok = true;
   }
@@ -1029,10 +1032,11 @@ struct cryo_solver {
     float4* v[6];
     float* f[14];
     scratch_fields(v, f);
-    copy_begin();
-    for (int k = 0; k < 6; ++k) d2d(v[k], z.sv[k].get(), z.n);
-    for (int k = 0; k < 14; ++k) d2d(f[k], z.sf[k].get(), z.n);
-    copy_end();
+    {
+      CopyBatch cb(*this);
+      for (int k = 0; k < 6; ++k) d2d(v[k], z.sv[k].get(), z.n);
+      for (int k = 0; k < 14; ++k) d2d(f[k], z.sf[k].get(), z.n);
+    }
   }
   std::vector<std::unique_ptr<Snap>> adj_ckpt, adj_tape;
   std::unique_ptr<Snap> adj_end;
@@ -1054,11 +1058,18 @@ struct cryo_solver {
   std::vector<CopyItem> copy_q;
   int copy_depth = 0;
   void copy_begin() { ++copy_depth; }
-  void copy_end() {
+  void copy_end() {  // launch errors surface at the next check_launch(); nothing here throws
     if (--copy_depth > 0) return;
     if (!copy_q.empty()) launch_copy_batch(copy_q.data(), static_cast<int>(copy_q.size()), stream);
     copy_q.clear();
   }
+  struct CopyBatch {  // copy_begin() ... copy_end(), also when the scope unwinds
+    cryo_solver& z;
+    explicit CopyBatch(cryo_solver& s) : z(s) { z.copy_begin(); }
+    ~CopyBatch() { z.copy_end(); }
+    CopyBatch(const CopyBatch&) = delete;
+    CopyBatch& operator=(const CopyBatch&) = delete;
+  };
   template <class T>
   void d2d(T* dst, const T* src, int count) {
     if (count <= 0) return;
@@ -1080,26 +1091,28 @@ struct cryo_solver {
     fit(z.phase, m), fit(z.flags, m), fit(z.id, m), fit(z.st, 1);
     if (with_a) fit(z.a, m), fit(z.ab, m);
     if (perm) fit(z.perm, m);
-    copy_begin();
-    d2d(z.x.get(), q.x, m), d2d(z.v.get(), q.v, m), d2d(z.vt.get(), q.vt, m), d2d(z.nrm.get(), q.nrm, m);
-    d2d(z.m.get(), q.m, m), d2d(z.m0.get(), q.m0, m), d2d(z.rho.get(), q.rho, m), d2d(z.T.get(), q.T, m);
-    d2d(z.phase.get(), q.phase, m), d2d(z.flags.get(), q.flags, m), d2d(z.id.get(), q.id, m);
-    d2d(z.st.get(), state.get(), 1);
-    if (with_a) d2d(z.a.get(), a.get(), m), d2d(z.ab.get(), ab.get(), m);
-    if (perm) d2d(z.perm.get(), perm, m);
-    copy_end();
+    {
+      CopyBatch cb(*this);
+      d2d(z.x.get(), q.x, m), d2d(z.v.get(), q.v, m), d2d(z.vt.get(), q.vt, m), d2d(z.nrm.get(), q.nrm, m);
+      d2d(z.m.get(), q.m, m), d2d(z.m0.get(), q.m0, m), d2d(z.rho.get(), q.rho, m), d2d(z.T.get(), q.T, m);
+      d2d(z.phase.get(), q.phase, m), d2d(z.flags.get(), q.flags, m), d2d(z.id.get(), q.id, m);
+      d2d(z.st.get(), state.get(), 1);
+      if (with_a) d2d(z.a.get(), a.get(), m), d2d(z.ab.get(), ab.get(), m);
+      if (perm) d2d(z.perm.get(), perm, m);
+    }
     z.n = m;
   }
   void snap_load(const Snap& z, bool with_a) {
     const int m = z.n;
     const Persist q = persist(set);
-    copy_begin();
-    d2d(q.x, z.x.get(), m), d2d(q.v, z.v.get(), m), d2d(q.vt, z.vt.get(), m), d2d(q.nrm, z.nrm.get(), m);
-    d2d(q.m, z.m.get(), m), d2d(q.m0, z.m0.get(), m), d2d(q.rho, z.rho.get(), m), d2d(q.T, z.T.get(), m);
-    d2d(q.phase, z.phase.get(), m), d2d(q.flags, z.flags.get(), m), d2d(q.id, z.id.get(), m);
-    d2d(state.get(), z.st.get(), 1);
-    if (with_a) d2d(a.get(), z.a.get(), m), d2d(ab.get(), z.ab.get(), m);
-    copy_end();
+    {
+      CopyBatch cb(*this);
+      d2d(q.x, z.x.get(), m), d2d(q.v, z.v.get(), m), d2d(q.vt, z.vt.get(), m), d2d(q.nrm, z.nrm.get(), m);
+      d2d(q.m, z.m.get(), m), d2d(q.m0, z.m0.get(), m), d2d(q.rho, z.rho.get(), m), d2d(q.T, z.T.get(), m);
+      d2d(q.phase, z.phase.get(), m), d2d(q.flags, z.flags.get(), m), d2d(q.id, z.id.get(), m);
+      d2d(state.get(), z.st.get(), 1);
+      if (with_a) d2d(a.get(), z.a.get(), m), d2d(ab.get(), z.ab.get(), m);
+    }
     n = m;
     lists_valid = false;
     tiles_ok = false;
@@ -1145,21 +1158,25 @@ struct cryo_solver {
     const Persist q = persist(set);
     if (!adj_pc()) return;
     launch_kick_sources(view(), P, state.get(), n, true, P.thermal != 0, true, stream);
-    copy_begin();
-    d2d(adj_v1.get(), q.v, n), d2d(adj_T1.get(), q.T, n), d2d(adj_fl.get(), q.flags, n);
-    copy_end();
+    {
+      CopyBatch cb(*this);
+      d2d(adj_v1.get(), q.v, n), d2d(adj_T1.get(), q.T, n), d2d(adj_fl.get(), q.flags, n);
+    }
     launch_pc_normalize(view(), grid(), P, lists(), n, partials.get(), stride, stream);
-    copy_begin();
-    d2d(adj_om.get(), view().t.out_m, n), d2d(adj_oh.get(), view().t.out_h, n);
-    copy_end();
+    {
+      CopyBatch cb(*this);
+      d2d(adj_om.get(), view().t.out_m, n), d2d(adj_oh.get(), view().t.out_h, n);
+    }
     launch_pc_apply(view(), grid(), P, lists(), n, stream);
-    copy_begin();
-    d2d(adj_m2.get(), q.m, n), d2d(adj_T2.get(), q.T, n);
-    copy_end();
-    copy_begin();
-    d2d(q.v, t.v.get(), n), d2d(q.T, t.T.get(), n), d2d(q.m, t.m.get(), n);
-    d2d(q.flags, t.flags.get(), n), d2d(q.phase, t.phase.get(), n);
-    copy_end();
+    {
+      CopyBatch cb(*this);
+      d2d(adj_m2.get(), q.m, n), d2d(adj_T2.get(), q.T, n);
+    }
+    {
+      CopyBatch cb(*this);
+      d2d(q.v, t.v.get(), n), d2d(q.T, t.T.get(), n), d2d(q.m, t.m.get(), n);
+      d2d(q.flags, t.flags.get(), n), d2d(q.phase, t.phase.get(), n);
+    }
   }
 
   void check_adjoint_scope() const {
@@ -1191,8 +1208,6 @@ struct cryo_solver {
     if (!loss) throw std::invalid_argument("adjoint: loss output is required");
     if (grad_theta && !grad) throw std::invalid_argument("adjoint: the contact-angle gradient needs grad_frame_acc");
     if (grad_lee && !grad) throw std::invalid_argument("adjoint: the Lee-coefficient gradient needs grad_frame_acc");
-    copy_depth = 0;  // no batch left open by an earlier failure
-    copy_q.clear();
     const bool fresh = !initialized;  // the start-up rates (and their a) belong to this run
     ensure_initialized();
     *loss = 0.0;
