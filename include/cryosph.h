@@ -220,12 +220,24 @@ int cryo_solver_autotune(cryo_solver* s, int32_t trial_steps, int32_t* variant, 
  * masses and temperatures are state), kick-drift-kick.  Requires
  * fixed_dt > 0, the Wendland C2 kernel, density summation, CSF only with
  * unsmoothed normals, the classic force and normal rotation, no transport
- * stress, SPS or shifting, and no frame rotation or manoeuvre table; a window
- * in which a particle splits or merges is not differentiable
- * (CRYO_ERR_INVALID_ARGUMENT otherwise). */
+ * stress, SPS or shifting, no manoeuvre table and no rotating frame (see
+ * cryo_solver_adjoint_slosh_frame); a window in which a particle splits or
+ * merges is not differentiable (CRYO_ERR_INVALID_ARGUMENT otherwise). */
 int cryo_solver_adjoint_slosh(cryo_solver* s, int64_t steps, const float* frame_acc, int32_t checkpoint_every,
                               double* loss, double* grad_frame_acc, double* wall_load, double* grad_contact_angle,
                               double* grad_lee);
+/* Extension (no counterpart in the reference): cryo_solver_adjoint_slosh in a
+ * rotating frame.  Step k also sets the frame's angular velocity
+ * frame_omega[3k..3k+2] and its rate frame_omega_dot[3k..3k+2] (either may be
+ * NULL: zero), as cryo_solver_set_frame_motion would (a -= wd x r + w x (w x r)
+ * + 2 w x v about the frame origin), and grad_frame_omega /
+ * grad_frame_omega_dot (NULL: not returned; they need grad_frame_acc) receive
+ * d loss / d those inputs.  A rotating frame set on the solver beforehand is
+ * replaced by these inputs; cryo_solver_adjoint_slosh rejects one. */
+int cryo_solver_adjoint_slosh_frame(cryo_solver* s, int64_t steps, const float* frame_acc, const float* frame_omega,
+                                    const float* frame_omega_dot, int32_t checkpoint_every, double* loss,
+                                    double* grad_frame_acc, double* grad_frame_omega, double* grad_frame_omega_dot,
+                                    double* wall_load, double* grad_contact_angle, double* grad_lee);
 
 /* Solver::time / step_count / last_dt / num_particles (sph_solver.hpp:153-163). */
 int cryo_solver_time(cryo_solver* s, double* t);

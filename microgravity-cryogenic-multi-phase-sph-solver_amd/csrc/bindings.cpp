@@ -817,6 +817,39 @@ PYBIND11_MODULE(CryoSPH, m) {
          "d frame_acc, wall load per step, d loss / d contact angle in degrees, (d loss / d "
          "lee_evaporation, d loss / d lee_condensation)), loss = sum_k dt |load_k,xy|^2; without gradient "
          "the derivatives are None.")
+      // Extension (cryo_solver_adjoint_slosh_frame): no counterpart in the reference.
+      .def("adjoint_slosh_frame", [](Solver& S, DArr acc, DArr omega, DArr omega_dot, int checkpoint_every,
+                                     bool gradient) -> py::tuple {
+        for (const DArr* a : {&acc, &omega, &omega_dot})
+          if (a->ndim() != 2 || a->shape(1) != 3 || a->shape(0) != acc.shape(0))
+            throw py::value_error("frame_acc, frame_omega and frame_omega_dot must have shape (steps, 3)");
+        const py::ssize_t k = acc.shape(0);
+        std::vector<float> a(3 * static_cast<std::size_t>(k)), w(a.size()), wd(a.size());
+        for (py::ssize_t i = 0; i < 3 * k; ++i) {
+          a[i] = static_cast<float>(acc.data()[i]);
+          w[i] = static_cast<float>(omega.data()[i]);
+          wd[i] = static_cast<float>(omega_dot.data()[i]);
+        }
+        py::array_t<double> load({k, py::ssize_t{3}}), grad({k, py::ssize_t{3}}), gw({k, py::ssize_t{3}}),
+            gwd({k, py::ssize_t{3}});
+        double loss = 0.0, dtheta = 0.0, dlee[2] = {0.0, 0.0};
+        int r;
+        {
+          py::gil_scoped_release rel;
+          r = cryo_solver_adjoint_slosh_frame(S.s, k, a.data(), w.data(), wd.data(), checkpoint_every, &loss,
+                                              gradient ? grad.mutable_data() : nullptr,
+                                              gradient ? gw.mutable_data() : nullptr,
+                                              gradient ? gwd.mutable_data() : nullptr, load.mutable_data(),
+                                              gradient ? &dtheta : nullptr, gradient ? dlee : nullptr);
+        }
+        check(r);
+        if (!gradient) return py::make_tuple(loss, py::none(), py::none(), py::none(), load, py::none(), py::none());
+        return py::make_tuple(loss, grad, gw, gwd, load, dtheta, py::make_tuple(dlee[0], dlee[1]));
+      }, py::arg("frame_acc"), py::arg("frame_omega"), py::arg("frame_omega_dot"), py::arg("checkpoint_every") = 20,
+         py::arg("gradient") = true,
+         "adjoint_slosh in a rotating frame: step k also sets the frame's angular velocity and rate; returns "
+         "(loss, d/d frame_acc, d/d frame_omega, d/d frame_omega_dot, wall loads, d/d contact angle, "
+         "(d/d lee_evaporation, d/d lee_condensation)).")
       .def("request_telemetry", [](Solver& S) {
         const int r = cryo_solver_request_telemetry(S.s);
         check(r);

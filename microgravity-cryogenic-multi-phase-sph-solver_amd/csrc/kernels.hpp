@@ -134,6 +134,8 @@ struct AdjArgs {
   int sat;
   FluidConst fluid;
   double* satv;    // {sum_L m bar, sum_V m T bar} of that feedback
+  // rotating frame: this step's d loss / d (omega, omega_dot), 6 doubles (null: not wanted)
+  double* wout;
 };
 // Several independent device-to-device copies in one launch (16-byte aligned
 // pointers): the adjoint's snapshots.
@@ -144,7 +146,7 @@ struct CopyItem {
 };
 void launch_copy_batch(const CopyItem* items, int count, hipStream_t s);
 void launch_adj_fill(float4* a, float4* b, float4* c, int n, hipStream_t s);
-void launch_adj_set_frame(DevState* st, const float* acc, hipStream_t s);
+void launch_adj_set_frame(DevState* st, const float* acc, const float* omega, const float* omega_dot, hipStream_t s);
 void launch_adj_load_sum(const PV& pv, int n, double* partials, int stride, double* out, hipStream_t s);
 void launch_adj_step(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, const DevState* st,
                      const AdjArgs& A, int n, double* grad_out, double* theta_out, hipStream_t s);

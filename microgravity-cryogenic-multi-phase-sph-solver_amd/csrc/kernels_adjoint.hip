@@ -115,10 +115,20 @@ __global__ __launch_bounds__(kBlock) void k_adj_fill(float4* a, float4* b, float
   c[i] = z;
 }
 
-__global__ __launch_bounds__(kBlock) void k_adj_set_frame(DevState* st, const float* acc) {
+// The step's frame (apply_frame, sph_solver.cu:660-667): acceleration and,
+// when given, angular velocity and its rate.
+__global__ __launch_bounds__(kBlock) void k_adj_set_frame(DevState* st, const float* acc, const float* omega,
+                                                         const float* omega_dot) {
   const V3 a{acc[0], acc[1], acc[2]};
   st->frame_acc = a;
   st->gravity = st->cfg_gravity - a;
+  if (omega || omega_dot) {
+    const V3 w = omega ? V3{omega[0], omega[1], omega[2]} : V3{0, 0, 0};
+    const V3 wd = omega_dot ? V3{omega_dot[0], omega_dot[1], omega_dot[2]} : V3{0, 0, 0};
+    st->omega = w;
+    st->omega_dot = wd;
+    st->frame_rot = (norm2(w) > 0.0f || norm2(wd) > 0.0f) ? 1 : 0;
+  }
 }
 
 // Sum of fw over fluid particles: block partials in rows [row, row + 3).
@@ -167,6 +177,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_prep(PV pv, SimParams P, AdjArgs
   const int i = gtid();
   double g[3] = {0.0, 0.0, 0.0};
   double hw = 0.0;
+  double wg[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // d loss / d (omega, omega_dot)
   if (i < n) {
     float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (is_fluid(pv.s.phase[i])) {
@@ -176,6 +187,13 @@ __global__ __launch_bounds__(kBlock) void k_adj_prep(PV pv, SimParams P, AdjArgs
       g[0] = r.x;  // a = g_eff + ...
       g[1] = r.y;
       g[2] = r.z;
+      if (A.wout) {  // a -= wd x rr + w x (w x rr) + 2 w x v, rr = x - origin
+        const V3 At{r.x, r.y, r.z}, w = st->omega;
+        const V3 rr = ld3(pv.s.x[i]) - st->origin, vv = ld3(pv.s.v[i]);
+        const V3 dw = At * (-dot(w, rr)) - rr * dot(At, w) + w * (2.0f * dot(At, rr)) + cross(At, vv) * 2.0f;
+        const V3 dwd = cross(At, rr);
+        wg[0] = dw.x, wg[1] = dw.y, wg[2] = dw.z, wg[3] = dwd.x, wg[4] = dwd.y, wg[5] = dwd.z;
+      }
     }
     A.at[i] = r;
     if (A.heat) {  // T1 = T + dTdt dt, dTdt = q / (m cp)
@@ -199,6 +217,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_prep(PV pv, SimParams P, AdjArgs
     }
   }
   block_sums_store<3>(g, A.partials + 3 * A.stride, A.stride);
+  if (A.wout) block_sums_store<6>(wg, A.partials + 15 * A.stride, A.stride);
   if (A.heat) {
     double h1[1] = {hw};
     block_sums_store<1>(h1, A.partials + 8 * A.stride, A.stride);
@@ -578,6 +597,11 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
   }
   const float rhob = rb + pb * dpdr;
   A.sb[i] = rhob * (vap ? I.m : pv.s.m0[i]) + tsb;  // rho = m0 sigma (liquid), m sigma (vapour)
+  if (st->frame_rot) {  // a -= wd x r + w x (w x r) + 2 w x v
+    const V3 At{cA.x, cA.y, cA.z}, w = st->omega, wd = st->omega_dot;
+    xr += av(cross(wd, At) - cross(w, cross(w, At)));
+    vr += av(cross(w, At) * 2.0f);
+  }
   if (A.mT) {
     const float4 pa = A.pc ? A.pa[i] : make_float4(A.cm[i], A.cT[i], 0.0f, 0.0f);  // (own shares)
     mb += pa.x + Vbk / I.rho;
@@ -1102,7 +1126,9 @@ void launch_adj_fill(float4* a, float4* b, float4* c, int n, hipStream_t s) {
   if (n <= 0) return;
   k_adj_fill<<<blocks_for(n), kBlock, 0, s>>>(a, b, c, n);
 }
-void launch_adj_set_frame(DevState* st, const float* acc, hipStream_t s) { k_adj_set_frame<<<1, 1, 0, s>>>(st, acc); }
+void launch_adj_set_frame(DevState* st, const float* acc, const float* omega, const float* omega_dot, hipStream_t s) {
+  k_adj_set_frame<<<1, 1, 0, s>>>(st, acc, omega, omega_dot);
+}
 void launch_adj_load_sum(const PV& pv, int n, double* partials, int stride, double* out, hipStream_t s) {
   const int nb = blocks_for(n > 0 ? n : 1);
   k_adj_load_sum<<<nb, kBlock, 0, s>>>(pv, n, partials, stride);
@@ -1143,6 +1169,7 @@ void launch_adj_step(const PV& pv, const GridDev& g, const SimParams& P, const N
   k_adj_density<<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
   // d loss / d acc = -(d loss / d g_eff): wall rows 0..2 + fluid rows 3..5
   k_adj_finish<<<1, 1024, 0, s>>>(A.partials, A.stride, nb, 0, 3, -1.0, grad_out, 3);
+  if (A.wout) k_adj_finish<<<1, 1024, 0, s>>>(A.partials, A.stride, nb, 15, -1, 1.0, A.wout, 6);
 }
 
 }  // namespace cryo

@@ -1050,7 +1050,8 @@ struct cryo_solver {
   DevBuf<float2> adj_pm2;
   DevBuf<uint8_t> adj_fl;
   DevBuf<int> adj_amax;
-  DevBuf<double> adj_hsum, adj_lee, adj_satv;
+  DevBuf<double> adj_hsum, adj_lee, adj_satv, adj_W;
+  DevBuf<float> adj_w, adj_wd;
   Snap* tape_slot = nullptr;  // set: step_issue() records its post-sort state and its rates here
 
   // d2d between copy_begin() and copy_end() queues the copy; copy_end() issues
@@ -1179,7 +1180,7 @@ struct cryo_solver {
     }
   }
 
-  void check_adjoint_scope() const {
+  void check_adjoint_scope(bool explicit_rotation = false) const {
     const cryo_config& c = cfg;
     if (!(c.fixed_dt > 0.0f)) throw std::invalid_argument("adjoint: needs fixed_dt > 0");
     if (c.kernel != CRYO_KERNEL_WENDLAND_C2) throw std::invalid_argument("adjoint: Wendland C2 kernel only");
@@ -1190,7 +1191,9 @@ struct cryo_solver {
       throw std::invalid_argument("adjoint: CSF only with unsmoothed normals, the classic force and normal rotation");
     if (c.transport_stress || c.sps_turbulence || c.particle_shifting)
       throw std::invalid_argument("adjoint: transport stress, SPS and shifting not differentiated");
-    if (has_maneuver || frame_rot) throw std::invalid_argument("adjoint: frame rotation / manoeuvre table not allowed");
+    if (has_maneuver) throw std::invalid_argument("adjoint: manoeuvre table not allowed (the frame is per-step input)");
+    if (frame_rot && !explicit_rotation)
+      throw std::invalid_argument("adjoint: a rotating frame is differentiated only with per-step rotation inputs");
   }
 
   // loss = sum_k dt (S_x^2 + S_y^2), S_k = sum of fw over fluid particles
@@ -1200,14 +1203,21 @@ struct cryo_solver {
   // re-run forward with its post-sort states taped, then walked back one step
   // at a time (the rates but the forces pass re-evaluated on the taped
   // state).  The solver ends in the forward end state.
+  // omega / omega_dot (optional, 3 per step): the frame's angular velocity and
+  // its rate per step (a -= wd x r + w x (w x r) + 2 w x v); grad_omega /
+  // grad_omega_dot (optional) receive their derivatives.
   void adjoint_slosh(long long steps, const float* acc, int every, double* loss, double* grad, double* force,
-                     double* grad_theta, double* grad_lee) {
-    check_adjoint_scope();
+                     double* grad_theta, double* grad_lee, const float* omega = nullptr,
+                     const float* omega_dot = nullptr, double* grad_omega = nullptr, double* grad_omega_dot = nullptr) {
+    const bool rot = omega || omega_dot;
+    check_adjoint_scope(rot);
     if (steps < 0) throw std::invalid_argument("adjoint: steps must be >= 0");
     if (every <= 0) throw std::invalid_argument("adjoint: checkpoint_every must be > 0");
     if (!loss) throw std::invalid_argument("adjoint: loss output is required");
     if (grad_theta && !grad) throw std::invalid_argument("adjoint: the contact-angle gradient needs grad_frame_acc");
     if (grad_lee && !grad) throw std::invalid_argument("adjoint: the Lee-coefficient gradient needs grad_frame_acc");
+    if ((grad_omega || grad_omega_dot) && !grad)
+      throw std::invalid_argument("adjoint: the rotation gradients need grad_frame_acc");
     const bool fresh = !initialized;  // the start-up rates (and their a) belong to this run
     ensure_initialized();
     *loss = 0.0;
@@ -1220,10 +1230,28 @@ struct cryo_solver {
     const bool csf = cfg.surface_tension == CRYO_ST_CSF && has_vapor;
     const std::size_t ns = static_cast<std::size_t>(steps);
     fit(adj_acc, 3 * ns), fit(adj_S, 3 * ns), fit(adj_G, 3 * ns), fit(adj_T, 2 * ns + 2);
-    fit(adj_part, 15 * static_cast<std::size_t>(stride));
+    fit(adj_part, 21 * static_cast<std::size_t>(stride));
     fit(adj_satv, 2);
     if (adj_pc()) fit(adj_lee, 2 * ns);
     CRYO_HIP(hipMemcpyAsync(adj_acc.get(), acc, sizeof(float) * 3 * ns, hipMemcpyHostToDevice, stream));
+    const float* dw = nullptr;
+    const float* dwd = nullptr;
+    if (omega) {
+      fit(adj_w, 3 * ns);
+      CRYO_HIP(hipMemcpyAsync(adj_w.get(), omega, sizeof(float) * 3 * ns, hipMemcpyHostToDevice, stream));
+      dw = adj_w.get();
+    }
+    if (omega_dot) {
+      fit(adj_wd, 3 * ns);
+      CRYO_HIP(hipMemcpyAsync(adj_wd.get(), omega_dot, sizeof(float) * 3 * ns, hipMemcpyHostToDevice, stream));
+      dwd = adj_wd.get();
+    }
+    const bool want_w = grad && (grad_omega || grad_omega_dot);
+    if (want_w) fit(adj_W, 6 * ns);
+    auto set_frame = [&](long long k) {
+      launch_adj_set_frame(state.get(), adj_acc.get() + 3 * k, dw ? dw + 3 * k : nullptr, dwd ? dwd + 3 * k : nullptr,
+                           stream);
+    };
     const long long K = every;
     const long long nseg = (steps + K - 1) / K;
     const long long k_last = (nseg - 1) * K;  // the last segment is taped by the forward pass itself
@@ -1237,7 +1265,7 @@ struct cryo_solver {
     }
     for (long long k = 0; k < steps; ++k) {
       if (grad && k % K == 0) snap_save(*adj_ckpt[static_cast<std::size_t>(k / K)], true, nullptr);
-      launch_adj_set_frame(state.get(), adj_acc.get() + 3 * k, stream);
+      set_frame(k);
       tape_slot = (grad && k >= k_last) ? adj_tape[static_cast<std::size_t>(k - k_last)].get() : nullptr;
       step_issue();
       tape_slot = nullptr;
@@ -1280,7 +1308,7 @@ struct cryo_solver {
         const long long k0 = seg * K, k1 = std::min(steps, k0 + K);
         if (seg < nseg - 1) snap_load(*adj_ckpt[static_cast<std::size_t>(seg)], true);
         for (long long k = k0; k < k1 && seg < nseg - 1; ++k) {
-          launch_adj_set_frame(state.get(), adj_acc.get() + 3 * k, stream);
+          set_frame(k);
           tape_slot = adj_tape[static_cast<std::size_t>(k - k0)].get();
           step_issue();
           tape_slot = nullptr;
@@ -1290,8 +1318,9 @@ struct cryo_solver {
           const Snap& t = *adj_tape[static_cast<std::size_t>(k - k0)];
           snap_load_rates(t);  // the step's sorted state and its rates
           if (mT) adj_pc_rerun(t);
-          const AdjArgs A = adj_args(cur, t.perm.get(), adj_S.get() + 3 * k, csf,
-                                     adj_pc() ? adj_lee.get() + 2 * k : nullptr, k > 0);
+          AdjArgs A = adj_args(cur, t.perm.get(), adj_S.get() + 3 * k, csf,
+                               adj_pc() ? adj_lee.get() + 2 * k : nullptr, k > 0);
+          A.wout = want_w ? adj_W.get() + 6 * k : nullptr;
           launch_adj_step(view(), grid(), P, lists(), state.get(), A, n, adj_G.get() + 3 * k, adj_T.get() + 2 * k,
                           stream);
           check_launch();
@@ -1319,10 +1348,16 @@ struct cryo_solver {
     const bool start_term = fresh && csf && grad_theta;
     if (grad && csf) CRYO_HIP(hipMemcpyAsync(T.data(), adj_T.get(), sizeof(double) * (2 * ns + (start_term ? 2 : 0)),
                                              hipMemcpyDeviceToHost, stream));
-    std::vector<double> lee(2 * ns, 0.0);
+    std::vector<double> lee(2 * ns, 0.0), W(want_w ? 6 * ns : 0);
     if (grad && adj_pc())
       CRYO_HIP(hipMemcpyAsync(lee.data(), adj_lee.get(), sizeof(double) * 2 * ns, hipMemcpyDeviceToHost, stream));
+    if (want_w) CRYO_HIP(hipMemcpyAsync(W.data(), adj_W.get(), sizeof(double) * 6 * ns, hipMemcpyDeviceToHost, stream));
     CRYO_HIP(hipStreamSynchronize(stream));
+    for (std::size_t q = 0; q < (want_w ? ns : 0); ++q)
+      for (int c = 0; c < 3; ++c) {
+        if (grad_omega) grad_omega[3 * q + c] = W[6 * q + c];
+        if (grad_omega_dot) grad_omega_dot[3 * q + c] = W[6 * q + 3 + c];
+      }
     if (grad_lee)
       for (std::size_t q = 0; q < ns; ++q) grad_lee[0] += lee[2 * q], grad_lee[1] += lee[2 * q + 1];
     if (grad_theta) {  // d loss / d theta = (cos_bar (-sin theta) + sin_bar cos theta) pi / 180
@@ -1337,6 +1372,11 @@ struct cryo_solver {
     *loss = Lsum;
     if (force) std::memcpy(force, S.data(), sizeof(double) * 3 * ns);
     sync_state();
+    // the host's frame is the last step's, as on the device
+    frame_acc = hs->frame_acc;
+    frame_omega = hs->omega;
+    frame_omega_dot = hs->omega_dot;
+    frame_rot = hs->frame_rot;
   }
 
   // ----------------------------------------------------------- autotune
@@ -1859,6 +1899,15 @@ int cryo_solver_adjoint_slosh(cryo_solver* s, int64_t steps, const float* frame_
   if (steps > 0 && !frame_acc) return fail("frame_acc is required", CRYO_ERR_INVALID_ARGUMENT);
   CRYO_GUARD(S(s).adjoint_slosh(steps, frame_acc, checkpoint_every, loss, grad_frame_acc, wall_load,
                                 grad_contact_angle, grad_lee))
+}
+int cryo_solver_adjoint_slosh_frame(cryo_solver* s, int64_t steps, const float* frame_acc, const float* frame_omega,
+                                    const float* frame_omega_dot, int32_t checkpoint_every, double* loss,
+                                    double* grad_frame_acc, double* grad_frame_omega, double* grad_frame_omega_dot,
+                                    double* wall_load, double* grad_contact_angle, double* grad_lee) {
+  if (steps > 0 && !frame_acc) return fail("frame_acc is required", CRYO_ERR_INVALID_ARGUMENT);
+  CRYO_GUARD(S(s).adjoint_slosh(steps, frame_acc, checkpoint_every, loss, grad_frame_acc, wall_load,
+                                grad_contact_angle, grad_lee, frame_omega, frame_omega_dot, grad_frame_omega,
+                                grad_frame_omega_dot))
 }
 int cryo_solver_run_until(cryo_solver* s, double t_end) {
   CRYO_GUARD({

@@ -1805,7 +1805,7 @@ struct Solver {
   // come from csrc/adjoint_pair.hpp; this restatement drives them in the
   // scatter form (each pair once, both members updated), serially, and is
   // itself checked against central differences of this file's forward step.
-  void check_adjoint_scope() const {
+  void check_adjoint_scope(bool explicit_rotation = false) const {
     const cryo_config& c = cfg;
     if (!(c.fixed_dt > 0)) throw std::invalid_argument("adjoint: needs fixed_dt > 0");
     if (c.kernel != CRYO_KERNEL_WENDLAND_C2) throw std::invalid_argument("adjoint: Wendland C2 kernel only");
@@ -1816,7 +1816,9 @@ struct Solver {
       throw std::invalid_argument("adjoint: CSF only with unsmoothed normals, the classic force and normal rotation");
     if (c.transport_stress || c.sps_turbulence || c.particle_shifting)
       throw std::invalid_argument("adjoint: transport stress, SPS and shifting not differentiated");
-    if (has_maneuver || P.frame_rot) throw std::invalid_argument("adjoint: frame rotation / manoeuvre table not allowed");
+    if (has_maneuver) throw std::invalid_argument("adjoint: manoeuvre table not allowed (the frame is per-step input)");
+    if (P.frame_rot && !explicit_rotation)
+      throw std::invalid_argument("adjoint: a rotating frame is differentiated only with per-step rotation inputs");
   }
 
   // Surface tension's share of the rates' adjoint: a += nrm coeff kappa
@@ -2207,7 +2209,7 @@ struct Solver {
 
   V3 adjoint_rates(const std::vector<u32>& perm, V3 Ssum, std::vector<V3>& xb, std::vector<V3>& vb,
                    std::vector<V3>& ab, std::vector<V3>& tb, std::vector<R>& mb, std::vector<R>& Tb, double& theta_b,
-                   double* lee, bool sat_prev) {
+                   double* lee, bool sat_prev, double* wb, double* wdb) {
     using namespace cryo_adj;
     using VR = Vec<R>;
     const int n = ps.n;
@@ -2237,6 +2239,20 @@ struct Solver {
       At[i] = vr(ab[i]) + v1b[i] * half;
       gb += At[i];  // a = g_eff + ...
       Tr[i] += T1b[i];
+    }
+    // rotating frame (sph_solver.cu:62-99 momentum): a -= wd x r + w x (w x r) + 2 w x v, r = x - origin
+    {
+      const V3 w = P.omega, wd = P.omega_dot;
+      for (int i = 0; i < n; ++i) {
+        if (!is_fluid(ps.phase[i])) continue;
+        const V3 A = v3(At[i]), r = ps.x[i] - P.frame_origin, v = ps.v[i];
+        xr[i] += vr(cross(wd, A) - cross(w, cross(w, A)));
+        vrb[i] += vr(cross(w, A) * R(2));
+        const V3 dwd = cross(A, r);
+        const V3 dw = A * (-dot(w, r)) - r * dot(A, w) + w * (R(2) * dot(A, r)) + cross(A, v) * R(2);
+        wdb[0] += double(dwd.x), wdb[1] += double(dwd.y), wdb[2] += double(dwd.z);
+        wb[0] += double(dw.x), wb[1] += double(dw.y), wb[2] += double(dw.z);
+      }
     }
     auto c_to_T = [&](int i, R cb) {  // c = sqrt(R T) for vapour
       if (ps.phase[i] == V && ps.T[i] > R(0) && cb != R(0)) Tr[i] += cb * P.gas_R / (R(2) * sound_speed(P, V, ps.T[i]));
@@ -2541,8 +2557,9 @@ struct Solver {
   }
 
   void adjoint_slosh(long long steps, const float* acc, double* loss, double* grad, double* force, double* grad_theta,
-                     double* grad_lee) {
-    check_adjoint_scope();
+                     double* grad_lee, const float* omega = nullptr, const float* omega_dot = nullptr,
+                     double* grad_omega = nullptr, double* grad_omega_dot = nullptr) {
+    check_adjoint_scope(omega || omega_dot);
     if (steps < 0) throw std::invalid_argument("adjoint: steps must be >= 0");
     const bool fresh = !initialized;  // the start-up rates (and their a) belong to this run
     ensure_initialized();
@@ -2551,12 +2568,18 @@ struct Solver {
     const V3 origin = P.frame_origin;
     const V3 zero{0, 0, 0};
     auto acc_of = [&](long long k) { return V3{R(acc[3 * k]), R(acc[3 * k + 1]), R(acc[3 * k + 2])}; };
+    auto w_of = [&](long long k) {
+      return omega ? V3{R(omega[3 * k]), R(omega[3 * k + 1]), R(omega[3 * k + 2])} : V3{0, 0, 0};
+    };
+    auto wd_of = [&](long long k) {
+      return omega_dot ? V3{R(omega_dot[3 * k]), R(omega_dot[3 * k + 1]), R(omega_dot[3 * k + 2])} : V3{0, 0, 0};
+    };
     std::vector<TapeEntry> tape(static_cast<std::size_t>(steps));
     std::vector<V3> S(static_cast<std::size_t>(steps));
     double L = 0;
     const int events0 = merges + splits;
     for (long long k = 0; k < steps; ++k) {
-      apply_frame(acc_of(k), zero, zero, origin);
+      apply_frame(acc_of(k), w_of(k), wd_of(k), origin);
       record = grad ? &tape[static_cast<std::size_t>(k)] : nullptr;
       step();
       record = nullptr;
@@ -2589,12 +2612,18 @@ struct Solver {
       TapeEntry& t = tape[static_cast<std::size_t>(k)];
       ps = t.ps;
       grid_build(grid, ps);  // already sorted: the identity permutation, the same cell table
-      apply_frame(acc_of(k), zero, zero, origin);
+      apply_frame(acc_of(k), w_of(k), wd_of(k), origin);
       P.dt = R(cfg.fixed_dt);
       P.T_sat = t.T_sat;
       evaluate_rates();
       const bool sat_prev = k > 0 && cfg.thermo_mode == CRYO_CLOSED && has_vapor && cfg.phase_change;
-      const V3 gb = adjoint_rates(t.perm, S[static_cast<std::size_t>(k)], xb, vb, ab, tb, mb, Tb, theta_b, lee, sat_prev);
+      double wbk[3] = {0, 0, 0}, wdbk[3] = {0, 0, 0};
+      const V3 gb =
+          adjoint_rates(t.perm, S[static_cast<std::size_t>(k)], xb, vb, ab, tb, mb, Tb, theta_b, lee, sat_prev, wbk, wdbk);
+      for (int q = 0; q < 3; ++q) {
+        if (grad_omega) grad_omega[3 * k + q] = wbk[q];
+        if (grad_omega_dot) grad_omega_dot[3 * k + q] = wdbk[q];
+      }
       grad[3 * k] = -double(gb.x);
       grad[3 * k + 1] = -double(gb.y);
       grad[3 * k + 2] = -double(gb.z);
@@ -2622,7 +2651,7 @@ struct Solver {
     ps = end_state;        // scratch included: a is the end state's
     P.T_sat = T_sat_end;
     grid_build(grid, ps);  // sorted already: the identity permutation
-    if (steps > 0) apply_frame(acc_of(steps - 1), zero, zero, origin);
+    if (steps > 0) apply_frame(acc_of(steps - 1), w_of(steps - 1), wd_of(steps - 1), origin);
   }
 };
 
@@ -2704,6 +2733,13 @@ int orc_adjoint_slosh(void* s, long long steps, const float* acc, int checkpoint
                       double* force, double* grad_contact_angle, double* grad_lee) {
   (void)checkpoint_every;  // the oracle tapes every step
   GUARD(static_cast<Solver*>(s)->adjoint_slosh(steps, acc, loss, grad, force, grad_contact_angle, grad_lee))
+}
+int orc_adjoint_slosh_frame(void* s, long long steps, const float* acc, const float* omega, const float* omega_dot,
+                            int checkpoint_every, double* loss, double* grad, double* grad_omega,
+                            double* grad_omega_dot, double* force, double* grad_contact_angle, double* grad_lee) {
+  (void)checkpoint_every;
+  GUARD(static_cast<Solver*>(s)->adjoint_slosh(steps, acc, loss, grad, force, grad_contact_angle, grad_lee, omega,
+                                               omega_dot, grad_omega, grad_omega_dot))
 }
 int orc_run_until(void* s, double t) {
   GUARD(Solver& S = *static_cast<Solver*>(s); while (S.time < t) S.step())

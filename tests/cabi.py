@@ -235,6 +235,30 @@ class AdjointResult(tuple):
     lee = None
 
 
+def _adjoint_slosh_frame(call, handle, acc, omega, omega_dot, checkpoint_every, grad):
+    """Rotating-frame form: returns an AdjointResult with .omega and .omega_dot
+    (d loss / d the per-step frame angular velocity and its rate)."""
+    acc = np.ascontiguousarray(np.asarray(acc, np.float32).reshape(-1, 3))
+    steps = len(acc)
+    w = np.ascontiguousarray(np.asarray(omega, np.float32).reshape(steps, 3))
+    wd = np.ascontiguousarray(np.asarray(omega_dot, np.float32).reshape(steps, 3))
+    loss = C.c_double()
+    g = np.zeros((steps, 3), np.float64) if grad else None
+    gw = np.zeros((steps, 3), np.float64) if grad else None
+    gwd = np.zeros((steps, 3), np.float64) if grad else None
+    f = np.zeros((steps, 3), np.float64)
+    th = C.c_double(0.0)
+    lee = np.zeros(2, np.float64)
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+    call("adjoint_slosh_frame", handle, C.c_int64(steps), _f32p(acc), _f32p(w), _f32p(wd),
+         C.c_int32(checkpoint_every), C.byref(loss), dp(g), dp(gw), dp(gwd), dp(f), C.byref(th) if grad else None,
+         dp(lee) if grad else None)
+    out = AdjointResult((loss.value, g, f, (th.value if grad else None)))
+    out.lee = (float(lee[0]), float(lee[1])) if grad else None
+    out.omega, out.omega_dot = gw, gwd
+    return out
+
+
 def _adjoint_slosh(call, handle, acc, checkpoint_every, grad):
     acc = np.ascontiguousarray(np.asarray(acc, np.float32).reshape(-1, 3))
     steps = len(acc)
@@ -317,6 +341,9 @@ class CpuSolver:
     def adjoint_slosh(self, acc, checkpoint_every=20, grad=True):
         """(loss, dloss/dacc or None, per-step sum of fw, dloss/dcontact_angle_deg or None), oracle."""
         return _adjoint_slosh(self._call, self.h, acc, checkpoint_every, grad)
+
+    def adjoint_slosh_frame(self, acc, omega, omega_dot, checkpoint_every=20, grad=True):
+        return _adjoint_slosh_frame(self._call, self.h, acc, omega, omega_dot, checkpoint_every, grad)
 
     def next_dt(self):
         d = C.c_double()
@@ -535,6 +562,10 @@ class HipSolver:
     def adjoint_slosh(self, acc, checkpoint_every=20, grad=True):
         """(loss, dloss/dacc or None, per-step sum of fw, dloss/dcontact_angle_deg or None)."""
         return _adjoint_slosh(lambda name, *a: self._call("solver_" + name, *a), self.h, acc, checkpoint_every, grad)
+
+    def adjoint_slosh_frame(self, acc, omega, omega_dot, checkpoint_every=20, grad=True):
+        return _adjoint_slosh_frame(lambda name, *a: self._call("solver_" + name, *a), self.h, acc, omega, omega_dot,
+                                    checkpoint_every, grad)
 
     def set_launch_variant(self, v):
         self._call("solver_set_launch_variant", self.h, C.c_int32(v))

@@ -149,6 +149,32 @@ def test_heat_and_phase_change_gradients_match_cpu_adjoints():
         assert np.array_equal(again[1], out[1]) and again.lee == out.lee
 
 
+def test_rotating_frame_gradients_match_fp64_adjoint():
+    """Per-step frame angular velocity and rate: the GPU's acceleration,
+    angular-velocity and angular-acceleration gradients against the FP64
+    adjoint (pinned by central differences), with and without heat and
+    phase change; bitwise independent of the checkpoint interval."""
+    say("test_rotating_frame_gradients_match_fp64_adjoint")
+    for lee in (None, 50.0):
+        fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=8, lee=lee)
+        rng = np.random.default_rng(3)
+        w = rng.normal(0.0, 2.0, size=acc.shape).astype(np.float32)
+        wd = rng.normal(0.0, 20.0, size=acc.shape).astype(np.float32)
+        out = hip(cfg, fluid, ps, lo, hi).adjoint_slosh_frame(acc, w, wd, checkpoint_every=3)
+        ref = orc(cfg, fluid, ps, lo, hi).adjoint_slosh_frame(acc, w, wd)
+        errs = [np.abs(a - b).max() / np.abs(b).max() for a, b in ((out[1], ref[1]), (out.omega, ref.omega),
+                                                                  (out.omega_dot, ref.omega_dot))]
+        say("lee", lee, "loss", out[0], "fp64", ref[0], "errors acc / omega / omega_dot", errs)
+        assert abs(out[0] - ref[0]) <= 1e-4 * ref[0]
+        assert max(errs) <= 1e-3
+        again = hip(cfg, fluid, ps, lo, hi).adjoint_slosh_frame(acc, w, wd, checkpoint_every=8)
+        assert np.array_equal(again.omega, out.omega) and np.array_equal(again.omega_dot, out.omega_dot)
+    s = hip(cfg, fluid, ps, lo, hi)
+    s.set_frame_motion(acc=(0.0, 0.0, 0.0), omega=(0.0, 0.0, 1.0))
+    with pytest.raises(ValueError):
+        s.adjoint_slosh(acc)
+
+
 def test_checkpoint_interval_does_not_change_the_gradient():
     say("test_checkpoint_interval_does_not_change_the_gradient")
     fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=10)
