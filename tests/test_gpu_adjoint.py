@@ -173,6 +173,26 @@ def test_rotating_frame_gradients_match_fp64_adjoint():
     s.set_frame_motion(acc=(0.0, 0.0, 0.0), omega=(0.0, 0.0, 1.0))
     with pytest.raises(ValueError):
         s.adjoint_slosh(acc)
+    # the Python module's method is the same call
+    import CryoSPH as cs
+
+    c = cs.SolverConfig()
+    c.dx = cfg.dx
+    c.gravity = cfg.gravity.tup()
+    c.transport_velocity = False
+    c.surface_tension = cs.SurfaceTensionModel.NoTension
+    c.thermal = c.phase_change = True
+    c.thermo_mode = cs.TankThermoMode.Vented
+    for f in ("velocity_scale", "fixed_dt", "artificial_viscosity", "artificial_viscosity_vapor", "conductivity_scale",
+              "heater_power", "vapor_mass_rebalance", "lee_evaporation", "lee_condensation"):
+        setattr(c, f, getattr(cfg, f))
+    m = cs.Solver(c, cs.FluidProperties.liquid_nitrogen())
+    m.set_particles(ps.x.astype(np.float64), ps.m, ps.rho, ps.T, ps.phase, domain_lo=tuple(map(float, lo)),
+                    domain_hi=tuple(map(float, hi)), velocity=ps.v.astype(np.float64), flags=ps.flags)
+    res = m.adjoint_slosh_frame(acc.astype(np.float64), w.astype(np.float64), wd.astype(np.float64), 3)
+    assert res[0] == out[0] and np.array_equal(res[1], out[1])
+    assert np.array_equal(res[2], out.omega) and np.array_equal(res[3], out.omega_dot)
+    assert res[6] == out.lee
 
 
 def test_checkpoint_interval_does_not_change_the_gradient():
