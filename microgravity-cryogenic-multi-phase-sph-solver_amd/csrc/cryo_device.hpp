@@ -484,18 +484,26 @@ struct ListWriter {
   int c = 0;
   int K;
   __device__ ListWriter(const NbrList& L, int i) : out(list_chunks(L, i)), K(L.K) {}
+  // The chunk is a shift register: the newest entry enters at w, so after
+  // four pushes x..w hold them in order (moves, no compares or selects).
   __device__ inline void push(int j) {
-    const int sl = c & 3;  // selects, not a switch: no divergent branches
-    buf.x = sl == 0 ? j : buf.x;
-    buf.y = sl == 1 ? j : buf.y;
-    buf.z = sl == 2 ? j : buf.z;
-    buf.w = sl == 3 ? j : buf.w;
-    if (sl == 3 && c < K) out[(c >> 2) * kListGroup] = buf;
+    buf.x = buf.y;
+    buf.y = buf.z;
+    buf.z = buf.w;
+    buf.w = j;
+    if ((c & 3) == 3 && c < K) out[(c >> 2) * kListGroup] = buf;
     ++c;
   }
-  // Writes the last partial chunk; returns the count to store (-1: overflow).
+  // Writes the last partial chunk (its m entries moved to the front);
+  // returns the count to store (-1: overflow).
   __device__ inline int finish() {
-    if ((c & 3) != 0 && c < K) out[(c >> 2) * kListGroup] = buf;
+    const int m = c & 3;
+    if (m != 0 && c < K) {
+      int4 o = make_int4(buf.y, buf.z, buf.w, 0);
+      if (m == 1) o = make_int4(buf.w, 0, 0, 0);
+      if (m == 2) o = make_int4(buf.z, buf.w, 0, 0);
+      out[(c >> 2) * kListGroup] = o;
+    }
     return c <= K ? c : -1;
   }
 };
