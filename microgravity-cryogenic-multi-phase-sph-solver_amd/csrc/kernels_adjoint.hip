@@ -405,6 +405,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_wall(PV pv, GridDev g, SimParams
   block_sums_store<3>(gs, A.partials, A.stride);
 }
 
+// TV: the transport-velocity terms are compiled in (TV); without them
+// the kernel holds fewer registers.
+template <bool TV>
 __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParams P, NbrList L, const DevState* st,
                                                      AdjArgs A, int n) {
   const int i = gtid();
@@ -427,10 +430,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
   AV gbk{0.0f, 0.0f, 0.0f}, wbk{0.0f, 0.0f, 0.0f};
   float wwbk = 0.0f, Vbk = 0.0f;
   // transport acceleration: own sums (owner active) and the neighbours' sums that read V_i = 1 / sigma_i
-  const float4 tvi = A.tvf ? A.tvs[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  const bool tv_own = tvi.w > 0.0f;
+  const float4 tvi = TV ? A.tvs[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const bool tv_own = TV && tvi.w > 0.0f;
   const AV Sbi{tvi.x, tvi.y, tvi.z};
-  const float Vti = A.tvf ? 1.0f / pv.t.sigma[i] : 0.0f, Vti2 = Vti * Vti;
+  const float Vti = TV ? 1.0f / pv.t.sigma[i] : 0.0f, Vti2 = Vti * Vti;
   float vt2b = 0.0f, vtb = 0.0f, trhob = 0.0f, tsb = 0.0f, tTb = 0.0f;
   AV tsum{0.0f, 0.0f, 0.0f};
   // masses, temperatures (sound speeds), the force sum for d(F / m) / dm, conduction
@@ -526,7 +529,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
       xr -= og2.r;
       Vbk += og2.V_j;
     }
-    if (A.tvf) {
+    if (TV) {
       const AV rkj = I.x - av(pv.s.x[j]);
       const float r2 = cryo_adj::dot(rkj, rkj);
       if (r2 < P.support2) {
@@ -1161,7 +1164,8 @@ void launch_adj_step(const PV& pv, const GridDev& g, const SimParams& P, const N
     k_adj_finish<<<1, 1024, 0, s>>>(A.partials, A.stride, nb, 6, -1, 1.0, theta_out, 2);
   }
   k_adj_wall<<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
-  k_adj_fluid<<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
+  if (A.tvf) k_adj_fluid<true><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
+  else k_adj_fluid<false><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
   if (A.sat) {
     const int nb_inv = launch_inventory(pv, st, n, A.partials + 12 * A.stride, A.stride, s);
     k_adj_sat<<<1, 1024, 0, s>>>(st, A, nb, nb_inv);
