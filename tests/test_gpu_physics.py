@@ -1,0 +1,245 @@
+"""Physics invariants of the reference's C++ suite, run on the gfx950 build at
+that suite's own sizes and horizons (thousands of steps, past where a
+step-by-step comparison with the oracle is meaningful):
+
+* transport-velocity momentum conservation (test_transport_velocity.cpp:14-40);
+* the centrifugal pressure of a spinning tank (test_maneuver.cpp:78-114);
+* the Laplace pressure jump of a drop (test_surface_tension.cpp:13-82);
+* mass and enthalpy closure under Lee phase change and under venting
+  (test_conservation.cpp:68-123).
+
+Set-ups, horizons and tolerances are the reference tests'.
+"""
+import numpy as np
+import pytest
+
+import CryoSPH as cs
+
+pytestmark = pytest.mark.gpu
+
+
+def lattice_block(lo, hi, dx):
+    """Cell-centred lattice of the box [lo, hi], x fastest (geometry.cpp:332-346)."""
+    f = np.float32
+    lo = np.asarray(lo, f)
+    dx = f(dx)
+    n = [int(np.floor((f(h) - l) / dx + f(1e-4))) for l, h in zip(lo, hi)]
+    k, j, i = np.meshgrid(np.arange(n[2]), np.arange(n[1]), np.arange(n[0]), indexing="ij")
+    x = np.stack([lo[0] + (i.astype(f) + f(0.5)) * dx, lo[1] + (j.astype(f) + f(0.5)) * dx,
+                  lo[2] + (k.astype(f) + f(0.5)) * dx], -1)
+    return x.reshape(-1, 3)
+
+
+def settle(solver, cycles):
+    """test_maneuver.cpp:26-32."""
+    for _ in range(cycles):
+        solver.run_until(solver.time + 0.02)
+        solver.damp_velocities(0.5)
+    solver.run_until(solver.time + 0.05)
+
+
+def conservation_config():
+    """test_conservation.cpp:14-24."""
+    cfg = cs.SolverConfig()
+    cfg.dx = 0.004
+    cfg.gravity = (0.0, 0.0, 0.0)
+    cfg.surface_tension = cs.SurfaceTensionModel.CSF
+    cfg.contact_angle_deg = 5
+    cfg.thermo_mode = cs.TankThermoMode.Closed
+    cfg.wall_thermal_bc = cs.WallThermalBC.Adiabatic
+    cfg.velocity_scale = 1.0
+    return cfg
+
+
+def test_transport_velocity_conserves_linear_momentum_without_walls():
+    """test_transport_velocity.cpp:14-40: a free LN2 block in zero g with a
+    shearing velocity field; the pair terms (pressure, viscosity, transport
+    stress) are antisymmetric, so 200 steps keep the liquid momentum within
+    1e-4 of M * 0.2 and the mass within 1e-6."""
+    print("test_transport_velocity_conserves_linear_momentum_without_walls", flush=True)
+    ln2 = cs.FluidProperties.liquid_nitrogen()
+    cfg = cs.SolverConfig()
+    cfg.dx = 0.004
+    cfg.gravity = (0.0, 0.0, 0.0)
+    cfg.surface_tension = cs.SurfaceTensionModel.NoTension
+    cfg.thermal = False
+    cfg.phase_change = False
+    cfg.velocity_scale = 0.5
+    assert cfg.transport_velocity
+    solver = cs.Solver(cfg, ln2)
+    x = lattice_block((0.0, 0.0, 0.0), (0.04, 0.04, 0.04), cfg.dx)
+    n = len(x)
+    assert n == 1000
+    f = np.float32
+    v = np.zeros((n, 3), f)
+    v[:, 0] = f(0.2) * np.sin(f(150) * x[:, 2])
+    v[:, 1] = f(0.1) * np.cos(f(120) * x[:, 0])
+    rho = np.full(n, ln2.rho_liquid, f)
+    dx = f(cfg.dx)
+    solver.set_particles(x, rho * dx * dx * dx, rho, np.full(n, ln2.T_sat, f), np.zeros(n, np.uint8),
+                         domain_lo=(-0.02, -0.02, -0.02), domain_hi=(0.06, 0.06, 0.06), velocity=v)
+    d0 = solver.diagnostics()
+    solver.run_steps(200)
+    d1 = solver.diagnostics()
+    p_scale = d0.mass_liquid * 0.2
+    for k in range(3):
+        assert abs(d1.liquid_momentum[k] - d0.liquid_momentum[k]) < 1e-4 * p_scale, (k, d0.liquid_momentum,
+                                                                                       d1.liquid_momentum)
+    assert d1.mass_total == pytest.approx(d0.mass_total, rel=1e-6)
+    print("momentum", d0.liquid_momentum, "->", d1.liquid_momentum, "scale", p_scale, flush=True)
+
+
+def test_steady_rotation_gives_centrifugal_pressure():
+    """test_maneuver.cpp:78-114: a full water cylinder (R = 0.05 m = 10 dx)
+    spinning at 20 rad/s settles to p = p0 + rho w^2 r^2 / 2; the fitted
+    dp / d(r^2) is within 4 % of rho w^2 / 2 and the net lateral load and the
+    axial moment vanish within 2 % of M w^2 R."""
+    print("test_steady_rotation_gives_centrifugal_pressure", flush=True)
+    water = cs.FluidProperties.water()
+    omega, R = 20.0, 0.05
+    cfg = cs.SolverConfig()
+    cfg.dx = 0.005
+    cfg.gravity = (0.0, 0.0, 0.0)
+    cfg.surface_tension = cs.SurfaceTensionModel.NoTension
+    cfg.thermal = False
+    cfg.phase_change = False
+    cfg.artificial_viscosity = 0.1
+    cfg.velocity_scale = omega * R
+    solver = cs.Solver(cfg, water)
+    fill = cs.FillSpec()
+    fill.mode = cs.FillMode.Full
+    solver.build_tank(cs.CylinderTank(R, 0.05), fill)
+    solver.set_frame_motion(angular_velocity=(0.0, 0.0, omega))
+    settle(solver, 40)
+    parts = solver.particles()
+    p = solver.pressure().astype(np.float64)
+    x = parts["position"].astype(np.float64)
+    r2 = x[:, 0] ** 2 + x[:, 1] ** 2
+    sel = (parts["phase"] == 0) & (r2 <= 0.035 ** 2) & (np.abs(x[:, 2]) <= 0.012)
+    assert sel.sum() > 300
+    k = np.polyfit(r2[sel], p[sel], 1)[0]
+    expected = 0.5 * water.rho_liquid * omega * omega
+    print("dp/d(r^2)", k, "expected", expected, "particles", int(sel.sum()), flush=True)
+    assert k / expected == pytest.approx(1.0, abs=0.04)
+    d = solver.diagnostics()
+    scale = d.mass_liquid * omega * omega * R
+    assert abs(d.slosh_force[0]) < 0.02 * scale
+    assert abs(d.slosh_force[1]) < 0.02 * scale
+    assert abs(d.slosh_moment[2]) < 0.02 * scale * R
+
+
+def test_laplace_pressure_jump():
+    """test_surface_tension.cpp:13-82: a CSF drop of radius 12 mm (sigma = 1,
+    density ratio 10) in a 48 mm box of vapour with three wall layers, damped
+    to rest: the pressure jump between the drop's core and the vapour shell
+    at 1.6-2 R is 2 sigma / R within 30 %."""
+    print("test_laplace_pressure_jump", flush=True)
+    fl = cs.FluidProperties.water()
+    fl.rho_vapor = 100.0
+    fl.mu_liquid = 0.05
+    fl.mu_vapor = 0.005
+    fl.sigma = 1.0
+    cfg = cs.SolverConfig()
+    cfg.dx = 0.002
+    cfg.gravity = (0.0, 0.0, 0.0)
+    cfg.thermal = False
+    cfg.phase_change = False
+    cfg.surface_tension = cs.SurfaceTensionModel.CSF
+    cfg.artificial_viscosity = 0.05
+    solver = cs.Solver(cfg, fl)
+    f = np.float32
+    R, L, dx = f(0.012), f(0.048), f(cfg.dx)
+    n = int(round(float(L / dx)))
+    idx = np.arange(-3, n + 3)
+    k, j, i = np.meshgrid(idx, idx, idx, indexing="ij")
+    k, j, i = k.ravel(), j.ravel(), i.ravel()
+    pos = np.stack([(i.astype(f) + f(0.5)) * dx - L / f(2), (j.astype(f) + f(0.5)) * dx - L / f(2),
+                    (k.astype(f) + f(0.5)) * dx - L / f(2)], -1).astype(f)
+    inside = (i >= 0) & (i < n) & (j >= 0) & (j < n) & (k >= 0) & (k < n)
+    liquid = inside & (np.linalg.norm(pos.astype(np.float64), axis=1) < R)
+    phase = np.where(inside, np.where(liquid, 0, 1), 2).astype(np.uint8)
+    rho = np.where(liquid | ~inside, f(fl.rho_liquid), f(fl.rho_vapor)).astype(f)
+    nrm = np.stack([np.where(i < 0, 1.0, np.where(i >= n, -1.0, 0.0)), np.where(j < 0, 1.0, np.where(j >= n, -1.0, 0.0)),
+                    np.where(k < 0, 1.0, np.where(k >= n, -1.0, 0.0))], -1)
+    ln = np.linalg.norm(nrm, axis=1)
+    nrm[ln > 0] /= ln[ln > 0, None]
+    nrm = nrm.astype(f)
+    # fluid first, then walls, as the reference pushes them
+    order = np.concatenate([np.flatnonzero(inside), np.flatnonzero(~inside)])
+    pad = float(L / f(2) + f(4) * dx)
+    solver.set_particles(pos[order], (rho * dx * dx * dx)[order], rho[order], np.full(len(order), fl.T_sat, f),
+                         phase[order], domain_lo=(-pad, -pad, -pad), domain_hi=(pad, pad, pad), normal=nrm[order])
+    for _ in range(30):
+        solver.run_until(solver.time + 0.004)
+        solver.damp_velocities(0.7)
+    solver.run_until(solver.time + 0.02)
+    parts = solver.particles()
+    p = solver.pressure().astype(np.float64)
+    r = np.linalg.norm(parts["position"].astype(np.float64), axis=1)
+    pin = (parts["phase"] == 0) & (r < 0.5 * R)
+    pout = (parts["phase"] == 1) & (r > 1.6 * R) & (r < 2.0 * R)
+    assert pin.sum() > 10 and pout.sum() > 10
+    dp = p[pin].mean() - p[pout].mean()
+    laplace = 2.0 * fl.sigma / float(R)
+    print("dp", dp, "Laplace", laplace, "ratio", dp / laplace, flush=True)
+    assert dp / laplace == pytest.approx(1.0, abs=0.3)
+    assert np.isfinite(solver.diagnostics().kinetic_energy)
+
+
+def test_phase_change_conserves_mass_and_enthalpy():
+    """test_conservation.cpp:68-101: LN2 box, half full, liquid 3 K superheated,
+    Lee 2: mass closes within 1e-5 and enthalpy within 2 % of the latent heat
+    of the evaporated mass; the liquid cools and the vapour gains mass."""
+    print("test_phase_change_conserves_mass_and_enthalpy", flush=True)
+    ln2 = cs.FluidProperties.liquid_nitrogen()
+    cfg = conservation_config()
+    cfg.lee_evaporation = 2.0
+    cfg.lee_condensation = 2.0
+    solver = cs.Solver(cfg, ln2)
+    fill = cs.FillSpec()
+    fill.fill_fraction = 0.5
+    fill.T_liquid = ln2.T_sat + 3
+    fill.T_vapor = ln2.T_sat
+    solver.build_tank(cs.BoxTank(0.04, 0.04, 0.04), fill)
+    solver.relax(0.03)
+    d0 = solver.diagnostics()
+    solver.run_until(0.04)
+    d1 = solver.diagnostics()
+    assert np.isfinite(d1.enthalpy)
+    assert d1.max_speed < 2.0
+    assert d1.evaporated_mass > 0.0
+    assert d1.mass_vapor > d0.mass_vapor
+    assert d1.mass_liquid < d0.mass_liquid
+    assert abs(d1.mass_total - d0.mass_total) < 1e-5 * d0.mass_total
+    latent = d1.evaporated_mass * ln2.h_fg
+    roundoff = 1e-6 * d0.mass_total * ln2.cp_liquid * ln2.T_sat
+    print("enthalpy change", d1.enthalpy - d0.enthalpy, "latent", latent, "roundoff", roundoff, flush=True)
+    assert abs(d1.enthalpy - d0.enthalpy) < 0.02 * latent + roundoff
+    assert latent > 10 * roundoff
+    assert d1.T_liquid_mean < d0.T_liquid_mean
+
+
+def test_venting_removes_excess_vapour():
+    """test_conservation.cpp:103-123: vented LN2 box, liquid 3 K superheated:
+    what left the liquid is still vapour or was vented (mass within 1e-5,
+    enthalpy within 2 % of the evaporated latent heat)."""
+    print("test_venting_removes_excess_vapour", flush=True)
+    ln2 = cs.FluidProperties.liquid_nitrogen()
+    cfg = conservation_config()
+    cfg.thermo_mode = cs.TankThermoMode.Vented
+    cfg.vent_relaxation = 0.2
+    solver = cs.Solver(cfg, ln2)
+    fill = cs.FillSpec()
+    fill.fill_fraction = 0.5
+    fill.T_liquid = ln2.T_sat + 3
+    solver.build_tank(cs.BoxTank(0.04, 0.04, 0.04), fill)
+    solver.relax(0.03)
+    d0 = solver.diagnostics()
+    solver.run_until(0.03)
+    d1 = solver.diagnostics()
+    assert abs(d1.mass_total + d1.vented_mass - d0.mass_total) < 1e-5 * d0.mass_total
+    assert d1.vented_mass > 0.0
+    roundoff = 1e-6 * d0.mass_total * ln2.cp_liquid * ln2.T_sat
+    err = abs(d1.enthalpy + d1.vented_enthalpy - d0.enthalpy)
+    print("enthalpy closure", err, "bar", 0.02 * d1.evaporated_mass * ln2.h_fg + roundoff, flush=True)
+    assert err < 0.02 * d1.evaporated_mass * ln2.h_fg + roundoff
