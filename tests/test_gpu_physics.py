@@ -2,9 +2,11 @@
 that suite's own sizes and horizons (thousands of steps, past where a
 step-by-step comparison with the oracle is meaningful):
 
-* transport-velocity momentum conservation (test_transport_velocity.cpp:14-40);
+* transport-velocity momentum conservation and a rising bubble
+  (test_transport_velocity.cpp:14-84);
 * the centrifugal pressure of a spinning tank (test_maneuver.cpp:78-114);
 * the Laplace pressure jump of a drop (test_surface_tension.cpp:13-82);
+* the equilibrium angle of sessile drops (test_sessile_drop.cpp:88-137);
 * mass and enthalpy closure under Lee phase change and under venting
   (test_conservation.cpp:68-123).
 
@@ -243,3 +245,109 @@ def test_venting_removes_excess_vapour():
     err = abs(d1.enthalpy + d1.vented_enthalpy - d0.enthalpy)
     print("enthalpy closure", err, "bar", 0.02 * d1.evaporated_mass * ln2.h_fg + roundoff, flush=True)
     assert err < 0.02 * d1.evaporated_mass * ln2.h_fg + roundoff
+
+
+def test_transport_velocity_bubble_rises_and_stays_inside_tank():
+    """test_transport_velocity.cpp:42-84: an LH2 capsule (r = 0.06 m, length
+    0.10 m, dx = 8 mm) with a central vapour bubble, relaxed, then 1 g until
+    t = 0.2 s: the bubble's centre of mass rises 25-90 mm, no fluid particle
+    ends more than 2 dx outside the tank and the speed stays below 3 m/s."""
+    print("test_transport_velocity_bubble_rises_and_stays_inside_tank", flush=True)
+    lh2 = cs.FluidProperties.liquid_hydrogen()
+    cfg = cs.SolverConfig()
+    cfg.dx = 0.008
+    cfg.gravity = (0.0, 0.0, 0.0)
+    cfg.surface_tension = cs.SurfaceTensionModel.NoTension
+    cfg.thermal = False
+    cfg.phase_change = False
+    cfg.velocity_scale = float(np.sqrt(np.float32(2 * 9.81 * 0.2)))
+    solver = cs.Solver(cfg, lh2)
+    tank = cs.CapsuleTank(0.06, 0.10)
+    fill = cs.FillSpec()
+    fill.mode = cs.FillMode.CentralBubble
+    fill.fill_fraction = 0.9
+    solver.build_tank(tank, fill)
+    solver.relax(0.02)
+    solver.set_gravity((0.0, 0.0, -9.81))
+
+    def bubble_z():
+        p = solver.particles()
+        vap = p["phase"] == 1
+        m = p["mass"][vap].astype(np.float64)
+        return float((m * p["position"][vap, 2]).sum() / m.sum())
+
+    z0 = bubble_z()
+    solver.run_until(0.2)
+    rise = bubble_z() - z0
+    print("bubble rise", rise, flush=True)
+    assert 0.025 < rise < 0.09
+    p = solver.particles()
+    fluid = np.flatnonzero(p["phase"] != 2)
+    outside = sum(tank.sdf(*map(float, p["position"][i])) > 2.0 * cfg.dx for i in fluid)
+    assert outside == 0
+    assert solver.diagnostics().max_speed < 3.0
+
+
+def contact_angle(x, phase, z_wall, dx):
+    """test_sessile_drop.cpp:38-83: a sphere fitted (least squares on
+    |m|^2 = 2 c.m + d) through the midpoints between each liquid particle and
+    its nearest vapour particle within 1.5 dx, skipping midpoints within 2 dx
+    of the wall; the angle between the wall and the sphere at the contact line."""
+    from scipy.spatial import cKDTree
+    liq = x[phase == 0].astype(np.float64)
+    vap = x[phase == 1].astype(np.float64)
+    d, j = cKDTree(vap).query(liq, distance_upper_bound=1.5 * dx)
+    ok = np.isfinite(d) & (d < 1.5 * dx)
+    mid = 0.5 * (liq[ok] + vap[j[ok]])
+    mid = mid[mid[:, 2] >= z_wall + 2 * dx]
+    assert len(mid) > 50
+    A = np.column_stack([2.0 * mid, np.ones(len(mid))])
+    s = np.linalg.solve(A.T @ A, A.T @ (mid * mid).sum(1))
+    R = np.sqrt(s[3] + s[0] ** 2 + s[1] ** 2 + s[2] ** 2)
+    return float(np.degrees(np.arccos(np.clip((z_wall - s[2]) / R, -1.0, 1.0))))
+
+
+@pytest.mark.parametrize("theta", [60.0, 90.0, 120.0])
+def test_sessile_drop_equilibrium_angle(theta):
+    """test_sessile_drop.cpp:88-137: a hemispherical drop (R0 = 16 mm) on the
+    floor of a 64 x 64 x 32 mm box of vapour (sigma = 1, density ratio 10),
+    damped for 0.4 s: the mean angle over five samples 10 ms apart is the
+    prescribed contact angle within 9 degrees."""
+    from cabi import box, default_fill, hip_build_tank_particles, hip_fluid_preset
+    print("test_sessile_drop_equilibrium_angle", theta, flush=True)
+    fl = cs.FluidProperties.water()
+    fl.rho_vapor = 100.0
+    fl.mu_liquid = 0.05
+    fl.mu_vapor = 0.005
+    fl.sigma = 1.0
+    cfg = cs.SolverConfig()
+    cfg.dx = 0.002
+    cfg.gravity = (0.0, 0.0, 0.0)
+    cfg.thermal = False
+    cfg.phase_change = False
+    cfg.contact_angle_deg = theta
+    solver = cs.Solver(cfg, fl)
+    f = np.float32
+    dx, R0, Lx, Lz = f(cfg.dx), 0.016, 0.064, 0.032
+    z_wall = -0.5 * Lz
+    ps = hip_build_tank_particles(box(Lx, Lx, Lz), hip_fluid_preset("H2O"), float(dx), default_fill(mode=2))
+    x = ps.x
+    r = np.sqrt(x[:, 0].astype(np.float64) ** 2 + x[:, 1].astype(np.float64) ** 2 + (x[:, 2] - z_wall) ** 2)
+    wall = ps.phase == 2
+    phase = np.where(wall, 2, np.where(r < R0, 0, 1)).astype(np.uint8)
+    rho = np.where(phase == 1, f(fl.rho_vapor), f(fl.rho_liquid)).astype(f)
+    pad = 4 * float(dx)
+    solver.set_particles(x, rho * dx * dx * dx, rho, np.full(len(x), fl.T_sat, f), phase,
+                         domain_lo=(-Lx / 2 - pad, -Lx / 2 - pad, z_wall - pad),
+                         domain_hi=(Lx / 2 + pad, Lx / 2 + pad, -z_wall + pad), normal=ps.nrm)
+    for _ in range(40):
+        solver.run_until(solver.time + 0.01)
+        solver.damp_velocities(0.8)
+    angles = []
+    for _ in range(5):
+        solver.run_until(solver.time + 0.01)
+        p = solver.particles()
+        angles.append(contact_angle(p["position"], p["phase"], z_wall, float(dx)))
+    measured = float(np.mean(angles))
+    print("prescribed", theta, "measured", measured, angles, flush=True)
+    assert abs(measured - theta) < 9.0
