@@ -97,6 +97,11 @@ def test_bad_inputs_raise():
     with pytest.raises(ValueError):
         solver.set_particles(np.zeros((4, 2)), np.ones(4), np.ones(4), np.ones(4), np.zeros(4, np.uint8),
                              domain_lo=(0, 0, 0), domain_hi=(1, 1, 1))
+    with pytest.raises(ValueError, match="no particles"):  # empty input (sph_solver.cu:408)
+        solver.set_particles(np.zeros((0, 3)), np.zeros(0), np.zeros(0), np.zeros(0), np.zeros(0, np.uint8),
+                             domain_lo=(0, 0, 0), domain_hi=(1, 1, 1))
+    with pytest.raises(RuntimeError, match="no particles loaded"):  # stepping an empty solver (sph_solver.cu:441)
+        cs.Solver(small_config(), ln2).run_steps(1)
     with pytest.raises(ValueError):
         solver.set_contact_angle(-5.0)
     akinci = cs.Solver(small_config(surface_tension=cs.SurfaceTensionModel.Akinci), ln2)
