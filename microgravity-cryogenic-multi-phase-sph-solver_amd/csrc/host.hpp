@@ -54,6 +54,9 @@ class Geometry {
   double volume() const;
   V3 outward_normal(double x, double y, double z, double eps) const;
   double mesh_winding(int mesh_id, double x, double y, double z) const;
+  const std::vector<cryo_geo_node>& node_list() const { return nodes_; }
+  bool has_mesh() const;
+  int depth() const;  // nodes on the longest root-to-leaf path
 
  private:
   struct MeshData {
@@ -67,7 +70,10 @@ class Geometry {
   std::vector<MeshData> meshes_;
 };
 
-HostParticles build_tank_particles(const Geometry& geo, const cryo_fluid& fluid, float dx, const cryo_fill& fill);
+// on_device: the lattice's signed distances and wall normals are evaluated
+// by a kernel on the current device (analytic and CSG tanks only).
+HostParticles build_tank_particles(const Geometry& geo, const cryo_fluid& fluid, float dx, const cryo_fill& fill,
+                                   bool on_device = false);
 
 // Owning device buffer (DeviceArray, cuda_utils.cuh:165-244).
 template <class T>

@@ -2245,12 +2245,13 @@ int cryo_mesh_winding_number(const cryo_mesh* mesh, double x, double y, double z
   })
 }
 
-int cryo_build_tank_particles(int32_t nn, const cryo_geo_node* nodes, int32_t nm, const cryo_mesh* meshes,
-                              const cryo_fluid* fluid, float dx, const cryo_fill* fill, int32_t* n, float* x,
-                              float* v, float* nrm, float* m, float* rho, float* T, uint8_t* phase, uint8_t* flags,
-                              uint32_t* id) {
+namespace {
+int build_tank_export(bool on_device, int32_t nn, const cryo_geo_node* nodes, int32_t nm, const cryo_mesh* meshes,
+                      const cryo_fluid* fluid, float dx, const cryo_fill* fill, int32_t* n, float* x, float* v,
+                      float* nrm, float* m, float* rho, float* T, uint8_t* phase, uint8_t* flags, uint32_t* id) {
   CRYO_GUARD({
-    const HostParticles hp = build_tank_particles(Geometry(nn, nodes, nm, meshes), *fluid, dx, *fill);
+    if (!fluid || !fill || !n) throw std::invalid_argument("cryo_build_tank_particles: null argument");
+    const HostParticles hp = build_tank_particles(Geometry(nn, nodes, nm, meshes), *fluid, dx, *fill, on_device);
     const int cnt = static_cast<int>(hp.size());
     if (x) {
       if (*n < cnt) throw std::out_of_range("cryo_build_tank_particles: arrays too small");
@@ -2268,6 +2269,20 @@ int cryo_build_tank_particles(int32_t nn, const cryo_geo_node* nodes, int32_t nm
     }
     *n = cnt;
   })
+}
+}  // namespace
+
+int cryo_build_tank_particles(int32_t nn, const cryo_geo_node* nodes, int32_t nm, const cryo_mesh* meshes,
+                              const cryo_fluid* fluid, float dx, const cryo_fill* fill, int32_t* n, float* x,
+                              float* v, float* nrm, float* m, float* rho, float* T, uint8_t* phase, uint8_t* flags,
+                              uint32_t* id) {
+  return build_tank_export(false, nn, nodes, nm, meshes, fluid, dx, fill, n, x, v, nrm, m, rho, T, phase, flags, id);
+}
+int cryo_build_tank_particles_device(int32_t nn, const cryo_geo_node* nodes, int32_t nm, const cryo_mesh* meshes,
+                                     const cryo_fluid* fluid, float dx, const cryo_fill* fill, int32_t* n, float* x,
+                                     float* v, float* nrm, float* m, float* rho, float* T, uint8_t* phase,
+                                     uint8_t* flags, uint32_t* id) {
+  return build_tank_export(true, nn, nodes, nm, meshes, fluid, dx, fill, n, x, v, nrm, m, rho, T, phase, flags, id);
 }
 
 int cryo_solver_build_tank(cryo_solver* s, int32_t nn, const cryo_geo_node* nodes, int32_t nm,

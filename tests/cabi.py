@@ -504,6 +504,7 @@ class HipSolver:
                                            C.c_void_p, C.c_void_p, C.c_float, C.c_void_p, C.c_void_p]
             lib.cryo_build_tank_particles.argtypes = [C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
                                                       C.c_float, C.c_void_p, C.c_void_p] + [C.c_void_p] * 9
+            lib.cryo_build_tank_particles_device.argtypes = lib.cryo_build_tank_particles.argtypes
             lib.cryo_solver_build_tank.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p,
                                                    C.c_void_p]
             cls._lib = lib
@@ -699,12 +700,15 @@ def hip_ensemble_run_steps(solvers, steps):
         raise RuntimeError(lib.cryo_last_error().decode())
 
 
-def hip_build_tank_particles(nodes, fluid: Fluid, dx, fill: Fill) -> ParticleSet:
-    """build_tank_particles through the product's host-side generator (no GPU needed)."""
+def hip_build_tank_particles(nodes, fluid: Fluid, dx, fill: Fill, device: bool = False) -> ParticleSet:
+    """build_tank_particles through the product's generator: the host one (no
+    GPU needed) or, with device=True, the one whose lattice distances and wall
+    normals come from a kernel."""
     lib = HipSolver.load()
+    gen = lib.cryo_build_tank_particles_device if device else lib.cryo_build_tank_particles
     n = C.c_int32(0)
-    rc = lib.cryo_build_tank_particles(len(nodes), C.cast(nodes, C.c_void_p), 0, None, C.byref(fluid), dx,
-                                       C.byref(fill), C.byref(n), *([None] * 9))
+    rc = gen(len(nodes), C.cast(nodes, C.c_void_p), 0, None, C.byref(fluid), dx, C.byref(fill), C.byref(n),
+             *([None] * 9))
     if rc < 0:
         raise RuntimeError(lib.cryo_last_error().decode())
     cnt = n.value
@@ -718,9 +722,8 @@ def hip_build_tank_particles(nodes, fluid: Fluid, dx, fill: Fill) -> ParticleSet
     fl = np.empty(cnt, np.uint8)
     id_ = np.empty(cnt, np.uint32)
     vp = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
-    rc = lib.cryo_build_tank_particles(len(nodes), C.cast(nodes, C.c_void_p), 0, None, C.byref(fluid), dx,
-                                       C.byref(fill), C.byref(n), vp(x), vp(v), vp(nrm), vp(m), vp(rho), vp(T), vp(ph),
-                                       vp(fl), vp(id_))
+    rc = gen(len(nodes), C.cast(nodes, C.c_void_p), 0, None, C.byref(fluid), dx,
+             C.byref(fill), C.byref(n), vp(x), vp(v), vp(nrm), vp(m), vp(rho), vp(T), vp(ph), vp(fl), vp(id_))
     if rc < 0:
         raise RuntimeError(lib.cryo_last_error().decode())
     return ParticleSet(x, v, nrm, m, rho, T, ph, fl, id_)
