@@ -4,6 +4,8 @@ which is bit-exact with the reference's build_tank_particles
 (geometry.cpp:242-330): every field of every particle must agree bit for bit,
 for analytic tanks and a CSG tree with translations and a difference, and at
 the full-size LH2 sphere (1.18 M particles)."""
+import time
+
 import numpy as np
 import pytest
 
@@ -34,7 +36,13 @@ def test_device_lattice_matches_host(name):
 def test_device_lattice_full_size_sphere():
     print("test_device_lattice_full_size_sphere", flush=True)
     fluid = cabi.hip_fluid_preset("LH2")
+    cabi.hip_build_tank_particles(cabi.sphere(0.1), fluid, 0.008, cabi.default_fill(), device=True)  # warm-up
+    t0 = time.perf_counter()
     h = cabi.hip_build_tank_particles(cabi.sphere(0.5), fluid, 0.008, cabi.default_fill())
+    t1 = time.perf_counter()
     d = cabi.hip_build_tank_particles(cabi.sphere(0.5), fluid, 0.008, cabi.default_fill(), device=True)
+    t2 = time.perf_counter()
+    # each call builds twice (count, then fill)
+    print(f"host generator {t1 - t0:.3f} s, device-evaluated {t2 - t1:.3f} s", flush=True)
     assert len(h) == 1176208
     same(h, d)
