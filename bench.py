@@ -205,10 +205,12 @@ def c5_case(cabi, steps, segments=16, seed=0, superheat=0.2):
 def adjoint_measure(cabi, steps, every):
     """Wall-clock of the forward pass alone and of forward + reverse sweep
     (cryo_solver_adjoint_slosh with and without the gradient) on the
-    differentiable-slosh workload, each on a fresh, initialised solver."""
+    differentiable-slosh workload, each on a fresh, initialised solver; the
+    fastest of three interleaved runs of each."""
     fluid, cfg, ps, lo, hi, acc = c5_case(cabi, steps)
     times = {}
-    for name, grad in (("forward", False), ("adjoint", True), ("forward2", False), ("adjoint2", True)):
+    for name, grad in (("forward", False), ("adjoint", True), ("forward2", False), ("adjoint2", True),
+                       ("forward3", False), ("adjoint3", True)):
         s = cabi.HipSolver(cfg, fluid)
         s.set_particles(ps, lo, hi)
         s.initialize()
@@ -220,8 +222,8 @@ def adjoint_measure(cabi, steps, every):
         if grad:
             gnorm, gtheta, glee = float(np.abs(out[1]).max()), out[3], out.lee
         del s
-    fwd = min(times["forward"], times["forward2"])
-    adj = min(times["adjoint"], times["adjoint2"])
+    fwd = min(times["forward"], times["forward2"], times["forward3"])
+    adj = min(times["adjoint"], times["adjoint2"], times["adjoint3"])
     return {"workload": f"LN2 cylinder r=0.1 h=0.25 fill 0.5 dx=0.0037 CSF contact angle 30 deg, Lee 5, "
                         f"liquid 0.2 K superheated, vented, {len(ps)} particles, {steps} steps of "
                         f"dt={cfg.fixed_dt:.3e} s, 16-segment frame-acceleration profile; "
