@@ -405,8 +405,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_wall(PV pv, GridDev g, SimParams
   block_sums_store<3>(gs, A.partials, A.stride);
 }
 
-// TV: the transport-velocity terms are compiled in (TV); without them
-// the kernel holds fewer registers.
+// TV: the transport-velocity terms are compiled in (launched so when A.tvf
+// is set); without them the kernel holds fewer registers.
 template <bool TV>
 __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParams P, NbrList L, const DevState* st,
                                                      AdjArgs A, int n) {
