@@ -327,6 +327,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_csf2(PV pv, GridDev g, SimParams
   block_sums_store<2>(th, A.partials + row * A.stride, A.stride);
 }
 
+template <bool TV>  // the transport-velocity terms compiled in (launched so when A.tvf is set)
 __global__ __launch_bounds__(kBlock) void k_adj_wall(PV pv, GridDev g, SimParams P, NbrList L, const DevState* st,
                                                     AdjArgs A, int n) {
   const int w = gtid();
@@ -365,7 +366,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_wall(PV pv, GridDev g, SimParams
         pb0 += o.p_w;
       }
       vwb -= o.vij;
-      if (A.tvf) {  // a transport owner's wall volume reads this wall's pressure
+      if (TV) {  // a transport owner's wall volume reads this wall's pressure
         const float4 tv = A.tvs[j];
         if (tv.w > 0.0f) {
           const float Ti = pv.s.T[j];
@@ -1163,7 +1164,8 @@ void launch_adj_step(const PV& pv, const GridDev& g, const SimParams& P, const N
     k_adj_csf2<<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n, 6);
     k_adj_finish<<<1, 1024, 0, s>>>(A.partials, A.stride, nb, 6, -1, 1.0, theta_out, 2);
   }
-  k_adj_wall<<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
+  if (A.tvf) k_adj_wall<true><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
+  else k_adj_wall<false><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
   if (A.tvf) k_adj_fluid<true><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
   else k_adj_fluid<false><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
   if (A.sat) {
