@@ -163,3 +163,17 @@ def _bad_angle(cs):
     c = cs.SolverConfig()
     c.contact_angle_deg = 200.0
     return c
+
+
+def test_device_lattice_rejects_too_deep_csg_before_any_device_work():
+    """cryo_build_tank_particles_device walks the CSG tree with a bounded stack:
+    a deeper tree is an invalid argument raised on the host, before any device
+    call; the host generator takes the same tree."""
+    fluid = cabi.hip_fluid_preset("LN2")
+    nodes = [(cabi.GEO_BOX, -1, -1, (0.04, 0.04, 0.04))]
+    nodes += [(cabi.GEO_TRANSLATED, k, -1, (0.0, 0.0, 0.0)) for k in range(40)]
+    with pytest.raises(RuntimeError, match="too deep"):
+        cabi.hip_build_tank_particles(cabi.geo_nodes(nodes), fluid, 0.004, cabi.default_fill(), device=True)
+    ps = cabi.hip_build_tank_particles(cabi.geo_nodes(nodes), fluid, 0.004, cabi.default_fill())
+    ref = cabi.hip_build_tank_particles(cabi.box(0.04, 0.04, 0.04), fluid, 0.004, cabi.default_fill())
+    assert np.array_equal(ps.x, ref.x) and np.array_equal(ps.phase, ref.phase)
