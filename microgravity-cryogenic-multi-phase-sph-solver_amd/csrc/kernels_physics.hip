@@ -786,6 +786,19 @@ struct TileRows {
 };
 
 // Row tables of tile `b`; called by the whole block, ends with a barrier.
+// Block b of an nblocks grid -> the tile it runs.  The hardware deals
+// workgroups round-robin to the 8 XCDs (XCD b % 8); within each group of
+// 8 x kTileChunk blocks, XCD x runs the chunk of kTileChunk consecutive tiles
+// [x kTileChunk, (x + 1) kTileChunk), so neighbouring tiles, whose staged halos
+// overlap, meet in one L2 while the chunks stay spread over the XCDs.  The
+// last, partial group keeps the identity.  A bijection of [0, nblocks).
+constexpr int kTileChunk = 16;
+__device__ inline int xcd_tile(int b, int nblocks) {
+  constexpr int span = 8 * kTileChunk;
+  if (b >= nblocks / span * span) return b;
+  return b / span * span + (b % 8) * kTileChunk + (b / 8) % kTileChunk;
+}
+
 __device__ inline void tile_rows(const GridDev& g, int b, TileRows& T) {
   const int tiles_x = (g.nx + kTileCells - 1) / kTileCells;
   const int tiles_y = (g.ny + kTileCells - 1) / kTileCells;
@@ -906,7 +919,7 @@ __global__ __launch_bounds__(kTileThreads) void k_forces_tiled(PV pv, GridDev g,
   __shared__ float4 s_x[kTileCap], s_rec[kTileCap], s_v[kTileCap], s_vt[kTileCap];
   __shared__ float s_V[kTileCap];
   __shared__ int s_work[kTileThreads], s_wave[kTileThreads / 64];
-  tile_rows(g, blockIdx.x, T);
+  tile_rows(g, xcd_tile(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x)), T);
   const int ntarget = T.tbase[T.ntrows];
   if (ntarget == 0) return;  // uniform over the block
   const int M = T.base[T.nrows];
