@@ -337,6 +337,21 @@ int cryo_solver_telemetry_pending_size(cryo_solver* s, int32_t* n);
 int cryo_solver_fetch_telemetry(cryo_solver* s, double* time, int64_t* step, int32_t* n, float* xyz,
                                 float* T, float* p, float* speed, uint8_t* phase);
 
+/* ------------------------------------------------------------ files ---- */
+/* Host-side writers; they need no device.
+ * write_vtk (io.cpp:52-87): legacy binary VTK POLYDATA (big-endian) of n
+ * particles from host arrays (x, v: 3n floats); pressure may be NULL, which
+ * leaves that point array out as the reference does for a size mismatch. */
+int cryo_write_vtk(const char* path, int32_t n, const float* x, const float* v, const float* T, const float* m,
+                   const float* rho, const uint8_t* phase, const float* pressure);
+/* write_json_frame (io.cpp:89-142) of a TelemetryFrame (telemetry.hpp:15-24)
+ * given as arrays: xyz 3n, T, p (NULL = zeros), speed, phase; walls are left
+ * out unless include_walls. */
+int cryo_write_json_frame(const char* path, double time, int64_t step, int32_t n, const float* xyz, const float* T,
+                          const float* p, const float* speed, const uint8_t* phase, int32_t include_walls);
+/* make_frame's speed column (io.cpp:100-117): |v| of each particle in float. */
+void cryo_frame_speed(int32_t n, const float* v, float* speed);
+
 /* ------------------------------------------------------- geometry ---- */
 /* Tank geometry as a flat node list (geometry.hpp:22-150).  Node k refers to
  * children by index < k; the last node is the root. */

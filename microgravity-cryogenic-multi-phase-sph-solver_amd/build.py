@@ -27,7 +27,7 @@ OBJ = HERE / "_build"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 LIB_SOURCES = ["kernels_grid.hip", "kernels_physics.hip", "kernels_step.hip", "kernels_adjoint.hip", "solver.cpp",
-               "geometry.cpp"]
+               "geometry.cpp", "io.cpp"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wno-unknown-pragmas",
           f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 # -simplifycfg-sink-common=false: merging the identical tails of two branches

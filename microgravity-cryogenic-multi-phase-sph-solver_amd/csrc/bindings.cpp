@@ -310,91 +310,26 @@ py::dict particles_dict(const Solver& S) {
   return d;
 }
 
-// Legacy binary VTK POLYDATA, big-endian (the reference's write_vtk format, io.cpp:52-87).
-template <class T>
-void put_be(std::ofstream& o, T v) {
-  unsigned char b[sizeof(T)];
-  std::memcpy(b, &v, sizeof(T));
-  for (std::size_t k = 0; k < sizeof(T); ++k) o.put(static_cast<char>(b[sizeof(T) - 1 - k]));
-}
+// Solver.write_vtk / write_json (bindings.cpp:491-497 of the reference): the
+// current state downloaded, then the C ABI's file writers (io.cpp:52-142 formats).
 void write_vtk(const Solver& S, const std::string& path) {
   const auto x = S.f3(CRYO_F_X), v = S.f3(CRYO_F_V);
   const auto T = S.f1<float>(CRYO_F_T), m = S.f1<float>(CRYO_F_M), rho = S.f1<float>(CRYO_F_RHO),
              p = S.f1<float>(CRYO_F_P);
   const auto ph = S.f1<uint8_t>(CRYO_F_PHASE);
-  const std::size_t n = T.size();
-  std::ofstream out(path, std::ios::binary);
-  if (!out) throw std::runtime_error("write_vtk: cannot open '" + path + "'");
-  out << "# vtk DataFile Version 3.0\nCryoSPH-CUDA particles\nBINARY\nDATASET POLYDATA\n";
-  out << "POINTS " << n << " float\n";
-  for (std::size_t i = 0; i < 3 * n; ++i) put_be(out, x[i]);
-  out << "\nVERTICES " << n << ' ' << 2 * n << '\n';
-  for (std::size_t i = 0; i < n; ++i) {
-    put_be(out, int32_t{1});
-    put_be(out, static_cast<int32_t>(i));
-  }
-  out << "\nPOINT_DATA " << n << '\n';
-  auto scalar = [&](const char* name, auto get) {
-    out << "SCALARS " << name << " float 1\nLOOKUP_TABLE default\n";
-    for (std::size_t i = 0; i < n; ++i) put_be(out, static_cast<float>(get(i)));
-    out << '\n';
-  };
-  scalar("temperature", [&](std::size_t i) { return T[i]; });
-  scalar("mass", [&](std::size_t i) { return m[i]; });
-  scalar("density", [&](std::size_t i) { return rho[i]; });
-  scalar("phase", [&](std::size_t i) { return static_cast<float>(ph[i]); });
-  scalar("pressure", [&](std::size_t i) { return p[i]; });
-  out << "VECTORS velocity float\n";
-  for (std::size_t i = 0; i < 3 * n; ++i) put_be(out, v[i]);
-  out << '\n';
-  if (!out) throw std::runtime_error("write_vtk: write failed for '" + path + "'");
+  check(cryo_write_vtk(path.c_str(), static_cast<int32_t>(T.size()), x.data(), v.data(), T.data(), m.data(),
+                       rho.data(), ph.data(), p.data()));
 }
 
-// JSON viewer frame (the reference's write_json_frame format, io.cpp:89-142).
 void write_json(const Solver& S, const std::string& path, bool include_walls) {
   const auto x = S.f3(CRYO_F_X), v = S.f3(CRYO_F_V);
   const auto T = S.f1<float>(CRYO_F_T), p = S.f1<float>(CRYO_F_P);
   const auto ph = S.f1<uint8_t>(CRYO_F_PHASE);
-  std::vector<int> keep;
-  for (std::size_t i = 0; i < ph.size(); ++i)
-    if (include_walls || ph[i] != CRYO_WALL) keep.push_back(static_cast<int>(i));
-  std::FILE* f = std::fopen(path.c_str(), "wb");
-  if (!f) throw std::runtime_error("write_json_frame: cannot open '" + path + "'");
-  std::fprintf(f, "{\"time\":%.9g,\"step\":%lld,\"n\":%zu", S.time(), S.steps(), keep.size());
-  std::fputs(",\"position\":[", f);
-  bool first = true;
-  for (int i : keep) {
-    std::fprintf(f, first ? "%.6g,%.6g,%.6g" : ",%.6g,%.6g,%.6g", double(x[3 * i]), double(x[3 * i + 1]),
-                 double(x[3 * i + 2]));
-    first = false;
-  }
-  std::fputs("],\"phase\":[", f);
-  first = true;
-  for (int i : keep) {
-    std::fprintf(f, first ? "%u" : ",%u", unsigned(ph[i]));
-    first = false;
-  }
-  std::fputc(']', f);
-  auto arr = [&](const char* name, auto get) {
-    std::fprintf(f, ",\"%s\":[", name);
-    bool fst = true;
-    for (int i : keep) {
-      const float val = get(i);
-      std::fprintf(f, fst ? "%.6g" : ",%.6g", std::isfinite(val) ? double(val) : 0.0);
-      fst = false;
-    }
-    std::fputc(']', f);
-  };
-  arr("temperature", [&](int i) { return T[i]; });
-  arr("pressure", [&](int i) { return p[i]; });
-  arr("speed", [&](int i) {
-    const float a = v[3 * i], b = v[3 * i + 1], c = v[3 * i + 2];
-    return std::sqrt(a * a + b * b + c * c);
-  });
-  std::fputs("}\n", f);
-  const bool bad = std::ferror(f);
-  std::fclose(f);
-  if (bad) throw std::runtime_error("write_json_frame: write failed for '" + path + "'");
+  const int32_t n = static_cast<int32_t>(T.size());
+  std::vector<float> speed(T.size());
+  cryo_frame_speed(n, v.data(), speed.data());
+  check(cryo_write_json_frame(path.c_str(), S.time(), S.steps(), n, x.data(), T.data(), p.data(), speed.data(),
+                              ph.data(), include_walls ? 1 : 0));
 }
 
 cryo_fill default_fill() {

@@ -24,6 +24,25 @@ namespace cryo {
                                __FILE__ + ":" + std::to_string(__LINE__));                               \
   } while (0)
 
+// Keeps `what` as the calling thread's cryo_last_error() message; returns code.
+int record_error(const char* what, int code);
+
+// Body of a C ABI entry point: C++ exceptions become status codes (the
+// reference's exception types: invalid_argument, out_of_range, runtime_error).
+#define CRYO_GUARD(...)                                                    \
+  try {                                                                    \
+    __VA_ARGS__;                                                           \
+    return CRYO_OK;                                                        \
+  } catch (const std::invalid_argument& e) {                               \
+    return ::cryo::record_error(e.what(), CRYO_ERR_INVALID_ARGUMENT);      \
+  } catch (const std::out_of_range& e) {                                   \
+    return ::cryo::record_error(e.what(), CRYO_ERR_OUT_OF_RANGE);          \
+  } catch (const std::exception& e) {                                      \
+    return ::cryo::record_error(e.what(), CRYO_ERR_RUNTIME);               \
+  } catch (...) {                                                          \
+    return ::cryo::record_error("unknown error", CRYO_ERR_RUNTIME);        \
+  }
+
 // HostParticles (particle_system.hpp:76-90): SoA host mirror.
 struct HostParticles {
   std::vector<V3> x, v, nrm;
@@ -64,6 +83,7 @@ class Geometry {
     std::vector<int32_t> t;
   };
   double sdf_node(int k, double x, double y, double z) const;
+  double mesh_sdf(int mesh_id, double x, double y, double z) const;
   void bounds_node(int k, V3& lo, V3& hi) const;
   double volume_node(int k) const;
   std::vector<cryo_geo_node> nodes_;

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -1722,32 +1723,107 @@ struct cryo_solver {
 };
 
 // ------------------------------------------------------------ C ABI glue
-namespace {
-
-int fail(const char* what, int code) {
+int cryo::record_error(const char* what, int code) {
   g_err = what;
   return code;
 }
 
-#define CRYO_GUARD(...)                                                  \
-  try {                                                                  \
-    __VA_ARGS__;                                                         \
-    return CRYO_OK;                                                      \
-  } catch (const std::invalid_argument& e) {                             \
-    return fail(e.what(), CRYO_ERR_INVALID_ARGUMENT);                    \
-  } catch (const std::out_of_range& e) {                                 \
-    return fail(e.what(), CRYO_ERR_OUT_OF_RANGE);                        \
-  } catch (const std::exception& e) {                                    \
-    return fail(e.what(), CRYO_ERR_RUNTIME);                             \
-  } catch (...) {                                                        \
-    return fail("unknown error", CRYO_ERR_RUNTIME);                      \
-  }
+namespace {
+
+int fail(const char* what, int code) { return cryo::record_error(what, code); }
+
 
 cryo_solver& S(cryo_solver* s) {
   if (!s) throw std::invalid_argument("null solver");
   return *s;
 }
 
+}  // namespace
+
+namespace {
+// The byte image of every input of one lattice build: two calls with equal
+// images build the same particles.
+std::string lattice_inputs(bool on_device, int32_t nn, const cryo_geo_node* nodes, int32_t nm, const cryo_mesh* meshes,
+                           const cryo_fluid* fluid, float dx, const cryo_fill* fill) {
+  std::string key;
+  const auto put = [&key](const void* p, std::size_t bytes) {
+    if (bytes) key.append(static_cast<const char*>(p), bytes);
+  };
+  put(&on_device, sizeof on_device);
+  put(&nn, sizeof nn);
+  put(&nm, sizeof nm);
+  put(&dx, sizeof dx);
+  put(fluid, sizeof *fluid);
+  put(fill, sizeof *fill);
+  if (nodes && nn > 0) put(nodes, sizeof(cryo_geo_node) * static_cast<std::size_t>(nn));
+  for (int32_t k = 0; meshes && k < nm; ++k) {
+    put(&meshes[k].n_vertices, sizeof(int32_t));
+    put(&meshes[k].n_triangles, sizeof(int32_t));
+    if (meshes[k].vertices && meshes[k].n_vertices > 0)
+      put(meshes[k].vertices, 3 * sizeof(double) * static_cast<std::size_t>(meshes[k].n_vertices));
+    if (meshes[k].triangles && meshes[k].n_triangles > 0)
+      put(meshes[k].triangles, 3 * sizeof(int32_t) * static_cast<std::size_t>(meshes[k].n_triangles));
+  }
+  return key;
+}
+
+// The particles of the last counting call (x == NULL), kept for the filling
+// call with the same inputs that normally follows it, so that the two-call
+// protocol builds the lattice once.
+struct LatticeMemo {
+  std::mutex lock;
+  std::string inputs;
+  std::shared_ptr<const HostParticles> particles;
+};
+LatticeMemo& lattice_memo() {
+  static LatticeMemo memo;
+  return memo;
+}
+
+int build_tank_export(bool on_device, int32_t nn, const cryo_geo_node* nodes, int32_t nm, const cryo_mesh* meshes,
+                      const cryo_fluid* fluid, float dx, const cryo_fill* fill, int32_t* n, float* x, float* v,
+                      float* nrm, float* m, float* rho, float* T, uint8_t* phase, uint8_t* flags, uint32_t* id) {
+  CRYO_GUARD({
+    if (!fluid || !fill || !n) throw std::invalid_argument("cryo_build_tank_particles: null argument");
+    const std::string inputs = lattice_inputs(on_device, nn, nodes, nm, meshes, fluid, dx, fill);
+    LatticeMemo& memo = lattice_memo();
+    std::shared_ptr<const HostParticles> built;
+    {
+      std::lock_guard<std::mutex> g(memo.lock);
+      if (memo.particles && memo.inputs == inputs) built = memo.particles;
+      if (x) {  // a filling call ends the pair
+        memo.particles.reset();
+        memo.inputs.clear();
+      }
+    }
+    if (!built) {
+      built = std::make_shared<const HostParticles>(
+          build_tank_particles(Geometry(nn, nodes, nm, meshes), *fluid, dx, *fill, on_device));
+      if (!x) {
+        std::lock_guard<std::mutex> g(memo.lock);
+        memo.inputs = inputs;
+        memo.particles = built;
+      }
+    }
+    const HostParticles& hp = *built;
+    const int cnt = static_cast<int>(hp.size());
+    if (x) {
+      if (*n < cnt) throw std::out_of_range("cryo_build_tank_particles: arrays too small");
+      for (int i = 0; i < cnt; ++i) {
+        x[3 * i] = hp.x[i].x; x[3 * i + 1] = hp.x[i].y; x[3 * i + 2] = hp.x[i].z;
+        if (v) { v[3 * i] = hp.v[i].x; v[3 * i + 1] = hp.v[i].y; v[3 * i + 2] = hp.v[i].z; }
+        if (nrm) { nrm[3 * i] = hp.nrm[i].x; nrm[3 * i + 1] = hp.nrm[i].y; nrm[3 * i + 2] = hp.nrm[i].z; }
+      }
+      if (m) std::copy(hp.m.begin(), hp.m.end(), m);
+      if (rho) std::copy(hp.rho.begin(), hp.rho.end(), rho);
+      if (T) std::copy(hp.T.begin(), hp.T.end(), T);
+      if (phase) std::copy(hp.phase.begin(), hp.phase.end(), phase);
+      if (flags) std::copy(hp.flags.begin(), hp.flags.end(), flags);
+      if (id) std::copy(hp.id.begin(), hp.id.end(), id);
+    }
+    *n = cnt;
+  })
+}
 }  // namespace
 
 extern "C" {
@@ -2245,32 +2321,6 @@ int cryo_mesh_winding_number(const cryo_mesh* mesh, double x, double y, double z
   })
 }
 
-namespace {
-int build_tank_export(bool on_device, int32_t nn, const cryo_geo_node* nodes, int32_t nm, const cryo_mesh* meshes,
-                      const cryo_fluid* fluid, float dx, const cryo_fill* fill, int32_t* n, float* x, float* v,
-                      float* nrm, float* m, float* rho, float* T, uint8_t* phase, uint8_t* flags, uint32_t* id) {
-  CRYO_GUARD({
-    if (!fluid || !fill || !n) throw std::invalid_argument("cryo_build_tank_particles: null argument");
-    const HostParticles hp = build_tank_particles(Geometry(nn, nodes, nm, meshes), *fluid, dx, *fill, on_device);
-    const int cnt = static_cast<int>(hp.size());
-    if (x) {
-      if (*n < cnt) throw std::out_of_range("cryo_build_tank_particles: arrays too small");
-      for (int i = 0; i < cnt; ++i) {
-        x[3 * i] = hp.x[i].x; x[3 * i + 1] = hp.x[i].y; x[3 * i + 2] = hp.x[i].z;
-        if (v) { v[3 * i] = hp.v[i].x; v[3 * i + 1] = hp.v[i].y; v[3 * i + 2] = hp.v[i].z; }
-        if (nrm) { nrm[3 * i] = hp.nrm[i].x; nrm[3 * i + 1] = hp.nrm[i].y; nrm[3 * i + 2] = hp.nrm[i].z; }
-      }
-      if (m) std::copy(hp.m.begin(), hp.m.end(), m);
-      if (rho) std::copy(hp.rho.begin(), hp.rho.end(), rho);
-      if (T) std::copy(hp.T.begin(), hp.T.end(), T);
-      if (phase) std::copy(hp.phase.begin(), hp.phase.end(), phase);
-      if (flags) std::copy(hp.flags.begin(), hp.flags.end(), flags);
-      if (id) std::copy(hp.id.begin(), hp.id.end(), id);
-    }
-    *n = cnt;
-  })
-}
-}  // namespace
 
 int cryo_build_tank_particles(int32_t nn, const cryo_geo_node* nodes, int32_t nm, const cryo_mesh* meshes,
                               const cryo_fluid* fluid, float dx, const cryo_fill* fill, int32_t* n, float* x,

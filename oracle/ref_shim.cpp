@@ -17,6 +17,7 @@
 
 #include "cryosph.h"
 #include "cryosph/core/geometry.hpp"
+#include "cryosph/core/io.hpp"
 #include "cryosph/core/particle_system.hpp"
 #include "cryosph/core/spatial_hash.hpp"
 #include "cryosph/physics/fluid_properties.hpp"
@@ -531,6 +532,32 @@ int ref_build_tank_particles(int nn, const cryo_geo_node* nodes, int nm, const c
       put(id, hp.id.data(), cnt);
     }
     *n = cnt;
+  })
+}
+
+// write_vtk (io.cpp:52-87) of float host arrays; pressure NULL = omitted.
+int ref_write_vtk(const char* path, int n, const float* x, const float* v, const float* T, const float* m,
+                  const float* rho, const uint8_t* phase, const float* pressure) {
+  GUARD({
+    HostParticles hp;
+    for (int i = 0; i < n; ++i)
+      hp.push(v3p(x + 3 * i), v3p(v + 3 * i), real(m[i]), real(rho[i]), real(T[i]), static_cast<Phase>(phase[i]));
+    std::vector<real> p;
+    if (pressure) p.assign(pressure, pressure + n);
+    write_vtk(path, hp, p);
+  })
+}
+
+// make_frame + write_json_frame (io.cpp:89-142) of float host arrays.
+int ref_write_json_frame(const char* path, double time, long long step, int n, const float* x, const float* v,
+                         const float* T, const float* pressure, const uint8_t* phase, int include_walls) {
+  GUARD({
+    HostParticles hp;
+    for (int i = 0; i < n; ++i)
+      hp.push(v3p(x + 3 * i), v3p(v + 3 * i), real(1), real(1), real(T[i]), static_cast<Phase>(phase[i]));
+    std::vector<real> p;
+    if (pressure) p.assign(pressure, pressure + n);
+    write_json_frame(path, make_frame(hp, p, time, step), include_walls != 0);
   })
 }
 

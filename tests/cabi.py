@@ -23,6 +23,8 @@ ORC_LIB = {32: ROOT / "oracle" / "liborc_f32.so", 64: ROOT / "oracle" / "liborc_
 REF_LIB = {32: ROOT / "oracle" / "_ref" / "libcryoref_f32.so", 64: ROOT / "oracle" / "_ref" / "libcryoref_f64.so"}
 
 LIQUID, VAPOR, WALL, DEAD = 0, 1, 2, 3
+# status codes of include/cryosph.h
+OK, ERR_INVALID_ARGUMENT, ERR_RUNTIME, ERR_OUT_OF_RANGE, ERR_NO_DEVICE = 0, -1, -2, -3, -4
 
 
 class Vec3(C.Structure):
@@ -442,12 +444,30 @@ class CpuSolver:
         return off, idx
 
 
-def cpu_build_tank_particles(kind, nodes, fluid: Fluid, dx, fill: Fill, bits=32) -> ParticleSet:
+def mesh_list(meshes):
+    """ctypes array of cryo_mesh over [(vertices (V, 3), triangles (T, 3)), ...];
+    returns (count, array or None, the numpy buffers it points into)."""
+    if not meshes:
+        return 0, None, []
+    keep = []
+    arr = (Mesh * len(meshes))()
+    for k, (verts, tris) in enumerate(meshes):
+        v = np.ascontiguousarray(verts, np.float64)
+        t = np.ascontiguousarray(tris, np.int32)
+        keep += [v, t]
+        arr[k].n_vertices, arr[k].n_triangles = len(v), len(t)
+        arr[k].vertices = v.ctypes.data_as(C.POINTER(C.c_double))
+        arr[k].triangles = t.ctypes.data_as(C.POINTER(C.c_int32))
+    return len(meshes), arr, keep
+
+
+def cpu_build_tank_particles(kind, nodes, fluid: Fluid, dx, fill: Fill, bits=32, meshes=None) -> ParticleSet:
     """build_tank_particles through the oracle or the reference build."""
     lib = CpuSolver.load(kind, bits)
     fn = getattr(lib, kind + "_build_tank_particles")
+    nm, marr, _keep = mesh_list(meshes)
     n = C.c_int(0)
-    rc = fn(C.c_int(len(nodes)), nodes, C.c_int(0), None, C.byref(fluid), C.c_double(dx), C.byref(fill),
+    rc = fn(C.c_int(len(nodes)), nodes, C.c_int(nm), marr, C.byref(fluid), C.c_double(dx), C.byref(fill),
             C.byref(n), None, None, None, None, None, None, None, None, None)
     _check(lib, kind + "_", rc)
     cnt = n.value
@@ -462,7 +482,7 @@ def cpu_build_tank_particles(kind, nodes, fluid: Fluid, dx, fill: Fill, bits=32)
     fl = np.empty(cnt, np.uint8)
     id_ = np.empty(cnt, np.uint32)
     vp = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
-    rc = fn(C.c_int(len(nodes)), nodes, C.c_int(0), None, C.byref(fluid), C.c_double(dx), C.byref(fill),
+    rc = fn(C.c_int(len(nodes)), nodes, C.c_int(nm), marr, C.byref(fluid), C.c_double(dx), C.byref(fill),
             C.byref(n), vp(x), vp(v), vp(nrm), vp(m), vp(rho), vp(T), vp(ph), vp(fl), vp(id_))
     _check(lib, kind + "_", rc)
     return ParticleSet(x, v, nrm, m, rho, T, ph, fl, id_)
@@ -700,14 +720,16 @@ def hip_ensemble_run_steps(solvers, steps):
         raise RuntimeError(lib.cryo_last_error().decode())
 
 
-def hip_build_tank_particles(nodes, fluid: Fluid, dx, fill: Fill, device: bool = False) -> ParticleSet:
+def hip_build_tank_particles(nodes, fluid: Fluid, dx, fill: Fill, device: bool = False, meshes=None) -> ParticleSet:
     """build_tank_particles through the product's generator: the host one (no
     GPU needed) or, with device=True, the one whose lattice distances and wall
     normals come from a kernel."""
     lib = HipSolver.load()
     gen = lib.cryo_build_tank_particles_device if device else lib.cryo_build_tank_particles
+    nm, marr, _keep = mesh_list(meshes)
+    mp = C.cast(marr, C.c_void_p) if marr is not None else None
     n = C.c_int32(0)
-    rc = gen(len(nodes), C.cast(nodes, C.c_void_p), 0, None, C.byref(fluid), dx, C.byref(fill), C.byref(n),
+    rc = gen(len(nodes), C.cast(nodes, C.c_void_p), nm, mp, C.byref(fluid), dx, C.byref(fill), C.byref(n),
              *([None] * 9))
     if rc < 0:
         raise RuntimeError(lib.cryo_last_error().decode())
@@ -722,7 +744,7 @@ def hip_build_tank_particles(nodes, fluid: Fluid, dx, fill: Fill, device: bool =
     fl = np.empty(cnt, np.uint8)
     id_ = np.empty(cnt, np.uint32)
     vp = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
-    rc = gen(len(nodes), C.cast(nodes, C.c_void_p), 0, None, C.byref(fluid), dx,
+    rc = gen(len(nodes), C.cast(nodes, C.c_void_p), nm, mp, C.byref(fluid), dx,
              C.byref(fill), C.byref(n), vp(x), vp(v), vp(nrm), vp(m), vp(rho), vp(T), vp(ph), vp(fl), vp(id_))
     if rc < 0:
         raise RuntimeError(lib.cryo_last_error().decode())

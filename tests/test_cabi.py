@@ -92,6 +92,57 @@ def test_lattice_generator_matches_reference(name):
         assert np.array_equal(getattr(r, f).view(np.uint8), getattr(h, f).view(np.uint8)), f
 
 
+def icosphere(radius, levels=1):
+    """Closed triangle mesh of a sphere: the icosahedron, each face split in
+    four `levels` times, vertices pushed out to `radius`."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    verts = [np.array(v, float) / np.linalg.norm(v) for v in verts]
+    for _ in range(levels):
+        mid = {}
+
+        def midpoint(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in mid:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                mid[key] = len(verts) - 1
+            return mid[key]
+        nxt = []
+        for a, b, c in faces:
+            ab, bc, ca = midpoint(a, b), midpoint(b, c), midpoint(c, a)
+            nxt += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nxt
+    return np.array(verts) * radius, np.array(faces, np.int32)
+
+
+@needs_ref
+@pytest.mark.parametrize("shift", [(0.0, 0.0, 0.0), (0.013, -0.007, 0.021)])
+def test_mesh_lattice_matches_reference(shift):
+    """A mesh tank (80-triangle icosphere: nearest-triangle distances and the
+    winding-number sign) alone and translated inside a union with a box, settled
+    30 % fill along a slanted direction: every field bit for bit."""
+    verts, tris = icosphere(0.05)
+    nodes = [(cabi.GEO_MESH, -1, -1, ())]
+    if any(shift):
+        nodes += [(cabi.GEO_TRANSLATED, 0, -1, shift), (cabi.GEO_BOX, -1, -1, (0.03, 0.03, 0.14)),
+                  (cabi.GEO_UNION, 1, 2, ())]
+    arr = cabi.geo_nodes(nodes)
+    arr[0].mesh_id = 0
+    fluid = cabi.hip_fluid_preset("LN2")
+    fill = cabi.default_fill(fill_fraction=0.3, settle_direction=(0.3, 0.2, -1.0))
+    r = cabi.cpu_build_tank_particles("ref", arr, fluid, 0.006, fill, meshes=[(verts, tris)])
+    h = cabi.hip_build_tank_particles(arr, fluid, 0.006, fill, meshes=[(verts, tris)])
+    assert len(r) == len(h) > 1000
+    assert int((h.phase == cabi.WALL).sum()) > 0 and int((h.phase == cabi.VAPOR).sum()) > 0
+    for f in ("x", "v", "nrm", "m", "rho", "T", "phase", "flags", "id"):
+        assert np.array_equal(getattr(r, f).view(np.uint8), getattr(h, f).view(np.uint8)), f
+
+
 def test_c1_lattice_count():
     """LN2 cylinder tank r=0.05 h=0.12 dx=4 mm: 26,360 particles = 14,520 fluid + 11,840 wall."""
     ps = cabi.hip_build_tank_particles(cabi.cylinder(0.05, 0.12), cabi.hip_fluid_preset("LN2"), 0.004,
@@ -125,44 +176,69 @@ def test_maneuver_profile_interpolation():
                                       C.c_double(0.5), a, w, wd) == -1
 
 
-def test_python_module_surface_without_gpu():
-    """test_bindings.py:25-54 parts that need no device."""
+def test_python_module_agrees_with_the_c_abi_without_gpu():
+    """The CryoSPH module's device-free surface is the C ABI's: every
+    SolverConfig default, the fluid presets and their saturation curve, the
+    geometry classes' distances / bounds / volumes at random points, and
+    ManeuverProfile.evaluate, value for value."""
     import CryoSPH as cs
-    assert cs.__version__ and isinstance(cs.cuda_backend, bool)
-    ln2 = cs.FluidProperties.liquid_nitrogen()
-    assert ln2.name == "LN2" and ln2.rho_liquid / ln2.rho_vapor > 100
-    assert ln2.saturation_temperature(ln2.p_sat) == pytest.approx(ln2.T_sat, rel=1e-6)
-    p2 = ln2.saturation_pressure(ln2.T_sat + 2.0)
-    assert p2 > ln2.p_sat and ln2.saturation_temperature(p2) == pytest.approx(ln2.T_sat + 2.0, rel=1e-5)
-    tank = cs.CylinderTank(radius=0.1, height=0.3)
-    assert tank.sdf(0, 0, 0) == pytest.approx(-0.1)
-    assert tank.volume() == pytest.approx(math.pi * 0.01 * 0.3)
-    lo, hi = tank.bounds
-    assert lo[2] == pytest.approx(-0.15) and hi[0] == pytest.approx(0.1)
-    cfg = cs.SolverConfig()
-    cfg.contact_angle_deg = 12.5
-    cfg.surface_tension = cs.SurfaceTensionModel.Akinci
-    cfg.gravity = (0.0, 0.0, 0.0)
-    assert cfg.gravity == (0.0, 0.0, 0.0) and cfg.contact_angle_deg == 12.5
-    assert cfg.surface_tension == cs.SurfaceTensionModel.Akinci
-    box = cs.Translated(cs.BoxTank(0.1, 0.1, 0.1), (0.05, 0.0, 0.05))
-    block = cs.Translated(cs.BoxTank(0.02, 0.02, 0.02), (0.05, 0.0, 0.01))
-    geo = cs.Difference(box, block)
-    assert geo.sdf(0.05, 0.0, 0.01) > 0 and geo.sdf(0.02, 0.0, 0.05) < 0
-    assert cs.Union(box, block).sdf(0.05, 0.0, 0.01) < 0
-    prof = cs.ManeuverProfile([0.0, 1.0], np.array([[0, 0, 0], [2.0, 0, 0]]), np.array([[0, 0, 0], [0, 0, 1.0]]))
-    a, w, wd = prof.evaluate(0.5)
-    assert a == pytest.approx((1.0, 0.0, 0.0)) and w[2] == pytest.approx(0.5) and wd[2] == pytest.approx(1.0)
-    with pytest.raises(ValueError):
-        cs.ManeuverProfile([0.0, 0.0], np.zeros((2, 3)))
-    with pytest.raises(ValueError):
-        cs.Solver(cfg if False else _bad_angle(cs), ln2)
-
-
-def _bad_angle(cs):
-    c = cs.SolverConfig()
-    c.contact_angle_deg = 200.0
-    return c
+    lib = cabi.HipSolver.load()
+    assert cs.__version__ == lib.cryo_version().decode()
+    cfg, ref = cs.SolverConfig(), cabi.hip_config_default()
+    for name, _ in cabi._CONFIG_FIELDS:
+        got = getattr(cfg, name)
+        want = getattr(ref, name)
+        if name == "gravity":
+            assert got == pytest.approx(want.tup()), name
+        else:
+            assert float(int(got) if not isinstance(got, float) else got) == pytest.approx(float(want)), name
+    for make, name in ((cs.FluidProperties.liquid_nitrogen, "LN2"), (cs.FluidProperties.liquid_hydrogen, "LH2"),
+                       (cs.FluidProperties.liquid_oxygen, "LOX"), (cs.FluidProperties.water, "H2O")):
+        mod, c = make(), cabi.hip_fluid_preset(name)
+        for k in ("rho_liquid", "rho_vapor", "mu_liquid", "k_liquid", "cp_vapor", "sigma", "h_fg", "T_sat", "p_sat"):
+            assert np.float32(getattr(mod, k)) == getattr(c, k), (name, k)
+        for T in (c.T_sat - 3.0, c.T_sat + 0.5, c.T_sat + 4.0):
+            p = mod.saturation_pressure(T)
+            assert np.float32(p) == lib.cryo_fluid_saturation_pressure(C.byref(c), np.float32(T))
+            assert mod.saturation_temperature(p) == pytest.approx(T, rel=2e-5)
+    box = cs.Translated(cs.BoxTank(0.1, 0.08, 0.12), (0.05, -0.01, 0.06))
+    block = cs.Translated(cs.SphereTank(0.03), (0.06, 0.0, 0.02))
+    shapes = {
+        "cylinder": (cs.CylinderTank(radius=0.07, height=0.2), cabi.cylinder(0.07, 0.2), math.pi * 0.07 ** 2 * 0.2),
+        "capsule": (cs.CapsuleTank(0.04, 0.1), cabi.capsule(0.04, 0.1),
+                    math.pi * 0.04 ** 2 * 0.1 + 4 / 3 * math.pi * 0.04 ** 3),
+        "difference": (cs.Difference(box, block), cabi.geo_nodes([
+            (cabi.GEO_BOX, -1, -1, (0.1, 0.08, 0.12)), (cabi.GEO_TRANSLATED, 0, -1, (0.05, -0.01, 0.06)),
+            (cabi.GEO_SPHERE, -1, -1, (0.03,)), (cabi.GEO_TRANSLATED, 2, -1, (0.06, 0.0, 0.02)),
+            (cabi.GEO_DIFFERENCE, 1, 3, ())]), None),
+        "union": (cs.Union(box, block), cabi.geo_nodes([
+            (cabi.GEO_BOX, -1, -1, (0.1, 0.08, 0.12)), (cabi.GEO_TRANSLATED, 0, -1, (0.05, -0.01, 0.06)),
+            (cabi.GEO_SPHERE, -1, -1, (0.03,)), (cabi.GEO_TRANSLATED, 2, -1, (0.06, 0.0, 0.02)),
+            (cabi.GEO_UNION, 1, 3, ())]), None),
+    }
+    pts = np.random.default_rng(5).uniform(-0.15, 0.2, (64, 3))
+    for name, (geo, nodes, vol) in shapes.items():
+        out = C.c_double()
+        for q in pts:
+            assert lib.cryo_geo_sdf(len(nodes), nodes, 0, None, *map(C.c_double, q), C.byref(out)) == 0
+            assert geo.sdf(*q) == out.value, name
+        lo3, hi3 = (C.c_float * 3)(), (C.c_float * 3)()
+        assert lib.cryo_geo_bounds(len(nodes), nodes, 0, None, lo3, hi3) == 0
+        lo, hi = geo.bounds
+        assert np.array_equal(np.float32(lo), np.array(lo3)) and np.array_equal(np.float32(hi), np.array(hi3))
+        if vol is not None:
+            assert geo.volume() == pytest.approx(vol, rel=1e-12)
+    times = np.array([0.0, 0.4, 1.5, 2.0])
+    acc = np.array([[0, 0, 0], [0.3, -0.1, 0.0], [0.3, 0.2, 0.05], [0.0, 0.0, 0.0]], np.float32)
+    om = np.array([[0, 0, 0], [0, 0.01, 0], [0, 0.02, 0], [0, 0.02, 0]], np.float32)
+    prof = cs.ManeuverProfile(times, acc, om)
+    for t in (-1.0, 0.0, 0.2, 0.4, 1.0, 1.7, 2.0, 5.0):
+        a, w, wd = (C.c_float * 3)(), (C.c_float * 3)(), (C.c_float * 3)()
+        assert lib.cryo_maneuver_evaluate(len(times), times.ctypes.data_as(C.c_void_p), acc.ctypes.data_as(C.c_void_p),
+                                          om.ctypes.data_as(C.c_void_p), C.c_double(t), a, w, wd) == 0
+        ma, mw, mwd = prof.evaluate(t)
+        assert np.array_equal(np.float32(ma), np.array(a)) and np.array_equal(np.float32(mw), np.array(w))
+        assert np.array_equal(np.float32(mwd), np.array(wd)), t
 
 
 def test_device_lattice_rejects_too_deep_csg_before_any_device_work():
