@@ -23,6 +23,7 @@ import ctypes as C
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 from pathlib import Path
@@ -246,23 +247,67 @@ def profile_section(lib, handle, index):
     return name.value.decode(), ms.value, cnt.value
 
 
-def cpu_baseline(cabi, water, cfg, ps, lo, hi, steps):
-    """Reference CPU/OpenMP build (oracle/_ref) on the host cores, a bounded sample."""
-    cores = min(16, os.cpu_count() or 1)
+def host_cpu_facts():
+    """What the CPU baseline runs on: CPUs this process may use, physical cores
+    among them (distinct (socket, core) pairs of `lscpu -p`), the model name."""
+    allowed = sorted(os.sched_getaffinity(0))
+    cores, model = len(allowed), "unknown"
+    try:
+        rows = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True, check=True).stdout
+        pairs = {tuple(r.split(",")[1:3]) for r in rows.splitlines()
+                 if r and not r.startswith("#") and int(r.split(",")[0]) in allowed}
+        cores = len(pairs) or cores
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True, check=True).stdout.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.CalledProcessError, ValueError):
+        pass
+    return {"nproc": len(allowed), "physical_cores": cores, "model": model}
+
+
+def cpu_baseline(dx, steps, warmup=5):
+    """The reference CPU/OpenMP build (oracle/_ref; the restatement if that is
+    absent) on the host cores, a bounded sample of the same workload: `warmup`
+    untimed steps after the start-up evaluation, then `steps` timed steps
+    (benchmarks/perf_scaling.cpp:86-104).  One OpenMP thread per physical core
+    this process may use, capped by OMP_NUM_THREADS where that is set (the
+    host's share of CPUs); threads bound close, one per core.  It runs in a
+    child process so that the OpenMP placement applies from the runtime's
+    start."""
+    facts = host_cpu_facts()
+    threads = facts["physical_cores"]
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    cmd = [sys.executable, str(Path(__file__).resolve()), "--cpu-baseline-child", "--dx", str(dx),
+           "--cpu-steps", str(steps), "--warmup", str(warmup)]
+    r = subprocess.run(cmd, env=env, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("CPU baseline child failed:\n" + r.stdout + r.stderr)
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    res.update({"cores": threads, "threads": threads, "host_physical_cores": facts["physical_cores"],
+                "host": facts})
+    res["sample"] += (f", {threads} OpenMP threads (OMP_PROC_BIND=close, OMP_PLACES=cores) on a host with "
+                      f"nproc={facts['nproc']}, {facts['physical_cores']} physical cores ({facts['model']})")
+    return res
+
+
+def cpu_baseline_child(dx, steps, warmup):
+    import cabi
+
+    water, cfg, ps, lo, hi = c2_particles(cabi, dx)
     kind = "reference" if cabi.REF_LIB[32].exists() else "port"
     pref = "ref" if kind == "reference" else "orc"
-    lib = cabi.CpuSolver.load(pref, 32)
-    getattr(lib, pref + "_set_threads")(cores)
     s = cabi.CpuSolver(pref, cfg, water, 32)
     s.set_particles(ps, lo, hi)
-    s.step()  # start-up evaluation + one warm-up step
+    s.run_steps(warmup)  # start-up evaluation + warm-up steps
     t0 = time.perf_counter()
     s.run_steps(steps)
     sec = time.perf_counter() - t0
     n = s.num_particles
-    return {"value": n * steps / sec, "unit": "particle-steps/s", "cores": cores, "kind": kind,
-            "sample": f"same workload (N={n}), {steps} timed steps after 1 warm-up step, "
-                      f"{sec:.1f} s, OMP threads={cores}"}
+    print(json.dumps({"value": n * steps / sec, "unit": "particle-steps/s", "kind": kind,
+                      "sample": f"same workload (N={n}), {steps} timed steps after {warmup} warm-up steps, "
+                                f"{sec:.1f} s"}))
 
 
 def copy_calibration(mib):
@@ -293,7 +338,11 @@ def main():
     ap.add_argument("--checkpoint-every", type=int, default=20)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (scales the HBM byte counters) and exit")
+    ap.add_argument("--cpu-baseline-child", action="store_true", help=argparse.SUPPRESS)
     a = ap.parse_args()
+    if a.cpu_baseline_child:
+        cpu_baseline_child(a.dx, a.cpu_steps, a.warmup)
+        return
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
         return
@@ -355,7 +404,7 @@ def main():
     if a.adjoint_steps > 0:
         out["adjoint"] = adjoint_measure(cabi, a.adjoint_steps, a.checkpoint_every)
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cabi, water, cfg, ps, lo, hi, a.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(a.dx, a.cpu_steps)
     print(json.dumps(out))
 
 

@@ -363,7 +363,8 @@ enum {
   CRYO_GEO_TRANSLATED = 4, /* a = child, p0..p2 = offset        */
   CRYO_GEO_DIFFERENCE = 5, /* a minus solid b                   */
   CRYO_GEO_UNION = 6,      /* a united with b                   */
-  CRYO_GEO_MESH = 7        /* mesh_id = index into the mesh list */
+  CRYO_GEO_MESH = 7,       /* mesh_id = index into the mesh list */
+  CRYO_GEO_CALLBACK = 8    /* mesh_id = index into the callback list (a caller's own distance function) */
 };
 typedef struct cryo_geo_node {
   int32_t type, a, b, mesh_id;
@@ -374,6 +375,18 @@ typedef struct cryo_mesh {
   const double* vertices;   /* 3 * n_vertices */
   const int32_t* triangles; /* 3 * n_triangles */
 } cryo_mesh;
+
+/* A caller-defined distance function: a Geometry subclass of the caller's
+ * (geometry.hpp:22-34 lets callers override sdf / bounds / volume).  sdf is
+ * called from several host threads at once and must be thread-safe; volume < 0
+ * means unknown.  Tanks holding one are built on the host only. */
+typedef double (*cryo_sdf_fn)(void* ctx, double x, double y, double z);
+typedef struct cryo_geo_callback {
+  cryo_sdf_fn sdf;
+  void* ctx;
+  float lo[3], hi[3];
+  double volume;
+} cryo_geo_callback;
 
 /* FillSpec (geometry.hpp:162-175). NaN temperatures mean T_sat. */
 typedef struct cryo_fill {
@@ -413,6 +426,27 @@ int cryo_build_tank_particles_device(int32_t n_nodes, const cryo_geo_node* nodes
 /* Solver::build_tank (sph_solver.cu:430-437). */
 int cryo_solver_build_tank(cryo_solver* s, int32_t n_nodes, const cryo_geo_node* nodes, int32_t n_meshes,
                            const cryo_mesh* meshes, const cryo_fill* fill);
+/* build_tank_particles and Solver::build_tank for trees that hold
+ * CRYO_GEO_CALLBACK nodes (a caller's own Geometry subclasses); otherwise as
+ * cryo_build_tank_particles / cryo_solver_build_tank. */
+int cryo_build_tank_particles_cb(int32_t n_nodes, const cryo_geo_node* nodes, int32_t n_meshes,
+                                 const cryo_mesh* meshes, int32_t n_callbacks, const cryo_geo_callback* callbacks,
+                                 const cryo_fluid* fluid, float dx, const cryo_fill* fill, int32_t* n, float* x,
+                                 float* v, float* nrm, float* m, float* rho, float* T, uint8_t* phase,
+                                 uint8_t* flags, uint32_t* id);
+int cryo_solver_build_tank_cb(cryo_solver* s, int32_t n_nodes, const cryo_geo_node* nodes, int32_t n_meshes,
+                              const cryo_mesh* meshes, int32_t n_callbacks, const cryo_geo_callback* callbacks,
+                              const cryo_fill* fill);
+
+/* -------------------------------------------------------- streams ---- */
+/* mem::create_stream / destroy_stream / synchronize (cuda_utils.cpp:48-60)
+ * as opaque handles (a blocking HIP stream on the current device), and
+ * Solver::stream (sph_solver.hpp:181): the solver's own compute stream, for
+ * a caller that waits on it. */
+int cryo_stream_create(void** stream);
+int cryo_stream_destroy(void* stream);
+int cryo_stream_synchronize(void* stream);
+int cryo_solver_stream(cryo_solver* s, void** stream);
 
 #ifdef __cplusplus
 }

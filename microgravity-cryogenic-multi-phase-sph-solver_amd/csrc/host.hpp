@@ -67,14 +67,16 @@ struct HostParticles {
 // Geometry tree from the flat node list of the C ABI.
 class Geometry {
  public:
-  Geometry(int n_nodes, const cryo_geo_node* nodes, int n_meshes, const cryo_mesh* meshes);
+  Geometry(int n_nodes, const cryo_geo_node* nodes, int n_meshes, const cryo_mesh* meshes, int n_callbacks = 0,
+           const cryo_geo_callback* callbacks = nullptr);
   double sdf(double x, double y, double z) const;
   void bounds(V3& lo, V3& hi) const;
   double volume() const;
   V3 outward_normal(double x, double y, double z, double eps) const;
   double mesh_winding(int mesh_id, double x, double y, double z) const;
   const std::vector<cryo_geo_node>& node_list() const { return nodes_; }
-  bool has_mesh() const;
+  bool has_mesh() const;      // mesh or caller-defined nodes (evaluated on the host only)
+  bool has_callback() const;
   int depth() const;  // nodes on the longest root-to-leaf path
 
  private:
@@ -88,6 +90,7 @@ class Geometry {
   double volume_node(int k) const;
   std::vector<cryo_geo_node> nodes_;
   std::vector<MeshData> meshes_;
+  std::vector<cryo_geo_callback> callbacks_;
 };
 
 // on_device: the lattice's signed distances and wall normals are evaluated

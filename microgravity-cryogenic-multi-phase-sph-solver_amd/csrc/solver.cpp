@@ -1781,14 +1781,18 @@ LatticeMemo& lattice_memo() {
 }
 
 int build_tank_export(bool on_device, int32_t nn, const cryo_geo_node* nodes, int32_t nm, const cryo_mesh* meshes,
-                      const cryo_fluid* fluid, float dx, const cryo_fill* fill, int32_t* n, float* x, float* v,
-                      float* nrm, float* m, float* rho, float* T, uint8_t* phase, uint8_t* flags, uint32_t* id) {
+                      int32_t ncb, const cryo_geo_callback* callbacks, const cryo_fluid* fluid, float dx,
+                      const cryo_fill* fill, int32_t* n, float* x, float* v, float* nrm, float* m, float* rho,
+                      float* T, uint8_t* phase, uint8_t* flags, uint32_t* id) {
   CRYO_GUARD({
     if (!fluid || !fill || !n) throw std::invalid_argument("cryo_build_tank_particles: null argument");
-    const std::string inputs = lattice_inputs(on_device, nn, nodes, nm, meshes, fluid, dx, fill);
+    // a caller's distance function may change between the two calls, so such
+    // trees are built afresh every time
+    const bool memoise = ncb <= 0;
+    const std::string inputs = memoise ? lattice_inputs(on_device, nn, nodes, nm, meshes, fluid, dx, fill) : "";
     LatticeMemo& memo = lattice_memo();
     std::shared_ptr<const HostParticles> built;
-    {
+    if (memoise) {
       std::lock_guard<std::mutex> g(memo.lock);
       if (memo.particles && memo.inputs == inputs) built = memo.particles;
       if (x) {  // a filling call ends the pair
@@ -1798,8 +1802,8 @@ int build_tank_export(bool on_device, int32_t nn, const cryo_geo_node* nodes, in
     }
     if (!built) {
       built = std::make_shared<const HostParticles>(
-          build_tank_particles(Geometry(nn, nodes, nm, meshes), *fluid, dx, *fill, on_device));
-      if (!x) {
+          build_tank_particles(Geometry(nn, nodes, nm, meshes, ncb, callbacks), *fluid, dx, *fill, on_device));
+      if (!x && memoise) {
         std::lock_guard<std::mutex> g(memo.lock);
         memo.inputs = inputs;
         memo.particles = built;
@@ -2326,20 +2330,36 @@ int cryo_build_tank_particles(int32_t nn, const cryo_geo_node* nodes, int32_t nm
                               const cryo_fluid* fluid, float dx, const cryo_fill* fill, int32_t* n, float* x,
                               float* v, float* nrm, float* m, float* rho, float* T, uint8_t* phase, uint8_t* flags,
                               uint32_t* id) {
-  return build_tank_export(false, nn, nodes, nm, meshes, fluid, dx, fill, n, x, v, nrm, m, rho, T, phase, flags, id);
+  return build_tank_export(false, nn, nodes, nm, meshes, 0, nullptr, fluid, dx, fill, n, x, v, nrm, m, rho, T, phase,
+                           flags, id);
+}
+int cryo_build_tank_particles_cb(int32_t nn, const cryo_geo_node* nodes, int32_t nm, const cryo_mesh* meshes,
+                                 int32_t ncb, const cryo_geo_callback* callbacks, const cryo_fluid* fluid, float dx,
+                                 const cryo_fill* fill, int32_t* n, float* x, float* v, float* nrm, float* m,
+                                 float* rho, float* T, uint8_t* phase, uint8_t* flags, uint32_t* id) {
+  return build_tank_export(false, nn, nodes, nm, meshes, ncb, callbacks, fluid, dx, fill, n, x, v, nrm, m, rho, T,
+                           phase, flags, id);
 }
 int cryo_build_tank_particles_device(int32_t nn, const cryo_geo_node* nodes, int32_t nm, const cryo_mesh* meshes,
                                      const cryo_fluid* fluid, float dx, const cryo_fill* fill, int32_t* n, float* x,
                                      float* v, float* nrm, float* m, float* rho, float* T, uint8_t* phase,
                                      uint8_t* flags, uint32_t* id) {
-  return build_tank_export(true, nn, nodes, nm, meshes, fluid, dx, fill, n, x, v, nrm, m, rho, T, phase, flags, id);
+  return build_tank_export(true, nn, nodes, nm, meshes, 0, nullptr, fluid, dx, fill, n, x, v, nrm, m, rho, T, phase,
+                           flags, id);
 }
 
 int cryo_solver_build_tank(cryo_solver* s, int32_t nn, const cryo_geo_node* nodes, int32_t nm,
                            const cryo_mesh* meshes, const cryo_fill* fill) {
+  return cryo_solver_build_tank_cb(s, nn, nodes, nm, meshes, 0, nullptr, fill);
+}
+
+int cryo_solver_build_tank_cb(cryo_solver* s, int32_t nn, const cryo_geo_node* nodes, int32_t nm,
+                              const cryo_mesh* meshes, int32_t ncb, const cryo_geo_callback* callbacks,
+                              const cryo_fill* fill) {
   CRYO_GUARD({
     cryo_solver& z = S(s);
-    const Geometry geo(nn, nodes, nm, meshes);
+    if (!fill) throw std::invalid_argument("cryo_solver_build_tank: null fill");
+    const Geometry geo(nn, nodes, nm, meshes, ncb, callbacks);
     const HostParticles hp = build_tank_particles(geo, z.fluid, z.cfg.dx, *fill);
     V3 lo, hi;
     geo.bounds(lo, hi);
@@ -2356,6 +2376,28 @@ int cryo_solver_build_tank(cryo_solver* s, int32_t nn, const cryo_geo_node* node
     const float l3[3] = {lo.x, lo.y, lo.z}, h3[3] = {hi.x, hi.y, hi.z};
     z.set_particles(cnt, xs.data(), vs.data(), ns.data(), hp.m.data(), hp.rho.data(), hp.T.data(), hp.phase.data(),
                     hp.flags.data(), hp.id.data(), l3, h3);
+  })
+}
+
+int cryo_stream_create(void** stream) {
+  CRYO_GUARD({
+    if (!stream) throw std::invalid_argument("cryo_stream_create: null argument");
+    check_device();
+    hipStream_t h = nullptr;
+    CRYO_HIP(hipStreamCreate(&h));  // blocking, default priority
+    *stream = h;
+  })
+}
+int cryo_stream_destroy(void* stream) {
+  CRYO_GUARD(if (stream) CRYO_HIP(hipStreamDestroy(static_cast<hipStream_t>(stream))))
+}
+int cryo_stream_synchronize(void* stream) {
+  CRYO_GUARD(CRYO_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream))))
+}
+int cryo_solver_stream(cryo_solver* s, void** stream) {
+  CRYO_GUARD({
+    if (!stream) throw std::invalid_argument("cryo_solver_stream: null argument");
+    *stream = S(s).stream;
   })
 }
 

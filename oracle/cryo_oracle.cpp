@@ -43,6 +43,9 @@ using R = ORC_REAL;
 using u8 = std::uint8_t;
 using u32 = std::uint32_t;
 
+// orc_round_positions (a diagnostic switch; off = the reference algorithm)
+inline bool g_round_positions = false;
+
 constexpr R kPi = R(3.14159265358979323846);  // cuda_utils.cuh:67
 constexpr u8 L = 0, V = 1, W = 2, D = 3;       // particle_system.hpp:13-23
 constexpr u8 F_HEATED = 1, F_SOURCE = 2, F_TARGET_VAPOR = 4, F_DYING = 8, F_SCRATCH = 2 | 4 | 8;
@@ -1550,6 +1553,7 @@ struct Solver {
         const V3 vt = tvf ? v + ps.ab[i] * half : v;
         ps.vt[i] = vt;
         ps.x[i] += vt * dt;
+        if (g_round_positions) ps.x[i] = V3{R(float(ps.x[i].x)), R(float(ps.x[i].y)), R(float(ps.x[i].z))};
         if (cont) ps.rho[i] += ps.drho[i] * dt;
       });
     }
@@ -3009,6 +3013,11 @@ double orc_saturation_pressure(const cryo_fluid* f, double T) {
 extern "C" {
 // Pins the OpenMP thread count: the FP64 block sums depend on it.
 void orc_set_threads(int n) { omp_set_num_threads(n); }
+// Diagnostic, not part of the reference algorithm: with on != 0 every drift
+// rounds the positions to float afterwards (in the FP64 build, the rest stays
+// double).  Measures how much of an FP32 run's error comes from storing
+// positions in float alone.
+void orc_round_positions(int on) { orc::g_round_positions = on != 0; }
 // Kernel value and radial derivative (SPH_kernels.cuh:14-58) for closed-form checks.
 void orc_kernel_eval(int type, double h, double r, double* W, double* dWdr) {
   const orc::Kern k = orc::Kern::make(type, orc::R(h));

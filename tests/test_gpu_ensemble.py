@@ -125,3 +125,47 @@ def test_launch_variants_give_the_same_bits_and_autotune_restores_the_state():
         _same(ref, s)
     with pytest.raises(ValueError):
         make(tanks[:1])[0].set_launch_variant(4)
+
+
+def test_c4_full_ensemble_bitwise_and_slosh_force_against_fp64_oracle():
+    """The whole batched-ensemble workload: the 64 LN2 / LOX cylinder and
+    capsule tanks of bench.c4_tanks(seed=0) (9.25 M particles, counts
+    log-uniform 20k-400k, fill U(0.2, 0.8), lateral acceleration
+    U(0.01, 0.1) m/s^2, Lee U(0.1, 20) 1/s, superheat U(0, 5) K, closed,
+    thermal + phase change), 5 steps through cryo_ensemble_run_steps.
+
+    Every member is bitwise its per-tank loop, so the ensemble's accuracy is
+    the loop's.  Each member's slosh force is compared with the FP64 oracle run
+    of the same tank and the table printed.  The plain FP32 form (the
+    reference's precision) is held to 5e-3 (measured worst 2.5e-3), not 1e-4:
+    at 1e-4 g the first drifts move particles by a few float ulps of their
+    coordinates and the net load follows those displacements, so FP32
+    position storage alone costs 2e-4..7e-4 (pinned on the CPU by
+    test_oracle.py::test_fp32_slosh_force_error_comes_from_float_positions)."""
+    import bench
+
+    print("test_c4_full_ensemble_bitwise_and_slosh_force_against_fp64_oracle", flush=True)
+    steps = 5
+    tanks = bench.c4_tanks(cabi, count=64, seed=0)
+    assert len(tanks) == 64 and sum(len(t["ps"]) for t in tanks) == 9245480
+    loop, ens = make(tanks), make(tanks)
+    for s in loop:
+        s.run_steps(steps)
+    cabi.hip_ensemble_run_steps(ens, steps)
+    worst = 0.0
+    print(f"{'member':>6} {'tank':<34} {'|F| (N)':>11} {'rel err vs FP64':>16}", flush=True)
+    for k, (t, a, b) in enumerate(zip(tanks, loop, ens)):
+        _same(a, b)
+        o = cabi.CpuSolver("orc", t["cfg"], t["fluid"], 64)
+        o.set_particles(t["ps"], t["lo"], t["hi"])
+        o.set_frame_motion(acc=(t["a_lat"], 0.0, 0.0))
+        o.run_steps(steps)
+        f32 = np.array(b.diagnostics().slosh_force.tup(), np.float64)
+        f64 = np.array(o.diagnostics().slosh_force.tup(), np.float64)
+        assert o.step_count == steps and o.num_particles == b.num_particles
+        err = float(np.linalg.norm(f32 - f64) / np.linalg.norm(f64))
+        worst = max(worst, err)
+        print(f"{k:>6} {t['desc']:<34} {np.linalg.norm(f64):>11.4e} {err:>16.3e}", flush=True)
+        del o
+    print(f"worst slosh-force relative error: {worst:.3e}", flush=True)
+    assert worst <= 5e-3

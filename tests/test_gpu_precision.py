@@ -108,13 +108,15 @@ def test_compensated_sums_against_fp64_oracle():
             assert plain[q] <= 1e-3 and comp[q] <= 1e-3, (q, plain[q], comp[q])
 
 
-C3_STEPS = 20
+C3_STEPS = 30
 
 
 def test_full_size_lh2_sphere_errors_against_fp64_oracle():
     """The same comparison on the full precision-form tank: LH2 sphere r = 0.5 m,
     dx = 8 mm (1,176,208 particles), the 0.05 m/s^2 lateral pulse for 2 s then a
-    0.02 rad/s slew, 20 fixed steps.  Written to
+    0.02 rad/s slew about y, 30 fixed steps.  The manoeuvre table is entered
+    1.96 s into the pulse, so that the steps cover its end, the 10 ms ramp
+    into the slew and 0.1 s of the slew.  Written to
     $CRYO_REPORT_DIR/precision_full.json when that is set."""
     say("test_full_size_lh2_sphere_errors_against_fp64_oracle")
     fluid = cabi.hip_fluid_preset("LH2")
@@ -128,8 +130,10 @@ def test_full_size_lh2_sphere_errors_against_fp64_oracle():
         ps.x[fl] += rng.uniform(-0.0004, 0.0004, size=(int(fl.sum()), 3)).astype(np.float32)
         return cfg, ps, ps.x.min(0) - np.float32(0.032), ps.x.max(0) + np.float32(0.032)
 
-    man = dict(time=[0.0, 0.01, 2.0, 2.01, 3.0], acc=[[0, 0, 0], [0.05, 0, 0], [0.05, 0, 0], [0, 0, 0], [0, 0, 0]],
-               omega=[[0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0.02], [0, 0, 0.02]])
+    t0 = -1.96
+    man = dict(time=[t0, t0 + 0.01, t0 + 2.0, t0 + 2.01, t0 + 3.0],
+               acc=[[0, 0, 0], [0.05, 0, 0], [0.05, 0, 0], [0, 0, 0], [0, 0, 0]],
+               omega=[[0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0.02, 0], [0, 0.02, 0]])
     cfg, ps, lo, hi = build(0)
     probe = HipSolver(cfg, fluid)
     probe.set_particles(ps, lo, hi)
@@ -145,6 +149,8 @@ def test_full_size_lh2_sphere_errors_against_fp64_oracle():
         s.run_steps(C3_STEPS)
         say(name, "N =", s.num_particles)
         d = s.diagnostics()
+        assert d.frame_angular_velocity.tup() == pytest.approx((0.0, 0.02, 0.0)), "the slew was not reached"
+        assert d.frame_acceleration.tup() == pytest.approx((0.0, 0.0, 0.0), abs=1e-7)
         ids = s.field("id")
         order = np.argsort(ids, kind="stable")
         results[name] = (d, ids[order], {f: np.asarray(s.field(f), np.float64)[order] for f in ("x", "v", "T")})
