@@ -307,6 +307,11 @@ int cryo_solver_evaluate_rates(cryo_solver* s);
 int cryo_hash_sort(int32_t n, const float* x, const uint8_t* phase, const float lo[3], const float hi[3],
                    float cell, uint32_t* perm, int32_t* n_alive, float radius, int32_t* counts,
                    uint64_t* id_sums);
+/* cryo_hash_sort for particles that carry their own (unique) ids: the id
+ * sums add those ids, perm still gives input positions. */
+int cryo_hash_sort_ids(int32_t n, const float* x, const uint8_t* phase, const uint32_t* id, const float lo[3],
+                       const float hi[3], float cell, uint32_t* perm, int32_t* n_alive, float radius, int32_t* counts,
+                       uint64_t* id_sums);
 
 /* ------------------------------------------------------ profiling ---- */
 /* HIP-event timing of the step's sections on the solver stream (no

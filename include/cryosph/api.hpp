@@ -358,8 +358,8 @@ class SpatialHash {
     std::vector<std::uint32_t> perm(h.size());
     int32_t alive = 0;
     std::vector<uint64_t> s64(sums ? h.size() : 0);
-    detail::check(cryo_hash_sort(n, xs.data(), h.phase.data(), lo, hi, cell_, perm.data(), &alive, radius, counts,
-                                 sums ? s64.data() : nullptr));
+    detail::check(cryo_hash_sort_ids(n, xs.data(), h.phase.data(), h.id.data(), lo, hi, cell_, perm.data(), &alive,
+                                     radius, counts, sums ? s64.data() : nullptr));
     if (sums) std::copy(s64.begin(), s64.end(), sums);
     perm.resize(static_cast<std::size_t>(alive));
     return perm;

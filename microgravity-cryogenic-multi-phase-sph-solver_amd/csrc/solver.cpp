@@ -2179,8 +2179,26 @@ int cryo_solver_evaluate_rates(cryo_solver* s) {
 
 int cryo_hash_sort(int32_t n, const float* x, const uint8_t* phase, const float lo[3], const float hi[3], float cell,
                    uint32_t* perm, int32_t* n_alive, float radius, int32_t* counts, uint64_t* id_sums) {
+  return cryo_hash_sort_ids(n, x, phase, nullptr, lo, hi, cell, perm, n_alive, radius, counts, id_sums);
+}
+
+int cryo_hash_sort_ids(int32_t n, const float* x, const uint8_t* phase, const uint32_t* id, const float lo[3],
+                       const float hi[3], float cell, uint32_t* perm, int32_t* n_alive, float radius, int32_t* counts,
+                       uint64_t* id_sums) {
   CRYO_GUARD({
     if (n < 0) throw std::invalid_argument("cryo_hash_sort: n < 0");
+    // the particles' ids ride through the sort; perm maps them back to input
+    // positions (ids must then be unique)
+    std::vector<uint32_t> where;
+    if (id) {
+      uint32_t top = 0;
+      for (int32_t k = 0; k < n; ++k) top = std::max(top, id[k]);
+      where.assign(static_cast<std::size_t>(top) + 1, UINT32_MAX);
+      for (int32_t k = 0; k < n; ++k) {
+        if (where[id[k]] != UINT32_MAX) throw std::invalid_argument("cryo_hash_sort: particle ids are not unique");
+        where[id[k]] = static_cast<uint32_t>(k);
+      }
+    }
     check_device();
     cryo_config c;
     config_default(&c);
@@ -2192,7 +2210,7 @@ int cryo_hash_sort(int32_t n, const float* x, const uint8_t* phase, const float 
       *n_alive = 0;
       return CRYO_OK;
     }
-    z.set_particles(n, x, nullptr, nullptr, ones.data(), ones.data(), ones.data(), phase, nullptr, nullptr, lo, hi);
+    z.set_particles(n, x, nullptr, nullptr, ones.data(), ones.data(), ones.data(), phase, nullptr, id, lo, hi);
     z.configure_grid(v3p(lo), v3p(hi), cell);
     const int before = z.n;
     z.grid_build(false);
@@ -2203,6 +2221,8 @@ int cryo_hash_sort(int32_t n, const float* x, const uint8_t* phase, const float 
     if (z.n > 0)
       CRYO_HIP(hipMemcpyAsync(perm, z.view().s.id, sizeof(uint32_t) * z.n, hipMemcpyDeviceToHost, z.stream));
     CRYO_HIP(hipStreamSynchronize(z.stream));
+    if (id)
+      for (int32_t k = 0; k < z.n; ++k) perm[k] = where[perm[k]];
     if (counts) z.neighbor_counts(radius, counts, id_sums);
   })
 }

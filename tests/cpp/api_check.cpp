@@ -195,7 +195,7 @@ void device_checks() {
   expect(solver.pressure().size() == solver.num_particles() && solver.curvature().size() == solver.num_particles(),
          "pressure and curvature");
   expect(solver.particle_system().capacity() >= 2 * solver.num_particles(), "capacity max(2n, n + 1024)");
-  expect(solver.grid().num_cells() > 0 && solver.grid().cell_size() == real(2 * 1.3 * 0.004f), "solver grid");
+  expect(solver.grid().num_cells() > 0 && solver.grid().cell_size() == solver.params().support, "solver grid");
   std::vector<int> counts;
   std::vector<std::uint64_t> sums;
   solver.grid().neighbor_stats(solver.particle_system(), solver.grid().cell_size(), counts, sums, solver.stream());
