@@ -277,6 +277,15 @@ int cryo_maneuver_evaluate(int32_t n, const double* time, const float* acc, cons
 
 /* Solver::probe_pressure (sph_solver.cu:738-748; kernel :197-216). */
 int cryo_solver_probe_pressure(cryo_solver* s, int32_t np, const float* points, float* out);
+/* cryo_solver_probe_pressure without allocating: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (HIP graph
+ * capture, tight serving loops).  At most 65536 points per call
+ * (CRYO_ERR_OUT_OF_RANGE beyond); the solver must already be initialised
+ * (a step or next_dt has run, CRYO_ERR_RUNTIME otherwise).  Points and
+ * results pass through one fixed 1 MiB device buffer per GPU that the
+ * library holds until the process ends, so calls that overlap on one GPU
+ * race.  Same results as cryo_solver_probe_pressure. */
+int cryo_solver_probe_pressure_noalloc(cryo_solver* s, int32_t np, const float* points, float* out);
 
 /* Spatial hash of the solver's current state (spatial_hash.cu:140-171):
  * per-particle count and id-sum of particles closer than `radius`. */

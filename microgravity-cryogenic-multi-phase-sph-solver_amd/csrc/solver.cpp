@@ -1722,6 +1722,34 @@ struct cryo_solver {
   }
 };
 
+// --------------------------------------- probe scratch (allocation-free entry)
+// cryo_solver_probe_pressure_noalloc keeps its probe points and results in
+// this fixed device array instead of buffers allocated per call: points in
+// [0, 3 kProbeScratchPoints), results after them (1 MiB).  Every device the
+// library runs on holds its own copy until the process ends; its address on
+// each device is looked up once (hipGetSymbolAddress) and used from then on,
+// never the array's name in host code.  Calls that overlap on one device
+// share it and race.
+constexpr int kProbeScratchPoints = 65536;
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) float scratch[4 * kProbeScratchPoints];
+std::mutex g_scratch_lock;
+float* g_scratch_on[kScratchDevices] = {};
+
+float* probe_scratch_on_current_device() {
+  int dev = 0;
+  CRYO_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kScratchDevices) throw std::runtime_error("probe scratch: device index out of range");
+  std::lock_guard<std::mutex> g(g_scratch_lock);
+  if (!g_scratch_on[dev]) {
+    void* p = nullptr;
+    CRYO_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw std::runtime_error("probe scratch: no device address");
+    g_scratch_on[dev] = static_cast<float*>(p);
+  }
+  return g_scratch_on[dev];
+}
+
 // ------------------------------------------------------------ C ABI glue
 int cryo::record_error(const char* what, int code) {
   g_err = what;
@@ -2116,6 +2144,28 @@ int cryo_solver_probe_pressure(cryo_solver* s, int32_t np, const float* points, 
     launch_probe(z.view(), z.grid(), z.P, dp.get(), dout.get(), np, z.stream);
     z.check_launch();
     CRYO_HIP(hipMemcpyAsync(out, dout.get(), sizeof(float) * np, hipMemcpyDeviceToHost, z.stream));
+    CRYO_HIP(hipStreamSynchronize(z.stream));
+  })
+}
+
+int cryo_solver_probe_pressure_noalloc(cryo_solver* s, int32_t np, const float* points, float* out) {
+  CRYO_GUARD({
+    if (np < 0 || np > kProbeScratchPoints)
+      throw std::out_of_range("cryo_solver_probe_pressure_noalloc: np must be within [0, " +
+                              std::to_string(kProbeScratchPoints) + "]");
+    cryo_solver& z = S(s);
+    // the start-up evaluation would allocate (cell tables, lists)
+    if (!z.initialized)
+      throw std::runtime_error(
+          "cryo_solver_probe_pressure_noalloc: the solver is not initialised yet (call next_dt or step first)");
+    if (np == 0) return CRYO_OK;
+    if (!points || !out) throw std::invalid_argument("cryo_solver_probe_pressure_noalloc: null argument");
+    float* const pts = probe_scratch_on_current_device();
+    float* const res = pts + 3 * kProbeScratchPoints;
+    CRYO_HIP(hipMemcpyAsync(pts, points, sizeof(float) * 3 * np, hipMemcpyHostToDevice, z.stream));
+    launch_probe(z.view(), z.grid(), z.P, pts, res, np, z.stream);
+    z.check_launch();
+    CRYO_HIP(hipMemcpyAsync(out, res, sizeof(float) * np, hipMemcpyDeviceToHost, z.stream));
     CRYO_HIP(hipStreamSynchronize(z.stream));
   })
 }
