@@ -80,10 +80,15 @@ typedef struct cryo_config {
   float vent_relaxation, merge_ratio, split_ratio, max_transfer_fraction, phase_change_density_limit,
       vapor_mass_rebalance;
   float cfl, viscous_number, force_number, capillary_number, thermal_number, dt_max, dt_min, fixed_dt;
-  /* Extension (no SolverConfig counterpart): 1 = precision form.  Neighbour
-   * sums use compensated (Kahan-Babuska) FP32 accumulation instead of the
-   * reference's plain running sums; results then differ from the reference in
-   * the last bits and are compared with the FP64 build instead.  Default 0. */
+  /* Extension (no SolverConfig counterpart): 1 = precision form, for small
+   * loads out of large, nearly cancelling forces (microgravity slosh).
+   * Positions are carried as double-float pairs (a float and its low-order
+   * part), pair distances taken from the pairs; neighbour sums use
+   * compensated (Kahan-Babuska) FP32 accumulation; the summation density and
+   * its EOS are evaluated in double.  Neighbour sets still come from the float
+   * positions.  Results differ from the reference in the last bits and are
+   * compared with the FP64 build instead.  The adjoint runs on this forward
+   * (its reverse sweep stays FP32).  Default 0. */
   int32_t compensated_sums;
 } cryo_config;
 

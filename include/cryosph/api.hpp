@@ -848,8 +848,9 @@ struct SolverConfig {
   real dt_max = real(1e-2);
   real dt_min = real(1e-9);
   real fixed_dt = 0;
-  // Extension without a reference counterpart: compensated (Kahan-Babuska)
-  // FP32 neighbour sums, the precision form.
+  // Extension without a reference counterpart: the precision form
+  // (double-float positions, compensated FP32 neighbour sums, the summation
+  // density's EOS in double), compared with the FP64 build.
   bool compensated_sums = false;
 
   cryo_config to_c() const {

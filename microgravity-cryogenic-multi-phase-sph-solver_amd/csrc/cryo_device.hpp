@@ -239,6 +239,27 @@ __host__ __device__ inline float eos_pressure(const SimParams& P, uint8_t ph, fl
   }
   return (P.clamp_neg && p < 0.0f) ? 0.0f : p;
 }
+// The same equation of state from a density carried in double (the
+// precision form's summation density, m times the compensated kernel sum):
+// near rest p is a small difference of large terms, and an FP32 density
+// would quantise it to B gamma 2^-24.  Rounded to float once, at the end.
+__host__ __device__ inline float eos_pressure_hp(const SimParams& P, uint8_t ph, double rho, float T) {
+  double p;
+  if (ph == kVapor) {
+    p = rho * static_cast<double>(P.gas_R) * static_cast<double>(T) + (static_cast<double>(P.p_bg) - P.vapor_K);
+  } else {
+    const double ratio = rho / static_cast<double>(P.rho0[0]);
+    double rg;
+    if (P.tait_gamma == 7.0f) {
+      const double r2 = ratio * ratio;
+      rg = r2 * r2 * r2 * ratio;
+    } else {
+      rg = pow(ratio, static_cast<double>(P.tait_gamma));
+    }
+    p = static_cast<double>(P.tait_B) * (rg - 1.0) + P.p_bg;
+  }
+  return (P.clamp_neg && p < 0.0) ? 0.0f : static_cast<float>(p);
+}
 __host__ __device__ inline float sound_speed(const SimParams& P, uint8_t ph, float T) {
   if (ph == kVapor) return sqrtf(P.gas_R * (T > 0.0f ? T : 0.0f));
   return P.c0;
@@ -586,7 +607,57 @@ struct Persist {
   float *m, *m0, *rho, *T;
   uint8_t *phase, *flags;
   uint32_t* id;
+  // Precision form only (null otherwise): the low-order part of each
+  // position, x + xlo with |xlo| <= ulp(x) / 2, so that positions resolve
+  // displacements far below float spacing (see pos_diff).
+  float4* xlo;
 };
+
+// ------------------------------------------------ double-float positions
+// Knuth's TwoSum of a and -b: s = fl(a - b) and the exact error (a - b) - s
+// (round to nearest, no fused operations).
+__host__ __device__ inline float diff_err(float a, float b, float s) {
+  const float nb = -b;
+  const float bv = s - a;
+  const float av = s - bv;
+  return (a - av) + (nb - bv);
+}
+// One coordinate of (a + al) - (b + bl), rounded once: the float parts'
+// difference, its exact error and the low parts' difference.
+__host__ __device__ inline float df_diff(float a, float al, float b, float bl) {
+  const float s = a - b;
+  return s + (diff_err(a, b, s) + (al - bl));
+}
+// Pair vector x_i - x_j.  HP = false: the float difference of the stored
+// positions, the reference's arithmetic.  HP = true (the precision form):
+// the double-float positions' difference, to which particle j's low part is
+// fetched here; li is particle i's.  Exactly antisymmetric in (i, j), as the
+// float difference is.
+template <bool HP>
+__device__ inline V3 pos_diff(V3 xi, V3 li, V3 xj, const float4* __restrict__ xlo, int j) {
+  if constexpr (!HP) {
+    return xi - xj;
+  } else {
+    const float4 lj = xlo[j];
+    return V3{df_diff(xi.x, li.x, xj.x, lj.x), df_diff(xi.y, li.y, xj.y, lj.y), df_diff(xi.z, li.z, xj.z, lj.z)};
+  }
+}
+template <bool HP>
+__device__ inline V3 pos_lo(const float4* __restrict__ xlo, int i) {
+  if constexpr (!HP) return V3{0.0f, 0.0f, 0.0f};
+  else return ld3(xlo[i]);
+}
+// x + d into the double-float (x, lo): the float sum, its exact rounding
+// error folded into the low part, then renormalised so |lo| <= ulp(x) / 2.
+__host__ __device__ inline void df_add(float& x, float& lo, float d) {
+  const float s = x + d;
+  const float bv = s - x;
+  const float err = (x - (s - bv)) + (d - bv);
+  const float t = lo + err;
+  const float hi = s + t;
+  lo = t - (hi - s);
+  x = hi;
+}
 struct Scratch {
   float4 *a, *ab, *ns, *vw, *fw, *mw;
   float *p, *drho, *dTdt, *sigma, *kappa, *nu_t, *alpha, *wall_w, *heat_w, *out_m, *out_h, *w_norm, *p_wv, *rmin;

@@ -32,7 +32,16 @@ __global__ __launch_bounds__(kBlock) void k_kick_drift_keys(PV pv, GridDev g, De
     pv.s.v[i] = st3(v);
     const V3 vt = tvf ? v + ld3(pv.t.ab[i]) * half : v;
     pv.s.vt[i] = st3(vt);
-    xi += vt * dt;
+    const V3 dx = vt * dt;
+    if (pv.s.xlo) {  // precision form: the displacement into the double-float position
+      V3 lo = ld3(pv.s.xlo[i]);
+      df_add(xi.x, lo.x, dx.x);
+      df_add(xi.y, lo.y, dx.y);
+      df_add(xi.z, lo.z, dx.z);
+      pv.s.xlo[i] = st3(lo);
+    } else {
+      xi += dx;
+    }
     pv.s.x[i] = st3(xi);
     if (continuity) pv.s.rho[i] += pv.t.drho[i] * dt;
   }
@@ -216,6 +225,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(Persist dst, Persist src, con
   dst.phase[i] = src.phase[j];
   dst.flags[i] = src.flags[j];
   dst.id[i] = src.id[j];
+  if (dst.xlo) dst.xlo[i] = src.xlo[j];
 }
 
 // spatial_hash.cu:140-171 (count and id-sum within radius, self included).

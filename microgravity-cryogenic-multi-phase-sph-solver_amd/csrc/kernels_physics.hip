@@ -79,9 +79,13 @@ __global__ __launch_bounds__(kBlock) void k_density(PV pv, GridDev g, SimParams 
   }
   Acc<KS> sum;
   float r2min = P.support2;
+  const V3 li = pos_lo<KS>(pv.s.xlo, i);
   for_each_cand_x(g, xi, P.kernel.support, pv.s.x, [&](int j, float4 xj) {
-    const float r2 = norm2(xi - ld3(xj));
-    if (r2 >= P.support2) return;
+    // neighbourhood by the float positions (the lists of every later pass);
+    // in the precision form the kernel takes the double-float distance
+    const float r2h = norm2(xi - ld3(xj));
+    if (r2h >= P.support2) return;
+    const float r2 = KS ? norm2(pos_diff<KS>(xi, li, ld3(xj), pv.s.xlo, j)) : r2h;
     sum += kW(P.kernel, sqrtf(r2));
     if (j != i && r2 < r2min) r2min = r2;
     lw.push(j);
@@ -91,28 +95,46 @@ __global__ __launch_bounds__(kBlock) void k_density(PV pv, GridDev g, SimParams 
   pv.t.rmin[i] = sqrtf(r2min);
   float rho = pv.s.rho[i];
   const float mi = pv.s.m[i];
-  if (P.density_mode == 0) {
-    rho = (ph == kLiquid ? pv.s.m0[i] : mi) * float(sum);
-    pv.s.rho[i] = rho;
-  }
   const float Ti = pv.s.T[i];
-  const float p = eos_pressure(P, ph, rho, Ti);
+  float p;
+  if constexpr (KS) {
+    // precision form: the summation density in double, the EOS on it
+    double rho_d = rho;
+    if (P.density_mode == 0) {
+      rho_d = static_cast<double>(ph == kLiquid ? pv.s.m0[i] : mi) * (static_cast<double>(sum.s) + sum.c);
+      rho = static_cast<float>(rho_d);
+      pv.s.rho[i] = rho;
+    }
+    p = eos_pressure_hp(P, ph, rho_d, Ti);
+  } else {
+    if (P.density_mode == 0) {
+      rho = (ph == kLiquid ? pv.s.m0[i] : mi) * float(sum);
+      pv.s.rho[i] = rho;
+    }
+    p = eos_pressure(P, ph, rho, Ti);
+  }
   pv.t.p[i] = p;
   pv.t.nrec[i] = make_float4(p, mi, rho, Ti);
   pv.t.vol[i] = mi / rho;
 }
 
 // SPH_kernels.cu:36-54 (start-up only).
+template <bool KS>
 __global__ __launch_bounds__(kBlock) void k_normalize_masses(PV pv, GridDev g, SimParams P, int n) {
   const int i = gtid();
   if (i >= n) return;
   if (!is_fluid(pv.s.phase[i])) return;
   const V3 xi = ld3(pv.s.x[i]);
-  float sum = 0.0f;
+  const V3 li = pos_lo<KS>(pv.s.xlo, i);
+  // the same sum the density pass forms (compensated in the precision form),
+  // so that the start state sits at rho0 in either form
+  Acc<KS> acc;
   for_each_nbr(g, xi, [&](int j) {
-    const float r2 = norm2(xi - ld3(pv.s.x[j]));
-    if (r2 < P.support2) sum += kW(P.kernel, sqrtf(r2));
+    const V3 xj = ld3(pv.s.x[j]);
+    if (norm2(xi - xj) >= P.support2) return;
+    acc += kW(P.kernel, sqrtf(KS ? norm2(pos_diff<KS>(xi, li, xj, pv.s.xlo, j)) : norm2(xi - xj)));
   });
+  const float sum = float(acc);
   if (sum > 0.0f) {
     const float m = pv.s.rho[i] / sum;
     pv.s.m[i] = m;
@@ -129,6 +151,7 @@ __global__ __launch_bounds__(kBlock) void k_wall_bc(PV pv, GridDev g, SimParams 
   if (pv.s.phase[i] != kWall) return;
   const V3 grav = st->gravity;
   const V3 xi = ld3(pv.s.x[i]);
+  const V3 li = pos_lo<KS>(pv.s.xlo, i);
   Acc<KS> asw[2], asp[2], asgr[2], srho;
   AccV3<KS> sv;
   struct Nb {
@@ -137,10 +160,10 @@ __global__ __launch_bounds__(kBlock) void k_wall_bc(PV pv, GridDev g, SimParams 
     int ph;
   };
   auto load = [&](int j) { return Nb{pv.s.x[j], pv.s.v[j], pv.s.rho[j], pv.t.p[j], pv.s.phase[j]}; };
-  for_each_pair4(L, g, i, xi, load, [&](int, const Nb d) {
+  for_each_pair4(L, g, i, xi, load, [&](int j, const Nb d) {
     const int phj = d.ph;
     if (!is_fluid(static_cast<uint8_t>(phj))) return;
-    const V3 rwf = xi - ld3(d.x);
+    const V3 rwf = pos_diff<KS>(xi, li, ld3(d.x), pv.s.xlo, j);
     const float r2 = norm2(rwf);
     if (r2 >= P.support2) return;
     const float w = kW(P.kernel, sqrtf(r2));
@@ -230,13 +253,14 @@ __device__ inline float interface_particle(const PV& pv, const SimParams& P, int
     return 0.0f;
   }
   const V3 xi = ld3(pv.s.x[i]);
+  const V3 li = pos_lo<KS>(pv.s.xlo, i);
   const float ci = ph == kLiquid ? 1.0f : 0.0f;
   AccV3<KS> agradc, awall_n, aakn, agradphi;
   Acc<KS> as_all, as_other, aww, ahw;
   const V3 vi = ld3(pv.s.v[i]);
   Acc<KS> Lm[3][3];
   visit([&](int j, const IfaceNbr d) {
-    const V3 rij = xi - ld3(d.x);
+    const V3 rij = pos_diff<KS>(xi, li, ld3(d.x), pv.s.xlo, j);
     const float r2 = norm2(rij);
     if (r2 >= P.support2) return;
     const float r = sqrtf(r2);
@@ -409,13 +433,14 @@ __global__ __launch_bounds__(kBlock) void k_smooth_normals(PV pv, GridDev g, Sim
     return;
   }
   const V3 xi = ld3(pv.s.x[i]);
+  const V3 li = pos_lo<KS>(pv.s.xlo, i);
   AccV3<KS> acc;
   for_each_pair(L, g, i, xi, [&](int j) {
     if (!is_fluid(pv.s.phase[j])) return;
     const V3 gj = ld3(pv.s.nrm[j]);
     const float gjm = norm(gj);
     if (!(gjm > P.normal_eps)) return;
-    const float r2 = norm2(xi - ld3(pv.s.x[j]));
+    const float r2 = norm2(pos_diff<KS>(xi, li, ld3(pv.s.x[j]), pv.s.xlo, j));
     if (r2 >= P.support2) return;
     acc += gj * ((pv.s.m[j] / pv.s.rho[j]) * kW(P.kernel, sqrtf(r2)) / gjm);
   });
@@ -432,7 +457,8 @@ struct NbrData {
   float4 rec;  // {p, m, rho, T} fluid; {p, p_wv, rho_tvf, T} wall
   float4 v;
   float4 vt;
-  float V;  // m / rho (fluid)
+  float V;   // m / rho (fluid)
+  float Vt;  // 1 / sigma (fluid, transport velocity on)
 };
 
 __device__ inline bool forces_need_vt(const SimParams& P, bool continuity) {
@@ -451,11 +477,15 @@ __device__ inline NbrData load_nbr(const PV& pv, const SimParams& P, int j, bool
   if (P.no_slip) vw = pv.t.vw[j];
   d.vt = v;
   if (need_vt) d.vt = pv.s.vt[j];
+  float sig = 1.0f;
+  if (P.tvf) sig = pv.t.sigma[j];
   // every gather of the pair in flight together, then one wait
   settle(d.x.x, d.x.y, d.x.z, d.ph, d.rec.x, d.rec.y, d.rec.z, d.rec.w, v.x, v.y, v.z, d.V);
   if (P.no_slip) settle(vw.x, vw.y, vw.z);
   if (need_vt) settle(d.vt.x, d.vt.y, d.vt.z);
+  if (P.tvf) settle(sig);
   d.v = d.ph == kWall ? vw : v;
+  d.Vt = P.tvf && is_fluid(d.ph) ? 1.0f / sig : 0.0f;
   return d;
 }
 
@@ -486,6 +516,7 @@ __device__ inline void forces_particle(const PV& pv, const SimParams& P, DevStat
       const float dt_now = st->dt;
       const int pi_idx = pidx(ph);
       const V3 xi = ld3(pv.s.x[i]);
+      const V3 li = pos_lo<KS>(pv.s.xlo, i);
       const V3 vi = ld3(pv.s.v[i]);
       const float mi = pv.s.m[i];
       const float rhoi = pv.s.rho[i];
@@ -540,7 +571,7 @@ __device__ inline void forces_particle(const PV& pv, const SimParams& P, DevStat
         const V3 xj = ld3(d.x);
         const int phj = d.ph;
         const float4 rec = d.rec;
-        const V3 rij = xi - xj;
+        const V3 rij = pos_diff<KS>(xi, li, xj, pv.s.xlo, j);
         const float r2 = norm2(rij);
         if (r2 >= P.support2) return;
         const float r = sqrtf(r2);
@@ -617,7 +648,7 @@ __device__ inline void forces_particle(const PV& pv, const SimParams& P, DevStat
         // fluid neighbour
         const float pj = rec.x, mj = rec.y, rhoj = rec.z;
         if (tvf_on) {
-          const float Vj = 1.0f / pv.t.sigma[j];
+          const float Vj = d.Vt;  // 1 / sigma_j
           sum_t += gw * (Vt2 + Vj * Vj);
         }
         const float Vj = d.V;  // mj / rhoj, divided once per particle
@@ -785,7 +816,6 @@ struct TileRows {
   int ntrows;
 };
 
-// Row tables of tile `b`; called by the whole block, ends with a barrier.
 // Block b of an nblocks grid -> the tile it runs.  The hardware deals
 // workgroups round-robin to the 8 XCDs (XCD b % 8); within each group of
 // 8 x kTileChunk blocks, XCD x runs the chunk of kTileChunk consecutive tiles
@@ -799,6 +829,7 @@ __device__ inline int xcd_tile(int b, int nblocks) {
   return b / span * span + (b % 8) * kTileChunk + (b / 8) % kTileChunk;
 }
 
+// Row tables of tile `b`; called by the whole block, ends with a barrier.
 __device__ inline void tile_rows(const GridDev& g, int b, TileRows& T) {
   const int tiles_x = (g.nx + kTileCells - 1) / kTileCells;
   const int tiles_y = (g.ny + kTileCells - 1) / kTileCells;
@@ -939,6 +970,7 @@ __global__ __launch_bounds__(kTileThreads) void k_forces_tiled(PV pv, GridDev g,
       float4 x, r, v, t;
       int ph;
       float V;
+      float sig;
     };
     auto load = [&](int u) {
       Slot q{};
@@ -954,6 +986,8 @@ __global__ __launch_bounds__(kTileThreads) void k_forces_tiled(PV pv, GridDev g,
         if (P.no_slip) w = pv.t.vw[j];
         float4 t = v;
         if (need_vt) t = pv.s.vt[j];
+        q.sig = 1.0f;
+        if (P.tvf) q.sig = pv.t.sigma[j];
         q.v = q.ph == kWall ? w : v;
         q.t = t;
       }
@@ -964,7 +998,9 @@ __global__ __launch_bounds__(kTileThreads) void k_forces_tiled(PV pv, GridDev g,
       if (s < M) {
         s_x[s] = make_float4(q.x.x, q.x.y, q.x.z, __int_as_float(q.ph));
         s_rec[s] = q.r;
-        s_v[s] = q.v;
+        // the velocity's free fourth component carries the neighbour's
+        // transport volume 1 / sigma, divided once per staged particle
+        s_v[s] = make_float4(q.v.x, q.v.y, q.v.z, P.tvf && is_fluid(q.ph) ? 1.0f / q.sig : 0.0f);
         s_V[s] = q.V;
         if (need_vt) s_vt[s] = q.t;
       }
@@ -997,20 +1033,27 @@ __global__ __launch_bounds__(kTileThreads) void k_forces_tiled(PV pv, GridDev g,
           if (q + 1 < nq) nxt = lp[(q + 1) * kListGroup];
           const int m = cnt - (q << 2);
           const int js[4] = {cur.x, cur.y, cur.z, cur.w};
-          NbrData d[4];
+          // the Akinci bodies hold more per-pair state: two records at a
+          // time (one with compensated sums) keep them within the register
+          // file (no scratch)
+          constexpr int kAhead = ST == 2 ? (KS ? 1 : 2) : 4;
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            if (u < m) {
-              const int s = rc.slot(js[u]);
-              const float4 xs = s_x[s], rs = s_rec[s], vs = s_v[s];
-              float4 vts = vs;
-              if (need_vt) vts = s_vt[s];
-              d[u] = NbrData{xs, __float_as_int(xs.w), rs, vs, vts, s_V[s]};
+          for (int h0 = 0; h0 < 4; h0 += kAhead) {
+            NbrData d[kAhead];
+#pragma unroll
+            for (int u = 0; u < kAhead; ++u) {
+              if (h0 + u < m) {
+                const int s = rc.slot(js[h0 + u]);
+                const float4 xs = s_x[s], rs = s_rec[s], vs = s_v[s];
+                float4 vts = vs;
+                if (need_vt) vts = s_vt[s];
+                d[u] = NbrData{xs, __float_as_int(xs.w), rs, vs, vts, s_V[s], vs.w};
+              }
             }
-          }
 #pragma unroll
-          for (int u = 0; u < 4; ++u)
-            if (u < m) f(js[u], d[u]);
+            for (int u = 0; u < kAhead; ++u)
+              if (h0 + u < m) f(js[h0 + u], d[u]);
+          }
           cur = nxt;
         }
       };
@@ -1200,12 +1243,13 @@ __global__ __launch_bounds__(kBlock) void k_shift_compute(PV pv, GridDev g, SimP
   const float coeff = P.shift_strength * mach * 4.0f * h * h;
   const float cap = 0.1f * P.dx;
   const V3 xi = ld3(pv.s.x[i]);
+  const V3 li = pos_lo<KS>(pv.s.xlo, i);
   AccV3<KS> sum;
   for_each_pair(L, g, i, xi, [&](int j) {
     if (j == i) return;
     const uint8_t phj = pv.s.phase[j];
     if (phj == kDead) return;
-    const V3 rij = xi - ld3(pv.s.x[j]);
+    const V3 rij = pos_diff<KS>(xi, li, ld3(pv.s.x[j]), pv.s.xlo, j);
     const float r2 = norm2(rij);
     if (r2 >= P.support2) return;
     const float r = sqrtf(r2);
@@ -1233,7 +1277,18 @@ __global__ __launch_bounds__(kBlock) void k_shift_compute(PV pv, GridDev g, SimP
 __global__ __launch_bounds__(kBlock) void k_shift_apply(PV pv, int n) {
   const int i = gtid();
   if (i >= n) return;
-  if (is_fluid(pv.s.phase[i])) pv.s.x[i] = st3(ld3(pv.s.x[i]) + ld3(pv.t.ns[i]));
+  if (!is_fluid(pv.s.phase[i])) return;
+  if (pv.s.xlo) {  // precision form: the shift into the double-float position
+    V3 x = ld3(pv.s.x[i]), lo = ld3(pv.s.xlo[i]);
+    const V3 d = ld3(pv.t.ns[i]);
+    df_add(x.x, lo.x, d.x);
+    df_add(x.y, lo.y, d.y);
+    df_add(x.z, lo.z, d.z);
+    pv.s.x[i] = st3(x);
+    pv.s.xlo[i] = st3(lo);
+  } else {
+    pv.s.x[i] = st3(ld3(pv.s.x[i]) + ld3(pv.t.ns[i]));
+  }
 }
 
 }  // namespace
@@ -1244,7 +1299,9 @@ void launch_density(const PV& pv, const GridDev& g, const SimParams& P, const Nb
   else k_density<false><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, n);
 }
 void launch_normalize_masses(const PV& pv, const GridDev& g, const SimParams& P, int n, hipStream_t s) {
-  if (n > 0) k_normalize_masses<<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, n);
+  if (n <= 0) return;
+  if (P.compensated) k_normalize_masses<true><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, n);
+  else k_normalize_masses<false><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, n);
 }
 void launch_wall_bc(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, const DevState* st, int n,
                     hipStream_t s) {

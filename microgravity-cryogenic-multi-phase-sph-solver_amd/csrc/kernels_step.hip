@@ -196,6 +196,7 @@ __global__ __launch_bounds__(kBlock) void k_pc_normalize(PV pv, GridDev g, SimPa
     if (fl & kSource) {
       const uint8_t target = (fl & kTargetVapor) ? kVapor : kLiquid;
       const V3 xi = ld3(pv.s.x[i]);
+      const V3 li = pos_lo<KS>(pv.s.xlo, i);
       Acc<KS> awsum;
       float wr_max = 0.0f;
       struct Nb {
@@ -208,7 +209,7 @@ __global__ __launch_bounds__(kBlock) void k_pc_normalize(PV pv, GridDev g, SimPa
       };
       for_each_pair4(L, g, i, xi, load, [&](int j, const Nb d) {
         if (j == i || d.ph != target || (d.fl & kDying)) return;
-        const float r2 = norm2(xi - ld3(d.x));
+        const float r2 = norm2(pos_diff<KS>(xi, li, ld3(d.x), pv.s.xlo, j));
         if (r2 >= P.support2) return;
         const float w = kW(P.kernel, sqrtf(r2));
         const float rhoj = d.rho;
@@ -261,6 +262,7 @@ __global__ __launch_bounds__(kBlock) void k_pc_apply(PV pv, GridDev g, SimParams
   const float mi = pv.s.m[i];
   const float Vi = mi / pv.s.rho[i];
   const V3 xi = ld3(pv.s.x[i]);
+  const V3 li = pos_lo<KS>(pv.s.xlo, i);
   Acc<KS> ain_m, ain_H;
   // receivers only near a source giving mass to this phase
   if (!(fl & kDying) && cells_any(g, xi, ph == kVapor ? kCellSrcV : kCellSrcL)) {
@@ -278,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void k_pc_apply(PV pv, GridDev g, SimParams
       if (!(om > 0.0f)) return;
       const uint8_t target = (d.fl & kTargetVapor) ? kVapor : kLiquid;
       if (target != ph) return;
-      const float r2 = norm2(xi - ld3(d.x));
+      const float r2 = norm2(pos_diff<KS>(xi, li, ld3(d.x), pv.s.xlo, j));
       if (r2 >= P.support2) return;
       const float share = om * Vi * kW(P.kernel, sqrtf(r2)) / d.wn;
       ain_m += share;
@@ -310,6 +312,7 @@ __global__ __launch_bounds__(kBlock) void k_rebalance(PV pv, GridDev g, SimParam
   if (pv.s.phase[i] != kVapor) return;
   const float beta = P.vapor_rebalance;
   const V3 xi = ld3(pv.s.x[i]);
+  const V3 li = pos_lo<KS>(pv.s.xlo, i);
   const float mi = pv.s.m[i];
   const float hi = enthalpy(P, kVapor, pv.s.T[i]);
   const V3 vi = ld3(pv.s.v[i]);
@@ -323,7 +326,7 @@ __global__ __launch_bounds__(kBlock) void k_rebalance(PV pv, GridDev g, SimParam
   auto load = [&](int j) { return Nb{pv.s.x[j], pv.s.v[j], pv.s.m[j], pv.s.T[j], pv.s.phase[j]}; };
   for_each_pair4(L, g, i, xi, load, [&](int j, const Nb d) {
     if (j == i || d.ph != kVapor) return;
-    const float r2 = norm2(xi - ld3(d.x));
+    const float r2 = norm2(pos_diff<KS>(xi, li, ld3(d.x), pv.s.xlo, j));
     if (r2 >= P.support2) return;
     const float flow = beta * P.vol0 * kW(P.kernel, sqrtf(r2)) * (d.m - mi);
     const bool from_j = flow > 0.0f;
@@ -402,15 +405,29 @@ __global__ __launch_bounds__(kBlock) void k_split(PV pv, SimParams P, const DevS
   const V3 xi = ld3(pv.s.x[i]);
   const V3 gc = ld3(pv.s.nrm[i]);
   const float gm = norm(gc);
+  // the child's (and, split symmetrically, the parent's) offset from the
+  // parent position; the precision form adds it to the double-float position
+  auto place = [&](int k, V3 off) {
+    if (pv.s.xlo) {
+      V3 x = xi, lo = ld3(pv.s.xlo[i]);
+      df_add(x.x, lo.x, off.x);
+      df_add(x.y, lo.y, off.y);
+      df_add(x.z, lo.z, off.z);
+      pv.s.x[k] = st3(x);
+      pv.s.xlo[k] = st3(lo);
+    } else {
+      pv.s.x[k] = st3(xi + off);
+    }
+  };
   if (gm > P.normal_eps) {
     const V3 nn = gc / gm;
     const V3 t = e - nn * dot(e, nn);
     e = normalized(t - nn * 0.5f);
-    pv.s.x[dst] = st3(xi + e * (0.5f * cbrtf_cr(vol)));
+    place(dst, e * (0.5f * cbrtf_cr(vol)));
   } else {
     const float delta = 0.25f * cbrtf_cr(vol);
-    pv.s.x[dst] = st3(xi + e * delta);
-    pv.s.x[i] = st3(xi - e * delta);
+    place(dst, e * delta);
+    place(i, -(e * delta));
   }
   pv.s.v[dst] = pv.s.v[i];
   pv.s.vt[dst] = pv.s.vt[i];
@@ -578,17 +595,27 @@ __global__ __launch_bounds__(kBlock) void k_diag(PV pv, SimParams P, int n, doub
     const double m = pv.s.m[i];
     const V3 v = ld3(pv.s.v[i]);
     if (ph == kLiquid || ph == kVapor) {
-      const int k = ph == kLiquid ? 0 : 1;
-      acc[k] = 1.0;
-      acc[3 + k] = m;
-      acc[5 + k] = 0.5 * m * static_cast<double>(norm2(v));
-      acc[7 + k] = m * static_cast<double>(enthalpy(P, ph, pv.s.T[i]));
-      acc[9 + k] = m * static_cast<double>(pv.s.T[i]);
+      // constant indices only (a runtime index would put acc in scratch)
+      const double ke = 0.5 * m * static_cast<double>(norm2(v));
+      const double mh = m * static_cast<double>(enthalpy(P, ph, pv.s.T[i]));
+      const double mT = m * static_cast<double>(pv.s.T[i]);
+      if (ph == kLiquid) {
+        acc[0] = 1.0, acc[3] = m, acc[5] = ke, acc[7] = mh, acc[9] = mT;
+      } else {
+        acc[1] = 1.0, acc[4] = m, acc[6] = ke, acc[8] = mh, acc[10] = mT;
+      }
       if (ph == kLiquid) {
         const V3 x = ld3(pv.s.x[i]);
-        acc[11] = m * x.x;
-        acc[12] = m * x.y;
-        acc[13] = m * x.z;
+        if (pv.s.xlo) {  // precision form: the double-float position
+          const V3 lo = ld3(pv.s.xlo[i]);
+          acc[11] = m * (static_cast<double>(x.x) + lo.x);
+          acc[12] = m * (static_cast<double>(x.y) + lo.y);
+          acc[13] = m * (static_cast<double>(x.z) + lo.z);
+        } else {
+          acc[11] = m * x.x;
+          acc[12] = m * x.y;
+          acc[13] = m * x.z;
+        }
         acc[14] = m * v.x;
         acc[15] = m * v.y;
         acc[16] = m * v.z;

@@ -218,6 +218,7 @@ struct cryo_solver {
 
   // particle arrays: two persistent sets (gather ping-pong) + scratch
   DevBuf<float4> x_[2], v_[2], vt_[2], nrm_[2];
+  DevBuf<float4> xlo_[2];  // precision form only: low-order parts of the positions
   DevBuf<float> m_[2], m0_[2], rho_[2], T_[2];
   DevBuf<uint8_t> phase_[2], flags_[2];
   DevBuf<uint32_t> id_[2];
@@ -383,7 +384,7 @@ struct cryo_solver {
     PV pv{};
     const int k = set;
     pv.s = Persist{x_[k].get(), v_[k].get(), vt_[k].get(), nrm_[k].get(), m_[k].get(), m0_[k].get(),
-                   rho_[k].get(), T_[k].get(), phase_[k].get(), flags_[k].get(), id_[k].get()};
+                   rho_[k].get(), T_[k].get(), phase_[k].get(), flags_[k].get(), id_[k].get(), xlo_[k].get()};
     pv.t = Scratch{a.get(), ab.get(), ns.get(), vw.get(), fw.get(), mw.get(), p.get(), drho.get(), dTdt.get(),
                    sigma.get(), kappa.get(), nu_t.get(), alpha.get(), wall_w.get(), heat_w.get(), out_m.get(),
                    out_h.get(), w_norm.get(), p_wv.get(), rmin.get(), nrec.get(), vol.get()};
@@ -392,7 +393,7 @@ struct cryo_solver {
   }
   Persist persist(int k) {
     return Persist{x_[k].get(), v_[k].get(), vt_[k].get(), nrm_[k].get(), m_[k].get(), m0_[k].get(),
-                   rho_[k].get(), T_[k].get(), phase_[k].get(), flags_[k].get(), id_[k].get()};
+                   rho_[k].get(), T_[k].get(), phase_[k].get(), flags_[k].get(), id_[k].get(), xlo_[k].get()};
   }
   GridDev grid() {
     GridDev gd = g;
@@ -418,6 +419,7 @@ struct cryo_solver {
       f(phase_[k], k == set);
       f(flags_[k], k == set);
       f(id_[k], k == set);
+      if (cfg.compensated_sums) f(xlo_[k], k == set);
     }
     f(a, true); f(ab, true); f(ns, true); f(vw, true); f(fw, true); f(mw, true); f(nrec, false); f(vol, false);
     f(p, true); f(drho, true); f(dTdt, true); f(sigma, true); f(kappa, true); f(nu_t, true); f(alpha, true);
@@ -636,6 +638,8 @@ struct cryo_solver {
     if (id) std::copy(id, id + n_in, ids.begin());
     else std::iota(ids.begin(), ids.end(), 0u);
     up(id_[set], ids.data());
+    if (xlo_[set].get())  // the given positions are exact: no low-order parts yet
+      CRYO_HIP(hipMemsetAsync(xlo_[set].get(), 0, sizeof(float4) * n_in, stream));
     CRYO_HIP(hipStreamSynchronize(stream));
     next_id = 0;
     for (uint32_t k : ids) next_id = std::max(next_id, k + 1);
@@ -965,7 +969,7 @@ struct cryo_solver {
   // optionally `a` (at step boundaries) and the step's sort permutation (taped
   // post-sort states).
   struct Snap {
-    DevBuf<float4> x, v, vt, nrm, a, ab, nrec;
+    DevBuf<float4> x, v, vt, nrm, a, ab, nrec, xlo;
     DevBuf<float> m, m0, rho, T, vol;
     DevBuf<int> lidx, lcnt, coff;  // neighbour lists and cell table of a taped rates evaluation
     DevBuf<uint8_t> cflags;
@@ -1091,6 +1095,7 @@ struct cryo_solver {
     const Persist q = persist(set);
     fit(z.x, m), fit(z.v, m), fit(z.vt, m), fit(z.nrm, m), fit(z.m, m), fit(z.m0, m), fit(z.rho, m), fit(z.T, m);
     fit(z.phase, m), fit(z.flags, m), fit(z.id, m), fit(z.st, 1);
+    if (q.xlo) fit(z.xlo, m);
     if (with_a) fit(z.a, m), fit(z.ab, m);
     if (perm) fit(z.perm, m);
     {
@@ -1098,6 +1103,7 @@ struct cryo_solver {
       d2d(z.x.get(), q.x, m), d2d(z.v.get(), q.v, m), d2d(z.vt.get(), q.vt, m), d2d(z.nrm.get(), q.nrm, m);
       d2d(z.m.get(), q.m, m), d2d(z.m0.get(), q.m0, m), d2d(z.rho.get(), q.rho, m), d2d(z.T.get(), q.T, m);
       d2d(z.phase.get(), q.phase, m), d2d(z.flags.get(), q.flags, m), d2d(z.id.get(), q.id, m);
+      if (q.xlo) d2d(z.xlo.get(), q.xlo, m);
       d2d(z.st.get(), state.get(), 1);
       if (with_a) d2d(z.a.get(), a.get(), m), d2d(z.ab.get(), ab.get(), m);
       if (perm) d2d(z.perm.get(), perm, m);
@@ -1112,6 +1118,7 @@ struct cryo_solver {
       d2d(q.x, z.x.get(), m), d2d(q.v, z.v.get(), m), d2d(q.vt, z.vt.get(), m), d2d(q.nrm, z.nrm.get(), m);
       d2d(q.m, z.m.get(), m), d2d(q.m0, z.m0.get(), m), d2d(q.rho, z.rho.get(), m), d2d(q.T, z.T.get(), m);
       d2d(q.phase, z.phase.get(), m), d2d(q.flags, z.flags.get(), m), d2d(q.id, z.id.get(), m);
+      if (q.xlo) d2d(q.xlo, z.xlo.get(), m);
       d2d(state.get(), z.st.get(), 1);
       if (with_a) d2d(a.get(), z.a.get(), m), d2d(ab.get(), z.ab.get(), m);
     }

@@ -6,6 +6,7 @@ step graphs on, the runner replays each member's step as a captured HIP
 graph (captured on the first two steps, one per persistent buffer set), so
 runs of 1 to 6 steps check the eager result against 0 to 4 replays.
 """
+import copy
 import sys
 from pathlib import Path
 
@@ -141,7 +142,12 @@ def test_c4_full_ensemble_bitwise_and_slosh_force_against_fp64_oracle():
     at 1e-4 g the first drifts move particles by a few float ulps of their
     coordinates and the net load follows those displacements, so FP32
     position storage alone costs 2e-4..7e-4 (pinned on the CPU by
-    test_oracle.py::test_fp32_slosh_force_error_comes_from_float_positions)."""
+    test_oracle.py::test_fp32_slosh_force_error_comes_from_float_positions).
+    The precision form (compensated_sums = 1: double-float positions,
+    compensated sums, the summation density's EOS in double) of every member
+    is compared with the same FP64 runs: it removes the position term, and
+    what is left (FP32 masses, velocities and temperatures) is held to the
+    bars below."""
     import bench
 
     print("test_c4_full_ensemble_bitwise_and_slosh_force_against_fp64_oracle", flush=True)
@@ -152,20 +158,32 @@ def test_c4_full_ensemble_bitwise_and_slosh_force_against_fp64_oracle():
     for s in loop:
         s.run_steps(steps)
     cabi.hip_ensemble_run_steps(ens, steps)
-    worst = 0.0
-    print(f"{'member':>6} {'tank':<34} {'|F| (N)':>11} {'rel err vs FP64':>16}", flush=True)
-    for k, (t, a, b) in enumerate(zip(tanks, loop, ens)):
+    for a, b in zip(loop, ens):
         _same(a, b)
+    del loop
+    errs, errs_p = [], []
+    print(f"{'member':>6} {'tank':<34} {'|F| (N)':>11} {'FP32 rel err':>13} {'precision form':>15}", flush=True)
+    for k, (t, b) in enumerate(zip(tanks, ens)):
         o = cabi.CpuSolver("orc", t["cfg"], t["fluid"], 64)
         o.set_particles(t["ps"], t["lo"], t["hi"])
         o.set_frame_motion(acc=(t["a_lat"], 0.0, 0.0))
         o.run_steps(steps)
-        f32 = np.array(b.diagnostics().slosh_force.tup(), np.float64)
         f64 = np.array(o.diagnostics().slosh_force.tup(), np.float64)
         assert o.step_count == steps and o.num_particles == b.num_particles
-        err = float(np.linalg.norm(f32 - f64) / np.linalg.norm(f64))
-        worst = max(worst, err)
-        print(f"{k:>6} {t['desc']:<34} {np.linalg.norm(f64):>11.4e} {err:>16.3e}", flush=True)
         del o
-    print(f"worst slosh-force relative error: {worst:.3e}", flush=True)
-    assert worst <= 5e-3
+        cp = copy.copy(t["cfg"])
+        cp.compensated_sums = 1
+        p = make([dict(t, cfg=cp)])[0]
+        p.run_steps(steps)
+        f32 = np.array(b.diagnostics().slosh_force.tup(), np.float64)
+        fp = np.array(p.diagnostics().slosh_force.tup(), np.float64)
+        del p
+        errs.append(float(np.linalg.norm(f32 - f64) / np.linalg.norm(f64)))
+        errs_p.append(float(np.linalg.norm(fp - f64) / np.linalg.norm(f64)))
+        print(f"{k:>6} {t['desc']:<34} {np.linalg.norm(f64):>11.4e} {errs[-1]:>13.3e} {errs_p[-1]:>15.3e}", flush=True)
+    e, ep = np.array(errs), np.array(errs_p)
+    print(f"FP32: worst {e.max():.3e} median {np.median(e):.3e}; precision form: worst {ep.max():.3e} "
+          f"median {np.median(ep):.3e}, {int((ep <= 1e-4).sum())} of 64 members within 1e-4", flush=True)
+    assert e.max() <= 5e-3
+    # measured: worst 1.9e-4, median 1.0e-5, 62 of 64 within 1e-4
+    assert ep.max() <= 5e-4 and np.median(ep) <= 5e-5 and (ep <= 1e-4).sum() >= 60
