@@ -47,18 +47,24 @@ STEP_BYTES = {"fluid": 590, "wall": 310}
 HBM_PEAK_GBS = 8000.0
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprof counter
-    summary (FETCH_SIZE and WRITE_SIZE passes, each scaled by what the same
-    counter reads for a plain 1 GiB device copy); None when absent."""
-    path = ROOT / "profiles" / "hbm_traffic_c2.txt"
+C2_DX = 0.007  # the workload the committed counter profiles were taken on
+VALU_CLOCK_HZ = 2.4e9  # MI355X peak engine clock
+SIMDS = 1024  # 256 CUs x 4 SIMDs
+
+
+def profile_column(name, kernel, column):
+    """Column `column` (0 = the kernel name) of `kernel`'s line in a committed
+    counter summary under profiles/ (counters cannot be read inside this run;
+    a kernel's byte and instruction counts per launch are fixed for a given
+    workload).  None when absent."""
+    path = ROOT / "profiles" / name
     if not path.exists():
         return None
     for line in path.read_text().splitlines():
         parts = line.split()
-        if len(parts) >= 2 and parts[0] == kernel:
+        if len(parts) > column and parts[0].startswith(kernel):
             try:
-                return float(parts[1])
+                return float(parts[column].rstrip("%"))
             except ValueError:
                 return None
     return None
@@ -373,6 +379,35 @@ def main():
     forces_bytes = n_fluid * FORCES_BYTES["fluid"] + n_wall * FORCES_BYTES["wall"]
     achieved = forces_bytes / (forces_ms * 1e-3) / 1e9
     step_bytes = n_fluid * STEP_BYTES["fluid"] + n_wall * STEP_BYTES["wall"]
+    # HBM roofline on algorithmic bytes (live kernel time), and the roof that
+    # binds: the kernel's VALU issue share, VALU instructions per launch (from
+    # the committed counter profile of this workload) x 4 cycles per wave64
+    # instruction over the SIMD-cycles of the live launch time.  Counter-based
+    # fields are given only for the workload the profiles were taken on.
+    on_profile = abs(a.dx - C2_DX) < 1e-9
+    traffic = profile_column("hbm_traffic_c2.txt", "k_forces_tiled", 1) if on_profile else None
+    valu_instr = profile_column("valu_l2_c2.txt", "k_forces_tiled", 2) if on_profile else None
+    valu_frac = (valu_instr * 4 / SIMDS / (forces_ms * 1e-3 * VALU_CLOCK_HZ)) if valu_instr else None
+    tfrac = (traffic / (forces_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None
+    if valu_frac is None:
+        limiter = None
+    elif valu_frac >= 0.8 or (tfrac or 0.0) >= 0.8:
+        limiter = "VALU issue" if valu_frac >= (tfrac or 0.0) else "HBM traffic"
+    else:
+        limiter = "latency: neither the VALU nor the HBM roof is reached"
+    roofline = {"bound": "valu" if valu_frac is not None and valu_frac > achieved / HBM_PEAK_GBS else "hbm",
+                "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                "traffic": traffic,
+                "traffic_source": "profiles/hbm_traffic_c2.txt (FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
+                if traffic else None,
+                "traffic_frac": tfrac,
+                "valu_frac": valu_frac,
+                "valu_source": "profiles/valu_l2_c2.txt (SQ_INSTS_VALU per launch) over the live kernel time"
+                if valu_frac is not None else None,
+                "limiter": limiter,
+                "kernel": "k_forces_tiled (transport velocity + momentum + wall loads + continuity + dt "
+                          "maxima; LDS-staged cell tiles) + its overflow instance",
+                "kernel_ms": forces_ms, "algorithmic_bytes": forces_bytes}
     out = {
         "metric": "particle-steps/s + % HBM roofline at 1M; ensemble speedup; adjoint/forward cost",
         "value": n * a.steps / sec,
@@ -389,11 +424,7 @@ def main():
         "config": {"workload": f"water column collapse 3.0x1.0x0.8 m box + 0.15x0.4x0.15 m block, column "
                                f"1.0x1.0x0.5 m, dx={a.dx}, continuity + delta-SPH, no-slip, FP32",
                    "n_particles": n, "n_fluid": n_fluid, "n_wall": n_wall, "dx": a.dx},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_forces_tiled"),
-                     "kernel": "k_forces_tiled (transport velocity + momentum + wall loads + continuity + dt "
-                               "maxima; LDS-staged cell tiles) + its overflow instance",
-                     "kernel_ms": forces_ms, "algorithmic_bytes": forces_bytes},
+        "roofline": roofline,
         "step_roofline": {"algorithmic_bytes": step_bytes,
                           "achieved": step_bytes / (ms_step * 1e-3) / 1e9,
                           "frac": step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
