@@ -70,6 +70,40 @@ def profile_column(name, kernel, column):
     return None
 
 
+class marker:
+    """A roctx range around a region (shown by rocprofv3 --marker-trace, which
+    the ensemble profile uses to find its window); a no-op when the roctx
+    library is absent."""
+
+    _lib = None
+    _tried = False
+
+    def __init__(self, name):
+        self.name = name.encode()
+
+    @classmethod
+    def lib(cls):
+        if not cls._tried:
+            cls._tried = True
+            for path in ("/opt/rocm/lib/librocprofiler-sdk-roctx.so.1", "librocprofiler-sdk-roctx.so.1"):
+                try:
+                    cls._lib = C.CDLL(path)
+                    break
+                except OSError:
+                    continue
+        return cls._lib
+
+    def __enter__(self):
+        if self.lib() is not None:
+            self.lib().roctxRangePushA(C.c_char_p(self.name))
+        return self
+
+    def __exit__(self, *exc):
+        if self.lib() is not None:
+            self.lib().roctxRangePop()
+        return False
+
+
 def c2_particles(cabi, dx):
     water = cabi.hip_fluid_preset("H2O")
     water.rho_liquid = 1000.0
@@ -159,11 +193,13 @@ def ensemble_measure(cabi, tanks, steps, tune_steps=3):
     picks = [s.autotune(tune_steps)[0] for s in solvers]
     tune = time.perf_counter() - t0
     t0 = time.perf_counter()
-    for s in solvers:
-        s.run_steps(steps)
+    with marker("per_tank_loop_tuned"):
+        for s in solvers:
+            s.run_steps(steps)
     loop_tuned = time.perf_counter() - t0
     t0 = time.perf_counter()
-    cabi.hip_ensemble_run_steps(solvers, steps)
+    with marker("ensemble_run_steps"):
+        cabi.hip_ensemble_run_steps(solvers, steps)
     ens = time.perf_counter() - t0
     for s in solvers:  # the same with captured step graphs (captures included)
         s.set_step_graphs(True)
@@ -342,6 +378,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=10, help="0 skips the 64-tank ensemble leg")
     ap.add_argument("--adjoint-steps", type=int, default=200, help="0 skips the adjoint leg")
     ap.add_argument("--checkpoint-every", type=int, default=20)
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="run only the 64-tank ensemble leg and print its JSON object")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (scales the HBM byte counters) and exit")
     ap.add_argument("--cpu-baseline-child", action="store_true", help=argparse.SUPPRESS)
@@ -351,6 +389,11 @@ def main():
         return
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
+        return
+    if a.ensemble_only:
+        import cabi
+
+        print(json.dumps(ensemble_measure(cabi, c4_tanks(cabi), max(a.ensemble_steps, 1))))
         return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported (how the step would shard "
