@@ -19,6 +19,7 @@ inputs resident in HBM.  Prints one JSON line.
 from __future__ import annotations
 
 import argparse
+import copy
 import ctypes as C
 import json
 import math
@@ -267,6 +268,23 @@ def adjoint_measure(cabi, steps, every):
         del s
     fwd = min(times["forward"], times["forward2"], times["forward3"])
     adj = min(times["adjoint"], times["adjoint2"], times["adjoint3"])
+    # the precision form (the forward whose gradient meets the FP64 bar)
+    pcfg = copy.copy(cfg)
+    pcfg.compensated_sums = 1
+    ptimes = {}
+    for name, grad in (("forward", False), ("adjoint", True), ("forward2", False), ("adjoint2", True)):
+        s = cabi.HipSolver(pcfg, fluid)
+        s.set_particles(ps, lo, hi)
+        s.initialize()
+        s.time  # synchronises
+        t0 = time.perf_counter()
+        pout = s.adjoint_slosh(acc, checkpoint_every=every, grad=grad)
+        ptimes[name] = time.perf_counter() - t0
+        if grad:
+            ploss, pgmax = pout[0], float(np.abs(pout[1]).max())
+        del s
+    pfwd = min(ptimes["forward"], ptimes["forward2"])
+    padj = min(ptimes["adjoint"], ptimes["adjoint2"])
     return {"workload": f"LN2 cylinder r=0.1 h=0.25 fill 0.5 dx=0.0037 CSF contact angle 30 deg, Lee 5, "
                         f"liquid 0.2 K superheated, vented, {len(ps)} particles, {steps} steps of "
                         f"dt={cfg.fixed_dt:.3e} s, 16-segment frame-acceleration profile; "
@@ -276,7 +294,10 @@ def adjoint_measure(cabi, steps, every):
             "particles": len(ps), "steps": steps, "checkpoint_every": every, "loss": loss, "grad_acc_max": gnorm,
             "grad_contact_angle": gtheta, "grad_lee_evaporation": glee[0], "grad_lee_condensation": glee[1],
             "forward_ms_per_step": 1e3 * fwd / steps, "forward_and_adjoint_ms_per_step": 1e3 * adj / steps,
-            "cost_ratio": adj / fwd}
+            "cost_ratio": adj / fwd,
+            "precision_form": {"loss": ploss, "grad_acc_max": pgmax, "forward_ms_per_step": 1e3 * pfwd / steps,
+                               "forward_and_adjoint_ms_per_step": 1e3 * padj / steps, "cost_ratio": padj / pfwd,
+                               "cost_ratio_vs_plain_forward": padj / fwd}}
 
 
 def profile_section(lib, handle, index):

@@ -384,3 +384,39 @@ def test_c5_precision_form_gradient_matches_fp64_adjoint_and_central_differences
     assert err64 <= 1e-3
     assert max(fd_err64) <= 1e-5
     assert max(fd_err) <= 1e-3
+
+
+def test_c5_200_step_gradient_matches_central_differences_of_the_gpu_forward():
+    """The workload as the bench runs it: 200 steps, a 16-segment profile,
+    checkpoint every 20, in the precision form.  An FP64 adjoint of 200 steps
+    at this size takes the host many minutes, so the gradient is held to
+    central differences of the GPU's own forward loss (precision form) on the
+    three components of largest magnitude, within 1e-3, with the step taken
+    as the float inputs actually differ.  The step is 1e-2 of the component:
+    over 200 steps the FP32-state forward answers a perturbation of 1e-3 with
+    rounding noise of a few 1e-3 of the difference (measured 3e-3..7e-3 at
+    1e-3, 2e-4..6e-4 at 1e-2, 7e-4..9e-4 at 5e-2 where the response stops
+    being linear).  The FP64 checks at 1e-3 are the 20-step test above."""
+    say("test_c5_200_step_gradient_matches_central_differences_of_the_gpu_forward")
+    sys.path.insert(0, str(__import__("pathlib").Path(__file__).resolve().parent.parent))
+    import bench
+    import cabi
+
+    steps = 200
+    fluid, cfg, ps, lo, hi, acc = bench.c5_case(cabi, steps=steps, segments=16)
+    cfg.compensated_sums = 1
+    loss, g = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=20)[:2]
+    say("loss", loss, "max |grad|", np.abs(g).max())
+    errs = []
+    for idx in np.argsort(-np.abs(g).ravel())[:3]:
+        k, c = divmod(int(idx), 3)
+        a_p, a_m = acc.copy(), acc.copy()
+        h = 1e-2 * abs(float(acc[k, c]))
+        a_p[k, c] = np.float32(float(acc[k, c]) + h)
+        a_m[k, c] = np.float32(float(acc[k, c]) - h)
+        lp = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(a_p, grad=False)[0]
+        lm = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(a_m, grad=False)[0]
+        fd = (lp - lm) / (float(a_p[k, c]) - float(a_m[k, c]))
+        errs.append(abs(g[k, c] - fd) / abs(fd))
+        say(f"d loss / d acc[{k},{c}]: central difference {fd:.6e}, adjoint {g[k, c]:.6e}, rel {errs[-1]:.2e}")
+    assert max(errs) <= 1e-3
