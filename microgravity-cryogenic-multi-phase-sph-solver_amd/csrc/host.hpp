@@ -74,6 +74,15 @@ class Geometry {
   double volume() const;
   V3 outward_normal(double x, double y, double z, double eps) const;
   double mesh_winding(int mesh_id, double x, double y, double z) const;
+  // A mesh as flat arrays (vertex k at v[3k..3k+2], triangle t at t[3t..3t+2])
+  struct MeshView {
+    const double* v;
+    const int32_t* t;
+    int tris;
+  };
+  MeshView mesh_view(int mesh_id) const;
+  int mesh_count() const { return static_cast<int>(meshes_.size()); }
+  std::size_t mesh_vertex_values(int mesh_id) const { return meshes_[mesh_id].v.size(); }
   const std::vector<cryo_geo_node>& node_list() const { return nodes_; }
   bool has_mesh() const;      // mesh or caller-defined nodes (evaluated on the host only)
   bool has_callback() const;

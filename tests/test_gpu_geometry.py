@@ -2,15 +2,16 @@
 the device (cryo_build_tank_particles_device) against the host generator,
 which is bit-exact with the reference's build_tank_particles
 (geometry.cpp:242-330): every field of every particle must agree bit for bit,
-for analytic tanks and a CSG tree with translations and a difference, and at
-the full-size LH2 sphere (1.18 M particles)."""
+for analytic tanks, a CSG tree with translations and a difference, mesh tanks
+(alone and inside a CSG union), and at the full-size LH2 sphere (1.18 M
+particles)."""
 import time
 
 import numpy as np
 import pytest
 
 import cabi
-from test_cabi import GEOMETRIES
+from test_cabi import GEOMETRIES, icosphere
 
 pytestmark = pytest.mark.gpu
 
@@ -45,4 +46,31 @@ def test_device_lattice_full_size_sphere():
     # each call builds twice (count, then fill)
     print(f"host generator {t1 - t0:.3f} s, device-evaluated {t2 - t1:.3f} s", flush=True)
     assert len(h) == 1176208
+    same(h, d)
+
+
+@pytest.mark.parametrize("shift", [(0.0, 0.0, 0.0), (0.013, -0.007, 0.021)])
+def test_device_mesh_lattice_matches_host(shift):
+    """The mesh tank of test_cabi.py's reference check (icosphere, alone and
+    translated inside a union with a box, slanted 30 % fill) with 1280
+    triangles: nearest-triangle distances and the winding-number sign
+    evaluated per lattice node on the device."""
+    print("test_device_mesh_lattice_matches_host", shift, flush=True)
+    verts, tris = icosphere(0.05, levels=3)
+    nodes = [(cabi.GEO_MESH, -1, -1, ())]
+    if any(shift):
+        nodes += [(cabi.GEO_TRANSLATED, 0, -1, shift), (cabi.GEO_BOX, -1, -1, (0.03, 0.03, 0.14)),
+                  (cabi.GEO_UNION, 1, 2, ())]
+    arr = cabi.geo_nodes(nodes)
+    arr[0].mesh_id = 0
+    fluid = cabi.hip_fluid_preset("LN2")
+    fill = cabi.default_fill(fill_fraction=0.3, settle_direction=(0.3, 0.2, -1.0))
+    t0 = time.perf_counter()
+    h = cabi.hip_build_tank_particles(arr, fluid, 0.004, fill, meshes=[(verts, tris)])
+    t1 = time.perf_counter()
+    d = cabi.hip_build_tank_particles(arr, fluid, 0.004, fill, device=True, meshes=[(verts, tris)])
+    t2 = time.perf_counter()
+    print(f"{len(h)} particles, {len(tris)} triangles: host {t1 - t0:.3f} s, device-evaluated {t2 - t1:.3f} s",
+          flush=True)
+    assert int((h.phase == cabi.WALL).sum()) > 0 and int((h.phase == cabi.VAPOR).sum()) > 0
     same(h, d)
