@@ -16,6 +16,7 @@
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "host.hpp"
@@ -921,10 +922,26 @@ struct cryo_solver {
   void step_finish() {
     if (cfg.phase_change != 0) {
       sync_state();
-      n = hs->n;
-      pending_dead = hs->merges - merges_seen;
-      merges_seen = hs->merges;
+      take_counts();
     }
+  }
+  // The same in two halves for callers that poll (the ensemble runner): the
+  // state copy is queued behind the step, and once the stream has drained the
+  // host takes the counts from it.
+  bool needs_counts() const { return cfg.phase_change != 0; }
+  void queue_state_copy() {
+    CRYO_HIP(hipMemcpyAsync(hs, state.get(), sizeof(DevState), hipMemcpyDeviceToHost, stream));
+  }
+  bool stream_idle() const {
+    const hipError_t e = hipStreamQuery(stream);
+    if (e == hipErrorNotReady) return false;
+    CRYO_HIP(e);
+    return true;
+  }
+  void take_counts() {
+    n = hs->n;
+    pending_dead = hs->merges - merges_seen;
+    merges_seen = hs->merges;
   }
 
   void scale_velocities(float f) { launch_scale_velocities(view(), n, f, stream); }
@@ -1977,8 +1994,11 @@ int cryo_ensemble_run_steps(cryo_solver* const* solvers, int32_t count, int64_t 
     // Every member's step is queued on its own stream before any member
     // waits, so the device runs the tanks' kernels side by side and a
     // member's host round trip overlaps the others' device work.  A member
-    // queues its next step as soon as its own bookkeeping is back, so the
-    // device never runs dry between ensemble steps.
+    // whose next step needs its particle count back (phase change) queues
+    // the state copy behind its step; the host polls the members' streams and
+    // queues a member's next step as soon as its own copy has landed,
+    // whichever member that is, so no member waits behind a slower one.
+    // Members without phase change queue all their steps at once.
     // Each member's step is a captured graph, replayed step after step.
     struct Graphs {
       cryo_solver* const* v;
@@ -1997,13 +2017,41 @@ int cryo_ensemble_run_steps(cryo_solver* const* solvers, int32_t count, int64_t 
       z.graphs_on = z.use_graphs;
     }
     if (steps > 0) {
-      for (int32_t t = 0; t < count; ++t) S(solvers[t]).step_issue_graph();
-      for (int64_t q = 1; q < steps; ++q)
-        for (int32_t t = 0; t < count; ++t) {
-          S(solvers[t]).step_finish();
-          S(solvers[t]).step_issue_graph();
+      std::vector<int64_t> issued(static_cast<std::size_t>(count), 0);
+      int32_t waiting = 0;  // members with a step whose counts are not back yet
+      for (int32_t t = 0; t < count; ++t) {
+        cryo_solver& z = S(solvers[t]);
+        if (!z.needs_counts()) {
+          for (int64_t q = 0; q < steps; ++q) z.step_issue_graph();
+          issued[t] = steps;
+          continue;
         }
-      for (int32_t t = 0; t < count; ++t) S(solvers[t]).step_finish();
+        z.step_issue_graph();
+        z.queue_state_copy();
+        issued[t] = 1;
+        ++waiting;
+      }
+      std::vector<char> pending(static_cast<std::size_t>(count), 0);
+      for (int32_t t = 0; t < count; ++t) pending[t] = S(solvers[t]).needs_counts() ? 1 : 0;
+      while (waiting > 0) {
+        bool progressed = false;
+        for (int32_t t = 0; t < count; ++t) {
+          if (!pending[t]) continue;
+          cryo_solver& z = S(solvers[t]);
+          if (!z.stream_idle()) continue;
+          z.take_counts();
+          progressed = true;
+          if (issued[t] < steps) {
+            z.step_issue_graph();
+            z.queue_state_copy();
+            ++issued[t];
+          } else {
+            pending[t] = 0;
+            --waiting;
+          }
+        }
+        if (!progressed) std::this_thread::yield();
+      }
     }
     for (int32_t t = 0; t < count; ++t) S(solvers[t]).sync_state();
   })
