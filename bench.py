@@ -417,7 +417,7 @@ def main():
         print(json.dumps(ensemble_measure(cabi, c4_tanks(cabi), max(a.ensemble_steps, 1))))
         return
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported (how the step would shard "
+        print(f"bench.py: the benchmark runs on one GPU; --gpus {a.gpus} is not supported (how the step would shard "
               "across GPUs is described in the design notes)", file=sys.stderr)
         sys.exit(2)
 

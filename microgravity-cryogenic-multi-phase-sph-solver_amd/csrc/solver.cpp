@@ -201,7 +201,7 @@ void check_device() {
   CRYO_HIP(hipGetDeviceProperties(&prop, dev));
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     throw std::runtime_error(std::string("CryoSPH: device arch '") + prop.gcnArchName +
-                             "' is not gfx950; this build only carries gfx950 code");
+                             "' is not gfx950; this library carries gfx950 code only");
 }
 
 }  // namespace
