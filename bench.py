@@ -174,8 +174,10 @@ def c4_tanks(cabi, count=64, seed=0):
 def ensemble_measure(cabi, tanks, steps, tune_steps=3):
     """The same tanks advanced `steps` steps by the per-tank loop of Solvers (the
     default launch shape), then autotuned (cryo_solver_autotune: each member
-    keeps the fastest launch variant of its neighbour passes; the results are
-    the same bits), then by the loop again and by cryo_ensemble_run_steps
+    keeps the fastest launch variant of its neighbour passes, then, where its
+    forces pass runs over LDS cell tiles, cryo_solver_autotune_tiles the fastest
+    cell-tile shape; the results are the same bits either way), then by the
+    loop again and by cryo_ensemble_run_steps
     (bitwise the loop's results; the GPU test checks that), wall-clock per
     ensemble step each way; and the ensemble once more with every member in
     the precision form, the form whose slosh force meets the FP64 bar."""
@@ -191,8 +193,14 @@ def ensemble_measure(cabi, tanks, steps, tune_steps=3):
     for s in solvers:
         s.run_steps(steps)
     loop = time.perf_counter() - t0
+    def tune_member(s):
+        v = s.autotune(tune_steps)[0]
+        shape = s.autotune_tiles(tune_steps)[0] if (v & 1) == 0 else s.tile_shape()
+        return v, shape
+
     t0 = time.perf_counter()
-    picks = [s.autotune(tune_steps)[0] for s in solvers]
+    tuned = [tune_member(s) for s in solvers]
+    picks = [v for v, _ in tuned]
     tune = time.perf_counter() - t0
     t0 = time.perf_counter()
     with marker("per_tank_loop_tuned"):
@@ -222,7 +230,7 @@ def ensemble_measure(cabi, tanks, steps, tune_steps=3):
         prec.append(s)
     for s in prec:
         s.run_steps(1)
-        s.autotune(tune_steps)
+        tune_member(s)
     t0 = time.perf_counter()
     cabi.hip_ensemble_run_steps(prec, steps)
     ens_prec = time.perf_counter() - t0
@@ -230,6 +238,8 @@ def ensemble_measure(cabi, tanks, steps, tune_steps=3):
     return {"tanks": len(tanks), "particles": n, "steps": steps,
             "per_tank_loop_ms_per_step": 1e3 * loop / steps,
             "autotune_s": tune, "launch_variants_chosen": [picks.count(v) for v in range(4)],
+            "tile_shapes_chosen": {"x".join(map(str, sh)): [sh for _, sh in tuned].count(sh)
+                                   for sh in sorted({sh for _, sh in tuned})},
             "per_tank_loop_tuned_ms_per_step": 1e3 * loop_tuned / steps,
             "ensemble_ms_per_step": 1e3 * ens / steps,
             "ensemble_step_graphs_ms_per_step": 1e3 * ens_graphs / steps,

@@ -203,6 +203,25 @@ int cryo_solver_set_step_graphs(cryo_solver* s, int32_t on);
 int cryo_solver_launch_variant(cryo_solver* s, int32_t* variant);
 int cryo_solver_autotune(cryo_solver* s, int32_t trial_steps, int32_t* variant, double* ms_per_step);
 
+/* Extension (no counterpart in the reference): the cell-tile shape of the
+ * LDS-staged interface and forces passes, in grid cells along x, y and z
+ * (default 3 x 3 x 3).  Accepted: 1 <= x <= 16, y, z >= 1, (y + 2)(z + 2) <= 25
+ * and y z <= 9; other shapes fail with CRYO_ERR_INVALID_ARGUMENT.  Every shape
+ * gives the same results bit for bit; the shape sets how many particles each
+ * workgroup stages in LDS and serves.  cryo_solver_tile_shape writes the
+ * current shape to cells_xyz[0..2].
+ * cryo_solver_autotune_tiles times `trial_steps` steps of each of `count`
+ * shapes shapes[3k..3k+2] (shapes NULL: six built-in candidates, 3x3x3, 4x3x2,
+ * 4x2x3, 2x3x3, 3x3x2 and 3x2x3, and count is ignored) at the current launch
+ * variant, from the solver's current state (after one untimed step each),
+ * keeps the fastest, returns its index in *pick (may be NULL) and each shape's
+ * milliseconds per step in ms_per_step[k] (may be NULL; count entries, six
+ * for the built-in list), and puts the solver's state back as it was. */
+int cryo_solver_set_tile_shape(cryo_solver* s, int32_t cells_x, int32_t cells_y, int32_t cells_z);
+int cryo_solver_tile_shape(cryo_solver* s, int32_t* cells_xyz);
+int cryo_solver_autotune_tiles(cryo_solver* s, int32_t trial_steps, const int32_t* shapes, int32_t count,
+                               int32_t* pick, double* ms_per_step);
+
 /* Extension (no counterpart in the reference, which has no adjoint or autodiff):
  * runs `steps` steps, step k with frame acceleration frame_acc[3k..3k+2]
  * (g_eff = gravity - acc, as cryo_solver_set_frame_motion sets it), and returns

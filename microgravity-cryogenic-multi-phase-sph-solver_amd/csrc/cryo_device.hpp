@@ -332,7 +332,15 @@ struct GridDev {
   // change pass, whether one of them gives mass to liquid / vapour receivers
   // (kCellSrcL, kCellSrcV)
   uint8_t* flags;
+  // cell-tile shape of the LDS-staged passes: cells along x, y, z
+  int tx = 3, ty = 3, tz = 3;
 };
+// Cell-tile shapes the tiled passes accept: at most 25 staged (z, y) rows,
+// (ty + 2)(tz + 2), and at most 9 target rows, ty tz (the sizes of their row
+// tables); at most 16 cells along x.
+__host__ __device__ inline bool tile_shape_ok(int tx, int ty, int tz) {
+  return tx >= 1 && ty >= 1 && tz >= 1 && tx <= 16 && (ty + 2) * (tz + 2) <= 25 && ty * tz <= 9;
+}
 constexpr int kCellWall = 1, kCellLiquid = 2, kCellVapor = 4, kCellFluid = 6, kCellSrcL = 8, kCellSrcV = 16;
 __host__ __device__ inline void cell_coords(const GridDev& g, V3 p, int& cx, int& cy, int& cz) {
   cx = clampv(static_cast<int>(floorf((p.x - g.ox) * g.inv_cell)), 0, g.nx - 1);

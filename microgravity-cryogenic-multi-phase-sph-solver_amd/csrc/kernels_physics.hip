@@ -799,11 +799,13 @@ __global__ __launch_bounds__(kBlock) void k_forces(PV pv, GridDev g, SimParams P
 // forward-moving row cursor maps index j to its LDS slot.  A tile whose rows
 // hold more than kTileCap particles gathers from global memory instead;
 // overflowed lists (count < 0) are left to the sweep instance of k_forces.
-constexpr int kTileCells = 3;
+// The tile shape (g.tx x g.ty x g.tz cells, 3 x 3 x 3 by default) is a launch
+// parameter: any shape with (ty + 2)(tz + 2) <= kTileRows staged rows and
+// ty tz <= kTileTargetRows target rows (tile_shape_ok) gives the same results.
 constexpr int kTileThreads = 512;
 constexpr int kTileCap = 2304;
-constexpr int kTileRows = (kTileCells + 2) * (kTileCells + 2);
-constexpr int kTileTargetRows = kTileCells * kTileCells;
+constexpr int kTileRows = 25;
+constexpr int kTileTargetRows = 9;
 
 struct TileRows {
   int start[kTileRows];     // first sorted index of each staged row
@@ -829,14 +831,20 @@ __device__ inline int xcd_tile(int b, int nblocks) {
   return b / span * span + (b % 8) * kTileChunk + (b / 8) % kTileChunk;
 }
 
+// Number of cell tiles (workgroups) of the tiled passes over grid g.
+inline long long tile_count(const GridDev& g) {
+  if (!tile_shape_ok(g.tx, g.ty, g.tz)) throw std::logic_error("cell-tile shape out of range");
+  return static_cast<long long>((g.nx + g.tx - 1) / g.tx) * ((g.ny + g.ty - 1) / g.ty) * ((g.nz + g.tz - 1) / g.tz);
+}
+
 // Row tables of tile `b`; called by the whole block, ends with a barrier.
 __device__ inline void tile_rows(const GridDev& g, int b, TileRows& T) {
-  const int tiles_x = (g.nx + kTileCells - 1) / kTileCells;
-  const int tiles_y = (g.ny + kTileCells - 1) / kTileCells;
-  const int cx0 = (b % tiles_x) * kTileCells;
-  const int cy0 = ((b / tiles_x) % tiles_y) * kTileCells;
-  const int cz0 = (b / (tiles_x * tiles_y)) * kTileCells;
-  const int cx1 = min(cx0 + kTileCells, g.nx), cy1 = min(cy0 + kTileCells, g.ny), cz1 = min(cz0 + kTileCells, g.nz);
+  const int tiles_x = (g.nx + g.tx - 1) / g.tx;
+  const int tiles_y = (g.ny + g.ty - 1) / g.ty;
+  const int cx0 = (b % tiles_x) * g.tx;
+  const int cy0 = ((b / tiles_x) % tiles_y) * g.ty;
+  const int cz0 = (b / (tiles_x * tiles_y)) * g.tz;
+  const int cx1 = min(cx0 + g.tx, g.nx), cy1 = min(cy0 + g.ty, g.ny), cz1 = min(cz0 + g.tz, g.nz);
   const int sx0 = max(cx0 - 1, 0), sx1 = min(cx1, g.nx - 1);
   const int sy0 = max(cy0 - 1, 0), sy1 = min(cy1, g.ny - 1);
   const int sz0 = max(cz0 - 1, 0), sz1 = min(cz1, g.nz - 1);
@@ -845,7 +853,8 @@ __device__ inline void tile_rows(const GridDev& g, int b, TileRows& T) {
   const int tyr = cy1 - cy0;
   const int ntrows = (cz1 - cz0) * tyr;
   const int t = threadIdx.x;
-  // row lengths in parallel: threads [0, 25) staged rows, [32, 41) target rows
+  // row lengths in parallel: threads [0, nrows) staged rows (nrows <= 25),
+  // [32, 32 + ntrows) target rows (ntrows <= 9)
   if (t < nrows) {
     const int row = ((sz0 + t / nyr) * g.ny + (sy0 + t % nyr)) * g.nx;
     const int s = g.off[row + sx0];
@@ -1317,8 +1326,7 @@ static void interface_variant(const PV& pv, const GridDev& g, const SimParams& P
     k_interface<SPS, KS><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, n, cm, heat_partials, 0);
     return;
   }
-  const long long tiles = static_cast<long long>((g.nx + kTileCells - 1) / kTileCells) *
-                          ((g.ny + kTileCells - 1) / kTileCells) * ((g.nz + kTileCells - 1) / kTileCells);
+  const long long tiles = tile_count(g);
   if (tiles > 0x7fffffffLL) throw std::runtime_error("interface pass: too many cell tiles");
   k_interface_tiled<SPS, KS><<<static_cast<int>(tiles), kTileThreads, 0, s>>>(pv, g, P, L, st, cm);
   k_interface<SPS, KS><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, n, cm, nullptr, 1);
@@ -1358,8 +1366,7 @@ static void forces_variant(const PV& pv, const GridDev& g, const SimParams& P, c
                            bool tiled, hipStream_t s) {
   if (!L.valid) throw std::logic_error("forces pass: neighbour lists not built");
   if (tiled) {
-    const long long tiles = static_cast<long long>((g.nx + kTileCells - 1) / kTileCells) *
-                            ((g.ny + kTileCells - 1) / kTileCells) * ((g.nz + kTileCells - 1) / kTileCells);
+    const long long tiles = tile_count(g);
     if (tiles > 0x7fffffffLL) throw std::runtime_error("forces pass: too many cell tiles");
     k_forces_tiled<ST, CONT, THERMAL, KS><<<static_cast<int>(tiles), kTileThreads, 0, s>>>(pv, g, P, L, st);
   } else {

@@ -241,6 +241,9 @@ struct cryo_solver {
   // way): LDS-staged cell tiles or per-particle list gathers, for the
   // interface and the forces pass.  autotune() times them.
   bool tile_iface = true, tile_forces = true;
+  // cell-tile shape of the tiled passes (cells along x, y, z); any accepted
+  // shape gives the same bits.  autotune_tiles() times candidates.
+  int tile_shape[3] = {3, 3, 3};
   const uint32_t* last_perm = nullptr;
   int n_sorted = 0;
   // reductions
@@ -400,6 +403,9 @@ struct cryo_solver {
     GridDev gd = g;
     gd.off = cell_off.get();
     gd.flags = cell_flags.get();
+    gd.tx = tile_shape[0];
+    gd.ty = tile_shape[1];
+    gd.tz = tile_shape[2];
     return gd;
   }
   NbrList lists() {
@@ -1411,10 +1417,18 @@ struct cryo_solver {
     tile_forces = (v & 1) == 0;
     tile_iface = (v & 2) == 0;
   }
-  // Times `trials` steps of every launch variant from the current state (after
-  // one untimed step each), keeps the fastest and puts the state back.  ms[v]
-  // (optional, 4 entries) receives each variant's milliseconds per step.
-  int autotune(int trials, double* ms) {
+  void set_tile_shape(int x, int y, int z) {
+    if (!tile_shape_ok(x, y, z))
+      throw std::invalid_argument("tile shape: need 1 <= x <= 16, y, z >= 1, (y + 2)(z + 2) <= 25 and y z <= 9");
+    tile_shape[0] = x, tile_shape[1] = y, tile_shape[2] = z;
+  }
+  // Times `trials` steps of each of `count` configurations from the current
+  // state (after one untimed step each; apply(k) selects configuration k),
+  // selects the fastest and puts the state back.  ms[k] (optional) receives
+  // configuration k's milliseconds per step.  Returns the chosen k; on an
+  // error the configuration `current` is selected again.
+  template <class Apply>
+  int time_configurations(int trials, int count, int current, Apply&& apply, double* ms) {
     if (trials <= 0) throw std::invalid_argument("autotune: trial_steps must be > 0");
     ensure_initialized();
     sync_state();
@@ -1423,12 +1437,11 @@ struct cryo_solver {
     snap_save_scratch(z);
     const int pd = pending_dead;
     const int ms_seen = merges_seen;
-    const int v0 = launch_variant();
     hipEvent_t e0, e1;
     CRYO_HIP(hipEventCreate(&e0));
     CRYO_HIP(hipEventCreate(&e1));
     double best = 0.0;
-    int pick = v0;
+    int pick = current;
     auto restore = [&]() {
       snap_load(z, true);
       snap_load_scratch(z);
@@ -1437,32 +1450,57 @@ struct cryo_solver {
       sync_state();
     };
     try {
-      for (int v = 0; v < 4; ++v) {
+      for (int k = 0; k < count; ++k) {
         restore();
-        set_launch_variant(v);
+        apply(k);
         step();
         CRYO_HIP(hipEventRecord(e0, stream));
-        for (int k = 0; k < trials; ++k) step();
+        for (int r = 0; r < trials; ++r) step();
         CRYO_HIP(hipEventRecord(e1, stream));
         CRYO_HIP(hipEventSynchronize(e1));
-        float t = 0.0f;
-        CRYO_HIP(hipEventElapsedTime(&t, e0, e1));
-        const double per = static_cast<double>(t) / trials;
-        if (ms) ms[v] = per;
-        if (v == 0 || per < best) best = per, pick = v;
+        float elapsed = 0.0f;
+        CRYO_HIP(hipEventElapsedTime(&elapsed, e0, e1));
+        const double per = static_cast<double>(elapsed) / trials;
+        if (ms) ms[k] = per;
+        if (k == 0 || per < best) best = per, pick = k;
       }
     } catch (...) {
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
-      set_launch_variant(v0);
+      apply(current);
       restore();
       throw;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    set_launch_variant(pick);
+    apply(pick);
     restore();
     return pick;
+  }
+  // Every launch variant (ms: 4 entries).
+  int autotune(int trials, double* ms) {
+    return time_configurations(trials, 4, launch_variant(), [this](int v) { set_launch_variant(v); }, ms);
+  }
+  // Tile shapes shapes[3k..3k+2], k < count, at the current launch variant;
+  // shapes == nullptr: the built-in candidates below (count ignored; ms then
+  // needs kTileCandidates entries).  Returns the index of the chosen shape.
+  static constexpr int kTileCandidates = 6;
+  int autotune_tiles(int trials, const int32_t* shapes, int count, double* ms) {
+    static constexpr int32_t builtin[kTileCandidates * 3] = {3, 3, 3, 4, 3, 2, 4, 2, 3, 2, 3, 3, 3, 3, 2, 3, 2, 3};
+    if (!shapes) shapes = builtin, count = kTileCandidates;
+    if (count <= 0) throw std::invalid_argument("autotune_tiles: need at least one tile shape");
+    for (int k = 0; k < count; ++k)
+      if (!tile_shape_ok(shapes[3 * k], shapes[3 * k + 1], shapes[3 * k + 2]))
+        throw std::invalid_argument("autotune_tiles: tile shape " + std::to_string(k) + " out of range");
+    const int keep[3] = {tile_shape[0], tile_shape[1], tile_shape[2]};
+    // the current shape is put back on an error: apply(count) means "keep"
+    return time_configurations(
+        trials, count, count,
+        [&](int k) {
+          if (k >= count) set_tile_shape(keep[0], keep[1], keep[2]);
+          else set_tile_shape(shapes[3 * k], shapes[3 * k + 1], shapes[3 * k + 2]);
+        },
+        ms);
   }
 
   // -------------------------------------------------------------- frame
@@ -1980,6 +2018,22 @@ int cryo_solver_autotune(cryo_solver* s, int32_t trial_steps, int32_t* variant, 
   CRYO_GUARD({
     const int v = S(s).autotune(trial_steps, ms_per_step);
     if (variant) *variant = v;
+  })
+}
+int cryo_solver_set_tile_shape(cryo_solver* s, int32_t cells_x, int32_t cells_y, int32_t cells_z) {
+  CRYO_GUARD(S(s).set_tile_shape(cells_x, cells_y, cells_z))
+}
+int cryo_solver_tile_shape(cryo_solver* s, int32_t* cells_xyz) {
+  if (!cells_xyz) return fail("cells_xyz is required", CRYO_ERR_INVALID_ARGUMENT);
+  CRYO_GUARD({
+    for (int k = 0; k < 3; ++k) cells_xyz[k] = S(s).tile_shape[k];
+  })
+}
+int cryo_solver_autotune_tiles(cryo_solver* s, int32_t trial_steps, const int32_t* shapes, int32_t count,
+                               int32_t* pick, double* ms_per_step) {
+  CRYO_GUARD({
+    const int k = S(s).autotune_tiles(trial_steps, shapes, count, ms_per_step);
+    if (pick) *pick = k;
   })
 }
 int cryo_ensemble_run_steps(cryo_solver* const* solvers, int32_t count, int64_t steps) {

@@ -606,6 +606,31 @@ class HipSolver:
         self._call("solver_autotune", self.h, C.c_int32(trial_steps), C.byref(v), ms)
         return v.value, list(ms)
 
+    def set_tile_shape(self, x, y, z):
+        self._call("solver_set_tile_shape", self.h, C.c_int32(x), C.c_int32(y), C.c_int32(z))
+
+    def tile_shape(self):
+        xyz = (C.c_int32 * 3)()
+        self._call("solver_tile_shape", self.h, xyz)
+        return tuple(xyz)
+
+    TILE_CANDIDATES = [(3, 3, 3), (4, 3, 2), (4, 2, 3), (2, 3, 3), (3, 3, 2), (3, 2, 3)]
+
+    def autotune_tiles(self, trial_steps=5, shapes=None):
+        """See cryo_solver_autotune_tiles: (chosen shape, ms per step of each shape);
+        shapes None: the built-in candidates (TILE_CANDIDATES)."""
+        k = C.c_int32()
+        if shapes is None:
+            ms = (C.c_double * len(self.TILE_CANDIDATES))()
+            self._call("solver_autotune_tiles", self.h, C.c_int32(trial_steps), None, C.c_int32(0), C.byref(k), ms)
+            shapes = self.TILE_CANDIDATES
+        else:
+            flat = (C.c_int32 * (3 * len(shapes)))(*[int(v) for sh in shapes for v in sh])
+            ms = (C.c_double * len(shapes))()
+            self._call("solver_autotune_tiles", self.h, C.c_int32(trial_steps), flat, C.c_int32(len(shapes)),
+                       C.byref(k), ms)
+        return tuple(shapes[k.value]), list(ms)
+
     def initialize(self):
         self._call("solver_initialize", self.h)
 

@@ -128,6 +128,59 @@ def test_launch_variants_give_the_same_bits_and_autotune_restores_the_state():
         make(tanks[:1])[0].set_launch_variant(4)
 
 
+def _akinci_tank():
+    """A liquid drop in a sphere with Akinci tension: its interface pass also
+    runs over LDS cell tiles."""
+    fluid = cabi.hip_fluid_preset("LN2")
+    cfg = cabi.default_config(dx=0.005, surface_tension=2, thermal=0, phase_change=0, gravity=(0, 0, -0.3),
+                              contact_angle_deg=60)
+    ps = cabi.hip_build_tank_particles(cabi.sphere(0.03), fluid, 0.005,
+                                       cabi.default_fill(include_vapor=0, fill_fraction=0.6))
+    rng = np.random.default_rng(3)
+    fl = ps.phase != cabi.WALL
+    ps.x[fl] += rng.uniform(-0.05 * 0.005, 0.05 * 0.005, size=(int(fl.sum()), 3)).astype(np.float32)
+    lo, hi = ps.x.min(0) - np.float32(0.02), ps.x.max(0) + np.float32(0.02)
+    return dict(cfg=cfg, fluid=fluid, ps=ps, lo=lo, hi=hi, a_lat=0.0, desc="LN2 drop, Akinci")
+
+
+def test_tile_shapes_give_the_same_bits_and_autotune_tiles_restores_the_state():
+    """Every accepted cell-tile shape of the LDS-staged passes, from single
+    cells to long thin tiles whose staged rows overflow LDS (their
+    global-memory path), steps a tank to the same bits; the shape autotuner's
+    trial steps leave no trace; shapes outside the row tables are refused."""
+    print("test_tile_shapes_give_the_same_bits_and_autotune_tiles_restores_the_state", flush=True)
+    shapes = HipSolver.TILE_CANDIDATES + [(1, 1, 1), (8, 1, 2), (16, 1, 1), (1, 3, 3), (2, 1, 6)]
+    for t in [small_tanks()[0], _akinci_tank()]:
+        print(t["desc"], flush=True)
+        ref = make([t])[0]
+        assert ref.tile_shape() == (3, 3, 3)
+        ref.run_steps(5)
+        for sh in shapes[1:]:
+            s = make([t])[0]
+            s.set_tile_shape(*sh)
+            assert s.tile_shape() == sh
+            s.run_steps(5)
+            _same(ref, s)
+        s = make([t])[0]
+        s.run_steps(2)
+        sh, ms = s.autotune_tiles(2)
+        print("autotune_tiles", sh, [round(v, 3) for v in ms], flush=True)
+        assert sh in HipSolver.TILE_CANDIDATES and s.tile_shape() == sh and min(ms) > 0
+        assert ms[HipSolver.TILE_CANDIDATES.index(sh)] == min(ms)
+        assert s.step_count == 2
+        s.run_steps(3)
+        _same(ref, s)
+        sh2, ms2 = s.autotune_tiles(1, shapes=[(1, 1, 1), (2, 2, 2)])
+        assert sh2 in ((1, 1, 1), (2, 2, 2)) and len(ms2) == 2 and s.tile_shape() == sh2
+    s = make([small_tanks()[0]])[0]
+    for bad in [(0, 3, 3), (3, 4, 3), (17, 1, 1), (3, 1, 10), (1, 0, 1)]:
+        with pytest.raises(ValueError):
+            s.set_tile_shape(*bad)
+        with pytest.raises(ValueError):
+            s.autotune_tiles(1, shapes=[(3, 3, 3), bad])
+    assert s.tile_shape() == (3, 3, 3)
+
+
 def test_c4_full_ensemble_bitwise_and_slosh_force_against_fp64_oracle():
     """The whole batched-ensemble workload: the 64 LN2 / LOX cylinder and
     capsule tanks of bench.c4_tanks(seed=0) (9.25 M particles, counts
