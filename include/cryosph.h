@@ -452,12 +452,14 @@ int cryo_build_tank_particles(int32_t n_nodes, const cryo_geo_node* nodes, int32
                               const cryo_mesh* meshes, const cryo_fluid* fluid, float dx, const cryo_fill* fill,
                               int32_t* n, float* x, float* v, float* nrm, float* m, float* rho, float* T,
                               uint8_t* phase, uint8_t* flags, uint32_t* id);
-/* build_tank_particles with the lattice's node classes and wall normals
- * evaluated on the current device (next to geometry.cpp:242-330, which
- * evaluates them on the host): the same particles bit for bit, for analytic,
- * CSG and mesh tanks (a mesh's distances against every triangle and its
- * winding-number sign per lattice node, on the device). Same arguments and
- * two-call protocol as cryo_build_tank_particles. */
+/* build_tank_particles run on the current device (next to
+ * geometry.cpp:242-330, which runs on the host): the lattice's node classes,
+ * their compaction to wall and fluid lists, the wall normals, the fill cut and
+ * the particle arrays are all computed there and downloaded; the same
+ * particles bit for bit, for analytic, CSG and mesh tanks (a mesh's distances
+ * against every triangle and its winding-number sign per lattice node, on the
+ * device). Caller-defined distance functions are refused (host only). Same
+ * arguments and two-call protocol as cryo_build_tank_particles. */
 int cryo_build_tank_particles_device(int32_t n_nodes, const cryo_geo_node* nodes, int32_t n_meshes,
                                      const cryo_mesh* meshes, const cryo_fluid* fluid, float dx,
                                      const cryo_fill* fill, int32_t* n, float* x, float* v, float* nrm, float* m,

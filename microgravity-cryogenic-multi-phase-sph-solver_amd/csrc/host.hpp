@@ -102,8 +102,9 @@ class Geometry {
   std::vector<cryo_geo_callback> callbacks_;
 };
 
-// on_device: the lattice's signed distances and wall normals are evaluated
-// by a kernel on the current device (analytic and CSG tanks only).
+// on_device: the whole generator runs on the current device (node classes,
+// compaction, wall normals, fill cut, particle arrays) and the result is
+// downloaded; caller-defined distance functions are refused.
 HostParticles build_tank_particles(const Geometry& geo, const cryo_fluid& fluid, float dx, const cryo_fill& fill,
                                    bool on_device = false);
 

@@ -1,5 +1,6 @@
-"""Lattice generation with the signed distances and wall normals evaluated on
-the device (cryo_build_tank_particles_device) against the host generator,
+"""Lattice generation on the device (cryo_build_tank_particles_device: the
+node classes, their compaction, wall normals, fill cut and particle assembly
+all evaluated there) against the host generator,
 which is bit-exact with the reference's build_tank_particles
 (geometry.cpp:242-330): every field of every particle must agree bit for bit,
 for analytic tanks, a CSG tree with translations and a difference, mesh tanks
@@ -31,6 +32,28 @@ def test_device_lattice_matches_host(name):
     fluid = cabi.hip_fluid_preset("LN2")
     h = cabi.hip_build_tank_particles(make(), fluid, dx, cabi.default_fill(**fill))
     d = cabi.hip_build_tank_particles(make(), fluid, dx, cabi.default_fill(**fill), device=True)
+    same(h, d)
+
+
+# the fill cut and the kept set on their edges: liquid only (the kept fluid
+# compacted), nothing liquid, everything liquid, the bubble form
+FILLS = {
+    "liquid_only": dict(include_vapor=0),
+    "no_liquid": dict(fill_fraction=0.0),
+    "no_liquid_liquid_only": dict(fill_fraction=0.0, include_vapor=0),
+    "all_liquid_liquid_only": dict(fill_fraction=1.0, include_vapor=0),
+    "bubble_liquid_only": dict(mode=1, fill_fraction=0.7, bubble_center=(0.01, 0.0, 0.02), include_vapor=0),
+    "tilted_settle": dict(fill_fraction=0.45, settle_direction=(0.2, -0.7, -1.0)),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FILLS))
+def test_device_lattice_fill_cases_match_host(name):
+    print("test_device_lattice_fill_cases_match_host", name, flush=True)
+    fluid = cabi.hip_fluid_preset("LN2")
+    fill = cabi.default_fill(**FILLS[name])
+    h = cabi.hip_build_tank_particles(cabi.capsule(0.04, 0.08), fluid, 0.004, fill)
+    d = cabi.hip_build_tank_particles(cabi.capsule(0.04, 0.08), fluid, 0.004, fill, device=True)
     same(h, d)
 
 
