@@ -738,23 +738,23 @@ PYBIND11_MODULE(CryoSPH, m) {
       }, [](Solver& S, std::array<int, 3> xyz) { check(cryo_solver_set_tile_shape(S.s, xyz[0], xyz[1], xyz[2])); })
       .def("autotune_tiles", [](Solver& S, int trial_steps, std::optional<std::vector<std::array<int, 3>>> shapes) {
         std::vector<int32_t> flat;
-        if (shapes) {
+        if (shapes)
           for (const auto& sh : *shapes) flat.insert(flat.end(), sh.begin(), sh.end());
-        } else {
-          flat = {3, 3, 3, 4, 3, 2, 4, 2, 3, 2, 3, 3, 3, 3, 2, 3, 2, 3};  // the built-in candidates
-        }
-        const int32_t count = static_cast<int32_t>(flat.size() / 3);
+        // no shapes: the library's six built-in candidates
+        const int32_t count = shapes ? static_cast<int32_t>(flat.size() / 3) : 6;
         std::vector<double> ms(count > 0 ? count : 1, 0.0);
         int32_t k = 0;
         int r;
         {
           py::gil_scoped_release rel;
-          r = cryo_solver_autotune_tiles(S.s, trial_steps, flat.data(), count, &k, ms.data());
+          r = cryo_solver_autotune_tiles(S.s, trial_steps, shapes ? flat.data() : nullptr, count, &k, ms.data());
         }
         check(r);
+        int32_t xyz[3] = {0, 0, 0};
+        check(cryo_solver_tile_shape(S.s, xyz));
         py::list times;
         for (int32_t i = 0; i < count; ++i) times.append(ms[i]);
-        return py::make_tuple(py::make_tuple(flat[3 * k], flat[3 * k + 1], flat[3 * k + 2]), times);
+        return py::make_tuple(py::make_tuple(xyz[0], xyz[1], xyz[2]), times);
       }, py::arg("trial_steps") = 5, py::arg("shapes") = py::none(),
          "Times trial_steps steps of each cell-tile shape (cells along x, y, z; default: six built-in "
          "candidates) at the current launch variant, keeps the fastest, restores the state; returns "

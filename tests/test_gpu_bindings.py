@@ -123,6 +123,35 @@ def test_particle_arrays_are_copies_with_reference_dtypes():
     assert s.pressure().shape == (n,) and s.curvature().shape == (n,)
 
 
+def test_module_launch_tuning_keeps_the_results():
+    """The module's extensions for launch shapes (launch_variant, tile_shape,
+    autotune, autotune_tiles) change no particle: a tuned solver and an untuned
+    one stay bit for bit equal."""
+    say("test_module_launch_tuning_keeps_the_results")
+    mcfg, _ = configs(dx=0.005, gravity=(0, 0, -0.3), thermal=False, phase_change=False, surface_tension=2)
+    fill, _ = fills(include_vapor=False, fill_fraction=0.6)
+    a, b = (cs.Solver(mcfg, cs.FluidProperties.liquid_nitrogen()) for _ in range(2))
+    for s in (a, b):
+        s.build_tank(cs.SphereTank(0.03), fill)
+        s.run_steps(2)
+    assert b.tile_shape == (3, 3, 3)
+    b.tile_shape = (4, 2, 3)
+    assert b.tile_shape == (4, 2, 3)
+    with pytest.raises(ValueError):
+        b.tile_shape = (3, 4, 3)
+    shape, ms = b.autotune_tiles(1)
+    assert len(ms) == 6 and min(ms) > 0 and b.tile_shape == tuple(shape)
+    shape, ms = b.autotune_tiles(1, shapes=[(1, 1, 1), (2, 2, 2)])
+    assert len(ms) == 2 and tuple(shape) in ((1, 1, 1), (2, 2, 2)) and b.tile_shape == tuple(shape)
+    v, ms4 = b.autotune(1)
+    assert b.launch_variant == v and len(ms4) == 4
+    for s in (a, b):
+        s.run_steps(3)
+    pa, pb = a.particles(), b.particles()
+    for k in pa:
+        assert np.array_equal(pa[k].view(np.uint8), pb[k].view(np.uint8)), k
+
+
 def test_closed_tank_mass_and_energy_budgets():
     """Closed LN2 capsule, superheated liquid, Lee 10, 120 steps: mass to
     2.5e-6, enthalpy change = heater input within 1 % of the latent heat
