@@ -242,7 +242,7 @@ int cryo_solver_autotune_tiles(cryo_solver* s, int32_t trial_steps, const int32_
  * heater, Lee phase change (sources, receiver normalisation, apply, vapour
  * rebalance, venting, a closed tank's saturation-temperature feedback;
  * masses and temperatures are state), kick-drift-kick.  Requires
- * fixed_dt > 0, the Wendland C2 kernel, density summation, CSF only with
+ * fixed_dt > 0 (either kernel: Wendland C2 or cubic spline), density summation, CSF only with
  * unsmoothed normals, the classic force and normal rotation, no transport
  * stress, SPS or shifting, no manoeuvre table and no rotating frame (see
  * cryo_solver_adjoint_slosh_frame); a window in which a particle splits or

@@ -7,7 +7,7 @@
 //                                             artificial viscosity, wall pressure,
 //                                             near-wall guard, no-slip shear, wall AV
 //   density        (SPH_kernels.cu:6-34)      sigma = sum_j W(r_ij)
-// with the Wendland C2 kernel (SPH_kernels.cuh:14-64).  Each function returns
+// with the Wendland C2 or the cubic-spline kernel (SPH_kernels.cuh:14-64).  Each function returns
 // d(c . term)/d(inputs) for the cotangent c of the quantity the term adds to.
 //
 // Templated on the scalar type: float on the GPU, double in the FP64 CPU
@@ -63,29 +63,48 @@ CRYO_ADJ_HD inline T dot(Vec<T> a, Vec<T> b) {
 // Constants of one rates evaluation.
 template <class T>
 struct Ctx {
-  T norm, inv_h, h;  // Wendland C2: W = norm (1 - q/2)^4 (2q + 1), q = r / h
+  T norm, inv_h, h;  // kernel normalisation, 1 / h, h
   T support2;        // (2h)^2
   T eta2;            // 0.01 h^2 (the AV / viscosity regulariser)
   T r_guard;         // 0.6 dx
   T dt;              // the step's dt (the guard's damping term)
   T vol0;            // wall particle volume dx^3
   int no_slip;
+  int type;          // 1 Wendland C2: W = norm (1 - q/2)^4 (2q + 1);
+                     // 0 cubic spline: W = norm (1 - 3q^2/2 + 3q^3/4) (q < 1), norm (2 - q)^3 / 4
 };
 
-// W(r), s = W'(r) / r and s' / r = (ds/dr) / r for Wendland C2, r < 2h.
+// W(r), s = W'(r) / r and s' / r = (ds/dr) / r, q = r / h < 2.
 template <class T>
-CRYO_ADJ_HD inline T wendland_w(const Ctx<T>& c, T r) {
+CRYO_ADJ_HD inline T kern_w(const Ctx<T>& c, T r) {
   const T q = r * c.inv_h;
+  if (c.type == 0) {
+    if (q < T(1)) return c.norm * (T(1) - T(1.5) * q * q + T(0.75) * q * q * q);
+    const T u = T(2) - q;
+    return c.norm * T(0.25) * u * u * u;
+  }
   const T t = T(1) - T(0.5) * q;
   return c.norm * t * t * t * t * (T(2) * q + T(1));
 }
 template <class T>
-CRYO_ADJ_HD inline T wendland_s(const Ctx<T>& c, T r) {
+CRYO_ADJ_HD inline T kern_s(const Ctx<T>& c, T r) {
+  if (c.type == 0) {
+    const T q = r * c.inv_h;
+    if (q < T(1)) return c.norm * c.inv_h * c.inv_h * (T(-3) + T(2.25) * q);
+    const T u = T(2) - q;
+    return T(-0.75) * c.norm * c.inv_h * u * u / r;
+  }
   const T t = T(1) - T(0.5) * r * c.inv_h;
   return T(-5) * c.norm * c.inv_h * c.inv_h * t * t * t;
 }
 template <class T>
-CRYO_ADJ_HD inline T wendland_spr(const Ctx<T>& c, T r) {
+CRYO_ADJ_HD inline T kern_spr(const Ctx<T>& c, T r) {
+  if (c.type == 0) {
+    const T q = r * c.inv_h;
+    if (q < T(1)) return T(2.25) * c.norm * c.inv_h * c.inv_h * c.inv_h / r;
+    const T u = T(2) - q;
+    return T(0.75) * c.norm * c.inv_h * u * (T(2) * c.inv_h / r + u / (r * r)) / r;
+  }
   const T t = T(1) - T(0.5) * r * c.inv_h;
   return T(7.5) * c.norm * c.inv_h * c.inv_h * c.inv_h * t * t / r;
 }
@@ -125,7 +144,7 @@ CRYO_ADJ_HD inline FFGrad<T> ff_vjp(const Ctx<T>& c, const Side<T>& I, const Sid
   if (!(r2 < c.support2)) return o;
   const T r = std::sqrt(r2);
   if (!(r > T(1e-12) * c.h)) return o;  // gw = 0 and the viscous term is skipped
-  const T s = wendland_s(c, r), spr = wendland_spr(c, r);
+  const T s = kern_s(c, r), spr = kern_spr(c, r);
   const T Vi = I.m / I.rho, Vj = J.m / J.rho;
   const T vsum = Vi * Vi + Vj * Vj;
   const T rs = I.rho + J.rho;
@@ -211,7 +230,7 @@ CRYO_ADJ_HD inline FWGrad<T> fw_vjp(const Ctx<T>& c, const Side<T>& I, Vec<T> xw
   if (!(r2 < c.support2)) return o;
   const T r = std::sqrt(r2);
   if (!(r > T(1e-12) * c.h)) return o;  // every term vanishes or is skipped
-  const T s = wendland_s(c, r), spr = wendland_spr(c, r);
+  const T s = kern_s(c, r), spr = kern_spr(c, r);
   const T Vi = I.m / I.rho;
   const T vsum = Vi * Vi + c.vol0 * c.vol0;
   const T cFr = dot(cF, rij);
@@ -292,7 +311,7 @@ CRYO_ADJ_HD inline WBGrad<T> wb_vjp(const Ctx<T>& c, const Side<T>& F, Vec<T> xw
   const T r2 = dot(rwf, rwf);
   if (!(r2 < c.support2)) return o;
   const T r = std::sqrt(r2);
-  const T W = wendland_w(c, r), s = wendland_s(c, r);
+  const T W = kern_w(c, r), s = kern_s(c, r);
   const T gr = dot(g, rwf);
   o.p = cp * W;
   o.rho = cp * gr * W;
@@ -341,7 +360,7 @@ CRYO_ADJ_HD inline ThGrad<T> th_vjp(const Ctx<T>& c, Vec<T> rij, T Vi, T Vj, T k
   const T r2 = dot(rij, rij);
   if (!(r2 < c.support2)) return o;
   const T r = std::sqrt(r2);
-  const T s = wendland_s(c, r);
+  const T s = kern_s(c, r);
   const T den = r2 + c.eta2;
   const T f = r2 * s / den;
   const T dT = Ti - Tj;
@@ -351,7 +370,7 @@ CRYO_ADJ_HD inline ThGrad<T> th_vjp(const Ctx<T>& c, Vec<T> rij, T Vi, T Vj, T k
   o.Ti = k * Vi * Vj * f;
   o.Tj = -o.Ti;
   if (r > T(1e-12) * c.h) {
-    const T spr = wendland_spr(c, r);
+    const T spr = kern_spr(c, r);
     // (1/r) df/dr = ((2 s + r^2 spr) den - 2 r^2 s) / den^2
     const T fr = ((T(2) * s + r2 * spr) * den - T(2) * r2 * s) / (den * den);
     o.r = rij * (k * Vi * Vj * dT * fr);
@@ -378,7 +397,7 @@ CRYO_ADJ_HD inline CsfGrad<T> csf_vjp(const Ctx<T>& c, Vec<T> rij, Vec<T> ns_i, 
   if (!(r2 < c.support2)) return o;
   const T r = std::sqrt(r2);
   if (!(r > T(1e-12) * c.h)) return o;
-  const T s = wendland_s(c, r), spr = wendland_spr(c, r);
+  const T s = kern_s(c, r), spr = kern_spr(c, r);
   const Vec<T> u = ns_i - ns_j;
   const Vec<T> gw = rij * s;
   o.ns_i = gw * (num_b * Vj);
@@ -406,7 +425,7 @@ CRYO_ADJ_HD inline GcGrad<T> gc_ff_vjp(const Ctx<T>& c, Vec<T> rij, T Vj, T dcol
   if (!(r2 < c.support2)) return o;
   const T r = std::sqrt(r2);
   if (!(r > T(1e-12) * c.h)) return o;
-  const T s = wendland_s(c, r), spr = wendland_spr(c, r);
+  const T s = kern_s(c, r), spr = kern_spr(c, r);
   const T gr = dot(gb, rij);
   o.r = (gb * s + rij * (gr * spr)) * (Vj * dcol);
   o.V_j = dcol * s * gr;
@@ -419,10 +438,10 @@ CRYO_ADJ_HD inline GcGrad<T> gc_fw_vjp(const Ctx<T>& c, Vec<T> rij, T alpha_w, T
   const T r2 = dot(rij, rij);
   if (!(r2 < c.support2)) return o;
   const T r = std::sqrt(r2);
-  const T s = wendland_s(c, r);
+  const T s = kern_s(c, r);
   o.r = rij * (c.vol0 * (dot(wnb, nrm_w) + wwb) * s);  // dW/dr_IJ = s r_IJ
   if (wall_colour && r > T(1e-12) * c.h) {
-    const T spr = wendland_spr(c, r);
+    const T spr = kern_spr(c, r);
     const T gr = dot(gb, rij);
     o.r += (gb * s + rij * (gr * spr)) * (c.vol0 * (alpha_w - c_i));
     o.V_j = c.vol0 * s * gr;
@@ -503,7 +522,7 @@ CRYO_ADJ_HD inline TvGrad<T> tvf_vjp(const Ctx<T>& c, Vec<T> rij, T Vt2, T Vj, V
   if (!(r2 < c.support2)) return o;
   const T r = std::sqrt(r2);
   if (!(r > T(1e-12) * c.h)) return o;
-  const T s = wendland_s(c, r), spr = wendland_spr(c, r);
+  const T s = kern_s(c, r), spr = kern_spr(c, r);
   const T sr = dot(Sb, rij);
   const T G = s * sr;  // Sb . gw
   o.r = (Sb * s + rij * (sr * spr)) * (Vt2 + Vj * Vj);

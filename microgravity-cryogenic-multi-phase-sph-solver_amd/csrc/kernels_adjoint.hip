@@ -91,7 +91,7 @@ __device__ inline float4 f4(AV a) { return make_float4(a.x, a.y, a.z, 0.0f); }
 
 __device__ inline cryo_adj::Ctx<float> adj_ctx(const SimParams& P, float dt) {
   return cryo_adj::Ctx<float>{P.kernel.norm, P.kernel.inv_h, P.kernel.h, P.support2, 0.01f * P.kernel.h * P.kernel.h,
-                              0.6f * P.dx, dt, P.vol0, P.no_slip};
+                              0.6f * P.dx, dt, P.vol0, P.no_slip, P.kernel.type};
 }
 __device__ inline cryo_adj::Side<float> side(const PV& pv, const SimParams& P, int i) {
   const uint8_t ph = pv.s.phase[i];
@@ -482,7 +482,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
           }
           const float r2 = cryo_adj::dot(rkw, rkw);
           if (hwb != 0.0f && (pv.s.flags[j] & kHeated) && r2 < P.support2)
-            xr += rkw * (hwb * P.vol0 * cryo_adj::wendland_s(c, sqrtf(r2)));
+            xr += rkw * (hwb * P.vol0 * cryo_adj::kern_s(c, sqrtf(r2)));
         }
       }
       const float4 cp = A.wcp[j], cv = A.wcv[j];
@@ -514,7 +514,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
         const AV rwk = xw - I.x;
         const float r2 = cryo_adj::dot(rwk, rwk);
         if (cv.w != 0.0f && r2 < P.support2)  // alpha_w = sw0 / swt: d/dx_k = (c_k - alpha_w) / swt (-s r_wk)
-          xr += rwk * (-cv.w * (ci - alpha_w) * cryo_adj::wendland_s(c, sqrtf(r2)));
+          xr += rwk * (-cv.w * (ci - alpha_w) * cryo_adj::kern_s(c, sqrtf(r2)));
       }
       return;
     }
@@ -646,7 +646,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_density(PV pv, GridDev g, SimPar
     const float r2 = cryo_adj::dot(rij, rij);
     if (!(r2 < P.support2)) return;
     const float sj = is_fluid(pv.s.phase[j]) ? A.sb[j] : 0.0f;  // wall sigma is a constant
-    xr += rij * ((sbi + sj) * cryo_adj::wendland_s(c, sqrtf(r2)));
+    xr += rij * ((sbi + sj) * cryo_adj::kern_s(c, sqrtf(r2)));
   });
   // x' = x + v_half dt, v' = v_half + a' dt/2, v_half = v + a dt/2
   const AV X = av(A.cx[i]) + xr;
@@ -754,7 +754,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_reb2(PV pv, GridDev g, SimParams
       const V3 rv = xi - ld3(pv.s.x[j]);
       const float r2 = norm2(rv);
       if (r2 >= P.support2) return;
-      const float r = sqrtf(r2), W = kW(P.kernel, r), sW = cryo_adj::wendland_s(c, r);
+      const float r = sqrtf(r2), W = kW(P.kernel, r), sW = cryo_adj::kern_s(c, r);
       const AV rij = av(rv);
       const float mj = A.m2[j], hj = enthalpy(P, kVapor, A.T2[j]);
       const AV vj = av(A.v1[j]);
@@ -861,7 +861,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_app2(PV pv, GridDev g, SimParams
     const V3 rv = xi - ld3(pv.s.x[j]);
     const float r2 = norm2(rv);
     if (r2 >= P.support2) return;
-    const float r = sqrtf(r2), W = kW(P.kernel, r), sW = cryo_adj::wendland_s(c, r);
+    const float r = sqrtf(r2), W = kW(P.kernel, r), sW = cryo_adj::kern_s(c, r);
     const float omj = A.om[j];
     if (recv && omj > 0.0f && pc_target(A.fl[j]) == ph) {  // i receives from j
       const float wnj = pv.t.w_norm[j];
@@ -1062,7 +1062,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_nrm(PV pv, GridDev g, SimParams 
       Vb += pni.z * W;
       return;
     }
-    const float sW = cryo_adj::wendland_s(c, r);
+    const float sW = cryo_adj::kern_s(c, r);
     const AV rij = av(rv);
     const float Vj = pv.s.m[j] / pv.s.rho[j];
     if (src && phj == tgt && !(A.fl[j] & kDying)) xr += rij * (pni.x * Vj * sW);

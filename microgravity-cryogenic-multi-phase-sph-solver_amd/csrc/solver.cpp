@@ -1214,7 +1214,6 @@ struct cryo_solver {
   void check_adjoint_scope(bool explicit_rotation = false) const {
     const cryo_config& c = cfg;
     if (!(c.fixed_dt > 0.0f)) throw std::invalid_argument("adjoint: needs fixed_dt > 0");
-    if (c.kernel != CRYO_KERNEL_WENDLAND_C2) throw std::invalid_argument("adjoint: Wendland C2 kernel only");
     if (c.density_mode != CRYO_DENSITY_SUMMATION) throw std::invalid_argument("adjoint: density summation only");
     if (c.surface_tension == CRYO_ST_AKINCI) throw std::invalid_argument("adjoint: Akinci surface tension not differentiated");
     if (c.surface_tension == CRYO_ST_CSF &&

@@ -1812,7 +1812,6 @@ struct Solver {
   void check_adjoint_scope(bool explicit_rotation = false) const {
     const cryo_config& c = cfg;
     if (!(c.fixed_dt > 0)) throw std::invalid_argument("adjoint: needs fixed_dt > 0");
-    if (c.kernel != CRYO_KERNEL_WENDLAND_C2) throw std::invalid_argument("adjoint: Wendland C2 kernel only");
     if (c.density_mode != CRYO_DENSITY_SUMMATION) throw std::invalid_argument("adjoint: density summation only");
     if (c.surface_tension == CRYO_ST_AKINCI) throw std::invalid_argument("adjoint: Akinci surface tension not differentiated");
     if (c.surface_tension == CRYO_ST_CSF &&
@@ -2054,7 +2053,7 @@ struct Solver {
           const R k = fb * beta * P.vol0;
           m2b[j] += k * W;
           m2b[i] -= k * W;
-          const VR d = vr(rij) * (k * (m2[j] - m2[i]) * cryo_adj::wendland_s(c, r));
+          const VR d = vr(rij) * (k * (m2[j] - m2[i]) * cryo_adj::kern_s(c, r));
           xr[i] += d;
           xr[j] -= d;
         });
@@ -2113,7 +2112,7 @@ struct Solver {
           omb[j] += shb * Vi * W / wn[j];
           Vb[i] += shb * om[j] * W / wn[j];
           wnb[j] -= shb * share / wn[j];
-          const VR d = vr(rij) * (shb * om[j] * Vi / wn[j] * cryo_adj::wendland_s(c, r));
+          const VR d = vr(rij) * (shb * om[j] * Vi / wn[j] * cryo_adj::kern_s(c, r));
           xr[i] += d;
           xr[j] -= d;
         });
@@ -2150,7 +2149,7 @@ struct Solver {
             const V3 rij = xi - ps.x[jmax];
             const R r = std::sqrt(norm2(rij));
             rb[jmax] -= wrb * wr_max / ps.rho[jmax];
-            const VR d = vr(rij) * (wrb / ps.rho[jmax] * cryo_adj::wendland_s(c, r));
+            const VR d = vr(rij) * (wrb / ps.rho[jmax] * cryo_adj::kern_s(c, r));
             xr[i] += d;
             xr[jmax] -= d;
           }
@@ -2163,7 +2162,7 @@ struct Solver {
             if (r2 >= P.support2) return;
             const R r = std::sqrt(r2);
             Vb[j] += wnb[i] * kW(P.kernel, r);
-            const VR d = vr(rij) * (wnb[i] * Vof(j) * cryo_adj::wendland_s(c, r));
+            const VR d = vr(rij) * (wnb[i] * Vof(j) * cryo_adj::kern_s(c, r));
             xr[i] += d;
             xr[j] -= d;
           });
@@ -2219,7 +2218,7 @@ struct Solver {
     const int n = ps.n;
     const R dt = P.dt, half = R(0.5) * dt;
     Ctx<R> c{P.kernel.norm, P.kernel.inv_h, P.kernel.h, P.support2, R(0.01) * P.kernel.h * P.kernel.h,
-             R(0.6) * P.dx, dt, P.vol0, P.no_slip};
+             R(0.6) * P.dx, dt, P.vol0, P.no_slip, P.kernel.type};
     auto vr = [](V3 a) { return VR{a.x, a.y, a.z}; };
     auto v3 = [](VR a) { return V3{a.x, a.y, a.z}; };
     const VR Fb{R(2) * dt * Ssum.x, R(2) * dt * Ssum.y, R(0)};
@@ -2411,7 +2410,7 @@ struct Solver {
             const V3 rij = ps.x[i] - ps.x[j];
             const R r2 = norm2(rij);
             if (r2 >= P.support2) return;
-            xr[i] += vr(rij) * (hb * P.vol0 * wendland_s(c, std::sqrt(r2)));
+            xr[i] += vr(rij) * (hb * P.vol0 * kern_s(c, std::sqrt(r2)));
           });
         }
       }
@@ -2440,7 +2439,7 @@ struct Solver {
         const R k = sab + (ps.phase[j] != ph ? sob : R(0));
         Vb[j] += k * kW(P.kernel, r);
         if (j == i) return;
-        const VR d = vr(rij) * (k * (ps.m[j] / ps.rho[j]) * wendland_s(c, r));
+        const VR d = vr(rij) * (k * (ps.m[j] / ps.rho[j]) * kern_s(c, r));
         xr[i] += d;
         xr[j] -= d;
       });
@@ -2485,7 +2484,7 @@ struct Solver {
           const V3 rwf = ps.x[w] - ps.x[f];
           const R r2 = norm2(rwf);
           if (r2 < P.support2)  // d alpha_w / d x_f = (c_f - alpha_w) / swt * dW/dx_f, dW/dx_f = -s r_wf
-            xr[f] += vr(rwf) * (-calpha * ((vap ? R(0) : R(1)) - ps.alpha[w]) * wendland_s(c, std::sqrt(r2)));
+            xr[f] += vr(rwf) * (-calpha * ((vap ? R(0) : R(1)) - ps.alpha[w]) * kern_s(c, std::sqrt(r2)));
         }
       });
     }
@@ -2510,7 +2509,7 @@ struct Solver {
         const V3 rij = ps.x[i] - ps.x[j];
         const R r2 = norm2(rij);
         if (r2 >= P.support2) return;
-        const VR d = vr(rij) * (sb[i] * wendland_s(c, std::sqrt(r2)));
+        const VR d = vr(rij) * (sb[i] * kern_s(c, std::sqrt(r2)));
         xr[i] += d;
         xr[j] -= d;
       });
@@ -2641,7 +2640,7 @@ struct Solver {
       using VR = cryo_adj::Vec<R>;
       const int n = ps.n;
       cryo_adj::Ctx<R> c{P.kernel.norm, P.kernel.inv_h, P.kernel.h, P.support2, R(0.01) * P.kernel.h * P.kernel.h,
-                         R(0.6) * P.dx, P.dt, P.vol0, P.no_slip};
+                         R(0.6) * P.dx, P.dt, P.vol0, P.no_slip, P.kernel.type};
       std::vector<VR> At(n), xr(n, VR{0, 0, 0});
       for (int i = 0; i < n; ++i) At[i] = is_fluid(ps.phase[i]) ? VR{ab[i].x, ab[i].y, ab[i].z} : VR{0, 0, 0};
       std::vector<R> Vb(n, R(0)), alphab(n, R(0));
@@ -3023,5 +3022,15 @@ void orc_kernel_eval(int type, double h, double r, double* W, double* dWdr) {
   const orc::Kern k = orc::Kern::make(type, orc::R(h));
   *W = double(orc::kW(k, orc::R(r)));
   *dWdr = double(orc::kdW(k, orc::R(r)));
+}
+// The adjoint's kernel functions (csrc/adjoint_pair.hpp): W, s = W'(r) / r and
+// s'(r) / r, for checks against differences of orc_kernel_eval.
+void orc_adjoint_kernel_eval(int type, double h, double r, double* W, double* s, double* spr) {
+  const orc::Kern k = orc::Kern::make(type, orc::R(h));
+  cryo_adj::Ctx<orc::R> c{k.norm, k.inv_h, k.h, k.support * k.support, orc::R(0), orc::R(0), orc::R(0), orc::R(0), 0,
+                          type};
+  *W = double(cryo_adj::kern_w(c, orc::R(r)));
+  *s = double(cryo_adj::kern_s(c, orc::R(r)));
+  *spr = double(cryo_adj::kern_spr(c, orc::R(r)));
 }
 }

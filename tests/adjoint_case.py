@@ -16,7 +16,7 @@ from cabi import WALL
 
 def adjoint_case(steps=12, seed=1, dx=0.004, L=0.04, dt=2.5e-4, acc_scale=0.3, csf=False, contact_angle=30.0,
                  tvf=False, lee=None, superheat=3.0, vented=True, rebalance=1.0, conduct=2000.0, heater=500.0,
-                 iso_walls=False):
+                 iso_walls=False, kernel=1):
     fluid = cabi.hip_fluid_preset("LN2")
     if csf:  # a surface tension large enough to weigh in the wall loads
         fluid.sigma = 0.2
@@ -25,7 +25,7 @@ def adjoint_case(steps=12, seed=1, dx=0.004, L=0.04, dt=2.5e-4, acc_scale=0.3, c
                               surface_tension=1 if csf else 0, contact_angle_deg=contact_angle,
                               transport_velocity=1 if tvf else 0, velocity_scale=0.3, fixed_dt=dt,
                               artificial_viscosity=0.5,
-                              artificial_viscosity_vapor=1.0)
+                              artificial_viscosity_vapor=1.0, kernel=kernel)
     if lee is not None:  # conduction and Lee phase change with both directions of transfer
         cfg.thermal = 1
         cfg.phase_change = 1

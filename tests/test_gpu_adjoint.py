@@ -117,6 +117,31 @@ def test_transport_velocity_gradients_match_fp64_adjoint():
     assert np.array_equal(out1[1], g) and out1[3] == dth
 
 
+def test_cubic_spline_gradients_match_cpu_adjoints():
+    """The cubic-spline kernel under CSF, the transport velocity, conduction and
+    Lee phase change: the GPU's acceleration, contact-angle and Lee gradients
+    against the FP32 CPU adjoint (same arithmetic, serial sums) and the FP64
+    one (pinned by central differences with this kernel in
+    test_oracle_adjoint.py)."""
+    say("test_cubic_spline_gradients_match_cpu_adjoints")
+    fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=10, csf=True, tvf=True, lee=50.0, kernel=0)
+    out = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=4)
+    out64 = orc(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
+    o32 = CpuSolver("orc", cfg, fluid, 32)
+    o32.set_particles(ps, lo, hi)
+    out32 = o32.adjoint_slosh(acc)
+    err = np.abs(out[1] - out64[1]).max() / np.abs(out64[1]).max()
+    err32 = np.abs(out[1] - out32[1]).max() / np.abs(out32[1]).max()
+    lee64 = [abs(out.lee[q] / out64.lee[q] - 1) for q in (0, 1)]
+    lee32 = [abs(out.lee[q] / out32.lee[q] - 1) for q in (0, 1)]
+    say("loss", out[0], "fp64", out64[0], "acc gradient error", err, "vs fp32 cpu", err32, "dL/dtheta", out[3],
+        "fp64", out64[3], "lee rel fp32", lee32, "rel fp64", lee64)
+    assert abs(out[0] - out64[0]) <= 1e-4 * out64[0]
+    assert err <= 1e-3 and err32 <= 1e-4
+    assert abs(out[3] - out64[3]) <= 1e-3 * abs(out64[3])
+    assert max(lee32) <= 2e-4 and max(lee64) <= 5e-3
+
+
 def test_heat_and_phase_change_gradients_match_cpu_adjoints():
     """Conduction, heater, Lee evaporation and condensation, normalisation,
     apply, vapour rebalance and venting on: the acceleration gradient and the

@@ -151,6 +151,39 @@ def test_kernel_closed_forms(ktype):
     assert ev(0.1, 0.2) == (0.0, 0.0) and ev(0.1, 0.25)[0] == 0.0
 
 
+@pytest.mark.parametrize("ktype", [0, 1])
+def test_adjoint_kernel_functions_are_the_kernel_derivatives(ktype):
+    """The adjoint's kernel functions (csrc/adjoint_pair.hpp: W, s = W'/r and
+    s'/r, shared by the GPU and FP64 adjoints) against the forward kernel: W
+    equal, W' and s' by central differences, across both branches of the
+    cubic spline (q < 1 and 1 <= q < 2)."""
+    lib = CpuSolver.load("orc", 64)
+    W, dW, s, spr = (C.c_double() for _ in range(4))
+    h = 0.013
+
+    def fwd(r):
+        lib.orc_kernel_eval(C.c_int(ktype), C.c_double(h), C.c_double(r), C.byref(W), C.byref(dW))
+        return W.value, dW.value
+
+    def adj(r):
+        lib.orc_adjoint_kernel_eval(C.c_int(ktype), C.c_double(h), C.c_double(r), C.byref(W), C.byref(s),
+                                    C.byref(spr))
+        return W.value, s.value, spr.value
+
+    scale = abs(adj(0.5 * h)[1])
+    for q in np.arange(0.05, 1.96, 0.05):
+        if abs(q - 1.0) < 1e-9:
+            continue
+        r = q * h
+        w_f, dw_f = fwd(r)
+        w_a, s_a, spr_a = adj(r)
+        assert w_a == pytest.approx(w_f, rel=1e-13, abs=1e-300)
+        assert s_a * r == pytest.approx(dw_f, rel=1e-12, abs=1e-12 * scale * h)
+        e = 1e-6 * h
+        ds = (adj(r + e)[1] - adj(r - e)[1]) / (2 * e)
+        assert spr_a * r == pytest.approx(ds, rel=1e-6, abs=1e-6 * scale / h)
+
+
 def test_oracle_sort_is_stable_and_neighbours_match_brute_force():
     """Stable sort by cell and brute-force neighbour sets (test_spatial_hash.cpp:25-93)."""
     lib = CpuSolver.load("orc", 32)

@@ -114,6 +114,36 @@ def test_fp64_adjoint_with_transport_velocity_matches_central_differences():
     assert np.abs(grad - fd).max() / np.abs(fd).max() <= 1e-6
 
 
+def test_fp64_adjoint_with_the_cubic_spline_kernel_matches_central_differences():
+    """The cubic-spline kernel (both of its branches) under every pair term of
+    the case: CSF with the contact angle, the transport velocity, conduction and
+    Lee phase change; acceleration and contact-angle gradients against central
+    differences of the FP64 forward."""
+    fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=10, csf=True, tvf=True, lee=50.0, kernel=0)
+    out = oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
+    loss, grad, dtheta = out[0], out[1], out[3]
+    assert loss > 0
+    fd = np.zeros_like(grad)
+    for k in range(acc.shape[0]):
+        for c in range(3):
+            a1, a2 = acc.copy(), acc.copy()
+            a1[k, c] += STEP
+            a2[k, c] -= STEP
+            h = float(a1[k, c]) - float(a2[k, c])
+            fd[k, c] = (oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(a1, grad=False)[0] -
+                        oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(a2, grad=False)[0]) / h
+    assert np.abs(grad - fd).max() / np.abs(fd).max() <= 1e-6
+    d, th0 = 0.01, float(cfg.contact_angle_deg)
+    losses = []
+    for th in (th0 + d, th0 - d):
+        cfg.contact_angle_deg = th
+        losses.append(oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, grad=False)[0])
+    cfg.contact_angle_deg = th0
+    fd_theta = (losses[0] - losses[1]) / (np.float32(th0 + d) - np.float32(th0 - d))
+    assert dtheta != 0.0
+    assert abs(dtheta - fd_theta) <= 1e-5 * abs(fd_theta), (dtheta, fd_theta)
+
+
 @pytest.mark.parametrize("iso_walls,vented", [(False, True), (True, True), (False, False)])
 def test_fp64_adjoint_with_heat_and_phase_change_matches_central_differences(iso_walls, vented):
     """Conduction (scaled up), the heater, Lee evaporation and condensation,
