@@ -7,6 +7,9 @@
 //                                             artificial viscosity, wall pressure,
 //                                             near-wall guard, no-slip shear, wall AV
 //   density        (SPH_kernels.cu:6-34)      sigma = sum_j W(r_ij)
+//   CSF            (surface_tension.cu:5-177) colour gradient, contact-angle
+//                                             rotation, curvature, classic or
+//                                             balanced force
 // with the Wendland C2 or the cubic-spline kernel (SPH_kernels.cuh:14-64).  Each function returns
 // d(c . term)/d(inputs) for the cotangent c of the quantity the term adds to.
 //
@@ -505,6 +508,38 @@ CRYO_ADJ_HD inline RotGrad<T> rot_vjp(Vec<T> g, Vec<T> wall_n, T ww, T cos_t, T 
 }
 
 // ---------------------------------------------------- transport velocity
+// Balanced CSF sum of an owner O (surface_tension.cu:120-177, csf_balanced),
+// over its fluid neighbours N:
+//   G_O += gw (V_O^2 + V_N^2) (rho_N c_O + rho_O c_N) / (rho_O + rho_N),  gw = r_ON s
+// (c: colour, 1 liquid and 0 vapour), a_O += G_O sigma kappa_O / m_O.  Gb: the
+// cotangent of G_O.  Returns the cotangents of r_ON, V_O, V_N, rho_O and rho_N.
+template <class T>
+struct BalGrad {
+  Vec<T> r;
+  T V_o, V_n, rho_o, rho_n;
+};
+template <class T>
+CRYO_ADJ_HD inline BalGrad<T> bal_vjp(const Ctx<T>& c, Vec<T> r_on, T Vo, T Vn, T rho_o, T rho_n, T co, T cn,
+                                      Vec<T> Gb) {
+  BalGrad<T> o{{T(0), T(0), T(0)}, T(0), T(0), T(0), T(0)};
+  const T r2 = dot(r_on, r_on);
+  if (!(r2 < c.support2)) return o;
+  const T r = std::sqrt(r2);
+  if (!(r > T(0))) return o;
+  const T s = kern_s(c, r), spr = kern_spr(c, r);
+  const T rs = rho_o + rho_n;
+  const T frac = (rho_n * co + rho_o * cn) / rs;
+  const T vs = Vo * Vo + Vn * Vn;
+  const T Gr = dot(Gb, r_on);
+  o.r = (Gb * s + r_on * (Gr * spr)) * (vs * frac);
+  const T Q = s * Gr;  // Gb . gw
+  o.V_o = Q * T(2) * Vo * frac;
+  o.V_n = Q * T(2) * Vn * frac;
+  o.rho_o = Q * vs * (cn - frac) / rs;
+  o.rho_n = Q * vs * (co - frac) / rs;
+  return o;
+}
+
 // Transport acceleration of an active owner I (SPH_kernels.cu:56-104):
 //   ab_I = K_I sum_J gw (Vt_I^2 + V_J^2),  K_I = -pb_I / (rho_I Vt_I),  Vt_I = 1 / sigma_I,
 //   V_J = 1 / sigma_J (fluid) or the wall volume below.

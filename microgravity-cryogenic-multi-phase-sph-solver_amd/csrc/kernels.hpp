@@ -136,6 +136,8 @@ struct AdjArgs {
   double* satv;    // {sum_L m bar, sum_V m T bar} of that feedback
   // rotating frame: this step's d loss / d (omega, omega_dot), 6 doubles (null: not wanted)
   double* wout;
+  // balanced CSF (csf and P.csf_balanced): {G_bar (xyz), the 1 / m factor's mass cotangent (w)}
+  float4* gbal = nullptr;
 };
 // Several independent device-to-device copies in one launch (16-byte aligned
 // pointers): the adjoint's snapshots.

@@ -226,34 +226,55 @@ __global__ __launch_bounds__(kBlock) void k_adj_prep(PV pv, SimParams P, AdjArgs
 
 __device__ inline bool live_normal(const PV& pv, int i) { return norm2(ld3(pv.t.ns[i])) > 0.25f; }
 
-// Curvature sums' cotangents: kappa = 3 num / den, a += nrm coeff kappa.
+// Curvature sums' cotangents: kappa = 3 num / den; a += nrm coeff kappa
+// (classic force) or a += G sigma kappa / m (balanced force, with G's
+// cotangent and the 1 / m factor's mass cotangent in A.gbal).
 __global__ __launch_bounds__(kBlock) void k_adj_csf1(PV pv, GridDev g, SimParams P, NbrList L, AdjArgs A, int n) {
   const int i = gtid();
   if (i >= n) return;
   float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float4 gbal = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const bool bal = P.csf_balanced != 0;
   if (is_fluid(pv.s.phase[i]) && live_normal(pv, i)) {
     const float coeff = 2.0f * P.sigma / (P.rho0[0] + P.rho0[1]);
     const float4 at = A.at[i], nr = pv.s.nrm[i];
     const V3 xi = ld3(pv.s.x[i]), ni = ld3(pv.t.ns[i]);
+    const float rhoi = pv.s.rho[i], mi = pv.s.m[i], Vi = mi / rhoi, ci = colour(pv.s.phase[i]);
     float num = 0.0f, den = 0.0f;  // the curvature sums (the forward pass's, recomputed)
+    V3 G{0.0f, 0.0f, 0.0f};        // and the balanced force's sum
     for_each_pair(L, g, i, xi, [&](int j) {
-      if (j == i || !is_fluid(pv.s.phase[j]) || !live_normal(pv, j)) return;
+      if (j == i || !is_fluid(pv.s.phase[j])) return;
+      const bool livej = live_normal(pv, j);
+      if (!livej && !bal) return;
       const V3 rij = xi - ld3(pv.s.x[j]);
       const float r2 = norm2(rij);
       if (!(r2 < P.support2)) return;
       const float r = sqrtf(r2);
       const float dW = kdW(P.kernel, r);
-      const float Vj = pv.s.m[j] / pv.s.rho[j];
-      num += Vj * dot(ni - ld3(pv.t.ns[j]), rij * (dW / r));
-      den += Vj * r * (-dW);
+      const float rhoj = pv.s.rho[j], Vj = pv.s.m[j] / rhoj;
+      const V3 gw = rij * (dW / r);
+      if (livej) {
+        num += Vj * dot(ni - ld3(pv.t.ns[j]), gw);
+        den += Vj * r * (-dW);
+      }
+      if (bal) G += gw * ((Vi * Vi + Vj * Vj) * (rhoj * ci + rhoi * colour(pv.s.phase[j])) / (rhoi + rhoj));
     });
     if (den > 0.0f) {
       const float kappa = 3.0f * num / den;
-      const float kb = coeff * (at.x * nr.x + at.y * nr.y + at.z * nr.z);
+      float kb;
+      if (bal) {
+        const float AG = at.x * G.x + at.y * G.y + at.z * G.z;
+        kb = P.sigma / mi * AG;
+        const float f = P.sigma * kappa / mi;
+        gbal = make_float4(at.x * f, at.y * f, at.z * f, -P.sigma * kappa * AG / (mi * mi));
+      } else {
+        kb = coeff * (at.x * nr.x + at.y * nr.y + at.z * nr.z);
+      }
       out = make_float4(3.0f * kb / den, -kappa * kb / den, kappa, 0.0f);
     }
   }
   A.cs[i] = out;
+  if (bal) A.gbal[i] = gbal;
 }
 
 // Per fluid particle: both sides of the curvature pairs, the normalisation,
@@ -305,7 +326,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_csf2(PV pv, GridDev g, SimParams
       Vb += o2.V_j;
     });
     const float4 at = A.at[k];
-    AV nrmb = livek ? AV{at.x, at.y, at.z} * (coeff * csk.z) : AV{0.0f, 0.0f, 0.0f};
+    AV nrmb = livek && !P.csf_balanced ? AV{at.x, at.y, at.z} * (coeff * csk.z) : AV{0.0f, 0.0f, 0.0f};
     const AV nrk = av(pv.s.nrm[k]);
     const float G = sqrtf(cryo_adj::dot(nrk, nrk));
     if (G > P.normal_eps) nrmb += (nsb - nsk * cryo_adj::dot(nsk, nsb)) * (1.0f / G);  // ns = nrm / |nrm|
@@ -449,6 +470,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
     const float hinv = st->heat_inv_norm;
     hwb = qi * P.heat_power * hinv - static_cast<float>(A.hsum[0]) * P.heat_power * hinv * hinv;
   }
+  const bool bal = A.csf && P.csf_balanced;
+  const float4 gbi4 = bal ? A.gbal[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const AV Gbi{gbi4.x, gbi4.y, gbi4.z};
   if (A.csf) {
     const float4 gb = A.gcb[i], wb = A.wnb[i];
     gbk = AV{gb.x, gb.y, gb.z};
@@ -551,6 +575,18 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
       }
     }
     const cryo_adj::Side<float> J = side(pv, P, j);
+    if (bal) {  // balanced CSF: i's G sum and j's G sum, which reads i
+      const AV rkj = I.x - J.x;
+      const float Vj = J.m / J.rho;
+      const float4 gj = A.gbal[j];
+      const cryo_adj::BalGrad<float> b1 = cryo_adj::bal_vjp(c, rkj, Vk, Vj, I.rho, J.rho, ci, colour(phj), Gbi);
+      const cryo_adj::BalGrad<float> b2 = cryo_adj::bal_vjp(c, AV{-rkj.x, -rkj.y, -rkj.z}, Vj, Vk, J.rho, I.rho,
+                                                            colour(phj), ci, AV{gj.x, gj.y, gj.z});
+      xr += b1.r;
+      xr -= b2.r;
+      Vbk += b1.V_o + b2.V_n;
+      rb += b1.rho_o + b2.rho_n;
+    }
     const float4 atj = A.at[j];
     const AV cAj{atj.x, atj.y, atj.z};
     const cryo_adj::FFGrad<float> o1 = cryo_adj::ff_vjp(c, I, J, cF, cA);         // i's pair term
@@ -608,7 +644,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
   }
   if (A.mT) {
     const float4 pa = A.pc ? A.pa[i] : make_float4(A.cm[i], A.cT[i], 0.0f, 0.0f);  // (own shares)
-    mb += pa.x + Vbk / I.rho;
+    mb += pa.x + Vbk / I.rho + gbi4.w;  // gbi4.w: the balanced force's 1 / m
     mb -= (cA.x * F.x + cA.y * F.y + cA.z * F.z) / (I.m * I.m);  // a = g + F / m + ...
     if (A.heat) mb -= pa.y * dt * pv.t.dTdt[i] / I.m;      // dTdt = q / (m cp)
     if (vap) mb += rhob * pv.t.sigma[i];

@@ -1815,8 +1815,8 @@ struct Solver {
     if (c.density_mode != CRYO_DENSITY_SUMMATION) throw std::invalid_argument("adjoint: density summation only");
     if (c.surface_tension == CRYO_ST_AKINCI) throw std::invalid_argument("adjoint: Akinci surface tension not differentiated");
     if (c.surface_tension == CRYO_ST_CSF &&
-        (c.csf_smooth_normals || c.csf_balanced || c.wetting_model != CRYO_WET_NORMAL_ROTATION))
-      throw std::invalid_argument("adjoint: CSF only with unsmoothed normals, the classic force and normal rotation");
+        (c.csf_smooth_normals || c.wetting_model != CRYO_WET_NORMAL_ROTATION))
+      throw std::invalid_argument("adjoint: CSF only with unsmoothed normals and normal rotation");
     if (c.transport_stress || c.sps_turbulence || c.particle_shifting)
       throw std::invalid_argument("adjoint: transport stress, SPS and shifting not differentiated");
     if (has_maneuver) throw std::invalid_argument("adjoint: manoeuvre table not allowed (the frame is per-step input)");
@@ -1830,7 +1830,8 @@ struct Solver {
   // (positions), Vb (V = m / rho) and alphab (wall colour); returns
   // d loss / d contact angle (degrees) of this evaluation.
   double csf_adjoint(const cryo_adj::Ctx<R>& c, const std::vector<cryo_adj::Vec<R>>& At,
-                     std::vector<cryo_adj::Vec<R>>& xr, std::vector<R>& Vb, std::vector<R>& alphab) {
+                     std::vector<cryo_adj::Vec<R>>& xr, std::vector<R>& Vb, std::vector<R>& alphab,
+                     std::vector<R>& rb, std::vector<R>& mr) {
     using namespace cryo_adj;
     using VR = Vec<R>;
     const int n = ps.n;
@@ -1841,11 +1842,35 @@ struct Solver {
     auto Vof = [&](int j) { return ps.m[j] / ps.rho[j]; };
     std::vector<R> numb(n, R(0)), denb(n, R(0));
     std::vector<VR> nsb(n, VR{0, 0, 0}), nrmb(n, VR{0, 0, 0});
+    // balanced force a += G sigma kappa / m: Gbal is the cotangent of G
+    const bool bal = P.csf_balanced != 0;
+    std::vector<VR> Gbal(bal ? n : 0, VR{0, 0, 0});
+    auto colour = [&](int k) { return ps.phase[k] == L ? R(1) : R(0); };
     for (int i = 0; i < n; ++i) {
       if (!is_fluid(ps.phase[i]) || !live(i)) continue;
       const R kappa = ps.kappa[i];
-      const R kb = coeff * dot(At[i], vr(ps.nrm[i]));
-      nrmb[i] += At[i] * (coeff * kappa);
+      R kb;
+      if (bal) {
+        const R rhoi = ps.rho[i], mi = ps.m[i], Vi = mi / rhoi, ci = colour(i);
+        V3 G{0, 0, 0};  // the forward's sum, recomputed
+        grid.each(ps.x[i], [&](int j) {
+          if (j == i || !is_fluid(ps.phase[j])) return;
+          const V3 rij = ps.x[i] - ps.x[j];
+          const R r2 = norm2(rij);
+          if (r2 >= P.support2) return;
+          const R r = std::sqrt(r2);
+          const V3 gw = rij * (kdW(P.kernel, r) / r);
+          const R rhoj = ps.rho[j], Vj = ps.m[j] / rhoj;
+          G += gw * ((Vi * Vi + Vj * Vj) * (rhoj * ci + rhoi * colour(j)) / (rhoi + rhoj));
+        });
+        const R AG = dot(At[i], vr(G));
+        kb = P.sigma / mi * AG;
+        Gbal[i] = At[i] * (P.sigma * kappa / mi);
+        mr[i] -= P.sigma * kappa * AG / (mi * mi);
+      } else {
+        kb = coeff * dot(At[i], vr(ps.nrm[i]));
+        nrmb[i] += At[i] * (coeff * kappa);
+      }
       R den = 0;
       grid.each(ps.x[i], [&](int j) {
         if (j == i || !is_fluid(ps.phase[j]) || !live(j)) return;
@@ -1870,6 +1895,23 @@ struct Solver {
         nsb[j] += o.ns_j;
         Vb[j] += o.V_j;
       });
+    }
+    if (bal) {
+      for (int i = 0; i < n; ++i) {
+        if (!is_fluid(ps.phase[i]) || !live(i)) continue;
+        const R Vi = ps.m[i] / ps.rho[i];
+        grid.each(ps.x[i], [&](int j) {
+          if (j == i || !is_fluid(ps.phase[j])) return;
+          const BalGrad<R> o = bal_vjp(c, vr(ps.x[i] - ps.x[j]), Vi, ps.m[j] / ps.rho[j], ps.rho[i], ps.rho[j],
+                                       colour(i), colour(j), Gbal[i]);
+          xr[i] += o.r;
+          xr[j] -= o.r;
+          Vb[i] += o.V_o;
+          Vb[j] += o.V_n;
+          rb[i] += o.rho_o;
+          rb[j] += o.rho_n;
+        });
+      }
     }
     std::vector<VR> gcb(n, VR{0, 0, 0}), wnb(n, VR{0, 0, 0});
     std::vector<R> wwb(n, R(0));
@@ -2261,7 +2303,7 @@ struct Solver {
       if (ps.phase[i] == V && ps.T[i] > R(0) && cb != R(0)) Tr[i] += cb * P.gas_R / (R(2) * sound_speed(P, V, ps.T[i]));
     };
     // surface tension
-    if (cfg.surface_tension == CRYO_ST_CSF && has_vapor) theta_b += csf_adjoint(c, At, xr, Vb, alphab);
+    if (cfg.surface_tension == CRYO_ST_CSF && has_vapor) theta_b += csf_adjoint(c, At, xr, Vb, alphab, rb, mr);
     // transport acceleration ab = K sum_J gw (Vt^2 + V_J^2), K = -pb / (rho Vt)
     // (SPH_kernels.cu:56-104), read by the next step's drift: tb is its cotangent
     std::vector<R> tsb(n, R(0));  // sigma cotangent of the transport sums
@@ -2643,8 +2685,8 @@ struct Solver {
                          R(0.6) * P.dx, P.dt, P.vol0, P.no_slip, P.kernel.type};
       std::vector<VR> At(n), xr(n, VR{0, 0, 0});
       for (int i = 0; i < n; ++i) At[i] = is_fluid(ps.phase[i]) ? VR{ab[i].x, ab[i].y, ab[i].z} : VR{0, 0, 0};
-      std::vector<R> Vb(n, R(0)), alphab(n, R(0));
-      theta_b += csf_adjoint(c, At, xr, Vb, alphab);
+      std::vector<R> Vb(n, R(0)), alphab(n, R(0)), rb(n, R(0)), mr(n, R(0));
+      theta_b += csf_adjoint(c, At, xr, Vb, alphab, rb, mr);
     }
     if (grad_theta) *grad_theta = theta_b;
     if (grad_lee) {
