@@ -75,6 +75,7 @@ struct Ctx {
   int no_slip;
   int type;          // 1 Wendland C2: W = norm (1 - q/2)^4 (2q + 1);
                      // 0 cubic spline: W = norm (1 - 3q^2/2 + 3q^3/4) (q < 1), norm (2 - q)^3 / 4
+  int tvf_stress;    // the transport-velocity stress term of momentum is on
 };
 
 // W(r), s = W'(r) / r and s' / r = (ds/dr) / r, q = r / h < 2.
@@ -121,6 +122,7 @@ struct Side {
   T eta;    // dynamic viscosity mu[phase]
   T alpha;  // AV coefficient of its phase
   int ph;
+  Vec<T> vt;  // transport velocity (read by the stress term only)
 };
 
 // Fluid-fluid term of momentum with owner I and neighbour J (j != i):
@@ -137,11 +139,12 @@ struct FFGrad {
   Vec<T> r, vij;
   T rho_i, rho_j, p_i, p_j, m_i, m_j, c_i, c_j;
   Vec<T> f;
+  Vec<T> vi, vj, vti, vtj;  // the stress term's own cotangents of v_I, v_J, vt_I, vt_J
 };
 template <class T>
 CRYO_ADJ_HD inline FFGrad<T> ff_vjp(const Ctx<T>& c, const Side<T>& I, const Side<T>& J, Vec<T> cF, Vec<T> cA) {
   FFGrad<T> o{{T(0), T(0), T(0)}, {T(0), T(0), T(0)}, T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0),
-              {T(0), T(0), T(0)}};
+              {T(0), T(0), T(0)}, {T(0), T(0), T(0)}, {T(0), T(0), T(0)}, {T(0), T(0), T(0)}, {T(0), T(0), T(0)}};
   const Vec<T> rij = I.x - J.x;
   const T r2 = dot(rij, rij);
   if (!(r2 < c.support2)) return o;
@@ -159,6 +162,26 @@ CRYO_ADJ_HD inline FFGrad<T> ff_vjp(const Ctx<T>& c, const Side<T>& I, const Sid
   o.f = rij * (-s * vsum * pt);
   T vsum_b = -pt * G;
   const T pt_b = -vsum * G;
+  T rhoi_s = T(0), rhoj_s = T(0);
+  // T4 (transport stress, SPH_kernels.cu:162-203 with tvf_stress):
+  //   force += A vsum / 2,  A = v_I rho_I (dv_I . gw) + v_J rho_J (dv_J . gw),  dv = vt - v
+  if (c.tvf_stress) {
+    const Vec<T> dvi = I.vt - I.v, dvj = J.vt - J.v;
+    const Vec<T> gw = rij * s;
+    const T di = dot(dvi, gw), dj = dot(dvj, gw);
+    const T a = dot(cF, I.v), b = dot(cF, J.v);
+    const T h = T(0.5) * vsum;
+    o.vi = (cF * di - gw * a) * (h * I.rho);
+    o.vj = (cF * dj - gw * b) * (h * J.rho);
+    o.vti = gw * (h * a * I.rho);
+    o.vtj = gw * (h * b * J.rho);
+    rhoi_s = h * a * di;
+    rhoj_s = h * b * dj;
+    vsum_b += T(0.5) * (a * I.rho * di + b * J.rho * dj);
+    const Vec<T> Dg = (dvi * (a * I.rho) + dvj * (b * J.rho)) * h;
+    o.r += Dg * s + rij * (dot(Dg, rij) * spr);
+    o.f += (I.v * (I.rho * di) + J.v * (J.rho * dj)) * h;
+  }
   // T2
   const Vec<T> vij = I.v - J.v;
   const T es = I.eta + J.eta;
@@ -200,8 +223,8 @@ CRYO_ADJ_HD inline FFGrad<T> ff_vjp(const Ctx<T>& c, const Side<T>& I, const Sid
       o.r += vij * vr_b + rij * (T(2) * r2_b);
     }
   }
-  o.rho_i = rhoi_b;
-  o.rho_j = rhoj_b;
+  o.rho_i = rhoi_b + rhoi_s;
+  o.rho_j = rhoj_b + rhoj_s;
   return o;
 }
 
@@ -222,12 +245,13 @@ struct FWGrad {
   Vec<T> r, vi, vij;
   T rho_i, p_i, p_w, m_i, c_i;
   Vec<T> f, acc;
+  Vec<T> vti;  // the stress term's cotangent of vt_I
 };
 template <class T>
 CRYO_ADJ_HD inline FWGrad<T> fw_vjp(const Ctx<T>& c, const Side<T>& I, Vec<T> xw, Vec<T> vw, T pw, Vec<T> cF,
                                     Vec<T> cA) {
   FWGrad<T> o{{T(0), T(0), T(0)}, {T(0), T(0), T(0)}, {T(0), T(0), T(0)}, T(0), T(0), T(0), T(0), T(0),
-              {T(0), T(0), T(0)}, {T(0), T(0), T(0)}};
+              {T(0), T(0), T(0)}, {T(0), T(0), T(0)}, {T(0), T(0), T(0)}};
   const Vec<T> rij = I.x - xw;
   const T r2 = dot(rij, rij);
   if (!(r2 < c.support2)) return o;
@@ -245,6 +269,20 @@ CRYO_ADJ_HD inline FWGrad<T> fw_vjp(const Ctx<T>& c, const Side<T>& I, Vec<T> xw
   o.p_i = T(-0.5) * vsum * G;
   o.p_w = o.p_i;
   T vsum_b = -pm * G;
+  T rho_s = T(0);
+  // W6 (transport stress): force += v_I (vsum / 2) rho_I (dv_I . gw), dv_I = vt_I - v_I
+  if (c.tvf_stress) {
+    const Vec<T> dvi = I.vt - I.v;
+    const Vec<T> gw = rij * s;
+    const T di = dot(dvi, gw), a = dot(cF, I.v), h = T(0.5) * vsum;
+    o.vi += (cF * di - gw * a) * (h * I.rho);
+    o.vti = gw * (h * a * I.rho);
+    rho_s = h * a * di;
+    vsum_b += T(0.5) * a * I.rho * di;
+    const Vec<T> Dg = dvi * (h * a * I.rho);
+    o.r += Dg * s + rij * (dot(Dg, rij) * spr);
+    o.f += I.v * (h * I.rho * di);
+  }
   // W2, W3
   if (r < c.r_guard) {
     const T q = T(1) - r / c.r_guard;
@@ -289,7 +327,7 @@ CRYO_ADJ_HD inline FWGrad<T> fw_vjp(const Ctx<T>& c, const Side<T>& I, Vec<T> xw
       o.r += vij * vr_b + rij * (T(2) * r2_b);
     }
   }
-  o.rho_i = vsum_b * (T(-2) * Vi * Vi / I.rho);
+  o.rho_i = vsum_b * (T(-2) * Vi * Vi / I.rho) + rho_s;
   o.m_i = vsum_b * T(2) * Vi / I.rho;
   return o;
 }

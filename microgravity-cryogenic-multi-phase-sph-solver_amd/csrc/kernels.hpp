@@ -138,6 +138,8 @@ struct AdjArgs {
   double* wout;
   // balanced CSF (csf and P.csf_balanced): {G_bar (xyz), the 1 / m factor's mass cotangent (w)}
   float4* gbal = nullptr;
+  // transport stress (P.tvf and P.tvf_stress): the rates' share of the vt cotangent
+  float4* vtr = nullptr;
 };
 // Several independent device-to-device copies in one launch (16-byte aligned
 // pointers): the adjoint's snapshots.

@@ -91,7 +91,7 @@ __device__ inline float4 f4(AV a) { return make_float4(a.x, a.y, a.z, 0.0f); }
 
 __device__ inline cryo_adj::Ctx<float> adj_ctx(const SimParams& P, float dt) {
   return cryo_adj::Ctx<float>{P.kernel.norm, P.kernel.inv_h, P.kernel.h, P.support2, 0.01f * P.kernel.h * P.kernel.h,
-                              0.6f * P.dx, dt, P.vol0, P.no_slip, P.kernel.type};
+                              0.6f * P.dx, dt, P.vol0, P.no_slip, P.kernel.type, P.tvf && P.tvf_stress ? 1 : 0};
 }
 __device__ inline cryo_adj::Side<float> side(const PV& pv, const SimParams& P, int i) {
   const uint8_t ph = pv.s.phase[i];
@@ -99,7 +99,8 @@ __device__ inline cryo_adj::Side<float> side(const PV& pv, const SimParams& P, i
                                pv.s.m[i],         pv.s.rho[i],
                                pv.t.p[i],         sound_speed(P, ph, pv.s.T[i]),
                                sel2(P.mu, pidx(ph)), ph == kVapor ? P.alpha_av_vap : P.alpha_av,
-                               static_cast<int>(ph)};
+                               static_cast<int>(ph),
+                               P.tvf && P.tvf_stress ? av(pv.s.vt[i]) : AV{0.0f, 0.0f, 0.0f}};
 }
 // Cotangent of the wall loads: d loss / d fw_i = 2 dt (S_x, S_y, 0).
 __device__ inline AV load_cot(const double* S, float dt) {
@@ -446,7 +447,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
   const AV cA{ati.x, ati.y, ati.z};
   const AV cF = cA * (1.0f / I.m);
   const AV cFw = cF + Fb, cAw = cA + Fb * I.m;
-  AV xr{0.0f, 0.0f, 0.0f}, vr{0.0f, 0.0f, 0.0f};
+  AV xr{0.0f, 0.0f, 0.0f}, vr{0.0f, 0.0f, 0.0f}, vtr{0.0f, 0.0f, 0.0f};
   float rb = 0.0f, pb = 0.0f;
   const float ci = colour(phi), Vk = I.m / I.rho;
   AV gbk{0.0f, 0.0f, 0.0f}, wbk{0.0f, 0.0f, 0.0f};
@@ -490,6 +491,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
       const cryo_adj::FWGrad<float> o = cryo_adj::fw_vjp(c, I, xw, vw, pw, cFw, cAw);
       xr += o.r;
       vr += o.vi + o.vij;
+      vtr += o.vti;
       rb += o.rho_i;
       pb += o.p_i;
       if (A.mT) {
@@ -595,6 +597,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
     xr -= o2.r;
     vr += o1.vij;
     vr -= o2.vij;
+    vr += o1.vi + o2.vj;  // the stress term's own shares
+    vtr += o1.vti + o2.vtj;
     rb += o1.rho_i + o2.rho_j;
     pb += o1.p_i + o2.p_j;
     if (A.mT) {
@@ -655,6 +659,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
   }
   A.xr[i] = f4(xr);
   A.vr[i] = f4(vr);
+  if (A.vtr) A.vtr[i] = f4(vtr);
 }
 
 __global__ __launch_bounds__(kBlock) void k_adj_density(PV pv, GridDev g, SimParams P, NbrList L, const DevState* st,
@@ -684,9 +689,12 @@ __global__ __launch_bounds__(kBlock) void k_adj_density(PV pv, GridDev g, SimPar
     const float sj = is_fluid(pv.s.phase[j]) ? A.sb[j] : 0.0f;  // wall sigma is a constant
     xr += rij * ((sbi + sj) * cryo_adj::kern_s(c, sqrtf(r2)));
   });
-  // x' = x + v_half dt, v' = v_half + a' dt/2, v_half = v + a dt/2
+  // x' = x + vt dt, vt = v_half (+ ab dt/2 with the transport velocity),
+  // v' = v_half + a' dt/2, v_half = v + a dt/2; vtb: the rates' share of vt's
+  // cotangent (transport stress)
   const AV X = av(A.cx[i]) + xr;
-  const AV Vh = av(A.pc ? A.v1b[i] : A.cv[i]) + av(A.vr[i]) + X * dt;
+  const AV vtb = A.vtr ? av(A.vtr[i]) : AV{0.0f, 0.0f, 0.0f};
+  const AV Vh = av(A.pc ? A.v1b[i] : A.cv[i]) + av(A.vr[i]) + X * dt + vtb;
   if (A.mT) {
     float nm = A.mr[i], nT = A.Tr[i];
     if (A.sat) {  // the closed tank's T_sat feedback (k_adj_sat)
@@ -703,7 +711,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_density(PV pv, GridDev g, SimPar
   A.nx[dst] = f4(X);
   A.nv[dst] = f4(Vh);
   A.na[dst] = f4(Vh * half);
-  if (A.tvf) A.nt[dst] = f4(X * (half * dt));  // x' = x + (v_half + ab dt/2) dt
+  if (A.tvf) A.nt[dst] = f4(A.vtr ? (X * dt + vtb) * half : X * (half * dt));  // vt = v_half + ab dt/2
 }
 
 // ------------------------------------------------------ phase change

@@ -1817,8 +1817,8 @@ struct Solver {
     if (c.surface_tension == CRYO_ST_CSF &&
         (c.csf_smooth_normals || c.wetting_model != CRYO_WET_NORMAL_ROTATION))
       throw std::invalid_argument("adjoint: CSF only with unsmoothed normals and normal rotation");
-    if (c.transport_stress || c.sps_turbulence || c.particle_shifting)
-      throw std::invalid_argument("adjoint: transport stress, SPS and shifting not differentiated");
+    if (c.sps_turbulence || c.particle_shifting)
+      throw std::invalid_argument("adjoint: SPS and shifting not differentiated");
     if (has_maneuver) throw std::invalid_argument("adjoint: manoeuvre table not allowed (the frame is per-step input)");
     if (P.frame_rot && !explicit_rotation)
       throw std::invalid_argument("adjoint: a rotating frame is differentiated only with per-step rotation inputs");
@@ -2260,7 +2260,7 @@ struct Solver {
     const int n = ps.n;
     const R dt = P.dt, half = R(0.5) * dt;
     Ctx<R> c{P.kernel.norm, P.kernel.inv_h, P.kernel.h, P.support2, R(0.01) * P.kernel.h * P.kernel.h,
-             R(0.6) * P.dx, dt, P.vol0, P.no_slip, P.kernel.type};
+             R(0.6) * P.dx, dt, P.vol0, P.no_slip, P.kernel.type, P.tvf && P.tvf_stress ? 1 : 0};
     auto vr = [](V3 a) { return VR{a.x, a.y, a.z}; };
     auto v3 = [](VR a) { return V3{a.x, a.y, a.z}; };
     const VR Fb{R(2) * dt * Ssum.x, R(2) * dt * Ssum.y, R(0)};
@@ -2268,9 +2268,10 @@ struct Solver {
     auto side = [&](int i) {
       const u8 ph = ps.phase[i];
       return Side<R>{vr(ps.x[i]), vr(ps.v[i]), ps.m[i], ps.rho[i], ps.p[i], sound_speed(P, ph, ps.T[i]),
-                     P.mu[pidx(ph)], ph == V ? P.alpha_av_vap : P.alpha_av, int(ph)};
+                     P.mu[pidx(ph)], ph == V ? P.alpha_av_vap : P.alpha_av, int(ph), vr(ps.vt[i])};
     };
     std::vector<VR> At(n, VR{0, 0, 0}), xr(n, VR{0, 0, 0}), vrb(n, VR{0, 0, 0}), vwb(n, VR{0, 0, 0});
+    std::vector<VR> vtr(n, VR{0, 0, 0});  // the transport stress's cotangent of vt
     std::vector<R> rb(n, R(0)), pb(n, R(0)), pwb0(n, R(0)), pwb1(n, R(0));
     std::vector<R> Vb(n, R(0)), alphab(n, R(0));
     // phase change, then the kick: v1 = v_half + a dt/2, T1 = T + dTdt dt
@@ -2377,6 +2378,7 @@ struct Solver {
           c_to_T(i, o.c_i);
           xr[i] += o.r;
           vrb[i] += o.vi + o.vij;
+          vtr[i] += o.vti;
           vwb[j] -= o.vij;
           rb[i] += o.rho_i;
           pb[i] += o.p_i;
@@ -2392,8 +2394,10 @@ struct Solver {
         c_to_T(j, o.c_j);
         xr[i] += o.r;
         xr[j] -= o.r;
-        vrb[i] += o.vij;
-        vrb[j] -= o.vij;
+        vrb[i] += o.vij + o.vi;
+        vrb[j] += o.vj - o.vij;
+        vtr[i] += o.vti;
+        vtr[j] += o.vtj;
         rb[i] += o.rho_i;
         rb[j] += o.rho_j;
         pb[i] += o.p_i;
@@ -2586,11 +2590,11 @@ struct Solver {
       if (ps.phase[i] == L) nm[perm[i]] += R(mlb);
       if (ps.phase[i] == V) nm[perm[i]] += R(mTb * ps.T[i]), nT[perm[i]] += R(mTb * ps.m[i]);
       const VR X = vr(xb[i]) + xr[i];
-      const VR Vh = v1b[i] + vrb[i] + X * dt;
+      const VR Vh = v1b[i] + vrb[i] + X * dt + vtr[i];  // vt = v_half + ab dt/2 (tvf)
       nx[perm[i]] = v3(X);
       nv[perm[i]] = v3(Vh);
       na[perm[i]] = v3(Vh * half);
-      if (P.tvf) nt[perm[i]] = v3(X * (half * dt));  // x' = x + (v_half + ab dt/2) dt
+      if (P.tvf) nt[perm[i]] = v3((X * dt + vtr[i]) * half);  // x' = x + (v_half + ab dt/2) dt
     }
     xb.swap(nx);
     vb.swap(nv);
@@ -2682,7 +2686,7 @@ struct Solver {
       using VR = cryo_adj::Vec<R>;
       const int n = ps.n;
       cryo_adj::Ctx<R> c{P.kernel.norm, P.kernel.inv_h, P.kernel.h, P.support2, R(0.01) * P.kernel.h * P.kernel.h,
-                         R(0.6) * P.dx, P.dt, P.vol0, P.no_slip, P.kernel.type};
+                         R(0.6) * P.dx, P.dt, P.vol0, P.no_slip, P.kernel.type, P.tvf && P.tvf_stress ? 1 : 0};
       std::vector<VR> At(n), xr(n, VR{0, 0, 0});
       for (int i = 0; i < n; ++i) At[i] = is_fluid(ps.phase[i]) ? VR{ab[i].x, ab[i].y, ab[i].z} : VR{0, 0, 0};
       std::vector<R> Vb(n, R(0)), alphab(n, R(0)), rb(n, R(0)), mr(n, R(0));
@@ -3070,7 +3074,7 @@ void orc_kernel_eval(int type, double h, double r, double* W, double* dWdr) {
 void orc_adjoint_kernel_eval(int type, double h, double r, double* W, double* s, double* spr) {
   const orc::Kern k = orc::Kern::make(type, orc::R(h));
   cryo_adj::Ctx<orc::R> c{k.norm, k.inv_h, k.h, k.support * k.support, orc::R(0), orc::R(0), orc::R(0), orc::R(0), 0,
-                          type};
+                          type, 0};
   *W = double(cryo_adj::kern_w(c, orc::R(r)));
   *s = double(cryo_adj::kern_s(c, orc::R(r)));
   *spr = double(cryo_adj::kern_spr(c, orc::R(r)));
