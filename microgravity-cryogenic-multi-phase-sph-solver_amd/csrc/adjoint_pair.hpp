@@ -223,8 +223,8 @@ CRYO_ADJ_HD inline FFGrad<T> ff_vjp(const Ctx<T>& c, const Side<T>& I, const Sid
       o.r += vij * vr_b + rij * (T(2) * r2_b);
     }
   }
-  o.rho_i = rhoi_b + rhoi_s;
-  o.rho_j = rhoj_b + rhoj_s;
+  o.rho_i = c.tvf_stress ? rhoi_b + rhoi_s : rhoi_b;
+  o.rho_j = c.tvf_stress ? rhoj_b + rhoj_s : rhoj_b;
   return o;
 }
 
@@ -327,7 +327,8 @@ CRYO_ADJ_HD inline FWGrad<T> fw_vjp(const Ctx<T>& c, const Side<T>& I, Vec<T> xw
       o.r += vij * vr_b + rij * (T(2) * r2_b);
     }
   }
-  o.rho_i = vsum_b * (T(-2) * Vi * Vi / I.rho) + rho_s;
+  o.rho_i = vsum_b * (T(-2) * Vi * Vi / I.rho);
+  if (c.tvf_stress) o.rho_i += rho_s;
   o.m_i = vsum_b * T(2) * Vi / I.rho;
   return o;
 }

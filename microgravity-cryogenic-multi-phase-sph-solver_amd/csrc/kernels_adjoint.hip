@@ -93,14 +93,15 @@ __device__ inline cryo_adj::Ctx<float> adj_ctx(const SimParams& P, float dt) {
   return cryo_adj::Ctx<float>{P.kernel.norm, P.kernel.inv_h, P.kernel.h, P.support2, 0.01f * P.kernel.h * P.kernel.h,
                               0.6f * P.dx, dt, P.vol0, P.no_slip, P.kernel.type, P.tvf && P.tvf_stress ? 1 : 0};
 }
+// VT: the transport velocity is read (by the momentum stress term).
+template <bool VT = false>
 __device__ inline cryo_adj::Side<float> side(const PV& pv, const SimParams& P, int i) {
   const uint8_t ph = pv.s.phase[i];
   return cryo_adj::Side<float>{av(pv.s.x[i]),     av(pv.s.v[i]),
                                pv.s.m[i],         pv.s.rho[i],
                                pv.t.p[i],         sound_speed(P, ph, pv.s.T[i]),
                                sel2(P.mu, pidx(ph)), ph == kVapor ? P.alpha_av_vap : P.alpha_av,
-                               static_cast<int>(ph),
-                               P.tvf && P.tvf_stress ? av(pv.s.vt[i]) : AV{0.0f, 0.0f, 0.0f}};
+                               static_cast<int>(ph), VT ? av(pv.s.vt[i]) : AV{0.0f, 0.0f, 0.0f}};
 }
 // Cotangent of the wall loads: d loss / d fw_i = 2 dt (S_x, S_y, 0).
 __device__ inline AV load_cot(const double* S, float dt) {
@@ -356,7 +357,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_wall(PV pv, GridDev g, SimParams
   double gs[3] = {0.0, 0.0, 0.0};
   if (w < n && pv.s.phase[w] == kWall) {
     const float dt = st->dt;
-    const cryo_adj::Ctx<float> c = adj_ctx(P, dt);
+    cryo_adj::Ctx<float> c = adj_ctx(P, dt);
+    c.tvf_stress = 0;  // the stress term reaches no wall-side cotangent (p_w, vw)
     const AV Fb = load_cot(A.S, dt);
     const AV xw = av(pv.s.x[w]), vw = av(pv.t.vw[w]);
     const float pw0 = pv.t.p[w], pw1 = pv.t.p_wv[w];
@@ -430,7 +432,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_wall(PV pv, GridDev g, SimParams
 
 // TV: the transport-velocity terms are compiled in (launched so when A.tvf
 // is set); without them the kernel holds fewer registers.
-template <bool TV>
+// EX: the transport stress, the balanced CSF force or the cubic-spline kernel
+// is on; the other instances fix Wendland C2 at compile time and hold none of
+// those terms' registers.
+template <bool TV, bool EX>
 __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParams P, NbrList L, const DevState* st,
                                                      AdjArgs A, int n) {
   const int i = gtid();
@@ -438,10 +443,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
   const uint8_t phi = pv.s.phase[i];
   if (!is_fluid(phi)) return;
   const float dt = st->dt;
-  const cryo_adj::Ctx<float> c = adj_ctx(P, dt);
+  cryo_adj::Ctx<float> c = adj_ctx(P, dt);
+  if constexpr (!EX) {  // the common instance: Wendland C2, no stress, no balanced force
+    c.tvf_stress = 0;
+    c.type = 1;
+  }
   const AV Fb = load_cot(A.S, dt);
   const AV grav = av(st->gravity);
-  const cryo_adj::Side<float> I = side(pv, P, i);
+  const cryo_adj::Side<float> I = side<EX>(pv, P, i);
   const bool vap = phi == kVapor;
   const float4 ati = A.at[i];
   const AV cA{ati.x, ati.y, ati.z};
@@ -471,7 +480,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
     const float hinv = st->heat_inv_norm;
     hwb = qi * P.heat_power * hinv - static_cast<float>(A.hsum[0]) * P.heat_power * hinv * hinv;
   }
-  const bool bal = A.csf && P.csf_balanced;
+  const bool bal = EX && A.csf && P.csf_balanced;
   const float4 gbi4 = bal ? A.gbal[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const AV Gbi{gbi4.x, gbi4.y, gbi4.z};
   if (A.csf) {
@@ -491,7 +500,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
       const cryo_adj::FWGrad<float> o = cryo_adj::fw_vjp(c, I, xw, vw, pw, cFw, cAw);
       xr += o.r;
       vr += o.vi + o.vij;
-      vtr += o.vti;
+      if (EX) vtr += o.vti;
       rb += o.rho_i;
       pb += o.p_i;
       if (A.mT) {
@@ -576,7 +585,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
         }
       }
     }
-    const cryo_adj::Side<float> J = side(pv, P, j);
+    const cryo_adj::Side<float> J = side<EX>(pv, P, j);
     if (bal) {  // balanced CSF: i's G sum and j's G sum, which reads i
       const AV rkj = I.x - J.x;
       const float Vj = J.m / J.rho;
@@ -597,8 +606,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
     xr -= o2.r;
     vr += o1.vij;
     vr -= o2.vij;
-    vr += o1.vi + o2.vj;  // the stress term's own shares
-    vtr += o1.vti + o2.vtj;
+    if (EX) {  // the stress term's own shares
+      vr += o1.vi + o2.vj;
+      vtr += o1.vti + o2.vtj;
+    }
     rb += o1.rho_i + o2.rho_j;
     pb += o1.p_i + o2.p_j;
     if (A.mT) {
@@ -648,7 +659,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
   }
   if (A.mT) {
     const float4 pa = A.pc ? A.pa[i] : make_float4(A.cm[i], A.cT[i], 0.0f, 0.0f);  // (own shares)
-    mb += pa.x + Vbk / I.rho + gbi4.w;  // gbi4.w: the balanced force's 1 / m
+    mb += pa.x + Vbk / I.rho;
+    if (bal) mb += gbi4.w;  // the balanced force's 1 / m
     mb -= (cA.x * F.x + cA.y * F.y + cA.z * F.z) / (I.m * I.m);  // a = g + F / m + ...
     if (A.heat) mb -= pa.y * dt * pv.t.dTdt[i] / I.m;      // dTdt = q / (m cp)
     if (vap) mb += rhob * pv.t.sigma[i];
@@ -659,7 +671,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
   }
   A.xr[i] = f4(xr);
   A.vr[i] = f4(vr);
-  if (A.vtr) A.vtr[i] = f4(vtr);
+  if (EX && A.vtr) A.vtr[i] = f4(vtr);
 }
 
 __global__ __launch_bounds__(kBlock) void k_adj_density(PV pv, GridDev g, SimParams P, NbrList L, const DevState* st,
@@ -1210,8 +1222,14 @@ void launch_adj_step(const PV& pv, const GridDev& g, const SimParams& P, const N
   }
   if (A.tvf) k_adj_wall<true><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
   else k_adj_wall<false><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
-  if (A.tvf) k_adj_fluid<true><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
-  else k_adj_fluid<false><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
+  const bool ex = A.vtr != nullptr || A.gbal != nullptr || P.kernel.type != 1;
+  if (A.tvf) {
+    if (ex) k_adj_fluid<true, true><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
+    else k_adj_fluid<true, false><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
+  } else {
+    if (ex) k_adj_fluid<false, true><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
+    else k_adj_fluid<false, false><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
+  }
   if (A.sat) {
     const int nb_inv = launch_inventory(pv, st, n, A.partials + 12 * A.stride, A.stride, s);
     k_adj_sat<<<1, 1024, 0, s>>>(st, A, nb, nb_inv);
