@@ -200,6 +200,21 @@ int cryo_solver_set_launch_variant(cryo_solver* s, int32_t variant);
  * Default off: on MI355X the replay has not been faster than plain launches for
  * these steps (the device, not the launch path, bounds them). */
 int cryo_solver_set_step_graphs(cryo_solver* s, int32_t on);
+/* Extension (no counterpart in the reference, which reads the particle count
+ * back after every phase-change step): on != 0 makes cryo_solver_run_steps and
+ * cryo_ensemble_run_steps queue a call's phase-change steps back to back with
+ * the particle count kept on the device.  The launches are sized for the count
+ * at the start of the call plus `room` particles (0: max(1024, n / 8)) and the
+ * kernels read the count when they run; splits append, a cell sort drops
+ * merged particles, all on the device, and the host takes the counts once at
+ * the end.  If a step's splits would pass that bound, the call restores the
+ * state it started from and runs the steps one at a time instead.  Results
+ * are the same bits either way.  Default off.  Fails with
+ * CRYO_ERR_INVALID_ARGUMENT for room < 0. */
+int cryo_solver_set_device_counts(cryo_solver* s, int32_t on, int32_t room);
+/* How many such calls passed their bound and ran step by step (since the
+ * solver was created): a count that grows says `room` is too small. */
+int cryo_solver_device_count_restarts(cryo_solver* s, int64_t* restarts);
 int cryo_solver_launch_variant(cryo_solver* s, int32_t* variant);
 int cryo_solver_autotune(cryo_solver* s, int32_t trial_steps, int32_t* variant, double* ms_per_step);
 

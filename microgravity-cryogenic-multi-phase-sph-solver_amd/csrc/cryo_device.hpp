@@ -218,6 +218,9 @@ struct DevState {
   unsigned next_id;
   unsigned split_requested;
   unsigned max_bits[5];  // atomicMax scratch for the maxima (float bits, all >= 0)
+  int n_sort;            // the count the step's cell sort runs over (dead particles included)
+  int merges_sorted;     // merges at the last cell sort (later ones are still in the arrays)
+  int count_overflow;    // a step's splits passed the launch bound of a queued run of steps
   int pad;
 };
 

@@ -18,18 +18,33 @@ struct Maneuver {
   V3 origin;
 };
 
+// A particle count as a kernel sees it.  Normally the host's count (`dev` =
+// nullptr).  While the host queues several phase-change steps without reading
+// the count back (splits append particles, a sort drops merged ones), `dev`
+// points at the count in the device state, read by each kernel when it runs,
+// and `ub`, an upper bound of it, sizes the launch: threads at or past the
+// count exit, block partial sums past it are zero.
+struct Count {
+  int ub = 0;
+  const int* dev = nullptr;
+  Count() = default;
+  Count(int n) : ub(n) {}  // a host count converts implicitly
+  Count(int bound, const int* d) : ub(bound), dev(d) {}
+  __device__ int get() const { return dev ? *dev : ub; }
+};
+
 // Fluid constants used by the ullage update (FluidProperties, fluid_properties.hpp:11-38).
 struct FluidConst {
   float rho_liquid, rho_vapor, T_sat, p_sat, R_gas, h_fg;
 };
 
 // grid
-void launch_kick_drift_keys(const PV& pv, const GridDev& g, DevState* st, int n, bool do_kick, bool continuity,
+void launch_kick_drift_keys(const PV& pv, const GridDev& g, DevState* st, Count n, bool do_kick, bool continuity,
                             bool tvf, uint32_t* keys, uint32_t* vals, hipStream_t s);
 int radix_tiles(int n);
-void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, int n,
+void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, Count n,
                        int key_bits, uint32_t* tile_hist, uint32_t* row_total, hipStream_t s, bool* result_in_alt);
-void launch_cell_offsets(const uint32_t* keys, int n, int num_cells, int* off, DevState* st, hipStream_t s);
+void launch_cell_offsets(const uint32_t* keys, Count n, int num_cells, int* off, DevState* st, hipStream_t s);
 void launch_cell_flags(const int* off, const uint8_t* phase, int num_cells, uint8_t* flags, hipStream_t s);
 void launch_gather(const Persist& dst, const Persist& src, const uint32_t* perm, const DevState* st, int n_ub,
                    hipStream_t s);
@@ -40,43 +55,44 @@ void launch_neighbor_fill(const float4* x, const GridDev& g, int n, float r2max,
 void launch_scan_one_block(uint32_t* data, int total, uint32_t* total_out, hipStream_t s);
 
 // physics
-void launch_density(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, int n, hipStream_t s);
+void launch_density(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, Count n, hipStream_t s);
 void launch_normalize_masses(const PV& pv, const GridDev& g, const SimParams& P, int n, hipStream_t s);
-void launch_wall_bc(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, const DevState* st, int n,
+void launch_wall_bc(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, const DevState* st, Count n,
                     hipStream_t s);
-int launch_interface(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, int n,
+int launch_interface(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, Count n,
                      bool csf_mode, double* heat_partials, bool tiled, hipStream_t s);
-void launch_smooth_normals(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, int n,
+void launch_smooth_normals(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, Count n,
                            hipStream_t s);
 void launch_heat_norm(DevState* st, const double* partials, int nb, bool active, hipStream_t s);
-void launch_forces(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, int n,
+void launch_forces(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, Count n,
                    bool csf_mode, bool akinci, bool continuity, bool thermal, bool tiled, hipStream_t s);
 void launch_probe(const PV& pv, const GridDev& g, const SimParams& P, const float* pts, float* out, int np,
                   hipStream_t s);
-void launch_shifting(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, const DevState* st, int n,
+void launch_shifting(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, const DevState* st, Count n,
                      hipStream_t s);
 
 // step
 int diag_quantities();
 void launch_begin_step(DevState* st, const SimParams& P, const DtLimits& L, const Maneuver& M, hipStream_t s);
-void launch_kick_sources(const PV& pv, const SimParams& P, DevState* st, int n, bool do_kick, bool thermal,
+void launch_kick_sources(const PV& pv, const SimParams& P, DevState* st, Count n, bool do_kick, bool thermal,
                          bool sources, hipStream_t s);
-int launch_pc_normalize(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, int n,
+int launch_pc_normalize(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, Count n,
                         double* partials, int stride, hipStream_t s);
-void launch_pc_apply(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, int n, hipStream_t s);
-void launch_rebalance(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, int n, hipStream_t s);
-void launch_rebalance_apply_mark(const PV& pv, const SimParams& P, int n, bool rebalance, uint32_t* flag,
+void launch_pc_apply(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, Count n, hipStream_t s);
+void launch_rebalance(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, Count n, hipStream_t s);
+void launch_rebalance_apply_mark(const PV& pv, const SimParams& P, Count n, bool rebalance, uint32_t* flag,
                                  hipStream_t s);
-void launch_exclusive_scan(const uint32_t* in, uint32_t* out, int n, uint32_t* block_sums, uint32_t* total,
+void launch_exclusive_scan(const uint32_t* in, uint32_t* out, Count n, uint32_t* block_sums, uint32_t* total,
                            hipStream_t s);
-void launch_split(const PV& pv, const SimParams& P, const DevState* st, int n, const uint32_t* flag,
+void launch_split(const PV& pv, const SimParams& P, const DevState* st, Count n, const uint32_t* flag,
                   const uint32_t* offset, hipStream_t s);
-void launch_split_commit(DevState* st, int n, int capacity, const uint32_t* total, hipStream_t s);
+void launch_split_commit(DevState* st, Count n, int capacity, const uint32_t* total, hipStream_t s);
 int launch_venting(const PV& pv, const SimParams& P, const DevState* st, int n_ub, float relax, double* partials,
                    int stride, hipStream_t s);
 int launch_inventory(const PV& pv, const DevState* st, int n_ub, double* partials, int stride, hipStream_t s);
 void launch_end_step(DevState* st, const double* partials, int stride, int nb_pc, int nb_vent, int nb_inv,
                      const FluidConst& fl, bool closed, bool advance_clock, hipStream_t s);
+void launch_pc_fold(DevState* st, const double* partials, int stride, int nb_pc, hipStream_t s);
 void launch_scale_velocities(const PV& pv, int n, float f, hipStream_t s);
 void launch_diagnostics(const PV& pv, const SimParams& P, int n, double* partials, int stride, unsigned* maxbits,
                         double* out, hipStream_t s);

@@ -18,9 +18,10 @@ __device__ inline int gtid() { return static_cast<int>(blockIdx.x * blockDim.x +
 // ---------------------------------------------------- kick-drift + cell keys
 // sph_solver.cu:20-34 (kick_drift) fused with the keys kernel of
 // spatial_hash.cu:61-68: the key uses the drifted position.
-__global__ __launch_bounds__(kBlock) void k_kick_drift_keys(PV pv, GridDev g, DevState* st, int n, int do_kick,
+__global__ __launch_bounds__(kBlock) void k_kick_drift_keys(PV pv, GridDev g, DevState* st, Count nc, int do_kick,
                                                            int continuity, int tvf, uint32_t* keys,
                                                            uint32_t* vals) {
+  const int n = nc.get();
   const int i = gtid();
   if (i >= n) return;
   const uint8_t ph = pv.s.phase[i];
@@ -54,8 +55,9 @@ constexpr int kRsThreads = 256;
 constexpr int kRsItems = 16;
 constexpr int kRsTile = kRsThreads * kRsItems;
 
-__global__ __launch_bounds__(kRsThreads) void k_radix_hist(const uint32_t* keys, int n, int shift,
+__global__ __launch_bounds__(kRsThreads) void k_radix_hist(const uint32_t* keys, Count nc, int shift,
                                                           uint32_t* tile_hist, int num_tiles) {
+  const int n = nc.get();
   __shared__ uint32_t h[256];
   const int tid = threadIdx.x;
   h[tid] = 0;
@@ -127,9 +129,10 @@ __global__ __launch_bounds__(256) void k_radix_scan_rows(uint32_t* tile_hist, in
 // Stable scatter: ranks inside a tile follow element order (round by round,
 // wave by wave, lane by lane), tiles follow tile order through tile_off.
 __global__ __launch_bounds__(kRsThreads) void k_radix_scatter(const uint32_t* keys_in, const uint32_t* vals_in,
-                                                             uint32_t* keys_out, uint32_t* vals_out, int n,
+                                                             uint32_t* keys_out, uint32_t* vals_out, Count nc,
                                                              int shift, const uint32_t* tile_off, int num_tiles,
                                                              const uint32_t* row_total) {
+  const int n = nc.get();
   __shared__ uint32_t running[256];
   __shared__ uint32_t digit_base[256];
   __shared__ uint32_t wave_cnt[kRsThreads / 64][256];
@@ -190,8 +193,9 @@ __global__ __launch_bounds__(kRsThreads) void k_radix_scatter(const uint32_t* ke
 
 // off[c] = first sorted index with key >= c, for c in [0, num_cells]:
 // cell c holds [off[c], off[c+1]); off[num_cells] = live particle count.
-__global__ __launch_bounds__(kBlock) void k_cell_offsets(const uint32_t* keys, int n, int num_cells, int* off,
+__global__ __launch_bounds__(kBlock) void k_cell_offsets(const uint32_t* keys, Count nc, int num_cells, int* off,
                                                         DevState* st) {
+  const int n = nc.get();
   const int c = gtid();
   if (c > num_cells) return;
   int lo = 0, hi = n;
@@ -204,6 +208,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_offsets(const uint32_t* keys, i
   if (c == num_cells) {
     st->n_alive = lo;
     st->n = lo;
+    st->merges_sorted = st->merges;
   }
 }
 
@@ -263,20 +268,20 @@ inline int blocks_for(long long n, int b = kBlock) { return static_cast<int>((n 
 }  // namespace
 
 // ------------------------------------------------------------ host wrappers
-void launch_kick_drift_keys(const PV& pv, const GridDev& g, DevState* st, int n, bool do_kick, bool continuity,
+void launch_kick_drift_keys(const PV& pv, const GridDev& g, DevState* st, Count n, bool do_kick, bool continuity,
                             bool tvf, uint32_t* keys, uint32_t* vals, hipStream_t s) {
-  if (n <= 0) return;
-  k_kick_drift_keys<<<blocks_for(n), kBlock, 0, s>>>(pv, g, st, n, do_kick ? 1 : 0, continuity ? 1 : 0,
+  if (n.ub <= 0) return;
+  k_kick_drift_keys<<<blocks_for(n.ub), kBlock, 0, s>>>(pv, g, st, n, do_kick ? 1 : 0, continuity ? 1 : 0,
                                                      tvf ? 1 : 0, keys, vals);
 }
 
 int radix_tiles(int n) { return (n + kRsTile - 1) / kRsTile; }
 
-void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, int n,
+void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, Count n,
                        int key_bits, uint32_t* tile_hist, uint32_t* row_total, hipStream_t s, bool* result_in_alt) {
   *result_in_alt = false;
-  if (n <= 0) return;
-  const int tiles = radix_tiles(n);
+  if (n.ub <= 0) return;
+  const int tiles = radix_tiles(n.ub);
   uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt;
   for (int shift = 0; shift < key_bits; shift += 8) {
     k_radix_hist<<<tiles, kRsThreads, 0, s>>>(ki, n, shift, tile_hist, tiles);
@@ -310,7 +315,7 @@ void launch_cell_flags(const int* off, const uint8_t* phase, int num_cells, uint
   k_cell_flags<<<blocks_for(num_cells), kBlock, 0, s>>>(off, phase, num_cells, flags);
 }
 
-void launch_cell_offsets(const uint32_t* keys, int n, int num_cells, int* off, DevState* st, hipStream_t s) {
+void launch_cell_offsets(const uint32_t* keys, Count n, int num_cells, int* off, DevState* st, hipStream_t s) {
   k_cell_offsets<<<blocks_for(static_cast<long long>(num_cells) + 1), kBlock, 0, s>>>(keys, n, num_cells, off, st);
 }
 

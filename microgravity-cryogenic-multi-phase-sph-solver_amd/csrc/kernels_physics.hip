@@ -56,7 +56,8 @@ __device__ inline float powf_cr(float x, float y) { return static_cast<float>(po
 // ------------------------------------------------------- density + EOS
 // Also builds the step's neighbour lists from the same (cell-culled) sweep.
 template <bool KS>
-__global__ __launch_bounds__(kBlock) void k_density(PV pv, GridDev g, SimParams P, NbrList L, int n) {
+__global__ __launch_bounds__(kBlock) void k_density(PV pv, GridDev g, SimParams P, NbrList L, Count nc) {
+  const int n = nc.get();
   const int i = gtid();
   if (i >= n) return;
   const uint8_t ph = pv.s.phase[i];
@@ -145,7 +146,8 @@ __global__ __launch_bounds__(kBlock) void k_normalize_masses(PV pv, GridDev g, S
 // --------------------------------------------------------------- wall BC
 template <bool KS>
 __global__ __launch_bounds__(kBlock) void k_wall_bc(PV pv, GridDev g, SimParams P, NbrList L, const DevState* st,
-                                                   int n) {
+                                                   Count nc) {
+  const int n = nc.get();
   const int i = gtid();
   if (i >= n) return;
   if (pv.s.phase[i] != kWall) return;
@@ -362,7 +364,8 @@ __device__ inline float interface_particle(const PV& pv, const SimParams& P, int
 // writes the heat partials as k_interface does.
 template <bool KS>
 __global__ __launch_bounds__(kBlock) void k_interface_sparse(PV pv, GridDev g, SimParams P, NbrList L, DevState* st,
-                                                            int n, int csf_mode, double* heat_partials) {
+                                                            Count nc, int csf_mode, double* heat_partials) {
+  const int n = nc.get();
   const int i = gtid();
   if (i == 0) {
     for (int k = 0; k < 5; ++k) st->max_bits[k] = 0u;
@@ -403,8 +406,9 @@ __global__ __launch_bounds__(kBlock) void k_interface_sparse(PV pv, GridDev g, S
 // just the particles whose list overflowed (the cell-tile instance does the
 // rest).
 template <bool SPS, bool KS>
-__global__ __launch_bounds__(kBlock) void k_interface(PV pv, GridDev g, SimParams P, NbrList L, DevState* st, int n,
+__global__ __launch_bounds__(kBlock) void k_interface(PV pv, GridDev g, SimParams P, NbrList L, DevState* st, Count nc,
                                                      int csf_mode, double* heat_partials, int only_overflow) {
+  const int n = nc.get();
   const int i = gtid();
   if (i == 0 && !only_overflow) {
     for (int k = 0; k < 5; ++k) st->max_bits[k] = 0u;
@@ -422,7 +426,8 @@ __global__ __launch_bounds__(kBlock) void k_interface(PV pv, GridDev g, SimParam
 
 // surface_tension.cu:87-118 with neighbour smoothing on.
 template <bool KS>
-__global__ __launch_bounds__(kBlock) void k_smooth_normals(PV pv, GridDev g, SimParams P, NbrList L, int n) {
+__global__ __launch_bounds__(kBlock) void k_smooth_normals(PV pv, GridDev g, SimParams P, NbrList L, Count nc) {
+  const int n = nc.get();
   const int i = gtid();
   if (i >= n) return;
   if (!is_fluid(pv.s.phase[i])) return;
@@ -773,7 +778,8 @@ __device__ inline void publish_maxima(DevState* st, Maxima mx) {
 // neighbours in the same order.  Two instances keep the sweep's code and
 // registers out of the list loop.
 template <int ST, bool CONT, bool THERMAL, bool KS, bool SWEEP>
-__global__ __launch_bounds__(kBlock) void k_forces(PV pv, GridDev g, SimParams P, NbrList L, DevState* st, int n) {
+__global__ __launch_bounds__(kBlock) void k_forces(PV pv, GridDev g, SimParams P, NbrList L, DevState* st, Count nc) {
+  const int n = nc.get();
   const int i = gtid();
   Maxima mx;
   const int cnt = i < n ? L.count[i] : 0;
@@ -1185,7 +1191,8 @@ __global__ __launch_bounds__(kTileThreads) void k_interface_tiled(PV pv, GridDev
 }
 
 // FP64 block partials of sum_fluid heat_w (sph_solver.cu:101-104).
-__global__ __launch_bounds__(kBlock) void k_heat_partials(PV pv, int n, double* partials) {
+__global__ __launch_bounds__(kBlock) void k_heat_partials(PV pv, Count nc, double* partials) {
+  const int n = nc.get();
   const int i = gtid();
   double v = 0.0;
   if (i < n && is_fluid(pv.s.phase[i])) v = static_cast<double>(pv.t.heat_w[i]);
@@ -1240,7 +1247,8 @@ __global__ __launch_bounds__(kBlock) void k_probe(PV pv, GridDev g, SimParams P,
 // SPH_kernels.cu:106-160 (delta+ shifting), pass 1 and pass 2.
 template <bool KS>
 __global__ __launch_bounds__(kBlock) void k_shift_compute(PV pv, GridDev g, SimParams P, NbrList L,
-                                                         const DevState* st, int n) {
+                                                         const DevState* st, Count nc) {
+  const int n = nc.get();
   const int i = gtid();
   if (i >= n) return;
   pv.t.ns[i] = st3(V3{0, 0, 0});
@@ -1283,7 +1291,8 @@ __global__ __launch_bounds__(kBlock) void k_shift_compute(PV pv, GridDev g, SimP
   pv.t.ns[i] = st3(dr);
 }
 
-__global__ __launch_bounds__(kBlock) void k_shift_apply(PV pv, int n) {
+__global__ __launch_bounds__(kBlock) void k_shift_apply(PV pv, Count nc) {
+  const int n = nc.get();
   const int i = gtid();
   if (i >= n) return;
   if (!is_fluid(pv.s.phase[i])) return;
@@ -1302,26 +1311,26 @@ __global__ __launch_bounds__(kBlock) void k_shift_apply(PV pv, int n) {
 
 }  // namespace
 
-void launch_density(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, int n, hipStream_t s) {
-  if (n <= 0) return;
-  if (P.compensated) k_density<true><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, n);
-  else k_density<false><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, n);
+void launch_density(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, Count n, hipStream_t s) {
+  if (n.ub <= 0) return;
+  if (P.compensated) k_density<true><<<blocks_for(n.ub), kBlock, 0, s>>>(pv, g, P, L, n);
+  else k_density<false><<<blocks_for(n.ub), kBlock, 0, s>>>(pv, g, P, L, n);
 }
 void launch_normalize_masses(const PV& pv, const GridDev& g, const SimParams& P, int n, hipStream_t s) {
   if (n <= 0) return;
   if (P.compensated) k_normalize_masses<true><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, n);
   else k_normalize_masses<false><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, n);
 }
-void launch_wall_bc(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, const DevState* st, int n,
+void launch_wall_bc(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, const DevState* st, Count n,
                     hipStream_t s) {
-  if (n <= 0) return;
-  if (P.compensated) k_wall_bc<true><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, st, n);
-  else k_wall_bc<false><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, st, n);
+  if (n.ub <= 0) return;
+  if (P.compensated) k_wall_bc<true><<<blocks_for(n.ub), kBlock, 0, s>>>(pv, g, P, L, st, n);
+  else k_wall_bc<false><<<blocks_for(n.ub), kBlock, 0, s>>>(pv, g, P, L, st, n);
 }
 template <bool SPS, bool KS>
 static void interface_variant(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st,
-                              int n, int cm, double* heat_partials, bool tiled, hipStream_t s) {
-  const int nb = blocks_for(n > 0 ? n : 1);
+                              Count n, int cm, double* heat_partials, bool tiled, hipStream_t s) {
+  const int nb = blocks_for(n.ub > 0 ? n.ub : 1);
   if (!tiled) {
     k_interface<SPS, KS><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, n, cm, heat_partials, 0);
     return;
@@ -1332,9 +1341,9 @@ static void interface_variant(const PV& pv, const GridDev& g, const SimParams& P
   k_interface<SPS, KS><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, n, cm, nullptr, 1);
   if (heat_partials) k_heat_partials<<<nb, kBlock, 0, s>>>(pv, n, heat_partials);
 }
-int launch_interface(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, int n,
+int launch_interface(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, Count n,
                      bool csf_mode, double* heat_partials, bool tiled, hipStream_t s) {
-  const int nb = blocks_for(n > 0 ? n : 1);
+  const int nb = blocks_for(n.ub > 0 ? n.ub : 1);
   const int cm = csf_mode ? 1 : 0;
   if (P.st_model != 2 && !P.sps && L.valid && g.flags) {
     if (P.compensated) k_interface_sparse<true><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, n, cm, heat_partials);
@@ -1351,18 +1360,18 @@ int launch_interface(const PV& pv, const GridDev& g, const SimParams& P, const N
   }
   return nb;
 }
-void launch_smooth_normals(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, int n,
+void launch_smooth_normals(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, Count n,
                            hipStream_t s) {
-  if (n <= 0) return;
-  if (P.compensated) k_smooth_normals<true><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, n);
-  else k_smooth_normals<false><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, n);
+  if (n.ub <= 0) return;
+  if (P.compensated) k_smooth_normals<true><<<blocks_for(n.ub), kBlock, 0, s>>>(pv, g, P, L, n);
+  else k_smooth_normals<false><<<blocks_for(n.ub), kBlock, 0, s>>>(pv, g, P, L, n);
 }
 void launch_heat_norm(DevState* st, const double* partials, int nb, bool active, hipStream_t s) {
   k_heat_norm<<<1, 1024, 0, s>>>(st, partials, nb, active ? 1 : 0);
 }
 
 template <int ST, bool CONT, bool THERMAL, bool KS>
-static void forces_variant(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, int n,
+static void forces_variant(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, Count n,
                            bool tiled, hipStream_t s) {
   if (!L.valid) throw std::logic_error("forces pass: neighbour lists not built");
   if (tiled) {
@@ -1370,12 +1379,12 @@ static void forces_variant(const PV& pv, const GridDev& g, const SimParams& P, c
     if (tiles > 0x7fffffffLL) throw std::runtime_error("forces pass: too many cell tiles");
     k_forces_tiled<ST, CONT, THERMAL, KS><<<static_cast<int>(tiles), kTileThreads, 0, s>>>(pv, g, P, L, st);
   } else {
-    k_forces<ST, CONT, THERMAL, KS, false><<<blocks_for(n > 0 ? n : 1), kBlock, 0, s>>>(pv, g, P, L, st, n);
+    k_forces<ST, CONT, THERMAL, KS, false><<<blocks_for(n.ub > 0 ? n.ub : 1), kBlock, 0, s>>>(pv, g, P, L, st, n);
   }
-  k_forces<ST, CONT, THERMAL, KS, true><<<blocks_for(n > 0 ? n : 1), kBlock, 0, s>>>(pv, g, P, L, st, n);
+  k_forces<ST, CONT, THERMAL, KS, true><<<blocks_for(n.ub > 0 ? n.ub : 1), kBlock, 0, s>>>(pv, g, P, L, st, n);
 }
 template <int ST, bool KS>
-static void forces_st(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, int n,
+static void forces_st(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, Count n,
                       bool continuity, bool thermal, bool tiled, hipStream_t s) {
   if (continuity) {
     if (thermal) forces_variant<ST, true, true, KS>(pv, g, P, L, st, n, tiled, s);
@@ -1386,13 +1395,13 @@ static void forces_st(const PV& pv, const GridDev& g, const SimParams& P, const 
   }
 }
 template <bool KS>
-static void forces_ks(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, int n,
+static void forces_ks(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, Count n,
                       bool csf_mode, bool akinci, bool continuity, bool thermal, bool tiled, hipStream_t s) {
   if (akinci) forces_st<2, KS>(pv, g, P, L, st, n, continuity, thermal, tiled, s);
   else if (csf_mode) forces_st<1, KS>(pv, g, P, L, st, n, continuity, thermal, tiled, s);
   else forces_st<0, KS>(pv, g, P, L, st, n, continuity, thermal, tiled, s);
 }
-void launch_forces(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, int n,
+void launch_forces(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, DevState* st, Count n,
                    bool csf_mode, bool akinci, bool continuity, bool thermal, bool tiled, hipStream_t s) {
   if (P.compensated) forces_ks<true>(pv, g, P, L, st, n, csf_mode, akinci, continuity, thermal, tiled, s);
   else forces_ks<false>(pv, g, P, L, st, n, csf_mode, akinci, continuity, thermal, tiled, s);
@@ -1402,12 +1411,12 @@ void launch_probe(const PV& pv, const GridDev& g, const SimParams& P, const floa
                   hipStream_t s) {
   if (np > 0) k_probe<<<blocks_for(np), kBlock, 0, s>>>(pv, g, P, pts, out, np);
 }
-void launch_shifting(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, const DevState* st, int n,
+void launch_shifting(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, const DevState* st, Count n,
                      hipStream_t s) {
-  if (n <= 0) return;
-  if (P.compensated) k_shift_compute<true><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, st, n);
-  else k_shift_compute<false><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, st, n);
-  k_shift_apply<<<blocks_for(n), kBlock, 0, s>>>(pv, n);
+  if (n.ub <= 0) return;
+  if (P.compensated) k_shift_compute<true><<<blocks_for(n.ub), kBlock, 0, s>>>(pv, g, P, L, st, n);
+  else k_shift_compute<false><<<blocks_for(n.ub), kBlock, 0, s>>>(pv, g, P, L, st, n);
+  k_shift_apply<<<blocks_for(n.ub), kBlock, 0, s>>>(pv, n);
 }
 
 }  // namespace cryo

@@ -78,6 +78,7 @@ __device__ inline float cbrtf_cr(float x) { return static_cast<float>(cbrt(stati
 __global__ void k_begin_step(DevState* st, SimParams P, DtLimits L, Maneuver M) {
   const float dt = compute_dt(P, L, st->v_max, st->a_max, st->Tv_max, st->nu_t_max, st->spacing_ratio);
   st->dt = dt;
+  st->n_sort = st->n;
   if (M.n > 0) {
     const double t = st->time + static_cast<double>(dt);
     V3 a, w, wd{0, 0, 0};
@@ -118,8 +119,9 @@ __global__ void k_begin_step(DevState* st, SimParams P, DtLimits L, Maneuver M) 
 
 // ----------------------------------------------- kick (+ Lee sources)
 // sph_solver.cu:36-46 and phase_change.cu:77-137 (sources read the kicked T).
-__global__ __launch_bounds__(kBlock) void k_kick_sources(PV pv, SimParams P, DevState* st, int n, int do_kick,
+__global__ __launch_bounds__(kBlock) void k_kick_sources(PV pv, SimParams P, DevState* st, Count nc, int do_kick,
                                                         int thermal, int sources) {
+  const int n = nc.get();
   const int i = gtid();
   if (i == 0 && do_kick && thermal && st->heat_inv_norm > 0.0f)
     st->heat_input += static_cast<double>(P.heat_power) * static_cast<double>(st->dt);
@@ -187,8 +189,9 @@ __global__ __launch_bounds__(kBlock) void k_kick_sources(PV pv, SimParams P, Dev
 // phase_change.cu:139-172, plus the partials of sph_solver.cu:108-124
 // (evaporated, condensed, merges) over every particle.
 template <bool KS>
-__global__ __launch_bounds__(kBlock) void k_pc_normalize(PV pv, GridDev g, SimParams P, NbrList L, int n,
+__global__ __launch_bounds__(kBlock) void k_pc_normalize(PV pv, GridDev g, SimParams P, NbrList L, Count nc,
                                                         double* partials, int stride) {
+  const int n = nc.get();
   const int i = gtid();
   double acc[3] = {0.0, 0.0, 0.0};
   if (i < n) {
@@ -253,7 +256,8 @@ __global__ __launch_bounds__(kBlock) void k_cell_sources(PV pv, GridDev g) {
 
 // phase_change.cu:174-215
 template <bool KS>
-__global__ __launch_bounds__(kBlock) void k_pc_apply(PV pv, GridDev g, SimParams P, NbrList L, int n) {
+__global__ __launch_bounds__(kBlock) void k_pc_apply(PV pv, GridDev g, SimParams P, NbrList L, Count nc) {
+  const int n = nc.get();
   const int i = gtid();
   if (i >= n) return;
   const uint8_t ph = pv.s.phase[i];
@@ -303,7 +307,8 @@ __global__ __launch_bounds__(kBlock) void k_pc_apply(PV pv, GridDev g, SimParams
 
 // phase_change.cu:217-249 (pass 1)
 template <bool KS>
-__global__ __launch_bounds__(kBlock) void k_rebalance(PV pv, GridDev g, SimParams P, NbrList L, int n) {
+__global__ __launch_bounds__(kBlock) void k_rebalance(PV pv, GridDev g, SimParams P, NbrList L, Count nc) {
+  const int n = nc.get();
   const int i = gtid();
   if (i >= n) return;
   pv.t.out_m[i] = 0.0f;
@@ -341,8 +346,9 @@ __global__ __launch_bounds__(kBlock) void k_rebalance(PV pv, GridDev g, SimParam
 
 // phase_change.cu:250-268 (pass 2) fused with mark_heavy_particles
 // (phase_change.cu:270-279) and a per-block flag count for the slot scan.
-__global__ __launch_bounds__(kBlock) void k_rebalance_apply_mark(PV pv, SimParams P, int n, int rebalance,
+__global__ __launch_bounds__(kBlock) void k_rebalance_apply_mark(PV pv, SimParams P, Count nc, int rebalance,
                                                                 uint32_t* flag) {
+  const int n = nc.get();
   const int i = gtid();
   if (i >= n) return;
   const uint8_t ph = pv.s.phase[i];
@@ -367,7 +373,8 @@ __global__ __launch_bounds__(kBlock) void k_rebalance_apply_mark(PV pv, SimParam
 
 // Exclusive scan of the split flags: per-block scan, block totals scanned by
 // one block, then offsets added (deterministic slots, never atomics).
-__global__ __launch_bounds__(1024) void k_scan_blocks(const uint32_t* in, uint32_t* out, int n, uint32_t* sums) {
+__global__ __launch_bounds__(1024) void k_scan_blocks(const uint32_t* in, uint32_t* out, Count nc, uint32_t* sums) {
+  const int n = nc.get();
   __shared__ uint32_t s[1024];
   const int i = blockIdx.x * 1024 + threadIdx.x;
   const uint32_t v = i < n ? in[i] : 0u;
@@ -382,14 +389,16 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(const uint32_t* in, uint32
   if (i < n) out[i] = s[threadIdx.x] - v;
   if (threadIdx.x == 1023) sums[blockIdx.x] = s[1023];
 }
-__global__ __launch_bounds__(1024) void k_scan_add(uint32_t* out, int n, const uint32_t* sums) {
+__global__ __launch_bounds__(1024) void k_scan_add(uint32_t* out, Count nc, const uint32_t* sums) {
+  const int n = nc.get();
   const int i = blockIdx.x * 1024 + threadIdx.x;
   if (i < n && blockIdx.x > 0) out[i] += sums[blockIdx.x];
 }
 
 // phase_change.cu:281-335
-__global__ __launch_bounds__(kBlock) void k_split(PV pv, SimParams P, const DevState* st, int n,
+__global__ __launch_bounds__(kBlock) void k_split(PV pv, SimParams P, const DevState* st, Count nc,
                                                  const uint32_t* flag, const uint32_t* offset) {
+  const int n = nc.get();
   const int i = gtid();
   if (i >= n) return;
   if (!flag[i]) return;
@@ -454,10 +463,15 @@ __global__ __launch_bounds__(kBlock) void k_split(PV pv, SimParams P, const DevS
 }
 
 // Split bookkeeping (sph_solver.cu:567-576): runs on one thread after k_split.
-__global__ void k_split_commit(DevState* st, int n, int capacity, const uint32_t* scan_total) {
+// `capacity` is the arrays' capacity, or with a device count the launch bound
+// of the queued steps: splits past it are flagged (the host then re-runs
+// those steps one at a time with the count in hand).
+__global__ void k_split_commit(DevState* st, Count nc, int capacity, const uint32_t* scan_total) {
+  const int n = nc.get();
   const uint32_t requested = *scan_total;
   const long long room = static_cast<long long>(capacity) - n;
   const long long added = requested < room ? static_cast<long long>(requested) : room;
+  if (nc.dev && static_cast<long long>(requested) > room) st->count_overflow = 1;
   st->n = n + static_cast<int>(added);
   st->next_id += requested;
   st->splits += static_cast<int>(added);
@@ -565,6 +579,20 @@ __global__ __launch_bounds__(1024) void k_end_step(DevState* st, const double* p
     st->time += static_cast<double>(st->dt);
     st->step += 1;
     st->last_dt = static_cast<double>(st->dt);
+  }
+}
+
+// The phase-change totals of k_end_step (the same ordered sums), folded in
+// before the particle arrays grow mid-step: growing re-sizes the partials.
+__global__ __launch_bounds__(1024) void k_pc_fold(DevState* st, const double* partials, int stride, int nb_pc) {
+  __shared__ double sh[1024];
+  const double ev = ordered_sum(partials + 0 * stride, nb_pc, sh);
+  const double co = ordered_sum(partials + 1 * stride, nb_pc, sh);
+  const double mg = ordered_sum(partials + 2 * stride, nb_pc, sh);
+  if (threadIdx.x == 0) {
+    st->evaporated += ev;
+    st->condensed += co;
+    st->merges += static_cast<int>(mg);
   }
 }
 
@@ -679,37 +707,37 @@ int diag_quantities() { return kDiagQ; }
 void launch_begin_step(DevState* st, const SimParams& P, const DtLimits& L, const Maneuver& M, hipStream_t s) {
   k_begin_step<<<1, 1, 0, s>>>(st, P, L, M);
 }
-void launch_kick_sources(const PV& pv, const SimParams& P, DevState* st, int n, bool do_kick, bool thermal,
+void launch_kick_sources(const PV& pv, const SimParams& P, DevState* st, Count n, bool do_kick, bool thermal,
                          bool sources, hipStream_t s) {
-  k_kick_sources<<<blocks_for(n > 0 ? n : 1), kBlock, 0, s>>>(pv, P, st, n, do_kick ? 1 : 0, thermal ? 1 : 0,
+  k_kick_sources<<<blocks_for(n.ub > 0 ? n.ub : 1), kBlock, 0, s>>>(pv, P, st, n, do_kick ? 1 : 0, thermal ? 1 : 0,
                                                               sources ? 1 : 0);
 }
-int launch_pc_normalize(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, int n,
+int launch_pc_normalize(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, Count n,
                         double* partials, int stride, hipStream_t s) {
-  const int nb = blocks_for(n > 0 ? n : 1);
+  const int nb = blocks_for(n.ub > 0 ? n.ub : 1);
   if (P.compensated) k_pc_normalize<true><<<nb, kBlock, 0, s>>>(pv, g, P, L, n, partials, stride);
   else k_pc_normalize<false><<<nb, kBlock, 0, s>>>(pv, g, P, L, n, partials, stride);
   return nb;
 }
-void launch_pc_apply(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, int n, hipStream_t s) {
-  if (n <= 0) return;
+void launch_pc_apply(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, Count n, hipStream_t s) {
+  if (n.ub <= 0) return;
   if (g.num_cells > 0) k_cell_sources<<<blocks_for(g.num_cells), kBlock, 0, s>>>(pv, g);
-  if (P.compensated) k_pc_apply<true><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, n);
-  else k_pc_apply<false><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, n);
+  if (P.compensated) k_pc_apply<true><<<blocks_for(n.ub), kBlock, 0, s>>>(pv, g, P, L, n);
+  else k_pc_apply<false><<<blocks_for(n.ub), kBlock, 0, s>>>(pv, g, P, L, n);
 }
-void launch_rebalance(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, int n, hipStream_t s) {
-  if (n <= 0) return;
-  if (P.compensated) k_rebalance<true><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, n);
-  else k_rebalance<false><<<blocks_for(n), kBlock, 0, s>>>(pv, g, P, L, n);
+void launch_rebalance(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, Count n, hipStream_t s) {
+  if (n.ub <= 0) return;
+  if (P.compensated) k_rebalance<true><<<blocks_for(n.ub), kBlock, 0, s>>>(pv, g, P, L, n);
+  else k_rebalance<false><<<blocks_for(n.ub), kBlock, 0, s>>>(pv, g, P, L, n);
 }
-void launch_rebalance_apply_mark(const PV& pv, const SimParams& P, int n, bool rebalance, uint32_t* flag,
+void launch_rebalance_apply_mark(const PV& pv, const SimParams& P, Count n, bool rebalance, uint32_t* flag,
                                  hipStream_t s) {
-  if (n > 0) k_rebalance_apply_mark<<<blocks_for(n), kBlock, 0, s>>>(pv, P, n, rebalance ? 1 : 0, flag);
+  if (n.ub > 0) k_rebalance_apply_mark<<<blocks_for(n.ub), kBlock, 0, s>>>(pv, P, n, rebalance ? 1 : 0, flag);
 }
-void launch_exclusive_scan(const uint32_t* in, uint32_t* out, int n, uint32_t* block_sums, uint32_t* total,
+void launch_exclusive_scan(const uint32_t* in, uint32_t* out, Count n, uint32_t* block_sums, uint32_t* total,
                            hipStream_t s) {
-  const int nb = (n + 1023) / 1024;
-  if (n > 0) {
+  const int nb = (n.ub + 1023) / 1024;
+  if (n.ub > 0) {
     k_scan_blocks<<<nb, 1024, 0, s>>>(in, out, n, block_sums);
     launch_scan_one_block(block_sums, nb, total, s);
     k_scan_add<<<nb, 1024, 0, s>>>(out, n, block_sums);
@@ -717,11 +745,11 @@ void launch_exclusive_scan(const uint32_t* in, uint32_t* out, int n, uint32_t* b
     k_zero_u32<<<1, 1, 0, s>>>(total, 1);
   }
 }
-void launch_split(const PV& pv, const SimParams& P, const DevState* st, int n, const uint32_t* flag,
+void launch_split(const PV& pv, const SimParams& P, const DevState* st, Count n, const uint32_t* flag,
                   const uint32_t* offset, hipStream_t s) {
-  if (n > 0) k_split<<<blocks_for(n), kBlock, 0, s>>>(pv, P, st, n, flag, offset);
+  if (n.ub > 0) k_split<<<blocks_for(n.ub), kBlock, 0, s>>>(pv, P, st, n, flag, offset);
 }
-void launch_split_commit(DevState* st, int n, int capacity, const uint32_t* total, hipStream_t s) {
+void launch_split_commit(DevState* st, Count n, int capacity, const uint32_t* total, hipStream_t s) {
   k_split_commit<<<1, 1, 0, s>>>(st, n, capacity, total);
 }
 int launch_venting(const PV& pv, const SimParams& P, const DevState* st, int n_ub, float relax, double* partials,
@@ -739,6 +767,9 @@ void launch_end_step(DevState* st, const double* partials, int stride, int nb_pc
                      const FluidConst& fl, bool closed, bool advance_clock, hipStream_t s) {
   k_end_step<<<1, 1024, 0, s>>>(st, partials, stride, nb_pc, nb_vent, nb_inv, fl, closed ? 1 : 0,
                                 advance_clock ? 1 : 0);
+}
+void launch_pc_fold(DevState* st, const double* partials, int stride, int nb_pc, hipStream_t s) {
+  if (nb_pc > 0) k_pc_fold<<<1, 1024, 0, s>>>(st, partials, stride, nb_pc);
 }
 void launch_scale_velocities(const PV& pv, int n, float f, hipStream_t s) {
   if (n > 0) k_scale_velocities<<<blocks_for(n), kBlock, 0, s>>>(pv, n, f);

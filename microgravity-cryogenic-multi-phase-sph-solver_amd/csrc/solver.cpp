@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -267,6 +268,24 @@ struct cryo_solver {
   int key_bits = 1;
   // host knowledge
   int n = 0, capacity = 0, pending_dead = 0, merges_seen = 0;
+  // Device counts: phase-change steps queued back to back without the count
+  // coming back in between.  Launches are sized for count_ub (the count at
+  // the start plus count_room, default max(1024, n / 8)); kernels read the
+  // count from the device state.  A step whose splits would pass count_ub
+  // flags it, and the run is restored and re-run one step at a time.
+  bool device_counts = false;  // run_steps and the ensemble runner use them
+  bool dev_count = false;      // set while such steps are queued
+  int count_ub = 0, count_room = 0;
+  int64_t count_restarts = 0;  // queued runs restored and re-run step by step
+  const int* state_field(std::size_t off) const {
+    return reinterpret_cast<const int*>(reinterpret_cast<const char*>(state.get()) + off);
+  }
+  // the count of the step's passes, and of its cell sort (dead included)
+  Count cn() const { return dev_count ? Count(count_ub, state_field(offsetof(DevState, n))) : Count(n); }
+  Count cn_sort() const {
+    return dev_count ? Count(count_ub, state_field(offsetof(DevState, n_sort))) : Count(n);
+  }
+  int n_bound() const { return dev_count ? count_ub : capacity; }  // bounds the count after a step's splits
   uint32_t next_id = 0;
   bool initialized = false, has_vapor = false;
   float char_length = 0.0f;
@@ -668,6 +687,7 @@ struct cryo_solver {
     s.last_dt = 0;
     s.heat_input = s.evaporated = s.condensed = s.vented = s.vented_h = 0;
     s.splits = s.merges = 0;
+    s.merges_sorted = s.count_overflow = 0;
     s.p_ullage = fluid.p_sat;
     s.fluid_volume0 = 0;
     s.dt = 0;
@@ -721,27 +741,29 @@ struct cryo_solver {
     PV pv = view();
     const GridDev gd = grid();
     prof_begin(0);
-    launch_kick_drift_keys(pv, gd, state.get(), n, do_kick, cfg.density_mode == CRYO_DENSITY_CONTINUITY,
+    const Count ns = cn_sort();
+    launch_kick_drift_keys(pv, gd, state.get(), ns, do_kick, cfg.density_mode == CRYO_DENSITY_CONTINUITY,
                            P.tvf != 0, keys_[0].get(), vals_[0].get(), stream);
     prof_end(0);
     prof_begin(1);
     bool alt = false;
-    launch_radix_sort(keys_[0].get(), vals_[0].get(), keys_[1].get(), vals_[1].get(), n, key_bits, tile_hist.get(),
+    launch_radix_sort(keys_[0].get(), vals_[0].get(), keys_[1].get(), vals_[1].get(), ns, key_bits, tile_hist.get(),
                       row_total.get(), stream, &alt);
     const uint32_t* keys = alt ? keys_[1].get() : keys_[0].get();
     const uint32_t* perm = alt ? vals_[1].get() : vals_[0].get();
-    launch_cell_offsets(keys, n, g.num_cells, cell_off.get(), state.get(), stream);
+    launch_cell_offsets(keys, ns, g.num_cells, cell_off.get(), state.get(), stream);
     prof_end(1);
     prof_begin(2);
-    launch_gather(persist(1 - set), persist(set), perm, state.get(), n, stream);
+    launch_gather(persist(1 - set), persist(set), perm, state.get(), ns.ub, stream);
     launch_cell_flags(cell_off.get(), persist(1 - set).phase, g.num_cells, cell_flags.get(), stream);
     prof_end(2);
     check_launch();
     set = 1 - set;
     last_perm = perm;
-    n_sorted = n;
     lists_valid = false;
     tiles_ok = true;
+    if (dev_count) return;  // the device drops them and holds the count
+    n_sorted = n;
     n -= pending_dead;  // dead particles sort last and are dropped
     pending_dead = 0;
   }
@@ -755,18 +777,19 @@ struct cryo_solver {
     const bool heat_sum = P.thermal && P.heat_power != 0.0f;
     prof_begin(3);
     lists_valid = false;
-    launch_density(pv, gd, P, lists(), n, stream);
+    const Count nc = cn();
+    launch_density(pv, gd, P, lists(), nc, stream);
     lists_valid = true;
     const NbrList L = lists();
     prof_end(3);
     prof_begin(4);
-    launch_wall_bc(pv, gd, P, L, state.get(), n, stream);
+    launch_wall_bc(pv, gd, P, L, state.get(), nc, stream);
     prof_end(4);
     prof_begin(5);
     const int nb =
-        launch_interface(pv, gd, P, L, state.get(), n, csf_mode, heat_sum ? partials.get() : nullptr,
+        launch_interface(pv, gd, P, L, state.get(), nc, csf_mode, heat_sum ? partials.get() : nullptr,
                          tiles_ok && tile_iface, stream);
-    if (csf_mode && P.csf_smooth) launch_smooth_normals(pv, gd, P, L, n, stream);
+    if (csf_mode && P.csf_smooth) launch_smooth_normals(pv, gd, P, L, nc, stream);
     if (P.thermal) launch_heat_norm(state.get(), partials.get(), nb, heat_sum, stream);
     prof_end(5);
     if (!forces) {
@@ -774,7 +797,7 @@ struct cryo_solver {
       return;
     }
     prof_begin(6);
-    launch_forces(pv, gd, P, L, state.get(), n, csf_mode, cfg.surface_tension == CRYO_ST_AKINCI,
+    launch_forces(pv, gd, P, L, state.get(), nc, csf_mode, cfg.surface_tension == CRYO_ST_AKINCI,
                   cfg.density_mode == CRYO_DENSITY_CONTINUITY, P.thermal != 0, tiles_ok && tile_forces, stream);
     prof_end(6);
     check_launch();
@@ -784,7 +807,7 @@ struct cryo_solver {
   void end_step(int nb_pc, int nb_vent, bool advance) {
     const bool closed = cfg.thermo_mode == CRYO_CLOSED && has_vapor;
     int nb_inv = 0;
-    if (closed) nb_inv = launch_inventory(view(), state.get(), capacity, partials.get() + 5 * stride, stride, stream);
+    if (closed) nb_inv = launch_inventory(view(), state.get(), n_bound(), partials.get() + 5 * stride, stride, stream);
     launch_end_step(state.get(), partials.get(), stride, nb_pc, nb_vent, nb_inv, fluid_const(), closed, advance,
                     stream);
     check_launch();
@@ -806,23 +829,27 @@ struct cryo_solver {
     const GridDev gd = grid();
     PV pv = view();
     const NbrList L = lists();
-    nb_pc = launch_pc_normalize(pv, gd, P, L, n, partials.get(), stride, stream);
-    launch_pc_apply(pv, gd, P, L, n, stream);
+    const Count nc = cn();
+    nb_pc = launch_pc_normalize(pv, gd, P, L, nc, partials.get(), stride, stream);
+    launch_pc_apply(pv, gd, P, L, nc, stream);
     const bool rebalance = P.vapor_rebalance > 0.0f && has_vapor;
-    if (rebalance) launch_rebalance(pv, gd, P, L, n, stream);
-    if (static_cast<std::size_t>(n) * 4 > static_cast<std::size_t>(capacity) * 3) {
+    if (rebalance) launch_rebalance(pv, gd, P, L, nc, stream);
+    // (with device counts the run grew the arrays before it was queued)
+    if (!dev_count && static_cast<std::size_t>(n) * 4 > static_cast<std::size_t>(capacity) * 3) {
+      launch_pc_fold(state.get(), partials.get(), stride, nb_pc, stream);  // growing drops the partials
+      nb_pc = 0;
       grow(capacity * 3 / 2 + 1024);
       pv = view();
     }
-    launch_rebalance_apply_mark(pv, P, n, rebalance, split_flag.get(), stream);
-    launch_exclusive_scan(split_flag.get(), split_off.get(), n, scan_sums.get(), scan_total.get(), stream);
-    launch_split(pv, P, state.get(), n, split_flag.get(), split_off.get(), stream);
+    launch_rebalance_apply_mark(pv, P, nc, rebalance, split_flag.get(), stream);
+    launch_exclusive_scan(split_flag.get(), split_off.get(), nc, scan_sums.get(), scan_total.get(), stream);
+    launch_split(pv, P, state.get(), nc, split_flag.get(), split_off.get(), stream);
     lists_valid = false;  // split moves parents and appends particles
     tiles_ok = false;
-    launch_split_commit(state.get(), n, capacity, scan_total.get(), stream);
+    launch_split_commit(state.get(), nc, n_bound(), scan_total.get(), stream);
     nb_vent = 0;
     if (cfg.thermo_mode == CRYO_VENTED && has_vapor)
-      nb_vent = launch_venting(pv, P, state.get(), capacity, cfg.vent_relaxation, partials.get() + 3 * stride, stride,
+      nb_vent = launch_venting(pv, P, state.get(), n_bound(), cfg.vent_relaxation, partials.get() + 3 * stride, stride,
                                stream);
     check_launch();
   }
@@ -843,7 +870,8 @@ struct cryo_solver {
   // SimParams cannot change.
   struct StepGraph {
     hipGraphExec_t exec = nullptr;
-    int n = -1, pending_dead = -1, capacity = -1;
+    bool dev = false;  // captured with device counts (keyed by count_ub, not n)
+    int n = -1, pending_dead = -1, capacity = -1, ub = -1;
     int set_after = 0, n_after = 0, n_sorted = 0, pd_after = 0;
     const uint32_t* last_perm = nullptr;
     bool lists_valid = false, tiles_ok = false;
@@ -858,17 +886,21 @@ struct cryo_solver {
   }
   void step_issue_graph() {
     ensure_initialized();
-    const bool grows = cfg.phase_change != 0 &&
+    const bool grows = !dev_count && cfg.phase_change != 0 &&
                        static_cast<std::size_t>(n) * 4 > static_cast<std::size_t>(capacity) * 3;  // allocates
     if (!graphs_on || prof_mask != 0 || tape_slot || grows || n == 0) {
       step_issue();
       return;
     }
     StepGraph& G = sg[set];
-    if (!G.exec || G.n != n || G.pending_dead != pending_dead || G.capacity != capacity) {
+    const bool same = G.dev == dev_count && G.capacity == capacity &&
+                      (dev_count ? G.ub == count_ub : G.n == n && G.pending_dead == pending_dead);
+    if (!G.exec || !same) {
       if (G.exec) (void)hipGraphExecDestroy(G.exec);
       G = StepGraph{};
       const int n0 = n, pd0 = pending_dead, cap0 = capacity;
+      G.dev = dev_count;
+      G.ub = count_ub;
       CRYO_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
       try {
         step_issue();
@@ -906,12 +938,13 @@ struct cryo_solver {
     const bool pc = cfg.phase_change != 0;
     PV pv = view();
     prof_begin(7);
-    launch_kick_sources(pv, P, state.get(), n, true, P.thermal != 0, pc && !shifting, stream);
+    const Count nc = cn();
+    launch_kick_sources(pv, P, state.get(), nc, true, P.thermal != 0, pc && !shifting, stream);
     if (shifting) {
-      launch_shifting(pv, grid(), P, lists(), state.get(), n, stream);
+      launch_shifting(pv, grid(), P, lists(), state.get(), nc, stream);
       lists_valid = false;  // positions moved
       tiles_ok = false;
-      if (pc) launch_kick_sources(pv, P, state.get(), n, false, false, true, stream);
+      if (pc) launch_kick_sources(pv, P, state.get(), nc, false, false, true, stream);
     }
     prof_end(7);
     int nb_pc = 0, nb_vent = 0;
@@ -950,6 +983,65 @@ struct cryo_solver {
     merges_seen = hs->merges;
   }
 
+  // A run of steps queued with device counts (device_counts set, phase change
+  // on): count_begin() applies the host's growth rule once for the run, saves
+  // the state and switches the launches to the device count; count_end(),
+  // once they are queued, takes the counts back, or, when a step's splits
+  // passed the launch bound, restores the saved state and returns false (the
+  // caller then runs the same steps one at a time).
+  bool uses_device_counts() const { return device_counts && cfg.phase_change != 0; }
+  struct CountRun;  // the saved state (after Snap)
+  std::unique_ptr<CountRun> count_run;
+  void count_begin() {
+    ensure_initialized();
+    sync_state();
+    const int room = count_room > 0 ? count_room : std::max(1024, n / 8);
+    if (static_cast<std::size_t>(n) * 4 > static_cast<std::size_t>(capacity) * 3 ||
+        static_cast<long long>(n) + room > capacity)
+      grow(static_cast<int>(std::max<long long>(static_cast<long long>(capacity) * 3 / 2 + 1024,
+                                                static_cast<long long>(n) + room)));
+    if (hs->count_overflow) {
+      hs->count_overflow = 0;
+      push_state();
+    }
+    if (!count_run) count_run = std::make_unique<CountRun>();
+    CountRun& r = *count_run;
+    snap_save(r.z, true, nullptr);
+    snap_save_scratch(r.z);
+    r.pending_dead = pending_dead, r.merges_seen = merges_seen, r.n_sorted = n_sorted, r.last_perm = last_perm;
+    count_ub = n + room;
+    dev_count = true;
+  }
+  bool count_end() {
+    dev_count = false;
+    sync_state();
+    CountRun& r = *count_run;
+    if (hs->count_overflow) {
+      snap_load(r.z, true);
+      snap_load_scratch(r.z);
+      pending_dead = r.pending_dead, merges_seen = r.merges_seen, n_sorted = r.n_sorted, last_perm = r.last_perm;
+      sync_state();
+      ++count_restarts;
+      return false;
+    }
+    n = hs->n;
+    n_sorted = hs->n_sort;
+    pending_dead = hs->merges - hs->merges_sorted;
+    merges_seen = hs->merges;
+    return true;
+  }
+  void count_abort() {  // an error while the run was queued: take what the device holds
+    if (!dev_count) return;
+    dev_count = false;
+    if (hipMemcpyAsync(hs, state.get(), sizeof(DevState), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess)
+      return;
+    n = hs->n;
+    n_sorted = hs->n_sort;
+    pending_dead = hs->merges - hs->merges_sorted;
+    merges_seen = hs->merges;
+  }
+
   void scale_velocities(float f) { launch_scale_velocities(view(), n, f, stream); }
 
   void relax(double duration, float damping) {  // sph_solver.cu:707-736
@@ -980,6 +1072,7 @@ struct cryo_solver {
     hs->last_dt = 0;
     hs->heat_input = hs->evaporated = hs->condensed = hs->vented = hs->vented_h = 0;
     hs->splits = hs->merges = 0;
+    hs->merges_sorted = 0;
     merges_seen = 0;
     push_state();
     evaluate_rates();
@@ -1066,6 +1159,11 @@ struct cryo_solver {
       for (int k = 0; k < 14; ++k) d2d(f[k], z.sf[k].get(), z.n);
     }
   }
+  struct CountRun {
+    Snap z;
+    int pending_dead = 0, merges_seen = 0, n_sorted = 0;
+    const uint32_t* last_perm = nullptr;
+  };
   std::vector<std::unique_ptr<Snap>> adj_ckpt, adj_tape;
   std::unique_ptr<Snap> adj_end;
   DevBuf<float4> adj_c[2][4];  // two cotangent sets of (x, v, a, ab)
@@ -1991,29 +2089,47 @@ int cryo_solver_step(cryo_solver* s) { CRYO_GUARD(S(s).step()) }
 int cryo_solver_run_steps(cryo_solver* s, int64_t k) {
   CRYO_GUARD({
     cryo_solver& z = S(s);
-    if (!z.use_graphs) {
-      for (int64_t q = 0; q < k; ++q) z.step();
-    } else {
-      struct Off {
-        cryo_solver& z;
-        ~Off() {
-          (void)hipStreamSynchronize(z.stream);
-          z.graph_reset();
-          z.graphs_on = false;
-        }
-      } off{z};
-      z.ensure_initialized();
-      z.graphs_on = true;
+    struct Off {
+      cryo_solver& z;
+      ~Off() {
+        z.count_abort();
+        if (!z.graphs_on) return;
+        (void)hipStreamSynchronize(z.stream);
+        z.graph_reset();
+        z.graphs_on = false;
+      }
+    } off{z};
+    z.ensure_initialized();
+    z.graphs_on = z.use_graphs;
+    auto one_by_one = [&]() {
       for (int64_t q = 0; q < k; ++q) {
         z.step_issue_graph();
         z.step_finish();
       }
+    };
+    if (z.uses_device_counts() && k > 1) {
+      z.count_begin();
+      for (int64_t q = 0; q < k; ++q) z.step_issue_graph();
+      if (!z.count_end()) one_by_one();
+    } else {
+      one_by_one();
     }
     z.sync_state();
   })
 }
 int cryo_solver_set_launch_variant(cryo_solver* s, int32_t variant) { CRYO_GUARD(S(s).set_launch_variant(variant)) }
 int cryo_solver_set_step_graphs(cryo_solver* s, int32_t on) { CRYO_GUARD(S(s).use_graphs = on != 0) }
+int cryo_solver_set_device_counts(cryo_solver* s, int32_t on, int32_t room) {
+  if (room < 0) return fail("device counts: room must be >= 0", CRYO_ERR_INVALID_ARGUMENT);
+  CRYO_GUARD({
+    S(s).device_counts = on != 0;
+    S(s).count_room = room;
+  })
+}
+int cryo_solver_device_count_restarts(cryo_solver* s, int64_t* restarts) {
+  if (!restarts) return fail("restarts is required", CRYO_ERR_INVALID_ARGUMENT);
+  CRYO_GUARD(*restarts = S(s).count_restarts)
+}
 int cryo_solver_launch_variant(cryo_solver* s, int32_t* variant) {
   if (!variant) return fail("variant is required", CRYO_ERR_INVALID_ARGUMENT);
   CRYO_GUARD(*variant = S(s).launch_variant())
@@ -2076,10 +2192,19 @@ int cryo_ensemble_run_steps(cryo_solver* const* solvers, int32_t count, int64_t 
     }
     if (steps > 0) {
       std::vector<int64_t> issued(static_cast<std::size_t>(count), 0);
+      std::vector<char> dc(static_cast<std::size_t>(count), 0);  // queued with device counts
+      struct Abort {
+        cryo_solver* const* v;
+        int32_t c;
+        ~Abort() {
+          for (int32_t t = 0; t < c; ++t) v[t]->count_abort();
+        }
+      } abort_runs{solvers, count};
       int32_t waiting = 0;  // members with a step whose counts are not back yet
       for (int32_t t = 0; t < count; ++t) {
         cryo_solver& z = S(solvers[t]);
-        if (!z.needs_counts()) {
+        if (!z.needs_counts() || (z.uses_device_counts() && steps > 1)) {
+          if (z.needs_counts()) z.count_begin(), dc[t] = 1;
           for (int64_t q = 0; q < steps; ++q) z.step_issue_graph();
           issued[t] = steps;
           continue;
@@ -2090,7 +2215,7 @@ int cryo_ensemble_run_steps(cryo_solver* const* solvers, int32_t count, int64_t 
         ++waiting;
       }
       std::vector<char> pending(static_cast<std::size_t>(count), 0);
-      for (int32_t t = 0; t < count; ++t) pending[t] = S(solvers[t]).needs_counts() ? 1 : 0;
+      for (int32_t t = 0; t < count; ++t) pending[t] = S(solvers[t]).needs_counts() && !dc[t] ? 1 : 0;
       while (waiting > 0) {
         bool progressed = false;
         for (int32_t t = 0; t < count; ++t) {
@@ -2109,6 +2234,15 @@ int cryo_ensemble_run_steps(cryo_solver* const* solvers, int32_t count, int64_t 
           }
         }
         if (!progressed) std::this_thread::yield();
+      }
+      for (int32_t t = 0; t < count; ++t) {
+        if (!dc[t]) continue;
+        cryo_solver& z = S(solvers[t]);
+        if (z.count_end()) continue;
+        for (int64_t q = 0; q < steps; ++q) {  // its splits passed the bound: one step at a time
+          z.step_issue_graph();
+          z.step_finish();
+        }
       }
     }
     for (int32_t t = 0; t < count; ++t) S(solvers[t]).sync_state();

@@ -594,6 +594,14 @@ class HipSolver:
     def set_step_graphs(self, on):
         self._call("solver_set_step_graphs", self.h, C.c_int32(1 if on else 0))
 
+    def set_device_counts(self, on, room=0):
+        self._call("solver_set_device_counts", self.h, C.c_int32(1 if on else 0), C.c_int32(room))
+
+    def device_count_restarts(self):
+        r = C.c_int64()
+        self._call("solver_device_count_restarts", self.h, C.byref(r))
+        return r.value
+
     def launch_variant(self):
         v = C.c_int32()
         self._call("solver_launch_variant", self.h, C.byref(v))

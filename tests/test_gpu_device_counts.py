@@ -1,0 +1,129 @@
+"""Device counts (cryo_solver_set_device_counts): a call's phase-change steps
+queued back to back, with the particle count kept on the device (splits append
+there, the next cell sort drops merged particles there), give the same bits as
+the same steps with the count read back after each one.  Covered on a tank
+that splits and merges particles by the thousand and then grows several times
+over: the default bound, the bound passed on purpose (room 1: the call restores
+its start state and runs step by step), step graphs replayed with device
+counts, and the ensemble runner.
+"""
+import numpy as np
+import pytest
+
+import cabi
+from cabi import HipSolver
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("x", "v", "vt", "m", "m0", "rho", "T", "id", "phase", "a")
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def churn_tank(device_counts=False, room=0, graphs=False):
+    """A closed 4 cm LN2 box, liquid 3 K superheated, Lee 20 s^-1 both ways,
+    merge ratio 0.9 and split ratio 1.05: quiet for about 40 steps, then
+    vapour particles split and light ones merge by the thousand every ten
+    steps, and from about step 60 the count grows several times over (the
+    particle arrays grow, and device-count calls pass their launch bound)."""
+    fluid = cabi.hip_fluid_preset("LN2")
+    cfg = cabi.default_config(dx=0.004, gravity=(0, 0, -0.5), contact_angle_deg=10, velocity_scale=1.0,
+                              thermo_mode=1, lee_evaporation=20.0, lee_condensation=20.0, split_ratio=1.05,
+                              merge_ratio=0.9)
+    s = HipSolver(cfg, fluid)
+    s.build_tank(cabi.box(0.04, 0.04, 0.04), cabi.default_fill(T_liquid=fluid.T_sat + 3.0))
+    if device_counts:
+        s.set_device_counts(True, room)
+    if graphs:
+        s.set_step_graphs(True)
+    return s
+
+
+def same(sa, sb, where):
+    assert sa.num_particles == sb.num_particles, where
+    assert sa.step_count == sb.step_count and sa.time == sb.time, where
+    for f in FIELDS:
+        fa, fb = sa.field(f), sb.field(f)
+        assert np.array_equal(fa.view(np.uint8), fb.view(np.uint8)), (where, f)
+    assert bytes(sa.diagnostics()) == bytes(sb.diagnostics()), where
+
+
+@pytest.mark.parametrize("room,graphs", [(0, False), (1, False), (0, True)])
+def test_device_counts_give_the_same_bits(room, graphs):
+    say("test_device_counts_give_the_same_bits room", room, "graphs", graphs)
+    ref = churn_tank()
+    dev = churn_tank(True, room, graphs)
+    n0 = ref.num_particles
+    done = 0
+    for n in (30, 10, 10, 10, 4, 3, 3):
+        ref.run_steps(n)
+        dev.run_steps(n)
+        done += n
+        same(ref, dev, f"after {done} steps")
+        d = ref.diagnostics()
+        say(f"  step {done}: particles {ref.num_particles} splits {d.splits} merges {d.merges} "
+            f"restarts {dev.device_count_restarts()}")
+    d = ref.diagnostics()
+    assert d.splits > 0 and d.merges > 0 and ref.num_particles > 2 * n0
+    if room == 1:  # one particle of room: every call with two or more splits restarts
+        assert dev.device_count_restarts() > 0
+
+
+def test_device_counts_single_steps_and_calls_in_a_row():
+    """Plain step() calls between device-count calls, and one-step calls
+    (which read the count back as usual), keep the bits."""
+    say("test_device_counts_single_steps_and_calls_in_a_row")
+    ref, dev = churn_tank(), churn_tank(True)
+    for k, n in enumerate((20, 1, 20, 2, 11)):
+        ref.run_steps(n)
+        dev.run_steps(n)
+        dev.step()
+        ref.step()
+        same(ref, dev, f"call {k}")
+    d = ref.diagnostics()
+    assert d.splits > 0 and d.merges > 0
+
+
+def test_ensemble_with_device_counts_matches_the_per_tank_loop():
+    say("test_ensemble_with_device_counts_matches_the_per_tank_loop")
+    import sys
+    from pathlib import Path
+
+    sys.path.insert(0, str(Path(__file__).resolve().parent.parent))
+    import bench
+
+    tanks = sorted(bench.c4_tanks(cabi, count=12, seed=7), key=lambda t: len(t["ps"]))[:3]
+
+    def members(dc, graphs=False):
+        out = []
+        for t in tanks:
+            s = HipSolver(t["cfg"], t["fluid"])
+            s.set_particles(t["ps"], t["lo"], t["hi"])
+            s.set_frame_motion(acc=(t["a_lat"], 0.0, 0.0))
+            if dc:
+                s.set_device_counts(True)
+            if graphs:
+                s.set_step_graphs(True)
+            out.append(s)
+        out.append(churn_tank(dc, 0, graphs))
+        return out
+
+    ref = members(False)
+    for s in ref:
+        s.run_steps(56)
+    for graphs in (False, True):
+        ens = members(True, graphs)
+        cabi.hip_ensemble_run_steps(ens, 40)
+        cabi.hip_ensemble_run_steps(ens, 16)
+        for k, (sa, sb) in enumerate(zip(ref, ens)):
+            same(sa, sb, f"member {k} graphs {graphs}")
+    d = ref[-1].diagnostics()
+    assert d.splits > 0 and d.merges > 0
+
+
+def test_device_counts_reject_negative_room():
+    s = churn_tank()
+    with pytest.raises(ValueError):
+        s.set_device_counts(True, -1)
