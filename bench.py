@@ -216,6 +216,18 @@ def ensemble_measure(cabi, tanks, steps, tune_steps=3):
     t0 = time.perf_counter()
     cabi.hip_ensemble_run_steps(solvers, steps)
     ens_graphs = time.perf_counter() - t0
+    for s in solvers:  # counts kept on the device: no host round trip per step
+        s.set_step_graphs(False)
+        s.set_device_counts(True)
+    t0 = time.perf_counter()
+    with marker("ensemble_device_counts"):
+        cabi.hip_ensemble_run_steps(solvers, steps)
+    ens_dc = time.perf_counter() - t0
+    for s in solvers:  # and each member's step a replayed graph
+        s.set_step_graphs(True)
+    t0 = time.perf_counter()
+    cabi.hip_ensemble_run_steps(solvers, steps)
+    ens_dc_graphs = time.perf_counter() - t0
     n = sum(s.num_particles for s in solvers)
     del solvers
     # the ensemble at the accuracy the slosh-force bar asks for: every member
@@ -243,6 +255,8 @@ def ensemble_measure(cabi, tanks, steps, tune_steps=3):
             "per_tank_loop_tuned_ms_per_step": 1e3 * loop_tuned / steps,
             "ensemble_ms_per_step": 1e3 * ens / steps,
             "ensemble_step_graphs_ms_per_step": 1e3 * ens_graphs / steps,
+            "ensemble_device_counts_ms_per_step": 1e3 * ens_dc / steps,
+            "ensemble_device_counts_step_graphs_ms_per_step": 1e3 * ens_dc_graphs / steps,
             "speedup": loop / ens, "speedup_vs_tuned_loop": loop_tuned / ens,
             "ensemble_particle_steps_per_s": n * steps / ens,
             "precision_form_ensemble_ms_per_step": 1e3 * ens_prec / steps,
