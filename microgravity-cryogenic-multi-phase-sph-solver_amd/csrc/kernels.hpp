@@ -156,6 +156,8 @@ struct AdjArgs {
   float4* gbal = nullptr;
   // transport stress (P.tvf and P.tvf_stress): the rates' share of the vt cotangent
   float4* vtr = nullptr;
+  // smoothed normals: each particle's cotangent of its smoothing sum
+  float4* sacc = nullptr;
 };
 // Several independent device-to-device copies in one launch (16-byte aligned
 // pointers): the adjoint's snapshots.

@@ -297,6 +297,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_csf2(PV pv, GridDev g, SimParams
     AV xr{0.0f, 0.0f, 0.0f}, nsb{0.0f, 0.0f, 0.0f};
     float Vb = 0.0f, ww = 0.0f;
     V3 gradc{0.0f, 0.0f, 0.0f}, wall_n{0.0f, 0.0f, 0.0f};
+    const bool smooth = A.sacc != nullptr;
+    V3 sacc{0.0f, 0.0f, 0.0f};  // the smoothed normal's sum (k_smooth_normals), recomputed
     for_each_pair(L, g, k, xk, [&](int j) {
       const V3 xj = ld3(pv.s.x[j]);
       const V3 rkj = xk - xj;
@@ -311,8 +313,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_csf2(PV pv, GridDev g, SimParams
         if (P.wall_colour) gradc += kgradW(P.kernel, rkj, r) * (P.vol0 * (pv.t.alpha[j] - ck));
         return;
       }
-      if (j == k || !is_fluid(phj)) return;
+      if (!is_fluid(phj)) return;
       const float Vj = pv.s.m[j] / pv.s.rho[j];
+      if (smooth) {
+        const V3 gj = ld3(pv.s.nrm[j]);
+        const float gjm = norm(gj);
+        if (gjm > P.normal_eps) sacc += gj * (Vj * kW(P.kernel, r) / gjm);
+      }
+      if (j == k) return;
       gradc += kgradW(P.kernel, rkj, r) * (Vj * (colour(phj) - ck));
       const AV nsj = av(pv.t.ns[j]);
       if (!livek || !(cryo_adj::dot(nsj, nsj) > 0.25f)) return;
@@ -327,25 +335,101 @@ __global__ __launch_bounds__(kBlock) void k_adj_csf2(PV pv, GridDev g, SimParams
       nsb += o2.ns_j;
       Vb += o2.V_j;
     });
-    const float4 at = A.at[k];
-    AV nrmb = livek && !P.csf_balanced ? AV{at.x, at.y, at.z} * (coeff * csk.z) : AV{0.0f, 0.0f, 0.0f};
-    const AV nrk = av(pv.s.nrm[k]);
-    const float G = sqrtf(cryo_adj::dot(nrk, nrk));
-    if (G > P.normal_eps) nrmb += (nsb - nsk * cryo_adj::dot(nsk, nsb)) * (1.0f / G);  // ns = nrm / |nrm|
-    const cryo_adj::RotGrad<float> ro = cryo_adj::rot_vjp(av(gradc), av(wall_n), ww, P.cos_theta, P.sin_theta,
-                                                          P.wall_adh_w, P.normal_eps, P.wetting != 1 ? 1 : 0, nrmb);
-    A.gcb[k] = make_float4(ro.g.x, ro.g.y, ro.g.z, ro.ww);
-    A.wnb[k] = f4(ro.wall_n);
     A.csv[k] = Vb;
     A.xc[k] = f4(xr);
-    th[0] = ro.cos_t;
-    th[1] = ro.sin_t;
+    if (smooth) {  // ns = sacc / |sacc|: k_adj_csf3 goes on from its cotangent
+      const float Am = norm(sacc);
+      A.sacc[k] = Am > 1e-20f ? f4((nsb - nsk * cryo_adj::dot(nsk, nsb)) * (1.0f / Am))
+                              : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    } else {
+      const float4 at = A.at[k];
+      AV nrmb = livek && !P.csf_balanced ? AV{at.x, at.y, at.z} * (coeff * csk.z) : AV{0.0f, 0.0f, 0.0f};
+      const AV nrk = av(pv.s.nrm[k]);
+      const float G = sqrtf(cryo_adj::dot(nrk, nrk));
+      if (G > P.normal_eps) nrmb += (nsb - nsk * cryo_adj::dot(nsk, nsb)) * (1.0f / G);  // ns = nrm / |nrm|
+      const cryo_adj::RotGrad<float> ro = cryo_adj::rot_vjp(av(gradc), av(wall_n), ww, P.cos_theta, P.sin_theta,
+                                                            P.wall_adh_w, P.normal_eps, P.wetting != 1 ? 1 : 0, nrmb);
+      A.gcb[k] = make_float4(ro.g.x, ro.g.y, ro.g.z, ro.ww);
+      A.wnb[k] = f4(ro.wall_n);
+      th[0] = ro.cos_t;
+      th[1] = ro.sin_t;
+    }
   } else if (k < n) {
     const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     A.gcb[k] = z;
     A.wnb[k] = z;
     A.csv[k] = 0.0f;
     A.xc[k] = z;
+    if (A.sacc) A.sacc[k] = z;
+  }
+  block_sums_store<2>(th, A.partials + row * A.stride, A.stride);
+}
+
+// Smoothed normals (k_smooth_normals): ns_i = acc_i / |acc_i| with acc_i =
+// sum_j V_j W_ij nrm_j / |nrm_j| over fluid j with a live colour gradient (i
+// itself included); A.sacc holds each particle's cotangent of acc_i.  Per
+// fluid particle k, over both roles of its pairs: the cotangents of k's unit
+// colour gradient, V_k and x_k, then the normal cotangent and the
+// contact-angle rotation of k_adj_csf2 (its colour gradient recomputed).
+__global__ __launch_bounds__(kBlock) void k_adj_csf3(PV pv, GridDev g, SimParams P, NbrList L, AdjArgs A, int n,
+                                                    int row) {
+  const int k = gtid();
+  double th[2] = {0.0, 0.0};
+  if (k < n && is_fluid(pv.s.phase[k])) {
+    const float coeff = 2.0f * P.sigma / (P.rho0[0] + P.rho0[1]);
+    const V3 xk = ld3(pv.s.x[k]);
+    const AV gk = av(pv.s.nrm[k]);
+    const float G = sqrtf(cryo_adj::dot(gk, gk));
+    const bool live_g = G > P.normal_eps;
+    const AV ghk = live_g ? gk * (1.0f / G) : AV{0.0f, 0.0f, 0.0f};
+    const float Vk = pv.s.m[k] / pv.s.rho[k];
+    const float ck = colour(pv.s.phase[k]);
+    const AV sk = av(A.sacc[k]);
+    AV ghb{0.0f, 0.0f, 0.0f}, xr{0.0f, 0.0f, 0.0f};
+    float Vb = 0.0f, ww = 0.0f;
+    V3 gradc{0.0f, 0.0f, 0.0f}, wall_n{0.0f, 0.0f, 0.0f};
+    for_each_pair(L, g, k, xk, [&](int j) {
+      const V3 rkj = xk - ld3(pv.s.x[j]);
+      const float r2 = norm2(rkj);
+      if (!(r2 < P.support2)) return;
+      const float r = sqrtf(r2);
+      const uint8_t phj = pv.s.phase[j];
+      if (phj == kWall) {  // as k_adj_csf2
+        const float w = kW(P.kernel, r);
+        ww += P.vol0 * w;
+        wall_n += ld3(pv.s.nrm[j]) * (P.vol0 * w);
+        if (P.wall_colour) gradc += kgradW(P.kernel, rkj, r) * (P.vol0 * (pv.t.alpha[j] - ck));
+        return;
+      }
+      if (!is_fluid(phj)) return;
+      const float Vj = pv.s.m[j] / pv.s.rho[j];
+      if (j != k) gradc += kgradW(P.kernel, rkj, r) * (Vj * (colour(phj) - ck));
+      if (!live_g) return;
+      const AV gj = av(pv.s.nrm[j]);
+      const float gjm = sqrtf(cryo_adj::dot(gj, gj));
+      if (!(gjm > P.normal_eps)) return;
+      const AV ghj = gj * (1.0f / gjm);
+      const AV sj = av(A.sacc[j]);
+      const float w = kW(P.kernel, r);
+      ghb += sj * (Vk * w);                   // j's sum holds V_k W nrm_k / |nrm_k|
+      Vb += w * cryo_adj::dot(sj, ghk);
+      if (r > 0.0f) {                         // W_kj's position derivative, both sums
+        const float s = kdW(P.kernel, r) / r;
+        xr += av(rkj) * (s * (Vj * cryo_adj::dot(sk, ghj) + Vk * cryo_adj::dot(sj, ghk)));
+      }
+    });
+    const float4 at = A.at[k], csk = A.cs[k];
+    AV nrmb = live_normal(pv, k) && !P.csf_balanced ? AV{at.x, at.y, at.z} * (coeff * csk.z) : AV{0.0f, 0.0f, 0.0f};
+    if (live_g) nrmb += (ghb - ghk * cryo_adj::dot(ghk, ghb)) * (1.0f / G);
+    const cryo_adj::RotGrad<float> ro = cryo_adj::rot_vjp(av(gradc), av(wall_n), ww, P.cos_theta, P.sin_theta,
+                                                          P.wall_adh_w, P.normal_eps, P.wetting != 1 ? 1 : 0, nrmb);
+    A.gcb[k] = make_float4(ro.g.x, ro.g.y, ro.g.z, ro.ww);
+    A.wnb[k] = f4(ro.wall_n);
+    A.csv[k] += Vb;
+    const float4 x0 = A.xc[k];
+    A.xc[k] = make_float4(x0.x + xr.x, x0.y + xr.y, x0.z + xr.z, 0.0f);
+    th[0] = ro.cos_t;
+    th[1] = ro.sin_t;
   }
   block_sums_store<2>(th, A.partials + row * A.stride, A.stride);
 }
@@ -1199,6 +1283,7 @@ void launch_adj_csf_only(const PV& pv, const GridDev& g, const SimParams& P, con
   const int nb = blocks_for(n > 0 ? n : 1);
   k_adj_csf1<<<nb, kBlock, 0, s>>>(pv, g, P, L, A, n);
   k_adj_csf2<<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n, 6);
+  if (A.sacc) k_adj_csf3<<<nb, kBlock, 0, s>>>(pv, g, P, L, A, n, 6);
   k_adj_finish<<<1, 1024, 0, s>>>(A.partials, A.stride, nb, 6, -1, 1.0, theta_out, 2);
 }
 void launch_adj_step(const PV& pv, const GridDev& g, const SimParams& P, const NbrList& L, const DevState* st,
@@ -1218,6 +1303,7 @@ void launch_adj_step(const PV& pv, const GridDev& g, const SimParams& P, const N
   if (A.csf) {
     k_adj_csf1<<<nb, kBlock, 0, s>>>(pv, g, P, L, A, n);
     k_adj_csf2<<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n, 6);
+    if (A.sacc) k_adj_csf3<<<nb, kBlock, 0, s>>>(pv, g, P, L, A, n, 6);
     k_adj_finish<<<1, 1024, 0, s>>>(A.partials, A.stride, nb, 6, -1, 1.0, theta_out, 2);
   }
   if (A.tvf) k_adj_wall<true><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);

@@ -1814,9 +1814,8 @@ struct Solver {
     if (!(c.fixed_dt > 0)) throw std::invalid_argument("adjoint: needs fixed_dt > 0");
     if (c.density_mode != CRYO_DENSITY_SUMMATION) throw std::invalid_argument("adjoint: density summation only");
     if (c.surface_tension == CRYO_ST_AKINCI) throw std::invalid_argument("adjoint: Akinci surface tension not differentiated");
-    if (c.surface_tension == CRYO_ST_CSF &&
-        (c.csf_smooth_normals || c.wetting_model != CRYO_WET_NORMAL_ROTATION))
-      throw std::invalid_argument("adjoint: CSF only with unsmoothed normals and normal rotation");
+    if (c.surface_tension == CRYO_ST_CSF && c.wetting_model != CRYO_WET_NORMAL_ROTATION)
+      throw std::invalid_argument("adjoint: CSF only with the normal-rotation wetting model");
     if (c.sps_turbulence || c.particle_shifting)
       throw std::invalid_argument("adjoint: SPS and shifting not differentiated");
     if (has_maneuver) throw std::invalid_argument("adjoint: manoeuvre table not allowed (the frame is per-step input)");
@@ -1913,6 +1912,42 @@ struct Solver {
         });
       }
     }
+    // smoothed normals (smooth_normals above): ns_i = acc_i / |acc_i|,
+    // acc_i = sum_j V_j W_ij nrm_j / |nrm_j| over fluid j (i itself included)
+    // with a live colour gradient; ghb is the cotangent of nrm_j / |nrm_j|
+    std::vector<VR> ghb(P.csf_smooth ? n : 0, VR{0, 0, 0});
+    if (P.csf_smooth) {
+      for (int i = 0; i < n; ++i) {
+        if (!is_fluid(ps.phase[i]) || !(norm(ps.nrm[i]) > P.normal_eps)) continue;
+        auto each_live = [&](auto&& f) {
+          grid.each(ps.x[i], [&](int j) {
+            if (!is_fluid(ps.phase[j])) return;
+            const R gjm = norm(ps.nrm[j]);
+            if (!(gjm > P.normal_eps)) return;
+            const R r2 = norm2(ps.x[i] - ps.x[j]);
+            if (r2 >= P.support2) return;
+            f(j, gjm, std::sqrt(r2));
+          });
+        };
+        V3 acc{0, 0, 0};  // the forward's sum, recomputed
+        each_live([&](int j, R gjm, R r) { acc += ps.nrm[j] * (Vof(j) * kW(P.kernel, r) / gjm); });
+        const R A = norm(acc);
+        if (!(A > R(1e-20))) continue;
+        const VR ns = vr(ps.ns[i]);
+        const VR ab = (nsb[i] - ns * dot(ns, nsb[i])) * (R(1) / A);
+        each_live([&](int j, R gjm, R r) {
+          const VR gh = vr(ps.nrm[j]) * (R(1) / gjm);
+          const R Vj = Vof(j), w = kW(P.kernel, r), agh = dot(ab, gh);
+          ghb[j] += ab * (Vj * w);
+          Vb[j] += w * agh;
+          if (r > R(0)) {
+            const VR d = vr(ps.x[i] - ps.x[j]) * (kdW(P.kernel, r) / r * Vj * agh);
+            xr[i] += d;
+            xr[j] -= d;
+          }
+        });
+      }
+    }
     std::vector<VR> gcb(n, VR{0, 0, 0}), wnb(n, VR{0, 0, 0});
     std::vector<R> wwb(n, R(0));
     double cos_b = 0, sin_b = 0;
@@ -1921,9 +1956,14 @@ struct Solver {
       if (!is_fluid(ph)) continue;
       const V3 g0 = ps.nrm[i];
       const R G = norm(g0);
-      if (G > P.normal_eps) {  // ns = nrm / |nrm| (no smoothing)
-        const VR ns = vr(ps.ns[i]);
-        nrmb[i] += (nsb[i] - ns * dot(ns, nsb[i])) * (R(1) / G);
+      if (G > P.normal_eps) {
+        if (P.csf_smooth) {  // the smoothed normals' nrm / |nrm|
+          const VR gh = vr(g0) * (R(1) / G);
+          nrmb[i] += (ghb[i] - gh * dot(gh, ghb[i])) * (R(1) / G);
+        } else {  // ns = nrm / |nrm|
+          const VR ns = vr(ps.ns[i]);
+          nrmb[i] += (nsb[i] - ns * dot(ns, nsb[i])) * (R(1) / G);
+        }
       }
       // the colour gradient before the rotation, the wall normal sum and weight
       const R ci = ph == L ? R(1) : R(0);

@@ -193,6 +193,52 @@ def test_fp64_adjoint_with_balanced_csf_matches_central_differences(lee):
             assert out.lee[which] != 0 and abs(out.lee[which] / fd_lee - 1) <= 2e-4, (name, out.lee[which], fd_lee)
 
 
+@pytest.mark.parametrize("lee", [None, 50.0])
+def test_fp64_adjoint_with_smoothed_normals_matches_central_differences(lee):
+    """CSF with smoothed normals (each fluid particle's normal the normalised
+    kernel-weighted sum of its live neighbours' unit colour gradients, itself
+    included): acceleration, contact-angle and Lee gradients against central
+    differences of the FP64 forward."""
+    fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=10, csf=True, lee=lee)
+    cfg.csf_smooth_normals = 1
+    out = oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
+    grad, dtheta = out[1], out[3]
+    fd = np.zeros_like(grad)
+    for k in range(acc.shape[0]):
+        for c in range(3):
+            a1, a2 = acc.copy(), acc.copy()
+            a1[k, c] += STEP
+            a2[k, c] -= STEP
+            h = float(a1[k, c]) - float(a2[k, c])
+            fd[k, c] = (oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(a1, grad=False)[0] -
+                        oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(a2, grad=False)[0]) / h
+    err = np.abs(grad - fd).max() / np.abs(fd).max()
+    print("acc", err)
+    assert err <= (3e-7 if lee is None else 2e-9), err
+    # the smoothed loss bends more in theta: central differences with a step of
+    # 0.04, 0.01 and 0.0025 degrees are 1e-3, 5e-5 and 1.5e-7 from the adjoint
+    d, th0 = 0.0025, float(cfg.contact_angle_deg)
+    losses = []
+    for th in (th0 + d, th0 - d):
+        cfg.contact_angle_deg = th
+        losses.append(oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, grad=False)[0])
+    cfg.contact_angle_deg = th0
+    fd_theta = (losses[0] - losses[1]) / (np.float32(th0 + d) - np.float32(th0 - d))
+    print("theta", dtheta, fd_theta, abs(dtheta / fd_theta - 1))
+    assert dtheta != 0.0
+    assert abs(dtheta - fd_theta) <= 1e-5 * abs(fd_theta), (dtheta, fd_theta)
+    if lee is not None:
+        for which, name in enumerate(("lee_evaporation", "lee_condensation")):
+            vals = []
+            for sgn in (1, -1):
+                setattr(cfg, name, lee * (1 + sgn * 1e-3))
+                vals.append(oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, grad=False)[0])
+            setattr(cfg, name, lee)
+            fd_lee = (vals[0] - vals[1]) / (2 * lee * 1e-3)
+            print(name, out.lee[which], fd_lee)
+            assert out.lee[which] != 0 and abs(out.lee[which] / fd_lee - 1) <= 2e-4, (name, out.lee[which], fd_lee)
+
+
 def test_fp64_adjoint_with_the_cubic_spline_kernel_matches_central_differences():
     """The cubic-spline kernel (both of its branches) under every pair term of
     the case: CSF with the contact angle, the transport velocity, conduction and
