@@ -60,8 +60,7 @@ def test_loss_is_the_integrated_wall_load():
 
 def test_adjoint_rejects_undifferentiated_physics():
     for csf, field, value in ((False, "surface_tension", 2), (False, "sps_turbulence", 1), (False, "fixed_dt", 0.0),
-                              (True, "wetting_model", 1),
-                              (True, "csf_smooth_normals", 1)):
+                              (True, "wetting_model", 1), (False, "particle_shifting", 1)):
         fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=2, csf=csf)
         setattr(cfg, field, value)
         with pytest.raises(ValueError):
