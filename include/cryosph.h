@@ -253,13 +253,13 @@ int cryo_solver_autotune_tiles(cryo_solver* s, int32_t trial_steps, const int32_
  * either way.  Differentiated: density summation, EOS, Shepard wall pressure,
  * ghost velocity and wall colour, momentum (pressure, viscosity, artificial
  * viscosity, wall terms), CSF surface tension (colour gradient, contact-angle
- * rotation, plain or smoothed normals, curvature, classic or balanced force), the transport velocity and its
+ * rotation and/or contact-line force, plain or smoothed normals, curvature,
+ * classic or balanced force), the transport velocity and its
  * momentum stress, conduction and the
  * heater, Lee phase change (sources, receiver normalisation, apply, vapour
  * rebalance, venting, a closed tank's saturation-temperature feedback;
  * masses and temperatures are state), kick-drift-kick.  Requires
- * fixed_dt > 0 (either kernel: Wendland C2 or cubic spline), density summation, CSF only with
- * the normal-rotation wetting model (not the contact-line force), no SPS
+ * fixed_dt > 0 (either kernel: Wendland C2 or cubic spline), density summation, no SPS
  * or shifting, no manoeuvre table and no rotating frame (see
  * cryo_solver_adjoint_slosh_frame); a window in which a particle splits or
  * merges is not differentiable (CRYO_ERR_INVALID_ARGUMENT otherwise). */

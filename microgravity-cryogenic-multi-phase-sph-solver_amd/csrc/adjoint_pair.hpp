@@ -491,6 +491,62 @@ CRYO_ADJ_HD inline GcGrad<T> gc_fw_vjp(const Ctx<T>& c, Vec<T> rij, T alpha_w, T
   return o;
 }
 
+// The wall sum gradphi_I += gw vol0 (wall w) of the contact-line force:
+// gpb its cotangent, the share of r_IJ (walls do not move).
+template <class T>
+CRYO_ADJ_HD inline Vec<T> gphi_fw_vjp(const Ctx<T>& c, Vec<T> rij, Vec<T> gpb) {
+  const T r2 = dot(rij, rij);
+  if (!(r2 < c.support2)) return Vec<T>{T(0), T(0), T(0)};
+  const T r = std::sqrt(r2);
+  if (!(r > T(1e-12) * c.h)) return Vec<T>{T(0), T(0), T(0)};
+  const T s = kern_s(c, r), spr = kern_spr(c, r);
+  return (gpb * s + rij * (dot(gpb, rij) * spr)) * c.vol0;
+}
+
+// Contact-line force of a fluid particle near a wall (surface_tension.cu:5-85):
+//   G = |g|, nh = g/G, nw = wall_n/|wall_n|, d = nh.nw, t = nh - nw d,
+//   delta = 2 |gradphi|, vw = t f,  f = -(cos(theta) + d) coeff G delta / |t|
+// with g the colour gradient before the contact-angle rotation; applied when
+// ww > 0 && G > normal_eps && |wall_n| > 0 && |t| > 1e-6, else vw = 0.
+// vb: cotangent of vw.
+template <class T>
+struct ClGrad {
+  Vec<T> g, wall_n, gradphi;
+  T cos_t;
+};
+template <class T>
+CRYO_ADJ_HD inline ClGrad<T> cl_vjp(Vec<T> g, Vec<T> wall_n, Vec<T> gradphi, T ww, T cos_t, T coeff, T normal_eps,
+                                    Vec<T> vb) {
+  ClGrad<T> o{{T(0), T(0), T(0)}, {T(0), T(0), T(0)}, {T(0), T(0), T(0)}, T(0)};
+  const T G = std::sqrt(dot(g, g));
+  const T Wn = std::sqrt(dot(wall_n, wall_n));
+  if (!(ww > T(0) && G > normal_eps && Wn > T(1e-20))) return o;
+  const Vec<T> nw = wall_n * (T(1) / Wn);
+  const Vec<T> nh = g * (T(1) / G);
+  const T d = dot(nh, nw);
+  const Vec<T> t = nh - nw * d;
+  const T tm = std::sqrt(dot(t, t));
+  if (!(tm > T(1e-6))) return o;
+  const T P = std::sqrt(dot(gradphi, gradphi));
+  const T delta = T(2) * P;
+  const T base = -coeff / tm;                  // f = base (cos_t + d) G delta
+  const T f = base * (cos_t + d) * G * delta;
+  const T q = dot(vb, t);                      // cotangent of f
+  o.cos_t = q * base * G * delta;
+  const T db = q * base * G * delta;           // d f / d d, then the t path below
+  const T Gb = q * base * (cos_t + d) * delta;
+  const T deltab = q * base * (cos_t + d) * G;
+  if (P > T(0)) o.gradphi = gradphi * (T(2) * deltab / P);
+  // t = nh - nw d with |t| in f: tb = f vb - (f / tm) t (t / tm)
+  const Vec<T> tb = vb * f - t * (f * q / (tm * tm));
+  const T dbt = db - dot(tb, nw);              // d's cotangent, both paths
+  const Vec<T> nhb = tb + nw * dbt;
+  const Vec<T> nwb = nh * dbt - tb * d;
+  o.g = (nhb - nh * dot(nh, nhb)) * (T(1) / G) + nh * Gb;
+  o.wall_n = (nwb - nw * dot(nw, nwb)) * (T(1) / Wn);
+  return o;
+}
+
 // Contact-angle rotation of the colour gradient (surface_tension.cu:60-80):
 //   G = |g|, nh = g/G, nw = wall_n/|wall_n|, t = nh - nw (nh.nw),
 //   target = -cos(theta) nw + sin(theta) t/|t|, beta = clamp(ww / adh_w, 0, 1),

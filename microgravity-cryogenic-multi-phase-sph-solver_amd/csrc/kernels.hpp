@@ -158,6 +158,8 @@ struct AdjArgs {
   float4* vtr = nullptr;
   // smoothed normals: each particle's cotangent of its smoothing sum
   float4* sacc = nullptr;
+  // contact-line force: each fluid particle's cotangent of its wall-indicator gradient
+  float4* gpb = nullptr;
 };
 // Several independent device-to-device copies in one launch (16-byte aligned
 // pointers): the adjoint's snapshots.

@@ -299,6 +299,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_csf2(PV pv, GridDev g, SimParams
     V3 gradc{0.0f, 0.0f, 0.0f}, wall_n{0.0f, 0.0f, 0.0f};
     const bool smooth = A.sacc != nullptr;
     V3 sacc{0.0f, 0.0f, 0.0f};  // the smoothed normal's sum (k_smooth_normals), recomputed
+    const bool cl = A.gpb != nullptr && !smooth;  // the contact-line force (k_adj_csf3 with smoothing)
+    V3 gradphi{0.0f, 0.0f, 0.0f};
     for_each_pair(L, g, k, xk, [&](int j) {
       const V3 xj = ld3(pv.s.x[j]);
       const V3 rkj = xk - xj;
@@ -310,6 +312,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_csf2(PV pv, GridDev g, SimParams
         const float w = kW(P.kernel, r);
         ww += P.vol0 * w;
         wall_n += ld3(pv.s.nrm[j]) * (P.vol0 * w);
+        if (cl) gradphi += kgradW(P.kernel, rkj, r) * P.vol0;
         if (P.wall_colour) gradc += kgradW(P.kernel, rkj, r) * (P.vol0 * (pv.t.alpha[j] - ck));
         return;
       }
@@ -349,10 +352,21 @@ __global__ __launch_bounds__(kBlock) void k_adj_csf2(PV pv, GridDev g, SimParams
       if (G > P.normal_eps) nrmb += (nsb - nsk * cryo_adj::dot(nsk, nsb)) * (1.0f / G);  // ns = nrm / |nrm|
       const cryo_adj::RotGrad<float> ro = cryo_adj::rot_vjp(av(gradc), av(wall_n), ww, P.cos_theta, P.sin_theta,
                                                             P.wall_adh_w, P.normal_eps, P.wetting != 1 ? 1 : 0, nrmb);
-      A.gcb[k] = make_float4(ro.g.x, ro.g.y, ro.g.z, ro.ww);
-      A.wnb[k] = f4(ro.wall_n);
+      AV gcb = ro.g, wnb = ro.wall_n;
       th[0] = ro.cos_t;
       th[1] = ro.sin_t;
+      if (cl) {  // a += vw, the contact-line force of the interface pass
+        cryo_adj::ClGrad<float> o{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, 0.0f};
+        if (livek)
+          o = cryo_adj::cl_vjp(av(gradc), av(wall_n), av(gradphi), ww, P.cos_theta, coeff, P.normal_eps,
+                               AV{at.x, at.y, at.z});
+        gcb += o.g;
+        wnb += o.wall_n;
+        th[0] += o.cos_t;
+        A.gpb[k] = f4(o.gradphi);
+      }
+      A.gcb[k] = make_float4(gcb.x, gcb.y, gcb.z, ro.ww);
+      A.wnb[k] = f4(wnb);
     }
   } else if (k < n) {
     const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -361,6 +375,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_csf2(PV pv, GridDev g, SimParams
     A.csv[k] = 0.0f;
     A.xc[k] = z;
     if (A.sacc) A.sacc[k] = z;
+    if (A.gpb) A.gpb[k] = z;
   }
   block_sums_store<2>(th, A.partials + row * A.stride, A.stride);
 }
@@ -387,7 +402,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_csf3(PV pv, GridDev g, SimParams
     const AV sk = av(A.sacc[k]);
     AV ghb{0.0f, 0.0f, 0.0f}, xr{0.0f, 0.0f, 0.0f};
     float Vb = 0.0f, ww = 0.0f;
-    V3 gradc{0.0f, 0.0f, 0.0f}, wall_n{0.0f, 0.0f, 0.0f};
+    V3 gradc{0.0f, 0.0f, 0.0f}, wall_n{0.0f, 0.0f, 0.0f}, gradphi{0.0f, 0.0f, 0.0f};
+    const bool cl = A.gpb != nullptr;
     for_each_pair(L, g, k, xk, [&](int j) {
       const V3 rkj = xk - ld3(pv.s.x[j]);
       const float r2 = norm2(rkj);
@@ -398,6 +414,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_csf3(PV pv, GridDev g, SimParams
         const float w = kW(P.kernel, r);
         ww += P.vol0 * w;
         wall_n += ld3(pv.s.nrm[j]) * (P.vol0 * w);
+        if (cl) gradphi += kgradW(P.kernel, rkj, r) * P.vol0;
         if (P.wall_colour) gradc += kgradW(P.kernel, rkj, r) * (P.vol0 * (pv.t.alpha[j] - ck));
         return;
       }
@@ -419,17 +436,29 @@ __global__ __launch_bounds__(kBlock) void k_adj_csf3(PV pv, GridDev g, SimParams
       }
     });
     const float4 at = A.at[k], csk = A.cs[k];
-    AV nrmb = live_normal(pv, k) && !P.csf_balanced ? AV{at.x, at.y, at.z} * (coeff * csk.z) : AV{0.0f, 0.0f, 0.0f};
+    const bool livek = live_normal(pv, k);
+    AV nrmb = livek && !P.csf_balanced ? AV{at.x, at.y, at.z} * (coeff * csk.z) : AV{0.0f, 0.0f, 0.0f};
     if (live_g) nrmb += (ghb - ghk * cryo_adj::dot(ghk, ghb)) * (1.0f / G);
     const cryo_adj::RotGrad<float> ro = cryo_adj::rot_vjp(av(gradc), av(wall_n), ww, P.cos_theta, P.sin_theta,
                                                           P.wall_adh_w, P.normal_eps, P.wetting != 1 ? 1 : 0, nrmb);
-    A.gcb[k] = make_float4(ro.g.x, ro.g.y, ro.g.z, ro.ww);
-    A.wnb[k] = f4(ro.wall_n);
+    AV gcb = ro.g, wnb = ro.wall_n;
+    th[0] = ro.cos_t;
+    th[1] = ro.sin_t;
+    if (cl) {  // a += vw, the contact-line force of the interface pass
+      cryo_adj::ClGrad<float> o{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, 0.0f};
+      if (livek)
+        o = cryo_adj::cl_vjp(av(gradc), av(wall_n), av(gradphi), ww, P.cos_theta, coeff, P.normal_eps,
+                             AV{at.x, at.y, at.z});
+      gcb += o.g;
+      wnb += o.wall_n;
+      th[0] += o.cos_t;
+      A.gpb[k] = f4(o.gradphi);
+    }
+    A.gcb[k] = make_float4(gcb.x, gcb.y, gcb.z, ro.ww);
+    A.wnb[k] = f4(wnb);
     A.csv[k] += Vb;
     const float4 x0 = A.xc[k];
     A.xc[k] = make_float4(x0.x + xr.x, x0.y + xr.y, x0.z + xr.z, 0.0f);
-    th[0] = ro.cos_t;
-    th[1] = ro.sin_t;
   }
   block_sums_store<2>(th, A.partials + row * A.stride, A.stride);
 }
@@ -516,9 +545,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_wall(PV pv, GridDev g, SimParams
 
 // TV: the transport-velocity terms are compiled in (launched so when A.tvf
 // is set); without them the kernel holds fewer registers.
-// EX: the transport stress, the balanced CSF force or the cubic-spline kernel
-// is on; the other instances fix Wendland C2 at compile time and hold none of
-// those terms' registers.
+// EX: the transport stress, the balanced CSF force, the contact-line force or
+// the cubic-spline kernel is on; the other instances fix Wendland C2 at
+// compile time and hold none of those terms' registers.
 template <bool TV, bool EX>
 __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParams P, NbrList L, const DevState* st,
                                                      AdjArgs A, int n) {
@@ -567,6 +596,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
   const bool bal = EX && A.csf && P.csf_balanced;
   const float4 gbi4 = bal ? A.gbal[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   const AV Gbi{gbi4.x, gbi4.y, gbi4.z};
+  AV gpbk{0.0f, 0.0f, 0.0f};
+  if (EX && A.gpb) gpbk = av(A.gpb[i]);
   if (A.csf) {
     const float4 gb = A.gcb[i], wb = A.wnb[i];
     gbk = AV{gb.x, gb.y, gb.z};
@@ -630,6 +661,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_fluid(PV pv, GridDev g, SimParam
         const cryo_adj::GcGrad<float> og = cryo_adj::gc_fw_vjp(c, I.x - xw, alpha_w, ci, av(pv.s.nrm[j]),
                                                                P.wall_colour, gbk, wbk, wwbk);
         xr += og.r;
+        if (EX && A.gpb) xr += cryo_adj::gphi_fw_vjp(c, I.x - xw, gpbk);  // the contact-line force's gradphi
         const AV rwk = xw - I.x;
         const float r2 = cryo_adj::dot(rwk, rwk);
         if (cv.w != 0.0f && r2 < P.support2)  // alpha_w = sw0 / swt: d/dx_k = (c_k - alpha_w) / swt (-s r_wk)
@@ -1308,7 +1340,7 @@ void launch_adj_step(const PV& pv, const GridDev& g, const SimParams& P, const N
   }
   if (A.tvf) k_adj_wall<true><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
   else k_adj_wall<false><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
-  const bool ex = A.vtr != nullptr || A.gbal != nullptr || P.kernel.type != 1;
+  const bool ex = A.vtr != nullptr || A.gbal != nullptr || A.gpb != nullptr || P.kernel.type != 1;
   if (A.tvf) {
     if (ex) k_adj_fluid<true, true><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);
     else k_adj_fluid<true, false><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n);

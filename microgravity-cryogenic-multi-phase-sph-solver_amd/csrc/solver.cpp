@@ -1167,7 +1167,7 @@ struct cryo_solver {
   std::vector<std::unique_ptr<Snap>> adj_ckpt, adj_tape;
   std::unique_ptr<Snap> adj_end;
   DevBuf<float4> adj_c[2][4];  // two cotangent sets of (x, v, a, ab)
-  DevBuf<float4> adj_at, adj_wcp, adj_wcv, adj_xr, adj_vr, adj_cs, adj_gcb, adj_wnb, adj_xc, adj_tvs, adj_gbal, adj_sacc,
+  DevBuf<float4> adj_at, adj_wcp, adj_wcv, adj_xr, adj_vr, adj_cs, adj_gcb, adj_wnb, adj_xc, adj_tvs, adj_gbal, adj_sacc, adj_gpb,
       adj_vtr;
   DevBuf<float> adj_sb, adj_acc, adj_csv;
   DevBuf<double> adj_S, adj_G, adj_T, adj_part;
@@ -1261,6 +1261,7 @@ struct cryo_solver {
               adj_tvs.get()};
     A.gbal = csf && P.csf_balanced ? adj_gbal.get() : nullptr;
     A.sacc = csf && P.csf_smooth ? adj_sacc.get() : nullptr;
+    A.gpb = csf && P.wetting != 0 ? adj_gpb.get() : nullptr;
     A.vtr = P.tvf && P.tvf_stress ? adj_vtr.get() : nullptr;
     A.heat = adj_heat() ? 1 : 0;
     A.pc = adj_pc() ? 1 : 0;
@@ -1318,8 +1319,6 @@ struct cryo_solver {
     if (!(c.fixed_dt > 0.0f)) throw std::invalid_argument("adjoint: needs fixed_dt > 0");
     if (c.density_mode != CRYO_DENSITY_SUMMATION) throw std::invalid_argument("adjoint: density summation only");
     if (c.surface_tension == CRYO_ST_AKINCI) throw std::invalid_argument("adjoint: Akinci surface tension not differentiated");
-    if (c.surface_tension == CRYO_ST_CSF && c.wetting_model != CRYO_WET_NORMAL_ROTATION)
-      throw std::invalid_argument("adjoint: CSF only with the normal-rotation wetting model");
     if (c.sps_turbulence || c.particle_shifting)
       throw std::invalid_argument("adjoint: SPS and shifting not differentiated");
     if (has_maneuver) throw std::invalid_argument("adjoint: manoeuvre table not allowed (the frame is per-step input)");
@@ -1420,6 +1419,7 @@ struct cryo_solver {
       if (csf) fit(adj_cs, m), fit(adj_csv, m), fit(adj_gcb, m), fit(adj_wnb, m), fit(adj_xc, m);
       if (csf && P.csf_balanced) fit(adj_gbal, m);
       if (csf && P.csf_smooth) fit(adj_sacc, m);
+      if (csf && P.wetting != 0) fit(adj_gpb, m);
       if (P.tvf && P.tvf_stress) fit(adj_vtr, m);
       fit(adj_tvs, m);
       launch_adj_fill(adj_c[0][0].get(), adj_c[0][1].get(), adj_c[0][2].get(), n, stream);

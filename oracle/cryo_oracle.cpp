@@ -1814,8 +1814,6 @@ struct Solver {
     if (!(c.fixed_dt > 0)) throw std::invalid_argument("adjoint: needs fixed_dt > 0");
     if (c.density_mode != CRYO_DENSITY_SUMMATION) throw std::invalid_argument("adjoint: density summation only");
     if (c.surface_tension == CRYO_ST_AKINCI) throw std::invalid_argument("adjoint: Akinci surface tension not differentiated");
-    if (c.surface_tension == CRYO_ST_CSF && c.wetting_model != CRYO_WET_NORMAL_ROTATION)
-      throw std::invalid_argument("adjoint: CSF only with the normal-rotation wetting model");
     if (c.sps_turbulence || c.particle_shifting)
       throw std::invalid_argument("adjoint: SPS and shifting not differentiated");
     if (has_maneuver) throw std::invalid_argument("adjoint: manoeuvre table not allowed (the frame is per-step input)");
@@ -1948,7 +1946,7 @@ struct Solver {
         });
       }
     }
-    std::vector<VR> gcb(n, VR{0, 0, 0}), wnb(n, VR{0, 0, 0});
+    std::vector<VR> gcb(n, VR{0, 0, 0}), wnb(n, VR{0, 0, 0}), gpb(n, VR{0, 0, 0});
     std::vector<R> wwb(n, R(0));
     double cos_b = 0, sin_b = 0;
     for (int i = 0; i < n; ++i) {
@@ -1967,7 +1965,7 @@ struct Solver {
       }
       // the colour gradient before the rotation, the wall normal sum and weight
       const R ci = ph == L ? R(1) : R(0);
-      V3 gradc{0, 0, 0}, wall_n{0, 0, 0};
+      V3 gradc{0, 0, 0}, wall_n{0, 0, 0}, gradphi{0, 0, 0};
       R ww = 0;
       grid.each(ps.x[i], [&](int j) {
         const V3 rij = ps.x[i] - ps.x[j];
@@ -1978,6 +1976,7 @@ struct Solver {
         if (ps.phase[j] == W) {
           ww += P.vol0 * w;
           wall_n += ps.nrm[j] * (P.vol0 * w);
+          gradphi += kgradW(P.kernel, rij, r) * P.vol0;
           if (P.wall_colour) gradc += kgradW(P.kernel, rij, r) * (P.vol0 * (ps.alpha[j] - ci));
           return;
         }
@@ -1992,6 +1991,13 @@ struct Solver {
       wwb[i] = ro.ww;
       cos_b += double(ro.cos_t);
       sin_b += double(ro.sin_t);
+      if (P.wetting != 0 && live(i)) {  // a += vw, the contact-line force (interface pass)
+        const ClGrad<R> cl = cl_vjp(vr(gradc), vr(wall_n), vr(gradphi), ww, P.cos_theta, coeff, P.normal_eps, At[i]);
+        gcb[i] += cl.g;
+        wnb[i] += cl.wall_n;
+        gpb[i] = cl.gradphi;
+        cos_b += double(cl.cos_t);
+      }
     }
     theta_b = (cos_b * -double(P.sin_theta) + sin_b * double(P.cos_theta)) * (3.14159265358979323846 / 180.0);
     for (int i = 0; i < n; ++i) {
@@ -2003,6 +2009,7 @@ struct Solver {
           const GcGrad<R> o = gc_fw_vjp(c, vr(ps.x[i] - ps.x[j]), ps.alpha[j], ci, vr(ps.nrm[j]), P.wall_colour,
                                         gcb[i], wnb[i], wwb[i]);
           xr[i] += o.r;
+          if (P.wetting != 0) xr[i] += gphi_fw_vjp(c, vr(ps.x[i] - ps.x[j]), gpb[i]);
           alphab[j] += o.V_j;
           return;
         }

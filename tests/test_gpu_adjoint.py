@@ -210,6 +210,36 @@ def test_smoothed_normal_gradients_match_cpu_adjoints(lee):
     assert np.array_equal(again[1], out[1]) and again[3] == out[3]
 
 
+@pytest.mark.parametrize("wetting,smooth", [(1, 0), (2, 0), (2, 1)])
+def test_contact_line_force_gradients_match_cpu_adjoints(wetting, smooth):
+    """The contact-line force (wetting model 1; 2: with the normal rotation;
+    and with smoothed normals) under conduction and Lee phase change: the
+    GPU's acceleration, contact-angle and Lee gradients against the FP32 CPU
+    adjoint (same arithmetic, serial sums) and the FP64 one (pinned by central
+    differences in test_oracle_adjoint.py)."""
+    say("test_contact_line_force_gradients_match_cpu_adjoints", wetting, smooth)
+    fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=10, csf=True, lee=50.0)
+    cfg.wetting_model = wetting
+    cfg.csf_smooth_normals = smooth
+    out = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=4)
+    out64 = orc(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
+    o32 = CpuSolver("orc", cfg, fluid, 32)
+    o32.set_particles(ps, lo, hi)
+    out32 = o32.adjoint_slosh(acc)
+    err = np.abs(out[1] - out64[1]).max() / np.abs(out64[1]).max()
+    err32 = np.abs(out[1] - out32[1]).max() / np.abs(out32[1]).max()
+    lee64 = [abs(out.lee[q] / out64.lee[q] - 1) for q in (0, 1)]
+    lee32 = [abs(out.lee[q] / out32.lee[q] - 1) for q in (0, 1)]
+    say("loss", out[0], "fp64", out64[0], "acc gradient error", err, "vs fp32 cpu", err32, "dL/dtheta", out[3],
+        "fp64", out64[3], "fp32", out32[3], "lee rel fp32", lee32, "rel fp64", lee64)
+    assert abs(out[0] - out64[0]) <= 1e-4 * out64[0]
+    assert err <= 1e-3 and err32 <= 1e-4
+    assert abs(out[3] - out64[3]) <= 1e-3 * abs(out64[3]) and abs(out[3] - out32[3]) <= 1e-4 * abs(out32[3])
+    assert max(lee32) <= 2e-4 and max(lee64) <= 5e-3
+    again = hip(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, checkpoint_every=10)
+    assert np.array_equal(again[1], out[1]) and again[3] == out[3]
+
+
 def test_cubic_spline_gradients_match_cpu_adjoints():
     """The cubic-spline kernel under CSF, the transport velocity, conduction and
     Lee phase change: the GPU's acceleration, contact-angle and Lee gradients
@@ -329,7 +359,7 @@ def test_checkpoint_interval_does_not_change_the_gradient():
 def test_adjoint_rejects_undifferentiated_physics_and_bad_arguments():
     say("test_adjoint_rejects_undifferentiated_physics_and_bad_arguments")
     for csf, field, value in ((False, "surface_tension", 2), (False, "sps_turbulence", 1), (False, "fixed_dt", 0.0),
-                              (False, "particle_shifting", 1), (True, "wetting_model", 1)):
+                              (False, "particle_shifting", 1)):
         fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=2, csf=csf)
         setattr(cfg, field, value)
         with pytest.raises(ValueError):

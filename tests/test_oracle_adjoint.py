@@ -60,7 +60,7 @@ def test_loss_is_the_integrated_wall_load():
 
 def test_adjoint_rejects_undifferentiated_physics():
     for csf, field, value in ((False, "surface_tension", 2), (False, "sps_turbulence", 1), (False, "fixed_dt", 0.0),
-                              (True, "wetting_model", 1), (False, "particle_shifting", 1)):
+                              (False, "particle_shifting", 1)):
         fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=2, csf=csf)
         setattr(cfg, field, value)
         with pytest.raises(ValueError):
@@ -236,6 +236,54 @@ def test_fp64_adjoint_with_smoothed_normals_matches_central_differences(lee):
             fd_lee = (vals[0] - vals[1]) / (2 * lee * 1e-3)
             print(name, out.lee[which], fd_lee)
             assert out.lee[which] != 0 and abs(out.lee[which] / fd_lee - 1) <= 2e-4, (name, out.lee[which], fd_lee)
+
+
+@pytest.mark.parametrize("wetting", [1, 2])
+def test_fp64_adjoint_with_the_contact_line_force_matches_central_differences(wetting):
+    """The contact-line force of fluid particles near a wall (wetting model 1;
+    2: with the normal rotation too), a += t f with t the wall-tangent part of
+    the colour gradient's direction and f proportional to (cos theta - cos of
+    the current angle), the gradient's size and the wall-indicator gradient:
+    acceleration and contact-angle gradients against central differences of the
+    FP64 forward, with conduction and phase change."""
+    fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=10, csf=True, lee=50.0)
+    cfg.wetting_model = wetting
+    out = oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
+    grad, dtheta = out[1], out[3]
+    fd = np.zeros_like(grad)
+    for k in range(acc.shape[0]):
+        for c in range(3):
+            a1, a2 = acc.copy(), acc.copy()
+            a1[k, c] += STEP
+            a2[k, c] -= STEP
+            h = float(a1[k, c]) - float(a2[k, c])
+            fd[k, c] = (oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(a1, grad=False)[0] -
+                        oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(a2, grad=False)[0]) / h
+    err = np.abs(grad - fd).max() / np.abs(fd).max()
+    print("acc", err)
+    assert err <= 2e-9, err
+    d, th0 = 0.0025, float(cfg.contact_angle_deg)
+    losses = []
+    for th in (th0 + d, th0 - d):
+        cfg.contact_angle_deg = th
+        losses.append(oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, grad=False)[0])
+    cfg.contact_angle_deg = th0
+    fd_theta = (losses[0] - losses[1]) / (np.float32(th0 + d) - np.float32(th0 - d))
+    print("theta", dtheta, fd_theta, abs(dtheta / fd_theta - 1))
+    assert dtheta != 0.0
+    assert abs(dtheta - fd_theta) <= 1e-5 * abs(fd_theta), (dtheta, fd_theta)
+    # Lee: with both wetting terms the loss bends more; a relative step of 1e-3
+    # is 2.8e-4 from the adjoint, 3e-4 is 3e-5 and 3e-5 is 1.1e-5
+    rel = 3e-4
+    for which, name in enumerate(("lee_evaporation", "lee_condensation")):
+        vals = []
+        for sgn in (1, -1):
+            setattr(cfg, name, 50.0 * (1 + sgn * rel))
+            vals.append(oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(acc, grad=False)[0])
+        setattr(cfg, name, 50.0)
+        fd_lee = (vals[0] - vals[1]) / (float(np.float32(50.0 * (1 + rel))) - float(np.float32(50.0 * (1 - rel))))
+        print(name, out.lee[which], fd_lee)
+        assert out.lee[which] != 0 and abs(out.lee[which] / fd_lee - 1) <= 2e-4, (name, out.lee[which], fd_lee)
 
 
 def test_fp64_adjoint_with_the_cubic_spline_kernel_matches_central_differences():
