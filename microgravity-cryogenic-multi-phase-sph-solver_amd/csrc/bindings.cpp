@@ -759,6 +759,21 @@ PYBIND11_MODULE(CryoSPH, m) {
          "Times trial_steps steps of each cell-tile shape (cells along x, y, z; default: six built-in "
          "candidates) at the current launch variant, keeps the fastest, restores the state; returns "
          "(shape, ms per step of each shape).  Results do not depend on the shape.")
+      // Extension (cryo_solver_set_step_graphs / set_device_counts): no counterpart in the reference.
+      .def("set_step_graphs", [](Solver& S, bool on) { check(cryo_solver_set_step_graphs(S.s, on ? 1 : 0)); },
+           py::arg("on"),
+           "run_steps replays the step as a captured HIP graph for the rest of each call (same results).")
+      .def("set_device_counts", [](Solver& S, bool on, int room) {
+        check(cryo_solver_set_device_counts(S.s, on ? 1 : 0, room));
+      }, py::arg("on"), py::arg("room") = 0,
+         "run_steps queues a call's phase-change steps with the particle count kept on the device; launches "
+         "are sized for the count plus `room` particles (0: max(1024, n/8)), a call whose splits pass it is "
+         "restored and run step by step (same results).")
+      .def_property_readonly("device_count_restarts", [](Solver& S) {
+        int64_t r = 0;
+        check(cryo_solver_device_count_restarts(S.s, &r));
+        return r;
+      })
       // Extension (cryo_solver_adjoint_slosh): no counterpart in the reference.
       .def("adjoint_slosh", [](Solver& S, DArr acc, int checkpoint_every, bool gradient) -> py::tuple {
         if (acc.ndim() != 2 || acc.shape(1) != 3) throw py::value_error("frame_acc must have shape (steps, 3)");

@@ -152,6 +152,34 @@ def test_module_launch_tuning_keeps_the_results():
         assert np.array_equal(pa[k].view(np.uint8), pb[k].view(np.uint8)), k
 
 
+def test_module_device_counts_and_step_graphs_keep_the_results():
+    """The module's set_device_counts and set_step_graphs: a closed tank with
+    phase change and splitting, run in calls of several steps with the count
+    on the device and the steps replayed as graphs, stays bit for bit the
+    solver that reads the count back every step."""
+    say("test_module_device_counts_and_step_graphs_keep_the_results")
+    ln2 = cs.FluidProperties.liquid_nitrogen()
+    mcfg, _ = configs(dx=0.004, gravity=(0.0, 0.0, -0.5), velocity_scale=1.0, thermo_mode=1, lee_evaporation=20.0,
+                      lee_condensation=20.0, split_ratio=1.05, merge_ratio=0.9)
+    fill, _ = fills(T_liquid=ln2.T_sat + 3.0)
+    a, b = (cs.Solver(mcfg, ln2) for _ in range(2))
+    for s in (a, b):
+        s.build_tank(cs.BoxTank(0.04, 0.04, 0.04), fill)
+    b.set_device_counts(True)
+    b.set_step_graphs(True)
+    with pytest.raises(ValueError):
+        b.set_device_counts(True, -1)
+    for n in (30, 15, 10):
+        for s in (a, b):
+            s.run_steps(n)
+        pa, pb = a.particles(), b.particles()
+        for k in pa:
+            assert np.array_equal(pa[k].view(np.uint8), pb[k].view(np.uint8)), (n, k)
+    d = a.diagnostics()
+    say("  splits", d.splits, "merges", d.merges, "restarts", b.device_count_restarts)
+    assert d.splits > 0 and d.merges > 0 and b.device_count_restarts >= 0
+
+
 def test_closed_tank_mass_and_energy_budgets():
     """Closed LN2 capsule, superheated liquid, Lee 10, 120 steps: mass to
     2.5e-6, enthalpy change = heater input within 1 % of the latent heat
