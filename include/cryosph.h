@@ -208,9 +208,10 @@ int cryo_solver_set_step_graphs(cryo_solver* s, int32_t on);
  * kernels read the count when they run; splits append, a cell sort drops
  * merged particles, all on the device, and the host takes the counts once at
  * the end.  If a step's splits would pass that bound, the call restores the
- * state it started from and runs the steps one at a time instead.  Results
- * are the same bits either way.  Default off.  Fails with
- * CRYO_ERR_INVALID_ARGUMENT for room < 0. */
+ * state it started from and runs the steps one at a time instead (the solver
+ * keeps a device copy of its particle state for that, about the size of the
+ * state itself).  Results are the same bits either way.  Default off.  Fails
+ * with CRYO_ERR_INVALID_ARGUMENT for room < 0. */
 int cryo_solver_set_device_counts(cryo_solver* s, int32_t on, int32_t room);
 /* How many such calls passed their bound and ran step by step (since the
  * solver was created): a count that grows says `room` is too small. */
