@@ -22,7 +22,7 @@ def say(*a):
     print(*a, flush=True)
 
 
-def churn_tank(device_counts=False, room=0, graphs=False):
+def churn_tank(device_counts=False, room=0, graphs=False, **extra):
     """A closed 4 cm LN2 box, liquid 3 K superheated, Lee 20 s^-1 both ways,
     merge ratio 0.9 and split ratio 1.05: quiet for about 40 steps, then
     vapour particles split and light ones merge by the thousand every ten
@@ -32,6 +32,8 @@ def churn_tank(device_counts=False, room=0, graphs=False):
     cfg = cabi.default_config(dx=0.004, gravity=(0, 0, -0.5), contact_angle_deg=10, velocity_scale=1.0,
                               thermo_mode=1, lee_evaporation=20.0, lee_condensation=20.0, split_ratio=1.05,
                               merge_ratio=0.9)
+    for k, v in extra.items():
+        setattr(cfg, k, v)
     s = HipSolver(cfg, fluid)
     s.build_tank(cabi.box(0.04, 0.04, 0.04), cabi.default_fill(T_liquid=fluid.T_sat + 3.0))
     if device_counts:
@@ -69,6 +71,30 @@ def test_device_counts_give_the_same_bits(room, graphs):
     assert d.splits > 0 and d.merges > 0 and ref.num_particles > 2 * n0
     if room == 1:  # one particle of room: every call with two or more splits restarts
         assert dev.device_count_restarts() > 0
+
+
+@pytest.mark.parametrize("extra", [dict(compensated_sums=1), dict(particle_shifting=1, shifting_strength=0.5),
+                                   dict(thermo_mode=0, vent_relaxation=0.05)],
+                         ids=["precision_form", "shifting", "vented"])
+def test_device_counts_other_step_paths(extra):
+    """The precision form (double-float positions in the saved state), particle
+    shifting (its second source pass) and a vented tank (venting launched over
+    the bound): device counts keep the bits through the churn."""
+    say("test_device_counts_other_step_paths", extra)
+    ref, dev = churn_tank(**extra), churn_tank(True, **extra)
+    done = 0
+    for n in (30, 15, 15):
+        ref.run_steps(n)
+        dev.run_steps(n)
+        done += n
+        same(ref, dev, f"{extra} after {done} steps")
+    d = ref.diagnostics()
+    say(f"  particles {ref.num_particles} splits {d.splits} merges {d.merges} vented {d.vented_mass} "
+        f"restarts {dev.device_count_restarts()}")
+    if "thermo_mode" in extra:  # the open tank vents its vapour instead of splitting it
+        assert d.vented_mass > 0
+    else:
+        assert d.splits > 0 and d.merges > 0
 
 
 def test_device_counts_single_steps_and_calls_in_a_row():
