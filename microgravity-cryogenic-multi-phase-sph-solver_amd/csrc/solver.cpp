@@ -1030,6 +1030,63 @@ struct cryo_solver {
     merges_seen = hs->merges;
     return true;
   }
+  // The whole-ensemble graph of cryo_ensemble_run_steps (this solver is member
+  // 0; every member initialised, none needing its count back between steps).
+  void ensemble_graph(cryo_solver* const* v, int32_t count, int64_t steps) {
+    std::vector<hipEvent_t> ev(static_cast<std::size_t>(count) + 1, nullptr);
+    struct Events {
+      std::vector<hipEvent_t>& e;
+      ~Events() {
+        for (hipEvent_t x : e)
+          if (x) (void)hipEventDestroy(x);
+      }
+    } owned_events{ev};
+    for (hipEvent_t& e : ev) CRYO_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    hipEvent_t fork = ev[static_cast<std::size_t>(count)];
+    for (int32_t t = 0; t < count; ++t) CRYO_HIP(hipStreamSynchronize(v[t]->stream));  // saved states copied
+    CRYO_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+      CRYO_HIP(hipEventRecord(fork, stream));
+      for (int32_t t = 0; t < count; ++t) {
+        cryo_solver& z = *v[t];
+        if (z.stream != stream) CRYO_HIP(hipStreamWaitEvent(z.stream, fork, 0));
+        z.step_issue();
+        z.step_issue();
+        if (z.stream != stream) {
+          CRYO_HIP(hipEventRecord(ev[static_cast<std::size_t>(t)], z.stream));
+          CRYO_HIP(hipStreamWaitEvent(stream, ev[static_cast<std::size_t>(t)], 0));
+        }
+      }
+    } catch (...) {
+      hipGraph_t partial = nullptr;
+      (void)hipStreamEndCapture(stream, &partial);
+      if (partial) (void)hipGraphDestroy(partial);
+      throw;
+    }
+    hipGraph_t graph = nullptr;
+    CRYO_HIP(hipStreamEndCapture(stream, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    CRYO_HIP(e);
+    struct Exec {
+      hipGraphExec_t x;
+      hipStream_t s;
+      ~Exec() {
+        (void)hipStreamSynchronize(s);
+        (void)hipGraphExecDestroy(x);
+      }
+    } owned_exec{exec, stream};
+    // the capture queued nothing; the host state of every member already
+    // reads two steps on, which is where each launch leaves the device
+    for (int64_t q = 0; q + 1 < steps; q += 2) CRYO_HIP(hipGraphLaunch(exec, stream));
+    CRYO_HIP(hipEventRecord(fork, stream));  // the members' streams go on behind the graph
+    for (int32_t t = 0; t < count; ++t) {
+      cryo_solver& z = *v[t];
+      if (z.stream != stream) CRYO_HIP(hipStreamWaitEvent(z.stream, fork, 0));
+      if (steps % 2) z.step_issue();
+    }
+  }
   void count_abort() {  // an error while the run was queued: take what the device holds
     if (!dev_count) return;
     dev_count = false;
@@ -2201,8 +2258,25 @@ int cryo_ensemble_run_steps(cryo_solver* const* solvers, int32_t count, int64_t 
           for (int32_t t = 0; t < c; ++t) v[t]->count_abort();
         }
       } abort_runs{solvers, count};
+      // One graph for the whole ensemble: when every member replays graphs and
+      // none needs its particle count back between steps (no phase change, or
+      // device counts), two steps of every member (one per persistent buffer
+      // set, so the graph ends where it began) are captured into one graph,
+      // each member on its own stream, forked from and joined to member 0's
+      // stream, and that graph is launched once per two ensemble steps; an odd
+      // last step is queued eagerly behind it.
+      bool one_graph = steps >= 2;
+      for (int32_t t = 0; t < count && one_graph; ++t) {
+        const cryo_solver& z = S(solvers[t]);
+        one_graph = z.use_graphs && z.prof_mask == 0 && z.n > 0 && (!z.needs_counts() || z.uses_device_counts());
+      }
+      if (one_graph) {
+        for (int32_t t = 0; t < count; ++t)
+          if (S(solvers[t]).needs_counts()) S(solvers[t]).count_begin(), dc[t] = 1;
+        S(solvers[0]).ensemble_graph(solvers, count, steps);
+      }
       int32_t waiting = 0;  // members with a step whose counts are not back yet
-      for (int32_t t = 0; t < count; ++t) {
+      for (int32_t t = 0; t < count && !one_graph; ++t) {
         cryo_solver& z = S(solvers[t]);
         if (!z.needs_counts() || (z.uses_device_counts() && steps > 1)) {
           if (z.needs_counts()) z.count_begin(), dc[t] = 1;

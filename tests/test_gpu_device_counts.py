@@ -141,12 +141,50 @@ def test_ensemble_with_device_counts_matches_the_per_tank_loop():
         s.run_steps(56)
     for graphs in (False, True):
         ens = members(True, graphs)
+        # with graphs every call of two or more steps runs the whole ensemble
+        # as one captured graph; 15 ends on an eager step, 1 is a plain step
         cabi.hip_ensemble_run_steps(ens, 40)
-        cabi.hip_ensemble_run_steps(ens, 16)
+        cabi.hip_ensemble_run_steps(ens, 15)
+        cabi.hip_ensemble_run_steps(ens, 1)
         for k, (sa, sb) in enumerate(zip(ref, ens)):
             same(sa, sb, f"member {k} graphs {graphs}")
     d = ref[-1].diagnostics()
     assert d.splits > 0 and d.merges > 0
+
+
+def test_one_graph_ensemble_without_phase_change_matches_the_per_tank_loop():
+    """Members without phase change and with graphs on: the ensemble runs as
+    one captured graph per two steps (an odd call ends on an eager step) and
+    stays bitwise the per-tank loop."""
+    say("test_one_graph_ensemble_without_phase_change_matches_the_per_tank_loop")
+    import copy
+    import sys
+    from pathlib import Path
+
+    sys.path.insert(0, str(Path(__file__).resolve().parent.parent))
+    import bench
+
+    tanks = sorted(bench.c4_tanks(cabi, count=12, seed=7), key=lambda t: len(t["ps"]))[:3]
+
+    def members(graphs):
+        out = []
+        for t in tanks:
+            cfg = copy.copy(t["cfg"])
+            cfg.phase_change = 0
+            s = HipSolver(cfg, t["fluid"])
+            s.set_particles(t["ps"], t["lo"], t["hi"])
+            s.set_frame_motion(acc=(t["a_lat"], 0.0, 0.0))
+            s.set_step_graphs(graphs)
+            out.append(s)
+        return out
+
+    ref, ens = members(False), members(True)
+    for s in ref:
+        s.run_steps(13)
+    cabi.hip_ensemble_run_steps(ens, 6)
+    cabi.hip_ensemble_run_steps(ens, 7)
+    for k, (sa, sb) in enumerate(zip(ref, ens)):
+        same(sa, sb, f"member {k}")
 
 
 def test_device_counts_reject_negative_room():
