@@ -198,7 +198,12 @@ int cryo_solver_set_launch_variant(cryo_solver* s, int32_t variant);
  * persistent buffer set, captured again when the particle count or capacity
  * changes) and replay it for the rest of the call.  Results are the same bits.
  * Default off: on MI355X the replay has not been faster than plain launches for
- * these steps (the device, not the launch path, bounds them). */
+ * these steps (the device, not the launch path, bounds them).  on = 2 also lets
+ * cryo_ensemble_run_steps, when every member is set so and none reads its
+ * particle count back between steps (no phase change, or device counts),
+ * capture two steps of every member as one graph and launch it once per two
+ * ensemble steps (same bits; slower here than separate member streams).  Other
+ * values fail with CRYO_ERR_INVALID_ARGUMENT. */
 int cryo_solver_set_step_graphs(cryo_solver* s, int32_t on);
 /* Extension (no counterpart in the reference, which reads the particle count
  * back after every phase-change step): on != 0 makes cryo_solver_run_steps and

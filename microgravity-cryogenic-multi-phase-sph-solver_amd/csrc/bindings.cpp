@@ -760,9 +760,10 @@ PYBIND11_MODULE(CryoSPH, m) {
          "candidates) at the current launch variant, keeps the fastest, restores the state; returns "
          "(shape, ms per step of each shape).  Results do not depend on the shape.")
       // Extension (cryo_solver_set_step_graphs / set_device_counts): no counterpart in the reference.
-      .def("set_step_graphs", [](Solver& S, bool on) { check(cryo_solver_set_step_graphs(S.s, on ? 1 : 0)); },
+      .def("set_step_graphs", [](Solver& S, int on) { check(cryo_solver_set_step_graphs(S.s, on)); },
            py::arg("on"),
-           "run_steps replays the step as a captured HIP graph for the rest of each call (same results).")
+           "1 (or True): run_steps replays the step as a captured HIP graph for the rest of each call; 2: also "
+           "lets the ensemble runner capture all members as one graph; 0: off (same results either way).")
       .def("set_device_counts", [](Solver& S, bool on, int room) {
         check(cryo_solver_set_device_counts(S.s, on ? 1 : 0, room));
       }, py::arg("on"), py::arg("room") = 0,

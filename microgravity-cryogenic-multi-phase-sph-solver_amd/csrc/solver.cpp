@@ -878,6 +878,7 @@ struct cryo_solver {
   };
   StepGraph sg[2];  // by persistent buffer set
   bool use_graphs = false, graphs_on = false;
+  bool ensemble_graph_on = false;  // the ensemble runner may capture all members as one graph
   void graph_reset() {
     for (StepGraph& x : sg) {
       if (x.exec) (void)hipGraphExecDestroy(x.exec);
@@ -2176,7 +2177,13 @@ int cryo_solver_run_steps(cryo_solver* s, int64_t k) {
   })
 }
 int cryo_solver_set_launch_variant(cryo_solver* s, int32_t variant) { CRYO_GUARD(S(s).set_launch_variant(variant)) }
-int cryo_solver_set_step_graphs(cryo_solver* s, int32_t on) { CRYO_GUARD(S(s).use_graphs = on != 0) }
+int cryo_solver_set_step_graphs(cryo_solver* s, int32_t on) {
+  if (on < 0 || on > 2) return fail("step graphs: on must be 0, 1 or 2", CRYO_ERR_INVALID_ARGUMENT);
+  CRYO_GUARD({
+    S(s).use_graphs = on != 0;
+    S(s).ensemble_graph_on = on == 2;
+  })
+}
 int cryo_solver_set_device_counts(cryo_solver* s, int32_t on, int32_t room) {
   if (room < 0) return fail("device counts: room must be >= 0", CRYO_ERR_INVALID_ARGUMENT);
   CRYO_GUARD({
@@ -2258,9 +2265,9 @@ int cryo_ensemble_run_steps(cryo_solver* const* solvers, int32_t count, int64_t 
           for (int32_t t = 0; t < c; ++t) v[t]->count_abort();
         }
       } abort_runs{solvers, count};
-      // One graph for the whole ensemble: when every member replays graphs and
-      // none needs its particle count back between steps (no phase change, or
-      // device counts), two steps of every member (one per persistent buffer
+      // One graph for the whole ensemble: when every member asks for it (step
+      // graphs mode 2) and none needs its particle count back between steps
+      // (no phase change, or device counts), two steps of every member (one per persistent buffer
       // set, so the graph ends where it began) are captured into one graph,
       // each member on its own stream, forked from and joined to member 0's
       // stream, and that graph is launched once per two ensemble steps; an odd
@@ -2268,7 +2275,7 @@ int cryo_ensemble_run_steps(cryo_solver* const* solvers, int32_t count, int64_t 
       bool one_graph = steps >= 2;
       for (int32_t t = 0; t < count && one_graph; ++t) {
         const cryo_solver& z = S(solvers[t]);
-        one_graph = z.use_graphs && z.prof_mask == 0 && z.n > 0 && (!z.needs_counts() || z.uses_device_counts());
+        one_graph = z.ensemble_graph_on && z.prof_mask == 0 && z.n > 0 && (!z.needs_counts() || z.uses_device_counts());
       }
       if (one_graph) {
         for (int32_t t = 0; t < count; ++t)

@@ -592,7 +592,7 @@ class HipSolver:
         self._call("solver_set_launch_variant", self.h, C.c_int32(v))
 
     def set_step_graphs(self, on):
-        self._call("solver_set_step_graphs", self.h, C.c_int32(1 if on else 0))
+        self._call("solver_set_step_graphs", self.h, C.c_int32(int(on)))
 
     def set_device_counts(self, on, room=0):
         self._call("solver_set_device_counts", self.h, C.c_int32(1 if on else 0), C.c_int32(room))

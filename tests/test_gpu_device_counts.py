@@ -39,7 +39,7 @@ def churn_tank(device_counts=False, room=0, graphs=False, **extra):
     if device_counts:
         s.set_device_counts(True, room)
     if graphs:
-        s.set_step_graphs(True)
+        s.set_step_graphs(graphs)
     return s
 
 
@@ -131,9 +131,9 @@ def test_ensemble_with_device_counts_matches_the_per_tank_loop():
             if dc:
                 s.set_device_counts(True)
             if graphs:
-                s.set_step_graphs(True)
+                s.set_step_graphs(2)
             out.append(s)
-        out.append(churn_tank(dc, 0, graphs))
+        out.append(churn_tank(dc, 0, 2 if graphs else False))
         return out
 
     ref = members(False)
@@ -141,8 +141,8 @@ def test_ensemble_with_device_counts_matches_the_per_tank_loop():
         s.run_steps(56)
     for graphs in (False, True):
         ens = members(True, graphs)
-        # with graphs every call of two or more steps runs the whole ensemble
-        # as one captured graph; 15 ends on an eager step, 1 is a plain step
+        # with graphs (mode 2) every call of two or more steps runs the whole
+        # ensemble as one captured graph; 15 ends on an eager step, 1 is a plain step
         cabi.hip_ensemble_run_steps(ens, 40)
         cabi.hip_ensemble_run_steps(ens, 15)
         cabi.hip_ensemble_run_steps(ens, 1)
@@ -174,7 +174,7 @@ def test_one_graph_ensemble_without_phase_change_matches_the_per_tank_loop():
             s = HipSolver(cfg, t["fluid"])
             s.set_particles(t["ps"], t["lo"], t["hi"])
             s.set_frame_motion(acc=(t["a_lat"], 0.0, 0.0))
-            s.set_step_graphs(graphs)
+            s.set_step_graphs(2 if graphs else 0)
             out.append(s)
         return out
 
