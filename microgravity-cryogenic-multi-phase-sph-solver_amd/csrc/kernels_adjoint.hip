@@ -281,6 +281,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_csf1(PV pv, GridDev g, SimParams
 
 // Per fluid particle: both sides of the curvature pairs, the normalisation,
 // then the contact-angle rotation of its colour gradient (recomputed).
+// EXT: the smoothed-normal or contact-line terms are compiled in (launched so
+// when A.sacc or A.gpb is set).
+template <bool EXT>
 __global__ __launch_bounds__(kBlock) void k_adj_csf2(PV pv, GridDev g, SimParams P, NbrList L, const DevState* st,
                                                     AdjArgs A, int n, int row) {
   const int k = gtid();
@@ -297,9 +300,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_csf2(PV pv, GridDev g, SimParams
     AV xr{0.0f, 0.0f, 0.0f}, nsb{0.0f, 0.0f, 0.0f};
     float Vb = 0.0f, ww = 0.0f;
     V3 gradc{0.0f, 0.0f, 0.0f}, wall_n{0.0f, 0.0f, 0.0f};
-    const bool smooth = A.sacc != nullptr;
+    const bool smooth = EXT && A.sacc != nullptr;
     V3 sacc{0.0f, 0.0f, 0.0f};  // the smoothed normal's sum (k_smooth_normals), recomputed
-    const bool cl = A.gpb != nullptr && !smooth;  // the contact-line force (k_adj_csf3 with smoothing)
+    const bool cl = EXT && A.gpb != nullptr && !smooth;  // the contact-line force (k_adj_csf3 with smoothing)
     V3 gradphi{0.0f, 0.0f, 0.0f};
     for_each_pair(L, g, k, xk, [&](int j) {
       const V3 xj = ld3(pv.s.x[j]);
@@ -374,8 +377,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_csf2(PV pv, GridDev g, SimParams
     A.wnb[k] = z;
     A.csv[k] = 0.0f;
     A.xc[k] = z;
-    if (A.sacc) A.sacc[k] = z;
-    if (A.gpb) A.gpb[k] = z;
+    if (EXT && A.sacc) A.sacc[k] = z;
+    if (EXT && A.gpb) A.gpb[k] = z;
   }
   block_sums_store<2>(th, A.partials + row * A.stride, A.stride);
 }
@@ -1314,7 +1317,8 @@ void launch_adj_csf_only(const PV& pv, const GridDev& g, const SimParams& P, con
                          const AdjArgs& A, int n, double* theta_out, hipStream_t s) {
   const int nb = blocks_for(n > 0 ? n : 1);
   k_adj_csf1<<<nb, kBlock, 0, s>>>(pv, g, P, L, A, n);
-  k_adj_csf2<<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n, 6);
+  if (A.sacc || A.gpb) k_adj_csf2<true><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n, 6);
+  else k_adj_csf2<false><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n, 6);
   if (A.sacc) k_adj_csf3<<<nb, kBlock, 0, s>>>(pv, g, P, L, A, n, 6);
   k_adj_finish<<<1, 1024, 0, s>>>(A.partials, A.stride, nb, 6, -1, 1.0, theta_out, 2);
 }
@@ -1334,7 +1338,8 @@ void launch_adj_step(const PV& pv, const GridDev& g, const SimParams& P, const N
   if (A.heat) k_adj_finish<<<1, 1024, 0, s>>>(A.partials, A.stride, nb, 8, -1, 1.0, A.hsum, 1);
   if (A.csf) {
     k_adj_csf1<<<nb, kBlock, 0, s>>>(pv, g, P, L, A, n);
-    k_adj_csf2<<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n, 6);
+    if (A.sacc || A.gpb) k_adj_csf2<true><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n, 6);
+    else k_adj_csf2<false><<<nb, kBlock, 0, s>>>(pv, g, P, L, st, A, n, 6);
     if (A.sacc) k_adj_csf3<<<nb, kBlock, 0, s>>>(pv, g, P, L, A, n, 6);
     k_adj_finish<<<1, 1024, 0, s>>>(A.partials, A.stride, nb, 6, -1, 1.0, theta_out, 2);
   }
