@@ -238,16 +238,17 @@ def test_fp64_adjoint_with_smoothed_normals_matches_central_differences(lee):
             assert out.lee[which] != 0 and abs(out.lee[which] / fd_lee - 1) <= 2e-4, (name, out.lee[which], fd_lee)
 
 
-@pytest.mark.parametrize("wetting", [1, 2])
-def test_fp64_adjoint_with_the_contact_line_force_matches_central_differences(wetting):
+@pytest.mark.parametrize("wetting,smooth", [(1, 0), (2, 0), (2, 1)])
+def test_fp64_adjoint_with_the_contact_line_force_matches_central_differences(wetting, smooth):
     """The contact-line force of fluid particles near a wall (wetting model 1;
     2: with the normal rotation too), a += t f with t the wall-tangent part of
     the colour gradient's direction and f proportional to (cos theta - cos of
     the current angle), the gradient's size and the wall-indicator gradient:
     acceleration and contact-angle gradients against central differences of the
-    FP64 forward, with conduction and phase change."""
+    FP64 forward, with conduction and phase change (and with smoothed normals)."""
     fluid, cfg, ps, lo, hi, acc = adjoint_case(steps=10, csf=True, lee=50.0)
     cfg.wetting_model = wetting
+    cfg.csf_smooth_normals = smooth
     out = oracle(cfg, fluid, ps, lo, hi).adjoint_slosh(acc)
     grad, dtheta = out[1], out[3]
     fd = np.zeros_like(grad)
